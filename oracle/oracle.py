@@ -1,0 +1,91 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes front end of the C oracle (cc_oracle.c).
+
+The oracle restates CatmullClark.h's refinement path on the CPU.  ``refine``
+returns levels 1..depth in the reference's cumulative layout.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from halfedgecatmullclark_amd.abi import HALFEDGE_DTYPE, crease_dtype, level_counts
+from halfedgecatmullclark_amd.cage import Cage
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+HALFEDGES, CREASES, UVS, POINTS, GATHER, NOCREASES = 1, 2, 4, 8, 16, 32
+ALL = HALFEDGES | CREASES | UVS | POINTS
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run `make -C oracle`)")
+        L = ctypes.CDLL(LIB_PATH)
+        for p in ("orc32_", "orc64_"):
+            f = getattr(L, p + "refine")
+            f.restype = ctypes.c_int32
+            f.argtypes = [ctypes.c_int32] * 5 + [ctypes.c_void_p] * 7 + [ctypes.c_int32] * 3 + [ctypes.c_void_p] * 3
+            g = getattr(L, p + "count")
+            g.restype = ctypes.c_int64
+            g.argtypes = [ctypes.c_int32] * 6
+            m = getattr(L, p + "map")
+            m.restype = ctypes.c_int32
+            m.argtypes = [ctypes.c_int32] * 4 + [ctypes.c_void_p] * 4 + [ctypes.c_int32] * 3
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a.size else ctypes.c_void_p(0)
+
+
+def refine(cage: Cage, depth: int, flags: int = ALL, real=np.float32, threads: int = 0) -> dict:
+    """Refine ``cage`` to ``depth``.  ``flags`` combines HALFEDGES, CREASES,
+    UVS, POINTS, GATHER (gather point rules) and NOCREASES (smooth rules)."""
+    real = np.dtype(real)
+    c = cage.contiguous()
+    n = level_counts(c.V, c.H, c.E, c.F, depth)
+    pts = np.ascontiguousarray(c.vertex_points, dtype=real)
+    uvs = np.ascontiguousarray(c.uvs, dtype=real)
+    cr = np.zeros(c.E, dtype=crease_dtype(real))
+    for k in ("nextID", "prevID", "sharpness"):
+        cr[k] = c.creases[k]
+    out_he = np.zeros(n["Hcum"][depth], dtype=HALFEDGE_DTYPE)
+    out_cr = np.zeros(n["Ccum"][depth], dtype=crease_dtype(real))
+    out_pt = np.zeros((n["Vcum"][depth], 3), dtype=real)
+    f = getattr(lib(), ("orc32_" if real.itemsize == 4 else "orc64_") + "refine")
+    rc = f(c.V, c.U, c.H, c.E, c.F, _p(c.halfedges), _p(c.v2h), _p(c.e2h), _p(c.f2h),
+           _p(pts), _p(uvs), _p(cr), depth, flags, threads, _p(out_he), _p(out_cr), _p(out_pt))
+    if rc != 0:
+        raise ValueError(f"oracle refine failed ({rc})")
+    return {"halfedges": out_he, "creases": out_cr, "points": out_pt, "counts": n}
+
+
+def count(cage: Cage, what: str, depth: int) -> int:
+    code = {"H": 0, "C": 1, "F": 2, "E": 3, "V": 4, "Hcum": 5, "Ccum": 6, "Vcum": 7}[what]
+    return int(lib().orc32_count(cage.V, cage.H, cage.E, cage.F, code, depth))
+
+
+def edge_to_halfedge(cage: Cage, edge: int, depth: int) -> int:
+    c = cage.contiguous()
+    return int(lib().orc32_map(c.V, c.H, c.E, c.F, _p(c.halfedges), _p(c.v2h), _p(c.e2h), _p(c.f2h), 0, edge, depth))
+
+
+def vertex_to_halfedge(cage: Cage, vertex: int, depth: int) -> int:
+    c = cage.contiguous()
+    return int(lib().orc32_map(c.V, c.H, c.E, c.F, _p(c.halfedges), _p(c.v2h), _p(c.e2h), _p(c.f2h), 1, vertex, depth))
+
+
+def level(arrays: dict, name: str, depth: int) -> np.ndarray:
+    """Slice of level ``depth`` (>= 1) out of a cumulative array."""
+    n = arrays["counts"]
+    key = {"halfedges": ("Hcum", "H"), "creases": ("Ccum", "C"), "points": ("Vcum", "V")}[name]
+    lo = n[key[0]][depth - 1]
+    return arrays[name][lo:lo + n[key[1]][depth]]

@@ -1,0 +1,117 @@
+"""TEST INFRASTRUCTURE ONLY: the reference's own CatmullClark.h, compiled by
+oracle/Makefile into oracle/_ref/ (float and double builds), driven through
+its public ABI.  Absent wherever the reference tree was not available at
+build time; tests that need it skip then.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from halfedgecatmullclark_amd.abi import HALFEDGE_DTYPE, MeshView, cc_Mesh, cc_Subd, crease_dtype, level_counts
+from halfedgecatmullclark_amd.cage import Cage
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(HERE, "_ref")
+_libs = {}
+
+DRIVERS = ("ccs_Refine_Scatter", "ccs_Refine_Gather", "ccs_Refine_NoCreases_Scatter",
+           "ccs_Refine_NoCreases_Gather", "ccs_RefineVertexPoints_Scatter",
+           "ccs_RefineVertexPoints_Gather", "ccs_RefineVertexPoints_NoCreases_Scatter",
+           "ccs_RefineVertexPoints_NoCreases_Gather", "ccs_RefineHalfedges", "ccs_RefineCreases",
+           "ccs_RefineVertexUvs")
+
+
+def available(real=np.float32) -> bool:
+    return os.path.exists(_path(real))
+
+
+def obj_to_ccm_path() -> str | None:
+    p = os.path.join(REF_DIR, "obj_to_ccm")
+    return p if os.path.exists(p) else None
+
+
+def _path(real):
+    return os.path.join(REF_DIR, "libccref32.so" if np.dtype(real).itemsize == 4 else "libccref64.so")
+
+
+def lib(real=np.float32):
+    key = np.dtype(real).itemsize
+    if key not in _libs:
+        L = ctypes.CDLL(_path(real), mode=ctypes.RTLD_LOCAL)
+        L.ccs_Create.restype = ctypes.POINTER(cc_Subd)
+        L.ccs_Create.argtypes = [ctypes.POINTER(cc_Mesh), ctypes.c_int32]
+        L.ccs_Release.argtypes = [ctypes.POINTER(cc_Subd)]
+        for name in DRIVERS:
+            getattr(L, name).argtypes = [ctypes.POINTER(cc_Subd)]
+        for name in ("ccs_CumulativeHalfedgeCountAtDepth", "ccs_CumulativeVertexCountAtDepth",
+                     "ccs_CumulativeEdgeCountAtDepth", "ccs_CumulativeFaceCountAtDepth",
+                     "ccs_CumulativeCreaseCountAtDepth", "ccm_HalfedgeCountAtDepth",
+                     "ccm_CreaseCountAtDepth", "ccm_FaceCountAtDepth", "ccm_EdgeCountAtDepth",
+                     "ccm_VertexCountAtDepth"):
+            getattr(L, name).restype = ctypes.c_int32
+            getattr(L, name).argtypes = [ctypes.POINTER(cc_Mesh), ctypes.c_int32]
+        for name in ("ccs_EdgeToHalfedgeID", "ccs_VertexPointToHalfedgeID"):
+            getattr(L, name).restype = ctypes.c_int32
+            getattr(L, name).argtypes = [ctypes.POINTER(cc_Subd), ctypes.c_int32, ctypes.c_int32]
+        L.ref_set_threads.argtypes = [ctypes.c_int]
+        _libs[key] = L
+    return _libs[key]
+
+
+class RefSubd:
+    """A ``cc_Subd`` created by the reference library for ``cage`` and depth."""
+
+    def __init__(self, cage: Cage, depth: int, real=np.float32):
+        self.real = np.dtype(real)
+        self.L = lib(real)
+        self.mesh = MeshView(cage, real)
+        self.depth = depth
+        self.n = level_counts(cage.V, cage.H, cage.E, cage.F, depth)
+        self.subd = self.L.ccs_Create(self.mesh.pointer, depth)
+
+    def run(self, *drivers: str, threads: int = 1) -> "RefSubd":
+        self.L.ref_set_threads(threads)
+        for name in drivers:
+            getattr(self.L, name)(self.subd)
+        return self
+
+    def arrays(self) -> dict:
+        s = self.subd.contents
+        nh, nc, nv = self.n["Hcum"][self.depth], self.n["Ccum"][self.depth], self.n["Vcum"][self.depth]
+        he = np.ctypeslib.as_array(ctypes.cast(s.halfedges, ctypes.POINTER(ctypes.c_int32)), (nh * 4,))
+        cr_bytes = (ctypes.c_char * (nc * crease_dtype(self.real).itemsize)).from_address(s.creases)
+        pr = ctypes.POINTER(ctypes.c_float if self.real.itemsize == 4 else ctypes.c_double)
+        pt = np.ctypeslib.as_array(ctypes.cast(s.vertexPoints, pr), (nv * 3,))
+        return {"halfedges": he.view(HALFEDGE_DTYPE).copy(),
+                "creases": np.frombuffer(bytes(cr_bytes), dtype=crease_dtype(self.real)).copy(),
+                "points": pt.reshape(nv, 3).copy(), "counts": self.n}
+
+    def edge_to_halfedge(self, e: int, d: int) -> int:
+        return int(self.L.ccs_EdgeToHalfedgeID(self.subd, e, d))
+
+    def vertex_to_halfedge(self, v: int, d: int) -> int:
+        return int(self.L.ccs_VertexPointToHalfedgeID(self.subd, v, d))
+
+    def close(self):
+        if self.subd:
+            self.L.ccs_Release(self.subd)
+            self.subd = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def refine(cage: Cage, depth: int, *drivers: str, real=np.float32, threads: int = 1) -> dict:
+    """Run the reference's drivers (default: ccs_Refine_Scatter) and return
+    copies of its cumulative arrays."""
+    r = RefSubd(cage, depth, real)
+    try:
+        return r.run(*(drivers or ("ccs_Refine_Scatter",)), threads=threads).arrays()
+    finally:
+        r.close()
