@@ -1,0 +1,215 @@
+"""ctypes front end of libcchip.so, the GPU drop-in for CatmullClark.h's
+refinement path.  Importing this module loads the library and fails loudly if
+it has not been built; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .abi import HALFEDGE_DTYPE, MeshView, cc_Mesh, cc_Subd, level_counts
+from .cage import CREASE_DTYPE, Cage
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcchip.so")
+
+HALFEDGES, CREASES, UVS, POINTS, GATHER, NOCREASES = 1, 2, 4, 8, 16, 32
+ALL = HALFEDGES | CREASES | UVS | POINTS
+PTR_HALFEDGES, PTR_CREASES, PTR_POINTS, PTR_CAGE_POINTS = 0, 1, 2, 3
+
+HOST_DRIVERS = ("ccs_Refine_Scatter", "ccs_Refine_Gather", "ccs_Refine_NoCreases_Scatter",
+                "ccs_Refine_NoCreases_Gather", "ccs_RefineVertexPoints_Scatter",
+                "ccs_RefineVertexPoints_Gather", "ccs_RefineVertexPoints_NoCreases_Scatter",
+                "ccs_RefineVertexPoints_NoCreases_Gather", "ccs_RefineHalfedges", "ccs_RefineCreases",
+                "ccs_RefineVertexUvs", "ccs_RefineCageHalfedges", "ccs_RefineCageCreases",
+                "ccs_RefineCageVertexUvs", "ccs_RefineCageVertexPoints_Scatter",
+                "ccs_RefineCageVertexPoints_Gather")
+
+
+class CCError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CCError(f"{LIB_PATH} is missing: build it with `python -m halfedgecatmullclark_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    sigs = {
+        "ccs_Create": (P(cc_Subd), [P(cc_Mesh), i32]),
+        "ccs_Release": (None, [P(cc_Subd)]),
+        "ccs_LastError": (i32, []),
+        "ccs_LastErrorString": (ctypes.c_char_p, []),
+        "ccs_RefineVertexPoints_Adjoint": (None, [P(cc_Subd), vp, vp]),
+        "ccd_Create": (vp, [P(cc_Mesh), i32]),
+        "ccd_Release": (None, [vp]),
+        "ccd_MaxDepth": (i32, [vp]),
+        "ccd_Count": (i64, [vp, i32, i32]),
+        "ccd_DevicePointer": (vp, [vp, i32]),
+        "ccd_UploadCagePoints": (i32, [vp, vp]),
+        "ccd_Refine": (i32, [vp, i32, vp]),
+        "ccd_RefinePoints": (i32, [vp, vp, vp, i32, vp]),
+        "ccd_RefinePointsAdjoint": (i32, [vp, vp, vp, i32, vp]),
+        "ccd_Download": (i32, [vp, P(cc_Subd), i32]),
+        "ccd_Synchronize": (i32, [vp]),
+        "ccd_SetTiming": (i32, [vp, i32]),
+        "ccd_TimingCount": (i32, [vp]),
+        "ccd_TimingName": (ctypes.c_char_p, [vp, i32]),
+        "ccd_TimingMs": (ctypes.c_float, [vp, i32]),
+        "ccm_Load": (P(cc_Mesh), [ctypes.c_char_p]),
+        "ccm_Save": (ctypes.c_bool, [P(cc_Mesh), ctypes.c_char_p]),
+        "ccm_Release": (None, [P(cc_Mesh)]),
+    }
+    for name in HOST_DRIVERS:
+        sigs[name] = (None, [P(cc_Subd)])
+    for name in ("ccs_CumulativeHalfedgeCountAtDepth", "ccs_CumulativeVertexCountAtDepth",
+                 "ccs_CumulativeEdgeCountAtDepth", "ccs_CumulativeFaceCountAtDepth",
+                 "ccs_CumulativeCreaseCountAtDepth", "ccm_HalfedgeCountAtDepth", "ccm_CreaseCountAtDepth",
+                 "ccm_FaceCountAtDepth", "ccm_EdgeCountAtDepth", "ccm_VertexCountAtDepth"):
+        sigs[name] = (i32, [P(cc_Mesh), i32])
+    for name in ("ccs_EdgeToHalfedgeID", "ccs_VertexPointToHalfedgeID", "ccs_VertexToHalfedgeID"):
+        sigs[name] = (i32, [P(cc_Subd), i32, i32])
+    for name, (res, args) in sigs.items():
+        try:
+            f = getattr(L, name)
+        except AttributeError:
+            continue  # calling it raises AttributeError
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def last_error() -> tuple[int, str]:
+    L = lib()
+    return int(L.ccs_LastError()), (L.ccs_LastErrorString() or b"").decode()
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        code, msg = last_error()
+        raise CCError(f"{what} failed ({rc}): {msg}")
+
+
+class HostSubd:
+    """A ``cc_Subd`` from ``ccs_Create`` (host arrays, GPU-refined)."""
+
+    def __init__(self, cage: Cage, depth: int):
+        self.L = lib()
+        self.mesh = MeshView(cage)
+        self.depth = depth
+        self.n = level_counts(cage.V, cage.H, cage.E, cage.F, depth)
+        self.subd = self.L.ccs_Create(self.mesh.pointer, depth)
+        if not self.subd:
+            raise CCError(f"ccs_Create failed: {last_error()[1]}")
+
+    def run(self, *drivers: str) -> "HostSubd":
+        for name in drivers:
+            getattr(self.L, name)(self.subd)
+            code, msg = last_error()
+            if code:
+                raise CCError(f"{name}: {msg}")
+        return self
+
+    def arrays(self) -> dict:
+        s = self.subd.contents
+        nh, nc, nv = self.n["Hcum"][self.depth], self.n["Ccum"][self.depth], self.n["Vcum"][self.depth]
+        he = np.ctypeslib.as_array(ctypes.cast(s.halfedges, ctypes.POINTER(ctypes.c_int32)), (nh * 4,))
+        cr = np.ctypeslib.as_array(ctypes.cast(s.creases, ctypes.POINTER(ctypes.c_int32)), (nc * 3,))
+        pt = np.ctypeslib.as_array(ctypes.cast(s.vertexPoints, ctypes.POINTER(ctypes.c_float)), (nv * 3,))
+        return {"halfedges": he.view(HALFEDGE_DTYPE).copy(), "creases": cr.view(CREASE_DTYPE).copy(),
+                "points": pt.reshape(nv, 3).copy(), "counts": self.n}
+
+    def close(self):
+        if getattr(self, "subd", None):
+            self.L.ccs_Release(self.subd)
+            self.subd = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def refine_host(cage: Cage, depth: int, *drivers: str) -> dict:
+    """Run host-semantics drivers (default ``ccs_Refine_Scatter``) and return copies of the arrays."""
+    s = HostSubd(cage, depth)
+    try:
+        return s.run(*(drivers or ("ccs_Refine_Scatter",))).arrays()
+    finally:
+        s.close()
+
+
+class DeviceSubd:
+    """A ``ccd_Subd``: the cage and levels 1..depth resident in HBM."""
+
+    def __init__(self, cage: Cage, depth: int):
+        self.L = lib()
+        self.cage = cage.contiguous()
+        self.mesh = MeshView(self.cage)
+        self.depth = depth
+        self.n = level_counts(cage.V, cage.H, cage.E, cage.F, depth)
+        self.handle = self.L.ccd_Create(self.mesh.pointer, depth)
+        if not self.handle:
+            raise CCError(f"ccd_Create failed: {last_error()[1]}")
+
+    def refine(self, steps: int = ALL, stream: int | None = None):
+        _check(self.L.ccd_Refine(self.handle, steps, stream), "ccd_Refine")
+
+    def refine_points(self, cage_points_ptr: int | None, out_ptr: int | None, steps: int = 0,
+                      stream: int | None = None):
+        _check(self.L.ccd_RefinePoints(self.handle, cage_points_ptr, out_ptr, steps, stream), "ccd_RefinePoints")
+
+    def refine_points_adjoint(self, grad_finest_ptr: int, grad_cage_ptr: int, steps: int = 0,
+                              stream: int | None = None):
+        _check(self.L.ccd_RefinePointsAdjoint(self.handle, grad_finest_ptr, grad_cage_ptr, steps, stream),
+               "ccd_RefinePointsAdjoint")
+
+    def synchronize(self):
+        _check(self.L.ccd_Synchronize(self.handle), "ccd_Synchronize")
+
+    def pointer(self, which: int) -> int:
+        return int(self.L.ccd_DevicePointer(self.handle, which) or 0)
+
+    def upload_cage_points(self, points: np.ndarray):
+        p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        _check(self.L.ccd_UploadCagePoints(self.handle, p.ctypes.data), "ccd_UploadCagePoints")
+
+    def download(self, steps: int = ALL) -> dict:
+        """Copy levels 1..depth to host numpy arrays (through a host cc_Subd)."""
+        nh, nc, nv = self.n["Hcum"][self.depth], self.n["Ccum"][self.depth], self.n["Vcum"][self.depth]
+        he = np.zeros(nh, HALFEDGE_DTYPE)
+        cr = np.zeros(nc, CREASE_DTYPE)
+        pt = np.zeros((nv, 3), np.float32)
+        sub = cc_Subd(self.mesh.pointer, pt.ctypes.data, he.ctypes.data, cr.ctypes.data, self.depth)
+        _check(self.L.ccd_Download(self.handle, ctypes.byref(sub), steps), "ccd_Download")
+        return {"halfedges": he, "creases": cr, "points": pt, "counts": self.n}
+
+    def set_timing(self, on: bool):
+        self.L.ccd_SetTiming(self.handle, int(on))
+
+    def timings(self) -> list[tuple[str, float]]:
+        n = self.L.ccd_TimingCount(self.handle)
+        return [(self.L.ccd_TimingName(self.handle, i).decode(), float(self.L.ccd_TimingMs(self.handle, i)))
+                for i in range(n)]
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.L.ccd_Release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

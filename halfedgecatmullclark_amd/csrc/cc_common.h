@@ -1,0 +1,164 @@
+// Shared host/device definitions of the refinement path: record layouts,
+// exact per-level counts, and the scalar rules every kernel uses.
+//
+// Every floating-point helper reproduces the operation order of
+// CatmullClark.h so that, compiled with -ffp-contract=off, results are
+// bitwise equal to the reference where the reference itself is
+// deterministic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cch {
+
+// Deepest level whose IDs fit int32 for the smallest cage (3 halfedges).
+constexpr int kMaxDepth = 15;
+
+// Record layouts, byte-identical to CatmullClark.h:30-55.
+struct CageHe { int32_t twin, next, prev, face, edge, vert, uv; };  // 28 B
+struct alignas(16) He { int32_t twin, edge, vert, uv; };           // 16 B
+struct Crease { int32_t next, prev; float sharp; };                 // 12 B
+struct P3 { float x, y, z; };                                       // 12 B
+struct UV { float u, v; };                                          //  8 B
+
+static_assert(sizeof(CageHe) == 28, "cc_Halfedge layout");
+static_assert(sizeof(He) == 16, "cc_Halfedge_SemiRegular layout");
+static_assert(sizeof(Crease) == 12, "cc_Crease layout");
+static_assert(sizeof(P3) == 12, "cc_VertexPoint layout");
+
+// Exact per-level counts (CatmullClark.h:455-585, :811-936) from the
+// recurrences F' = H, E' = 2E + H, H' = 4H, C' = 2C, V' = V + E + F.
+// off*[d] is the offset of level d (d >= 1) in the cumulative arrays.
+struct Counts {
+    int64_t H[kMaxDepth + 2], C[kMaxDepth + 2], F[kMaxDepth + 2], E[kMaxDepth + 2], V[kMaxDepth + 2];
+    int64_t offH[kMaxDepth + 2], offC[kMaxDepth + 2], offV[kMaxDepth + 2];
+
+    void init(int64_t V0, int64_t H0, int64_t E0, int64_t F0) {
+        H[0] = H0; C[0] = E0; F[0] = F0; E[0] = E0; V[0] = V0;
+        offH[0] = offC[0] = offV[0] = 0;
+        offH[1] = offC[1] = offV[1] = 0;
+        for (int d = 1; d <= kMaxDepth + 1; ++d) {
+            H[d] = 4 * H[d - 1];
+            C[d] = 2 * C[d - 1];
+            F[d] = H[d - 1];
+            E[d] = 2 * E[d - 1] + H[d - 1];
+            V[d] = V[d - 1] + E[d - 1] + F[d - 1];
+            if (d + 1 <= kMaxDepth + 1) {
+                offH[d + 1] = offH[d] + H[d];
+                offC[d + 1] = offC[d] + C[d];
+                offV[d + 1] = offV[d] + V[d];
+            }
+        }
+    }
+    // cumulative counts over levels 1..d
+    int64_t cumH(int d) const { return offH[d + 1]; }
+    int64_t cumC(int d) const { return offC[d + 1]; }
+    int64_t cumV(int d) const { return offV[d + 1]; }
+};
+
+// int32 per-level table passed by value to kernels that walk the O(depth)
+// edge/vertex maps.
+struct LevelTab {
+    int32_t E[kMaxDepth + 1], V[kMaxDepth + 1], F[kMaxDepth + 1];
+};
+
+// Device view of a cage (kernel argument).
+struct DCage {
+    const CageHe *he;
+    const int32_t *v2h, *e2h, *f2h;
+    const P3 *pts;
+    const UV *uvs;
+    const Crease *cr;
+    int32_t V, U, H, E, F;
+};
+
+// ---- scalar rules (CatmullClark.h:296-392, :667-689) ----------------------
+__host__ __device__ __forceinline__ int32_t qnext(int32_t h) {
+    return (int32_t)(((uint32_t)h & ~3u) | (((uint32_t)h + 1u) & 3u));
+}
+__host__ __device__ __forceinline__ int32_t qprev(int32_t h) {
+    return (int32_t)(((uint32_t)h & ~3u) | (((uint32_t)h - 1u) & 3u));
+}
+__host__ __device__ __forceinline__ int32_t i4(int32_t x) { return (int32_t)((uint32_t)x * 4u); }
+__host__ __device__ __forceinline__ int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
+__host__ __device__ __forceinline__ float fminr(float a, float b) { return a < b ? a : b; }
+__host__ __device__ __forceinline__ float fmaxr(float a, float b) { return a > b ? a : b; }
+__host__ __device__ __forceinline__ float sat(float x) { return fmaxr(0.0f, fminr(x, 1.0f)); }
+__host__ __device__ __forceinline__ float sgn(float x) { return x < 0.0f ? -1.0f : (x > 0.0f ? 1.0f : 0.0f); }
+__host__ __device__ __forceinline__ float lerp1(float x, float y, float u) { return x + u * (y - x); }
+
+__host__ __device__ __forceinline__ P3 add(P3 a, P3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ __forceinline__ P3 mul(P3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__host__ __device__ __forceinline__ P3 lerp(P3 a, P3 b, float u) {
+    return {lerp1(a.x, b.x, u), lerp1(a.y, b.y, u), lerp1(a.z, b.z, u)};
+}
+__host__ __device__ __forceinline__ P3 zero3() { return {0.0f, 0.0f, 0.0f}; }
+
+// UV codec (CatmullClark.h:374-392).  The encode keeps the low 32 bits of a
+// 64-bit truncating conversion, which is what x86-64 compilers emit for the
+// reference's float -> uint32_t conversion, so out-of-range inputs agree too.
+__host__ __device__ __forceinline__ UV uv_decode(int32_t code) {
+    uint32_t t = (uint32_t)code;
+    return {(float)(t & 0xFFFFu) / 65535.0f, (float)((t >> 16) & 0xFFFFu) / 65535.0f};
+}
+__host__ __device__ __forceinline__ uint32_t f2u_trunc(float x) { return (uint32_t)(int64_t)x; }
+__host__ __device__ __forceinline__ int32_t uv_encode(UV uv) {
+    uint32_t u = f2u_trunc(uv.u * 65535.0f);
+    uint32_t v = f2u_trunc(uv.v * 65535.0f);
+    return (int32_t)((u & 0xFFFFu) | ((v & 0xFFFFu) << 16));
+}
+__host__ __device__ __forceinline__ UV uv_mid(UV a, UV b) {
+    return {lerp1(a.u, b.u, 0.5f), lerp1(a.v, b.v, 0.5f)};
+}
+
+// O(depth) maps (CatmullClark.h:1206-1395): a representative halfedge of an
+// edge / vertex at level d >= 1.
+__host__ __device__ inline int32_t edge_rep_first(const DCage &c, int32_t e) {
+    if (e >= 2 * c.E) {
+        int32_t h = e - 2 * c.E;
+        return imax(4 * h + 1, 4 * c.he[h].next + 2);
+    } else if (e & 1) {
+        return 4 * c.he[c.e2h[e >> 1]].next + 3;
+    }
+    return 4 * c.e2h[e >> 1] + 0;
+}
+
+__host__ __device__ inline int32_t edge_rep(const DCage &c, const LevelTab &t, int32_t e, int d) {
+    uint32_t path = 1u;
+    int32_t rep = 0;
+    int k = d;
+    for (; k > 1; --k) {
+        int32_t Ep = t.E[k - 1];
+        if (e >= 2 * Ep) {
+            int32_t h = e - 2 * Ep;
+            rep = imax(4 * h + 1, 4 * qnext(h) + 2);
+            break;
+        }
+        path = (path << 1) | (uint32_t)(e & 1);
+        e >>= 1;
+    }
+    if (k == 1) rep = edge_rep_first(c, e);
+    for (; path > 1u; path >>= 1) rep = (path & 1u) ? 4 * qnext(rep) + 3 : 4 * rep + 0;
+    return rep;
+}
+
+__host__ __device__ inline int32_t vertex_rep(const DCage &c, const LevelTab &t, int32_t v, int d) {
+    int shift = 0;
+    int32_t rep = 0;
+    int k = d;
+    for (; k > 1; --k) {
+        int32_t Vp = t.V[k - 1], Fp = t.F[k - 1];
+        if (v >= Vp + Fp) { rep = 4 * edge_rep(c, t, v - Vp - Fp, k - 1) + 1; break; }
+        if (v >= Vp) { rep = 4 * (4 * (v - Vp)) + 2; break; }
+        shift += 2;
+    }
+    if (k == 1) {
+        if (v >= c.V + c.F) rep = 4 * c.e2h[v - c.V - c.F] + 1;
+        else if (v >= c.V) rep = 4 * c.f2h[v - c.V] + 2;
+        else rep = 4 * c.v2h[v] + 0;
+    }
+    return rep << shift;
+}
+
+}  // namespace cch

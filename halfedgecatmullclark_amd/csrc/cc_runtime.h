@@ -1,0 +1,47 @@
+// The device mirror of a cc_Subd (the opaque ccd_Subd of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "cc_internal.h"
+#include "cc_kernels.h"
+
+struct ccd_Subd {
+    struct Timed {
+        const char *name;
+        hipEvent_t start, stop;
+    };
+
+    int device = 0;
+    int32_t D = 0, V0 = 0, U0 = 0, H0 = 0, E0 = 0, F0 = 0;
+    cch::Counts n;
+    cch::LevelTab tab;
+    // cage (level 0) on the device
+    cch::CageHe *c_he = nullptr;
+    int32_t *c_v2h = nullptr, *c_e2h = nullptr, *c_f2h = nullptr;
+    cch::P3 *c_pts = nullptr;
+    cch::UV *c_uvs = nullptr;
+    cch::Crease *c_cr = nullptr;
+    // levels 1..D, cumulative, laid out as cc_Subd's arrays
+    cch::He *he = nullptr;
+    cch::Crease *cr = nullptr;
+    cch::P3 *pts = nullptr;
+    hipStream_t stream = nullptr;
+    bool timing = false;
+    std::vector<Timed> timed;
+
+    bool alloc(const cc_Mesh *cage, int32_t depth);
+    void free_all();
+    bool upload_cage(const cc_Mesh *cage, hipStream_t st);
+    cch::DCage cage_view(const cch::P3 *points) const;
+    cch::Lvl level_view(int d, const cch::P3 *levelPts, cch::P3 *nextPts) const;
+    bool refine(int32_t steps, int dmax, hipStream_t st);
+    bool refine_topology(int32_t steps, int dmax, hipStream_t st);
+    bool refine_points(int32_t steps, int dmax, const cch::P3 *cagePts, cch::P3 *out, hipStream_t st);
+    void begin_timing();
+    void zero(float *p, int64_t count, hipStream_t st);
+    template <typename K, typename... Args>
+    void launch(const char *name, K kernel, int64_t threads, hipStream_t st, Args... args);
+};

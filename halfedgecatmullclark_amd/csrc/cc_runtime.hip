@@ -1,0 +1,539 @@
+// Runtime of the C ABI: device mirrors of cc_Subd (ccd_Subd), the per-level
+// launch sequence, the drop-in ccs_Refine* entry points with the reference's
+// host semantics (CatmullClark.h:2631-2683, :2803-2812, :2928-2939,
+// :3033-3042, :3051-3082), the device-resident ccd_* API and the error
+// channel.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "cc_internal.h"
+#include "cc_kernels.h"
+#include "cc_runtime.h"
+
+using namespace cch;
+
+// ---- error channel -----------------------------------------------------------
+namespace {
+thread_local int32_t t_err = 0;
+thread_local char t_msg[512] = "";
+}  // namespace
+
+namespace cch {
+void set_error(int32_t code, const char *msg) {
+    t_err = code;
+    snprintf(t_msg, sizeof(t_msg), "%s", msg);
+}
+void clear_error() {
+    t_err = 0;
+    t_msg[0] = '\0';
+}
+}  // namespace cch
+
+extern "C" int32_t ccs_LastError(void) { return t_err; }
+extern "C" const char *ccs_LastErrorString(void) { return t_msg; }
+
+namespace {
+
+// reports a HIP failure on the error channel and stderr; returns true on error
+bool hip_failed(hipError_t e, const char *what) {
+    if (e == hipSuccess) return false;
+    char buf[400];
+    snprintf(buf, sizeof(buf), "%s: %s", what, hipGetErrorString(e));
+    fprintf(stderr, "cc: %s\n", buf);
+    set_error(100 + (int32_t)e, buf);
+    return true;
+}
+
+#define CC_TRY(expr, what) \
+    do { if (hip_failed((expr), (what))) return false; } while (0)
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+constexpr int kBlock = 256;
+
+}  // namespace
+
+// ---- device mirror -----------------------------------------------------------
+bool ccd_Subd::alloc(const cc_Mesh *cage, int32_t depth) {
+    D = depth;
+    V0 = cage->vertexCount; U0 = cage->uvCount; H0 = cage->halfedgeCount; E0 = cage->edgeCount; F0 = cage->faceCount;
+    n.init(V0, H0, E0, F0);
+    tab = level_tab(n);
+    CC_TRY(hipGetDevice(&device), "hipGetDevice");
+    CC_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    CC_TRY(hipMalloc(&c_he, sizeof(CageHe) * (size_t)H0), "hipMalloc(cage halfedges)");
+    CC_TRY(hipMalloc(&c_v2h, sizeof(int32_t) * (size_t)(V0 + 1)), "hipMalloc(cage v2h)");
+    CC_TRY(hipMalloc(&c_e2h, sizeof(int32_t) * (size_t)(E0 + 1)), "hipMalloc(cage e2h)");
+    CC_TRY(hipMalloc(&c_f2h, sizeof(int32_t) * (size_t)(F0 + 1)), "hipMalloc(cage f2h)");
+    CC_TRY(hipMalloc(&c_pts, sizeof(P3) * (size_t)(V0 + 1)), "hipMalloc(cage points)");
+    CC_TRY(hipMalloc(&c_uvs, sizeof(UV) * (size_t)(U0 + 1)), "hipMalloc(cage uvs)");
+    CC_TRY(hipMalloc(&c_cr, sizeof(Crease) * (size_t)(E0 + 1)), "hipMalloc(cage creases)");
+    CC_TRY(hipMalloc(&he, sizeof(He) * (size_t)n.cumH(D)), "hipMalloc(halfedges)");
+    CC_TRY(hipMalloc(&cr, sizeof(Crease) * (size_t)n.cumC(D)), "hipMalloc(creases)");
+    CC_TRY(hipMalloc(&pts, sizeof(P3) * (size_t)n.cumV(D)), "hipMalloc(points)");
+    return true;
+}
+
+void ccd_Subd::free_all() {
+    DeviceGuard g(device);
+    void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto &t : timed) {
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
+    }
+    timed.clear();
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+bool ccd_Subd::upload_cage(const cc_Mesh *cage, hipStream_t st) {
+    if (cage->vertexCount != V0 || cage->uvCount != U0 || cage->halfedgeCount != H0 ||
+        cage->edgeCount != E0 || cage->faceCount != F0) {
+        set_error(10, "cage counts differ from the ones the device mirror was created for");
+        return false;
+    }
+    CC_TRY(hipMemcpyAsync(c_he, cage->halfedges, sizeof(CageHe) * (size_t)H0, hipMemcpyHostToDevice, st), "upload cage");
+    CC_TRY(hipMemcpyAsync(c_v2h, cage->vertexToHalfedgeIDs, sizeof(int32_t) * (size_t)V0, hipMemcpyHostToDevice, st), "upload cage");
+    CC_TRY(hipMemcpyAsync(c_e2h, cage->edgeToHalfedgeIDs, sizeof(int32_t) * (size_t)E0, hipMemcpyHostToDevice, st), "upload cage");
+    CC_TRY(hipMemcpyAsync(c_f2h, cage->faceToHalfedgeIDs, sizeof(int32_t) * (size_t)F0, hipMemcpyHostToDevice, st), "upload cage");
+    CC_TRY(hipMemcpyAsync(c_pts, cage->vertexPoints, sizeof(P3) * (size_t)V0, hipMemcpyHostToDevice, st), "upload cage");
+    if (U0 > 0) CC_TRY(hipMemcpyAsync(c_uvs, cage->uvs, sizeof(UV) * (size_t)U0, hipMemcpyHostToDevice, st), "upload cage");
+    CC_TRY(hipMemcpyAsync(c_cr, cage->creases, sizeof(Crease) * (size_t)E0, hipMemcpyHostToDevice, st), "upload cage");
+    return true;
+}
+
+DCage ccd_Subd::cage_view(const P3 *points) const {
+    DCage c;
+    c.he = c_he; c.v2h = c_v2h; c.e2h = c_e2h; c.f2h = c_f2h;
+    c.pts = points ? points : c_pts;
+    c.uvs = c_uvs; c.cr = c_cr;
+    c.V = V0; c.U = U0; c.H = H0; c.E = E0; c.F = F0;
+    return c;
+}
+
+Lvl ccd_Subd::level_view(int d, const P3 *levelPts, P3 *nextPts) const {
+    Lvl L;
+    L.he = he + n.offH[d];
+    L.cr = cr + n.offC[d];
+    L.old = levelPts;
+    L.vp = nextPts;
+    L.fp = nextPts + n.V[d];
+    L.ep = L.fp + n.F[d];
+    L.H = (int32_t)n.H[d]; L.V = (int32_t)n.V[d]; L.F = (int32_t)n.F[d]; L.E = (int32_t)n.E[d]; L.C = (int32_t)n.C[d];
+    return L;
+}
+
+void ccd_Subd::begin_timing() {
+    for (auto &t : timed) {
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
+    }
+    timed.clear();
+}
+
+template <typename K, typename... Args>
+void ccd_Subd::launch(const char *name, K kernel, int64_t threads, hipStream_t st, Args... args) {
+    if (threads <= 0) return;
+    const unsigned grid = (unsigned)((threads + kBlock - 1) / kBlock);
+    Timed t{name, nullptr, nullptr};
+    if (timing) {
+        (void)hipEventCreate(&t.start);
+        (void)hipEventCreate(&t.stop);
+        (void)hipEventRecord(t.start, st);
+    }
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, st, args...);
+    if (timing) {
+        (void)hipEventRecord(t.stop, st);
+        timed.push_back(t);
+    }
+}
+
+void ccd_Subd::zero(float *p, int64_t count, hipStream_t st) {
+    if (count <= 0) return;
+    const int64_t blocks = (count + kBlock - 1) / kBlock;
+    const unsigned grid = (unsigned)(blocks < 65536 ? blocks : 65536);
+    hipLaunchKernelGGL(k_zero, dim3(grid), dim3(kBlock), 0, st, p, count);
+}
+
+// Topology, creases and uvs of levels 1..dmax (CatmullClark.h:2803-2812,
+// :2928-2939, :3033-3042).  uvMode: 0 leave uvIDs, 1 refine, 2 write zero.
+bool ccd_Subd::refine_topology(int32_t steps, int dmax, hipStream_t st) {
+    const bool topo = (steps & CCD_HALFEDGES) != 0;
+    int uvMode = ((steps & CCD_UVS) && U0 > 0) ? 1 : 0;
+    if (topo && uvMode == 0 && U0 == 0) uvMode = 2;
+    if (topo || uvMode == 1) {
+        const DCage c = cage_view(nullptr);
+        launch("cage_halfedges", k_cage_halfedges, H0, st, c, he, (int)topo, uvMode);
+        for (int d = 1; d < dmax; ++d) {
+            launch("level_halfedges", k_level_halfedges, n.H[d], st, (const He *)(he + n.offH[d]), he + n.offH[d + 1],
+                   (int32_t)n.H[d], (int32_t)n.V[d], (int32_t)n.F[d], (int32_t)n.E[d], (int)topo, uvMode);
+        }
+    }
+    if (steps & CCD_CREASES) {
+        launch("cage_creases", k_creases, E0, st, (const Crease *)c_cr, E0, cr);
+        for (int d = 1; d < dmax; ++d)
+            launch("level_creases", k_creases, n.C[d], st, (const Crease *)(cr + n.offC[d]), (int32_t)n.C[d], cr + n.offC[d + 1]);
+    }
+    return true;
+}
+
+// Points of levels 1..dmax (CatmullClark.h:2631-2683): the cage step, then
+// face, edge and vertex points per level.
+bool ccd_Subd::refine_points(int32_t steps, int dmax, const P3 *cagePts, P3 *out, hipStream_t st) {
+    const bool gather = (steps & CCD_GATHER) != 0;
+    const int creased = (steps & CCD_NOCREASES) ? 0 : 1;
+    const DCage c = cage_view(cagePts);
+    {
+        P3 *vp = out, *fp = out + V0, *ep = out + V0 + F0;
+        launch("cage_faces", k_cage_faces, F0, st, c, fp, (int)gather);
+        launch("cage_edges", k_cage_edges, E0, st, c, (const P3 *)fp, ep, (int)gather, creased);
+        if (gather) {
+            launch("cage_vertices", k_cage_vertices_gather, V0, st, c, (const P3 *)fp, (const P3 *)ep, vp, creased);
+        } else {
+            zero(&vp->x, 3 * (int64_t)V0, st);
+            launch("cage_vertices", k_cage_vertices_scatter, H0, st, c, (const P3 *)fp, (const P3 *)ep, vp, creased);
+        }
+    }
+    for (int d = 1; d < dmax; ++d) {
+        const Lvl L = level_view(d, out + n.offV[d], out + n.offV[d + 1]);
+        launch("faces", k_faces, n.F[d], st, L, (int)gather);
+        if (gather) {
+            launch("edges", k_edges_gather, n.E[d], st, L, c, tab, d, creased);
+            launch("vertices", k_vertices_gather, n.V[d], st, L, c, tab, d, creased);
+        } else {
+            launch("edges", k_edges_scatter, n.H[d], st, L, creased);
+            zero(&L.vp->x, 3 * n.V[d], st);
+            launch("vertices", k_vertices_scatter, n.H[d], st, L, creased);
+        }
+    }
+    return true;
+}
+
+bool ccd_Subd::refine(int32_t steps, int dmax, hipStream_t st) {
+    if (timing) begin_timing();
+    if (!refine_topology(steps, dmax, st)) return false;
+    if (steps & CCD_POINTS) {
+        if (!refine_points(steps, dmax, nullptr, pts, st)) return false;
+    }
+    CC_TRY(hipGetLastError(), "kernel launch");
+    return true;
+}
+
+// ---- drop-in host API ----------------------------------------------------------
+namespace {
+
+std::mutex g_mu;
+std::unordered_map<const cc_Subd *, ccd_Subd *> g_mirrors;
+
+ccd_Subd *create_mirror(const cc_Mesh *cage, int32_t depth) {
+    ccd_Subd *s = new (std::nothrow) ccd_Subd();
+    if (!s) { set_error(2, "out of host memory"); return nullptr; }
+    if (!s->alloc(cage, depth)) {
+        s->free_all();
+        delete s;
+        return nullptr;
+    }
+    return s;
+}
+
+ccd_Subd *mirror_for(const cc_Subd *subd) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_mirrors.find(subd);
+    const cc_Mesh *c = subd->cage;
+    if (it != g_mirrors.end()) {
+        ccd_Subd *s = it->second;
+        if (s->D == subd->maxDepth && s->V0 == c->vertexCount && s->U0 == c->uvCount && s->H0 == c->halfedgeCount &&
+            s->E0 == c->edgeCount && s->F0 == c->faceCount)
+            return s;
+        s->free_all();
+        delete s;
+        g_mirrors.erase(it);
+    }
+    ccd_Subd *s = create_mirror(c, subd->maxDepth);
+    if (s) g_mirrors[subd] = s;
+    return s;
+}
+
+void drop_mirror(const cc_Subd *subd) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_mirrors.find(subd);
+    if (it != g_mirrors.end()) {
+        it->second->free_all();
+        delete it->second;
+        g_mirrors.erase(it);
+    }
+}
+
+// Run `steps` for levels 1..dmax of a host cc_Subd on the GPU.  The data a
+// step reads from the subd (the topology and creases that a points-only call
+// consumes, the other field of a partially rewritten halfedge record) is
+// uploaded first, so the result equals what the reference computes from the
+// current host arrays.
+bool host_refine(cc_Subd *subd, int32_t steps, int dmax, const char *fname) {
+    clear_error();
+    if (!subd || !subd->cage) {
+        set_error(11, "null subd or cage");
+        fprintf(stderr, "cc: %s: null subd or cage\n", fname);
+        return false;
+    }
+    const char *why = nullptr;
+    if (!ref_counts_fit_int32(subd->cage, subd->maxDepth, &why)) {
+        char buf[256];
+        snprintf(buf, sizeof(buf), "%s: %s", fname, why);
+        fprintf(stderr, "cc: %s\n", buf);
+        set_error(12, buf);
+        return false;
+    }
+    ccd_Subd *s = mirror_for(subd);
+    if (!s) return false;
+    DeviceGuard g(s->device);
+    hipStream_t st = s->stream;
+    const cc_Mesh *cage = subd->cage;
+    if (!s->upload_cage(cage, st)) return false;
+    const Counts &n = s->n;
+    const bool uvActive = (steps & CCD_UVS) && s->U0 > 0;
+    const bool writesHe = (steps & CCD_HALFEDGES) || uvActive;
+    const bool partialHe = (steps & CCD_HALFEDGES) ? (!uvActive && s->U0 > 0) : uvActive;
+    const bool pointsNeedTopo = (steps & CCD_POINTS) && !(steps & CCD_HALFEDGES);
+    if (partialHe || pointsNeedTopo) {
+        const int64_t count = partialHe ? n.cumH(dmax) : n.cumH(dmax - 1);
+        CC_TRY(hipMemcpyAsync(s->he, subd->halfedges, sizeof(He) * (size_t)count, hipMemcpyHostToDevice, st), "upload halfedges");
+    }
+    if ((steps & CCD_POINTS) && !(steps & CCD_CREASES) && dmax > 1) {
+        CC_TRY(hipMemcpyAsync(s->cr, subd->creases, sizeof(Crease) * (size_t)n.cumC(dmax - 1), hipMemcpyHostToDevice, st), "upload creases");
+    }
+    if (!s->refine(steps, dmax, st)) return false;
+    if (writesHe)
+        CC_TRY(hipMemcpyAsync(subd->halfedges, s->he, sizeof(He) * (size_t)n.cumH(dmax), hipMemcpyDeviceToHost, st), "download halfedges");
+    if (steps & CCD_CREASES)
+        CC_TRY(hipMemcpyAsync(subd->creases, s->cr, sizeof(Crease) * (size_t)n.cumC(dmax), hipMemcpyDeviceToHost, st), "download creases");
+    if (steps & CCD_POINTS)
+        CC_TRY(hipMemcpyAsync(subd->vertexPoints, s->pts, sizeof(P3) * (size_t)n.cumV(dmax), hipMemcpyDeviceToHost, st), "download points");
+    CC_TRY(hipStreamSynchronize(st), "refine");
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+cc_Subd *ccs_Create(const cc_Mesh *cage, int32_t maxDepth) {
+    clear_error();
+    const char *why = nullptr;
+    if (!cage || !ref_counts_fit_int32(cage, maxDepth, &why)) {
+        char buf[256];
+        snprintf(buf, sizeof(buf), "ccs_Create: %s", cage ? why : "null cage");
+        fprintf(stderr, "cc: %s\n", buf);
+        set_error(12, buf);
+        return NULL;
+    }
+    Counts n;
+    n.init(cage->vertexCount, cage->halfedgeCount, cage->edgeCount, cage->faceCount);
+    cc_Subd *s = (cc_Subd *)malloc(sizeof(cc_Subd));
+    if (!s) { set_error(2, "ccs_Create: out of host memory"); return NULL; }
+    s->maxDepth = maxDepth;
+    s->cage = cage;
+    s->halfedges = (cc_Halfedge_SemiRegular *)malloc(sizeof(cc_Halfedge_SemiRegular) * (size_t)n.cumH(maxDepth));
+    s->creases = (cc_Crease *)malloc(sizeof(cc_Crease) * (size_t)n.cumC(maxDepth) + 1);
+    s->vertexPoints = (cc_VertexPoint *)malloc(sizeof(cc_VertexPoint) * (size_t)n.cumV(maxDepth));
+    if (!s->halfedges || !s->creases || !s->vertexPoints) {
+        free(s->halfedges); free(s->creases); free(s->vertexPoints); free(s);
+        set_error(2, "ccs_Create: out of host memory");
+        fprintf(stderr, "cc: ccs_Create: out of host memory\n");
+        return NULL;
+    }
+    return s;
+}
+
+void ccs_Release(cc_Subd *subd) {
+    if (!subd) return;
+    drop_mirror(subd);
+    free(subd->halfedges);
+    free(subd->creases);
+    free(subd->vertexPoints);
+    free(subd);
+}
+
+#define CC_HOST_ENTRY(name, steps, dmax) \
+    void name(cc_Subd *subd) { (void)host_refine(subd, (steps), (dmax), #name); }
+
+CC_HOST_ENTRY(ccs_Refine_Scatter, CCD_ALL, subd ? subd->maxDepth : 1)
+CC_HOST_ENTRY(ccs_Refine_Gather, CCD_ALL | CCD_GATHER, subd ? subd->maxDepth : 1)
+CC_HOST_ENTRY(ccs_Refine_NoCreases_Scatter, CCD_ALL | CCD_NOCREASES, subd ? subd->maxDepth : 1)
+CC_HOST_ENTRY(ccs_Refine_NoCreases_Gather, CCD_ALL | CCD_GATHER | CCD_NOCREASES, subd ? subd->maxDepth : 1)
+CC_HOST_ENTRY(ccs_RefineVertexPoints_Scatter, CCD_POINTS, subd ? subd->maxDepth : 1)
+CC_HOST_ENTRY(ccs_RefineVertexPoints_Gather, CCD_POINTS | CCD_GATHER, subd ? subd->maxDepth : 1)
+CC_HOST_ENTRY(ccs_RefineVertexPoints_NoCreases_Scatter, CCD_POINTS | CCD_NOCREASES, subd ? subd->maxDepth : 1)
+CC_HOST_ENTRY(ccs_RefineVertexPoints_NoCreases_Gather, CCD_POINTS | CCD_GATHER | CCD_NOCREASES, subd ? subd->maxDepth : 1)
+CC_HOST_ENTRY(ccs_RefineHalfedges, CCD_HALFEDGES, subd ? subd->maxDepth : 1)
+CC_HOST_ENTRY(ccs_RefineCreases, CCD_CREASES, subd ? subd->maxDepth : 1)
+CC_HOST_ENTRY(ccs_RefineCageHalfedges, CCD_HALFEDGES, 1)
+CC_HOST_ENTRY(ccs_RefineCageCreases, CCD_CREASES, 1)
+CC_HOST_ENTRY(ccs_RefineCageVertexUvs, CCD_UVS, 1)
+CC_HOST_ENTRY(ccs_RefineCageVertexPoints_Scatter, CCD_POINTS, 1)
+CC_HOST_ENTRY(ccs_RefineCageVertexPoints_Gather, CCD_POINTS | CCD_GATHER, 1)
+
+// no-op without cage uvs, like the reference (CatmullClark.h:2930)
+void ccs_RefineVertexUvs(cc_Subd *subd) {
+    if (subd && subd->cage && subd->cage->uvCount <= 0) { clear_error(); return; }
+    (void)host_refine(subd, CCD_UVS, subd ? subd->maxDepth : 1, "ccs_RefineVertexUvs");
+}
+
+// ---- device-resident API -------------------------------------------------------
+ccd_Subd *ccd_Create(const cc_Mesh *cage, int32_t maxDepth) {
+    clear_error();
+    if (!cage || maxDepth < 1 || maxDepth > kMaxDepth || cage->halfedgeCount < 3 || cage->edgeCount < 1 ||
+        cage->faceCount < 1 || cage->vertexCount < 1) {
+        set_error(12, "ccd_Create: invalid cage or depth");
+        return nullptr;
+    }
+    Counts n;
+    n.init(cage->vertexCount, cage->halfedgeCount, cage->edgeCount, cage->faceCount);
+    for (int d = 0; d <= maxDepth; ++d) {
+        if (n.H[d] > INT32_MAX || n.V[d] + n.E[d] + n.F[d] > INT32_MAX || 2 * n.E[d] + n.H[d] > INT32_MAX) {
+            set_error(12, "ccd_Create: IDs at this depth overflow int32");
+            return nullptr;
+        }
+    }
+    ccd_Subd *s = create_mirror(cage, maxDepth);
+    if (!s) return nullptr;
+    DeviceGuard g(s->device);
+    if (!s->upload_cage(cage, s->stream) || hip_failed(hipStreamSynchronize(s->stream), "ccd_Create")) {
+        s->free_all();
+        delete s;
+        return nullptr;
+    }
+    return s;
+}
+
+void ccd_Release(ccd_Subd *s) {
+    if (!s) return;
+    s->free_all();
+    delete s;
+}
+
+int32_t ccd_MaxDepth(const ccd_Subd *s) { return s ? s->D : 0; }
+
+int64_t ccd_Count(const ccd_Subd *s, int32_t what, int32_t d) {
+    if (!s || d < 0 || d > kMaxDepth) return -1;
+    switch (what) {
+    case 0: return s->n.H[d];
+    case 1: return s->n.C[d];
+    case 2: return s->n.F[d];
+    case 3: return s->n.E[d];
+    case 4: return s->n.V[d];
+    case 5: return s->n.cumH(d);
+    case 6: return s->n.cumC(d);
+    case 7: return s->n.cumV(d);
+    default: return -1;
+    }
+}
+
+void *ccd_DevicePointer(const ccd_Subd *s, int32_t which) {
+    if (!s) return nullptr;
+    switch (which) {
+    case CCD_PTR_HALFEDGES: return s->he;
+    case CCD_PTR_CREASES: return s->cr;
+    case CCD_PTR_POINTS: return s->pts;
+    case CCD_PTR_CAGE_POINTS: return s->c_pts;
+    default: return nullptr;
+    }
+}
+
+int32_t ccd_UploadCagePoints(ccd_Subd *s, const cc_VertexPoint *hostPoints) {
+    clear_error();
+    if (!s || !hostPoints) { set_error(11, "ccd_UploadCagePoints: null argument"); return t_err; }
+    DeviceGuard g(s->device);
+    if (hip_failed(hipMemcpyAsync(s->c_pts, hostPoints, sizeof(P3) * (size_t)s->V0, hipMemcpyHostToDevice, s->stream),
+                   "ccd_UploadCagePoints") ||
+        hip_failed(hipStreamSynchronize(s->stream), "ccd_UploadCagePoints"))
+        return t_err;
+    return 0;
+}
+
+int32_t ccd_Refine(ccd_Subd *s, int32_t steps, void *stream) {
+    clear_error();
+    if (!s) { set_error(11, "ccd_Refine: null subd"); return t_err; }
+    DeviceGuard g(s->device);
+    hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    if (!s->refine(steps, s->D, st)) return t_err ? t_err : 1;
+    return 0;
+}
+
+int32_t ccd_RefinePoints(ccd_Subd *s, const float *devCagePoints, float *devPoints, int32_t steps, void *stream) {
+    clear_error();
+    if (!s) { set_error(11, "ccd_RefinePoints: null subd"); return t_err; }
+    DeviceGuard g(s->device);
+    hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    if (s->timing) s->begin_timing();
+    P3 *out = devPoints ? reinterpret_cast<P3 *>(devPoints) : s->pts;
+    if (!s->refine_points(steps | CCD_POINTS, s->D, reinterpret_cast<const P3 *>(devCagePoints), out, st)) return t_err ? t_err : 1;
+    if (hip_failed(hipGetLastError(), "ccd_RefinePoints")) return t_err;
+    return 0;
+}
+
+int32_t ccd_Download(const ccd_Subd *s, cc_Subd *dst, int32_t steps) {
+    clear_error();
+    if (!s || !dst) { set_error(11, "ccd_Download: null argument"); return t_err; }
+    if (dst->maxDepth != s->D) { set_error(13, "ccd_Download: depth mismatch"); return t_err; }
+    DeviceGuard g(s->device);
+    hipStream_t st = s->stream;
+    if ((steps & (CCD_HALFEDGES | CCD_UVS)) &&
+        hip_failed(hipMemcpyAsync(dst->halfedges, s->he, sizeof(He) * (size_t)s->n.cumH(s->D), hipMemcpyDeviceToHost, st), "download"))
+        return t_err;
+    if ((steps & CCD_CREASES) &&
+        hip_failed(hipMemcpyAsync(dst->creases, s->cr, sizeof(Crease) * (size_t)s->n.cumC(s->D), hipMemcpyDeviceToHost, st), "download"))
+        return t_err;
+    if ((steps & CCD_POINTS) &&
+        hip_failed(hipMemcpyAsync(dst->vertexPoints, s->pts, sizeof(P3) * (size_t)s->n.cumV(s->D), hipMemcpyDeviceToHost, st), "download"))
+        return t_err;
+    if (hip_failed(hipStreamSynchronize(st), "download")) return t_err;
+    return 0;
+}
+
+int32_t ccd_Synchronize(ccd_Subd *s) {
+    clear_error();
+    if (!s) { set_error(11, "ccd_Synchronize: null subd"); return t_err; }
+    DeviceGuard g(s->device);
+    if (hip_failed(hipDeviceSynchronize(), "ccd_Synchronize")) return t_err;
+    return 0;
+}
+
+int32_t ccd_SetTiming(ccd_Subd *s, int32_t enable) {
+    if (!s) return 11;
+    s->timing = enable != 0;
+    return 0;
+}
+
+int32_t ccd_TimingCount(const ccd_Subd *s) { return s ? (int32_t)s->timed.size() : 0; }
+
+const char *ccd_TimingName(const ccd_Subd *s, int32_t i) {
+    return (s && i >= 0 && i < (int32_t)s->timed.size()) ? s->timed[i].name : "";
+}
+
+float ccd_TimingMs(const ccd_Subd *s, int32_t i) {
+    if (!s || i < 0 || i >= (int32_t)s->timed.size()) return -1.0f;
+    float ms = -1.0f;
+    (void)hipEventSynchronize(s->timed[i].stop);
+    if (hipEventElapsedTime(&ms, s->timed[i].start, s->timed[i].stop) != hipSuccess) return -1.0f;
+    return ms;
+}
+
+}  // extern "C"

@@ -1,0 +1,269 @@
+/*
+ * CatmullClarkHIP.h -- MI355X (gfx950) drop-in for the uniform Catmull-Clark
+ * refinement path of CatmullClark.h (jdupuy/HalfedgeCatmullClark).
+ *
+ * Link against libcchip.so.  The types below are byte-identical to the
+ * reference's (CatmullClark.h:17-71, :136-142) so existing callers compile
+ * unchanged; every ccm_* / ccs_* function keeps the reference's name,
+ * signature and meaning.  The ccs_Refine* functions run on the GPU and keep
+ * the reference's host semantics: they read the cage from host memory, write
+ * every level 1..maxDepth into the cc_Subd's host arrays and return when the
+ * results are there.  Failures never abort: the call logs to stderr, leaves
+ * the outputs unmodified where it can, and records a code for
+ * ccs_LastError() on the calling thread.
+ *
+ * Extensions (no counterpart in the reference) are marked as such:
+ *   - ccs_LastError / ccs_LastErrorString (error channel);
+ *   - ccs_RefineCage* (the cage step 0 -> 1 on its own);
+ *   - ccs_RefineVertexPoints_Adjoint (transpose of the point refinement);
+ *   - ccd_* (device-resident refinement for callers that keep data in HBM);
+ *   - ccb_* (batched refinement of many cages).
+ */
+#ifndef CC_INCLUDE_CC_HIP_H
+#define CC_INCLUDE_CC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#include <stdint.h>
+#include <stdbool.h>
+
+/* ---- data layouts (CatmullClark.h:17-71, :136-142) ---------------------- */
+typedef union {
+    struct { float x, y, z; };
+    float array[3];
+} cc_VertexPoint;
+
+typedef union {
+    struct { float u, v; };
+    float array[2];
+} cc_VertexUv;
+
+typedef struct {
+    int32_t nextID;
+    int32_t prevID;
+    float sharpness;
+} cc_Crease;
+
+typedef struct {
+    int32_t twinID;
+    int32_t nextID;
+    int32_t prevID;
+    int32_t faceID;
+    int32_t edgeID;
+    int32_t vertexID;
+    int32_t uvID;
+} cc_Halfedge;
+
+typedef struct {
+    int32_t twinID;
+    int32_t edgeID;
+    int32_t vertexID;
+    int32_t uvID;
+} cc_Halfedge_SemiRegular;
+
+typedef struct {
+    int32_t vertexCount;
+    int32_t uvCount;
+    int32_t halfedgeCount;
+    int32_t edgeCount;
+    int32_t faceCount;
+    int32_t *vertexToHalfedgeIDs;
+    int32_t *edgeToHalfedgeIDs;
+    int32_t *faceToHalfedgeIDs;
+    cc_VertexPoint *vertexPoints;
+    cc_VertexUv *uvs;
+    cc_Halfedge *halfedges;
+    cc_Crease *creases;
+} cc_Mesh;
+
+typedef struct {
+    const cc_Mesh *cage;
+    cc_VertexPoint *vertexPoints;
+    cc_Halfedge_SemiRegular *halfedges;
+    cc_Crease *creases;
+    int32_t maxDepth;
+} cc_Subd;
+
+/* ---- cage lifecycle and I/O (CatmullClark.h:74-83) ---------------------- */
+cc_Mesh *ccm_Load(const char *filename);                       /* :3186 */
+cc_Mesh *ccm_Create(int32_t vertexCount, int32_t uvCount, int32_t halfedgeCount,
+                    int32_t edgeCount, int32_t faceCount);     /* :763  */
+void ccm_Release(cc_Mesh *mesh);                               /* :798  */
+bool ccm_Save(const cc_Mesh *mesh, const char *filename);      /* :3227 */
+
+/* ---- cage counts (CatmullClark.h:86-101) --------------------------------- */
+int32_t ccm_FaceCount(const cc_Mesh *mesh);
+int32_t ccm_EdgeCount(const cc_Mesh *mesh);
+int32_t ccm_HalfedgeCount(const cc_Mesh *mesh);
+int32_t ccm_CreaseCount(const cc_Mesh *mesh);
+int32_t ccm_VertexCount(const cc_Mesh *mesh);
+int32_t ccm_UvCount(const cc_Mesh *mesh);
+int32_t ccm_HalfedgeCountAtDepth(const cc_Mesh *cage, int32_t depth);      /* :523 */
+int32_t ccm_CreaseCountAtDepth(const cc_Mesh *cage, int32_t depth);        /* :540 */
+int32_t ccm_FaceCountAtDepth(const cc_Mesh *cage, int32_t depth);          /* :473 */
+int32_t ccm_FaceCountAtDepth_Fast(const cc_Mesh *cage, int32_t depth);     /* :465 */
+int32_t ccm_EdgeCountAtDepth(const cc_Mesh *cage, int32_t depth);          /* :504 */
+int32_t ccm_EdgeCountAtDepth_Fast(const cc_Mesh *cage, int32_t depth);     /* :494 */
+int32_t ccm_VertexCountAtDepth(const cc_Mesh *cage, int32_t depth);        /* :578 */
+int32_t ccm_VertexCountAtDepth_Fast(const cc_Mesh *cage, int32_t depth);   /* :563 */
+
+/* ---- cage accessors (CatmullClark.h:104-131) ----------------------------- */
+int32_t ccm_HalfedgeTwinID(const cc_Mesh *mesh, int32_t halfedgeID);
+int32_t ccm_HalfedgeNextID(const cc_Mesh *mesh, int32_t halfedgeID);
+int32_t ccm_HalfedgePrevID(const cc_Mesh *mesh, int32_t halfedgeID);
+int32_t ccm_HalfedgeFaceID(const cc_Mesh *mesh, int32_t halfedgeID);
+int32_t ccm_HalfedgeEdgeID(const cc_Mesh *mesh, int32_t halfedgeID);
+int32_t ccm_HalfedgeVertexID(const cc_Mesh *mesh, int32_t halfedgeID);
+int32_t ccm_HalfedgeUvID(const cc_Mesh *mesh, int32_t halfedgeID);
+float ccm_HalfedgeSharpness(const cc_Mesh *mesh, int32_t halfedgeID);
+cc_VertexPoint ccm_HalfedgeVertexPoint(const cc_Mesh *mesh, int32_t halfedgeID);
+cc_VertexUv ccm_HalfedgeVertexUv(const cc_Mesh *mesh, int32_t halfedgeID);
+int32_t ccm_CreaseNextID(const cc_Mesh *mesh, int32_t edgeID);
+int32_t ccm_CreasePrevID(const cc_Mesh *mesh, int32_t edgeID);
+float ccm_CreaseSharpness(const cc_Mesh *mesh, int32_t edgeID);
+cc_VertexPoint ccm_VertexPoint(const cc_Mesh *mesh, int32_t vertexID);
+cc_VertexUv ccm_Uv(const cc_Mesh *mesh, int32_t uvID);
+int32_t ccm_HalfedgeNextID_Quad(int32_t halfedgeID);
+int32_t ccm_HalfedgePrevID_Quad(int32_t halfedgeID);
+int32_t ccm_HalfedgeFaceID_Quad(int32_t halfedgeID);
+int32_t ccm_VertexToHalfedgeID(const cc_Mesh *mesh, int32_t vertexID);
+int32_t ccm_EdgeToHalfedgeID(const cc_Mesh *mesh, int32_t edgeID);
+int32_t ccm_FaceToHalfedgeID(const cc_Mesh *mesh, int32_t faceID);
+int32_t ccm_FaceToHalfedgeID_Quad(int32_t faceID);
+int32_t ccm_NextVertexHalfedgeID(const cc_Mesh *mesh, int32_t halfedgeID);
+int32_t ccm_PrevVertexHalfedgeID(const cc_Mesh *mesh, int32_t halfedgeID);
+
+/* ---- subd lifecycle and counts (CatmullClark.h:145-160) ------------------ */
+/* ccs_Create refuses (returns NULL, ccs_LastError() != 0) a depth below 1 or
+ * one whose cumulative counts overflow the reference's int32 formulas. */
+cc_Subd *ccs_Create(const cc_Mesh *cage, int32_t maxDepth);                 /* :953 */
+void ccs_Release(cc_Subd *subd);                                            /* :977 */
+int32_t ccs_MaxDepth(const cc_Subd *subd);
+int32_t ccs_VertexCount(const cc_Subd *subd);
+int32_t ccs_CumulativeFaceCount(const cc_Subd *subd);
+int32_t ccs_CumulativeEdgeCount(const cc_Subd *subd);
+int32_t ccs_CumulativeCreaseCount(const cc_Subd *subd);
+int32_t ccs_CumulativeVertexCount(const cc_Subd *subd);
+int32_t ccs_CumulativeHalfedgeCount(const cc_Subd *subd);
+int32_t ccs_CumulativeHalfedgeCountAtDepth(const cc_Mesh *cage, int32_t depth); /* :861 */
+int32_t ccs_CumulativeVertexCountAtDepth(const cc_Mesh *cage, int32_t depth);   /* :916 */
+int32_t ccs_CumulativeFaceCountAtDepth(const cc_Mesh *cage, int32_t depth);     /* :815 */
+int32_t ccs_CumulativeEdgeCountAtDepth(const cc_Mesh *cage, int32_t depth);     /* :830 */
+int32_t ccs_CumulativeCreaseCountAtDepth(const cc_Mesh *cage, int32_t depth);   /* :890 */
+
+/* ---- subd accessors (CatmullClark.h:163-195) ----------------------------- */
+int32_t ccs_HalfedgeTwinID(const cc_Subd *subd, int32_t halfedgeID, int32_t depth);
+int32_t ccs_HalfedgeNextID(const cc_Subd *subd, int32_t halfedgeID, int32_t depth);
+int32_t ccs_HalfedgePrevID(const cc_Subd *subd, int32_t halfedgeID, int32_t depth);
+int32_t ccs_HalfedgeFaceID(const cc_Subd *subd, int32_t halfedgeID, int32_t depth);
+int32_t ccs_HalfedgeEdgeID(const cc_Subd *subd, int32_t halfedgeID, int32_t depth);
+int32_t ccs_HalfedgeVertexID(const cc_Subd *subd, int32_t halfedgeID, int32_t depth);
+cc_VertexPoint ccs_HalfedgeVertexPoint(const cc_Subd *subd, int32_t halfedgeID, int32_t depth);
+cc_VertexUv ccs_HalfedgeVertexUv(const cc_Subd *subd, int32_t halfedgeID, int32_t depth);
+float ccs_HalfedgeSharpness(const cc_Subd *subd, int32_t halfedgeID, int32_t depth);
+int32_t ccs_CreaseNextID_Fast(const cc_Subd *subd, int32_t edgeID, int32_t depth);
+int32_t ccs_CreaseNextID(const cc_Subd *subd, int32_t edgeID, int32_t depth);
+int32_t ccs_CreasePrevID_Fast(const cc_Subd *subd, int32_t edgeID, int32_t depth);
+int32_t ccs_CreasePrevID(const cc_Subd *subd, int32_t edgeID, int32_t depth);
+float ccs_CreaseSharpness_Fast(const cc_Subd *subd, int32_t edgeID, int32_t depth);
+float ccs_CreaseSharpness(const cc_Subd *subd, int32_t edgeID, int32_t depth);
+cc_VertexPoint ccs_VertexPoint(const cc_Subd *subd, int32_t vertexID, int32_t depth);
+int32_t ccs_NextVertexHalfedgeID(const cc_Subd *subd, int32_t halfedgeID, int32_t depth);
+int32_t ccs_PrevVertexHalfedgeID(const cc_Subd *subd, int32_t halfedgeID, int32_t depth);
+/* Declared as ccs_VertexToHalfedgeID (:187) but defined as
+ * ccs_VertexPointToHalfedgeID (:1336) in the reference; both are exported. */
+int32_t ccs_VertexToHalfedgeID(const cc_Subd *subd, int32_t vertexID, int32_t depth);
+int32_t ccs_VertexPointToHalfedgeID(const cc_Subd *subd, int32_t vertexID, int32_t depth);
+int32_t ccs_EdgeToHalfedgeID(const cc_Subd *mesh, int32_t edgeID, int32_t depth);   /* :1230 */
+int32_t ccs_FaceToHalfedgeID(const cc_Subd *mesh, int32_t faceID, int32_t depth);   /* :1190 */
+
+/* ---- refinement (CatmullClark.h:197-212), GPU-backed --------------------- */
+void ccs_Refine_Gather(cc_Subd *subd);                          /* :3066 */
+void ccs_Refine_Scatter(cc_Subd *subd);                         /* :3060 */
+void ccs_RefineVertexPoints_Gather(cc_Subd *subd);              /* :2659 */
+void ccs_RefineVertexPoints_Scatter(cc_Subd *subd);             /* :2631 */
+void ccs_RefineHalfedges(cc_Subd *subd);                        /* :2803 */
+void ccs_RefineCreases(cc_Subd *subd);                          /* :3033 */
+void ccs_RefineVertexUvs(cc_Subd *subd);                        /* :2928 */
+void ccs_Refine_NoCreases_Gather(cc_Subd *subd);                /* :3078 */
+void ccs_Refine_NoCreases_Scatter(cc_Subd *subd);               /* :3072 */
+void ccs_RefineVertexPoints_NoCreases_Gather(cc_Subd *subd);    /* :2672 */
+void ccs_RefineVertexPoints_NoCreases_Scatter(cc_Subd *subd);   /* :2645 */
+
+/* ---- extension: error channel ------------------------------------------- */
+/* 0 when the last ccs_/ccd_/ccb_ call on this thread succeeded. */
+int32_t ccs_LastError(void);
+const char *ccs_LastErrorString(void);
+
+/* ---- extension: the cage step (level 0 -> 1) alone ----------------------
+ * The reference keeps these static (ccs__RefineCageHalfedges :2693,
+ * ccs__RefineCageCreases :2950, ccs__RefineCageVertexUvs :2825, the
+ * ccs__Cage*Points_* kernels :1408-1991).  Each writes level 1 only. */
+void ccs_RefineCageHalfedges(cc_Subd *subd);
+void ccs_RefineCageCreases(cc_Subd *subd);
+void ccs_RefineCageVertexUvs(cc_Subd *subd);
+void ccs_RefineCageVertexPoints_Scatter(cc_Subd *subd);
+void ccs_RefineCageVertexPoints_Gather(cc_Subd *subd);
+
+/* ---- extension: adjoint of the point refinement --------------------------
+ * Refined points are linear in the cage points at fixed topology and
+ * sharpness.  Given dL/d(points at level maxDepth) (V_D records) this writes
+ * dL/d(cage points) (V_0 records), the transpose of ccs_RefineVertexPoints_*
+ * with creases.  Topology and creases must have been refined already. */
+void ccs_RefineVertexPoints_Adjoint(const cc_Subd *subd,
+                                    const cc_VertexPoint *dL_dFinest,
+                                    cc_VertexPoint *dL_dCage);
+
+/* ---- extension: device-resident refinement ------------------------------
+ * A ccd_Subd owns device copies of a cage and of levels 1..maxDepth laid out
+ * exactly as cc_Subd's arrays.  Calls are asynchronous on `stream` (a
+ * hipStream_t; NULL selects the ccd_Subd's own stream).  Return 0 on
+ * success, a nonzero code otherwise (also readable via ccs_LastError). */
+typedef struct ccd_Subd ccd_Subd;
+
+enum {
+    CCD_HALFEDGES = 1,    /* twin/edge/vertex IDs of levels 1..D           */
+    CCD_CREASES = 2,      /* crease links and sharpness                    */
+    CCD_UVS = 4,          /* uvIDs (only when the cage has uvs)            */
+    CCD_POINTS = 8,       /* vertex points                                 */
+    CCD_GATHER = 16,      /* gather point rules instead of scatter rules   */
+    CCD_NOCREASES = 32,   /* plain Catmull-Clark rules (ignore sharpness)  */
+    CCD_ALL = 15
+};
+
+enum { CCD_PTR_HALFEDGES = 0, CCD_PTR_CREASES = 1, CCD_PTR_POINTS = 2, CCD_PTR_CAGE_POINTS = 3 };
+
+ccd_Subd *ccd_Create(const cc_Mesh *cage, int32_t maxDepth);
+void ccd_Release(ccd_Subd *subd);
+int32_t ccd_MaxDepth(const ccd_Subd *subd);
+/* exact count of `what` (0 H, 1 C, 2 F, 3 E, 4 V at `depth`; 5 Hcum, 6 Ccum,
+ * 7 Vcum over levels 1..depth) */
+int64_t ccd_Count(const ccd_Subd *subd, int32_t what, int32_t depth);
+void *ccd_DevicePointer(const ccd_Subd *subd, int32_t which);
+int32_t ccd_UploadCagePoints(ccd_Subd *subd, const cc_VertexPoint *hostPoints);
+int32_t ccd_Refine(ccd_Subd *subd, int32_t steps, void *stream);
+/* points only, from cage points already on the device (or the uploaded ones
+ * when devCagePoints is NULL) into devPoints (or the ccd_Subd's own buffer
+ * when NULL); devPoints holds levels 1..D in cumulative order. */
+int32_t ccd_RefinePoints(ccd_Subd *subd, const float *devCagePoints, float *devPoints,
+                         int32_t steps, void *stream);
+/* adjoint on the device: devGradFinest holds V_D x 3 floats, devGradCage
+ * receives V_0 x 3 floats. */
+int32_t ccd_RefinePointsAdjoint(ccd_Subd *subd, const float *devGradFinest, float *devGradCage,
+                                int32_t steps, void *stream);
+int32_t ccd_Download(const ccd_Subd *subd, cc_Subd *dst, int32_t steps);
+int32_t ccd_Synchronize(ccd_Subd *subd);
+/* per-kernel timing with HIP events on the launch stream: enable, run, then
+ * read the kernel names and mean milliseconds of the last ccd_Refine. */
+int32_t ccd_SetTiming(ccd_Subd *subd, int32_t enable);
+int32_t ccd_TimingCount(const ccd_Subd *subd);
+const char *ccd_TimingName(const ccd_Subd *subd, int32_t i);
+float ccd_TimingMs(const ccd_Subd *subd, int32_t i);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* CC_INCLUDE_CC_HIP_H */
