@@ -1,0 +1,234 @@
+"""Benchmark of the GPU refinement path (one MI355X).
+
+Workload: the 64x32 torus cage (2,048 quads, 8,192 halfedges, R=1, r=0.3,
+N(0, 0.01) jitter, one longitudinal crease ring of sharpness 1.5 and one of
+4.0, grid uvs) refined to depth 8 with every step of ccs_Refine_Scatter
+(halfedges, creases, uvs, vertex points).  One "step" is one full
+refinement of all 8 levels from the cage resident in HBM; value is refined
+halfedges (levels 1..8, 715,816,960) per second.
+
+Also reported on the same line:
+  - the gather point rules on the same workload;
+  - forward + adjoint against forward alone on the 32x16 torus at depth 5
+    (the cage-fitting workload);
+  - roofline of the dominant kernel (per-kernel HIP events on the launch
+    stream over the timed steps, algorithmic bytes per launch);
+  - the reference's own OpenMP CPU path (CatmullClark.h built by
+    oracle/Makefile) timed on this host's cores on the same torus at depth 6.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "refined halfedges/s + HBM GB/s at depth 8; fwd+adjoint and batched vs plain"
+
+
+def level_bytes(n, d):
+    """Algorithmic bytes of the fused level kernel d -> d+1: each input read
+    once and each output written once.  Reads: the level-d halfedge records
+    (16 B), level-d points (12 B), level-d creases (12 B); writes: the four
+    16-B child records per parent halfedge and the level-(d+1) face and edge
+    points (12 B)."""
+    return 16 * n["H"][d] + 12 * n["V"][d] + 12 * n["C"][d] + 64 * n["H"][d] + 12 * (n["F"][d] + n["E"][d])
+
+
+def vertex_kernel_bytes(n, d):
+    """Algorithmic bytes of the vertex-point kernel d -> d+1: the level-d
+    records (16 B), points and creases (12 B) and the level-(d+1) face and
+    edge points it reads once, the level-(d+1) vertex points it writes."""
+    return 16 * n["H"][d] + 12 * n["V"][d] + 12 * n["C"][d] + 12 * (n["F"][d] + n["E"][d]) + 12 * n["V"][d]
+
+
+def refinement_bytes(n, D):
+    """Algorithmic bytes of the whole refinement (all steps, levels 0..D-1):
+    per step d -> d+1 read the level-d records (28 B per cage halfedge at
+    d = 0), points and creases once, write the child records, all level-(d+1)
+    points and creases once."""
+    total = 0
+    for d in range(D):
+        rec = 28 if d == 0 else 16
+        total += rec * n["H"][d] + 64 * n["H"][d] + 12 * n["V"][d] + 12 * n["V"][d + 1] + 12 * n["C"][d] + 24 * n["C"][d]
+    return total
+
+
+def cpu_baseline(seconds_budget: float = 20.0):
+    """The reference's OpenMP ccs_Refine_Scatter on this host (torus 64x32,
+    depth 6), repeated up to the time budget."""
+    try:
+        import numpy as np  # noqa: F401
+        from halfedgecatmullclark_amd import cage as C
+        from oracle import ref
+    except Exception as e:  # pragma: no cover
+        return {"value": None, "unit": "refined halfedges/s", "cores": 0, "kind": "reference",
+                "sample": f"unavailable: {e}"}
+    if not ref.available():
+        return {"value": None, "unit": "refined halfedges/s", "cores": 0, "kind": "reference",
+                "sample": "reference build (oracle/_ref) not present"}
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16)
+    depth = 6
+    r = ref.RefSubd(C.torus(64, 32), depth)
+    r.run("ccs_Refine_Scatter", threads=cores)  # warm up (page faults)
+    times = []
+    t_start = time.perf_counter()
+    while time.perf_counter() - t_start < seconds_budget and len(times) < 50:
+        t0 = time.perf_counter()
+        r.run("ccs_Refine_Scatter", threads=cores)
+        times.append(time.perf_counter() - t0)
+    r.close()
+    times.sort()
+    med = times[len(times) // 2]
+    hc = r.n["Hcum"][depth]
+    return {"value": hc / med, "unit": "refined halfedges/s", "cores": cores, "kind": "reference",
+            "sample": f"ccs_Refine_Scatter (OpenMP, {cores} threads), torus 64x32 at depth 6 "
+                      f"({hc} refined halfedges), median of {len(times)} runs: {med * 1e3:.1f} ms"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the refinement of one cage runs on one device; "
+              "run it with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from halfedgecatmullclark_amd import capi
+    from halfedgecatmullclark_amd import cage as C
+
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    sptr = stream.cuda_stream
+
+    D = 8
+    cage = C.torus(64, 32, seed=0)
+    ds = capi.DeviceSubd(cage, D)
+    n = ds.n
+    hcum = n["Hcum"][D]
+
+    def timed(steps_flags, k, w, per_kernel=False):
+        for _ in range(w):
+            ds.refine(steps_flags, sptr)
+        ds.set_timing(per_kernel)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        kern = {}
+        e0.record(stream)
+        for _ in range(k):
+            ds.refine(steps_flags, sptr)
+            if per_kernel:
+                for i, (name, ms) in enumerate(ds.timings()):
+                    kern.setdefault((i, name), []).append(ms)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ds.set_timing(False)
+        return e0.elapsed_time(e1) / k, kern
+
+    ms_scatter, kern = timed(capi.ALL, args.steps, args.warmup, per_kernel=True)
+    ms_gather, _ = timed(capi.ALL | capi.GATHER, max(3, args.steps // 2), 1)
+
+    # dominant kernel: largest mean duration among the launches of one step;
+    # the per-level kernels appear once per level, in level order
+    seen, labelled = {}, []
+    for (i, name) in sorted(kern):
+        seen[name] = seen.get(name, 0) + 1
+        labelled.append(((i, name), seen[name]))
+    ((idx, name), level) = max(labelled, key=lambda kv: sum(kern[kv[0]]) / len(kern[kv[0]]))
+    durs = kern[(idx, name)]
+    mean_ms = sum(durs) / len(durs)
+    if name == "level_fused":
+        nbytes = level_bytes(n, level)
+    elif name == "level_vertices":
+        nbytes = vertex_kernel_bytes(n, level)
+    else:
+        nbytes = None
+    achieved = nbytes / (mean_ms * 1e-3) / 1e9 if nbytes else None
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "pmc_level_fused_torus64x32_d8.json")
+    if os.path.exists(prof):
+        try:
+            traffic = json.load(open(prof)).get("traffic_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    # forward + adjoint on the cage-fitting workload (32x16 torus, depth 5)
+    from halfedgecatmullclark_amd.autograd import Subdivider
+    sub = Subdivider(C.torus(32, 16, seed=0), 5)
+    V = sub.cage_points().clone()
+    out = torch.empty((sub.Vcum, 3), device=dev)
+    gcage = torch.empty((sub.V0, 3), device=dev)
+    target = torch.randn((sub.VD, 3), device=dev)
+    K3 = max(20, args.steps)
+
+    def fwd():
+        sub.ds.refine_points(V.data_ptr(), out.data_ptr(), 0, sptr)
+
+    def fwd_adj():
+        fwd()
+        r = out[sub.finest_offset:] - target
+        sub.ds.refine_points_adjoint(r.data_ptr(), gcage.data_ptr(), 0, sptr)
+
+    for f in (fwd, fwd_adj):
+        for _ in range(3):
+            f()
+    res = {}
+    for label, f in (("fwd", fwd), ("fwd_adj", fwd_adj)):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(K3):
+            f()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        res[label] = e0.elapsed_time(e1) / K3
+    sub.close()
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    ds.close()
+
+    line = {
+        "metric": METRIC,
+        "value": hcum / (ms_scatter * 1e-3),
+        "unit": "refined halfedges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_scatter,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded torus cage)",
+        "config": {"workload": "torus 64x32 cage (8192 halfedges, crease rings s=1.5 and s=4.0, uvs) "
+                               "refined to depth 8 with ccs_Refine_Scatter semantics",
+                   "depth": D, "refined_halfedges": hcum, "finest_halfedges": n["H"][D]},
+        "hbm_gbs_algorithmic": refinement_bytes(n, D) / (ms_scatter * 1e-3) / 1e9,
+        "gather": {"ms_per_step": ms_gather, "value": hcum / (ms_gather * 1e-3)},
+        "fwd_adjoint": {"workload": "torus 32x16 cage, depth 5, loss 0.5*||S(V)-T||^2",
+                        "fwd_ms": res["fwd"], "fwd_adj_ms": res["fwd_adj"],
+                        "ratio": res["fwd_adj"] / res["fwd"]},
+        "roofline": {"bound": "hbm", "kernel": f"{name} (level {level} -> {level + 1})",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+                     "kernel_ms": mean_ms, "algorithmic_bytes": nbytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -61,8 +61,8 @@ class MeshView:
         c = cage.contiguous()
         real = np.dtype(real)
         self.cage = c
-        self.points = np.ascontiguousarray(c.vertex_points, dtype=real)
-        self.uvs = np.ascontiguousarray(c.uvs, dtype=real)
+        self.points = np.ascontiguousarray(np.asarray(cage.vertex_points).reshape(-1, 3), dtype=real)
+        self.uvs = np.ascontiguousarray(np.asarray(cage.uvs).reshape(-1, 2), dtype=real)
         cr = np.zeros(c.E, dtype=crease_dtype(real))
         cr["nextID"] = c.creases["nextID"]
         cr["prevID"] = c.creases["prevID"]

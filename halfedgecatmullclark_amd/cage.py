@@ -257,7 +257,8 @@ def to_obj(points, faces, uvs=None, face_uvs=None, creases=()) -> str:
         else:
             lines.append("f " + " ".join(str(v + 1) for v in f))
     for a, b, s in creases:
-        lines.append(f"t crease 2/1/0 {a} {b} {s:.9g}")
+        # nine significant digits of the float32 value read back exactly
+        lines.append(f"t crease 2/1/0 {a} {b} {float(np.float32(s)):.9g}")
     return "\n".join(lines) + "\n"
 
 

@@ -15,9 +15,10 @@ from .cage import CREASE_DTYPE, Cage
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcchip.so")
 
-HALFEDGES, CREASES, UVS, POINTS, GATHER, NOCREASES = 1, 2, 4, 8, 16, 32
+HALFEDGES, CREASES, UVS, POINTS, GATHER, NOCREASES, FP64 = 1, 2, 4, 8, 16, 32, 64
 ALL = HALFEDGES | CREASES | UVS | POINTS
 PTR_HALFEDGES, PTR_CREASES, PTR_POINTS, PTR_CAGE_POINTS = 0, 1, 2, 3
+OPT_FUSED, OPT_STORE, OPT_XCD_REMAP, OPT_FAN = 0, 1, 2, 3
 
 HOST_DRIVERS = ("ccs_Refine_Scatter", "ccs_Refine_Gather", "ccs_Refine_NoCreases_Scatter",
                 "ccs_Refine_NoCreases_Gather", "ccs_RefineVertexPoints_Scatter",
@@ -50,6 +51,7 @@ def lib() -> ctypes.CDLL:
         "ccs_LastError": (i32, []),
         "ccs_LastErrorString": (ctypes.c_char_p, []),
         "ccs_RefineVertexPoints_Adjoint": (None, [P(cc_Subd), vp, vp]),
+        "ccs_RefineVertexPoints_Adjoint_f64": (None, [P(cc_Subd), vp, vp]),
         "ccd_Create": (vp, [P(cc_Mesh), i32]),
         "ccd_Release": (None, [vp]),
         "ccd_MaxDepth": (i32, [vp]),
@@ -62,6 +64,7 @@ def lib() -> ctypes.CDLL:
         "ccd_Download": (i32, [vp, P(cc_Subd), i32]),
         "ccd_Synchronize": (i32, [vp]),
         "ccd_SetTiming": (i32, [vp, i32]),
+        "ccd_SetOption": (i32, [vp, i32, i32]),
         "ccd_TimingCount": (i32, [vp]),
         "ccd_TimingName": (ctypes.c_char_p, [vp, i32]),
         "ccd_TimingMs": (ctypes.c_float, [vp, i32]),
@@ -119,6 +122,18 @@ class HostSubd:
             if code:
                 raise CCError(f"{name}: {msg}")
         return self
+
+    def adjoint(self, grad_finest: np.ndarray, real=np.float32) -> np.ndarray:
+        """dL/d(cage points) from dL/d(finest points), on the GPU."""
+        real = np.dtype(real)
+        g = np.ascontiguousarray(np.asarray(grad_finest).reshape(-1, 3), dtype=real)
+        out = np.zeros((self.n["V"][0], 3), dtype=real)
+        name = "ccs_RefineVertexPoints_Adjoint" if real.itemsize == 4 else "ccs_RefineVertexPoints_Adjoint_f64"
+        getattr(self.L, name)(self.subd, g.ctypes.data, out.ctypes.data)
+        code, msg = last_error()
+        if code:
+            raise CCError(f"{name}: {msg}")
+        return out
 
     def arrays(self) -> dict:
         s = self.subd.contents
@@ -194,6 +209,9 @@ class DeviceSubd:
         sub = cc_Subd(self.mesh.pointer, pt.ctypes.data, he.ctypes.data, cr.ctypes.data, self.depth)
         _check(self.L.ccd_Download(self.handle, ctypes.byref(sub), steps), "ccd_Download")
         return {"halfedges": he, "creases": cr, "points": pt, "counts": self.n}
+
+    def set_option(self, key: int, value: int):
+        _check(self.L.ccd_SetOption(self.handle, key, value), "ccd_SetOption")
 
     def set_timing(self, on: bool):
         self.L.ccd_SetTiming(self.handle, int(on))

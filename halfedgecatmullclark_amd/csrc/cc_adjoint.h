@@ -1,0 +1,37 @@
+// Adjoint of the point refinement (transpose of the scatter rules).
+#pragma once
+
+#include "cc_common.h"
+#include "cc_fused.h"
+
+namespace cch {
+
+// Gradients of one level step d -> d+1: gV, gF, gE are the slices of the
+// level-(d+1) gradient (the face and edge slices receive the vertex and edge
+// rules' shares), gP receives the level-d gradient.
+template <typename T>
+struct AdjArgs {
+    const He *he;
+    const Crease *cr;
+    int32_t H, C;
+    T *gV, *gF, *gE;
+    T *gP;
+};
+
+struct AdjPlan {
+    const Counts *n;
+    int D;
+    const He *he;        // cumulative halfedges of levels 1..D
+    const Crease *cr;    // cumulative creases of levels 1..D
+    DCage cage;
+    LevelTab tab;
+    LevelPtrs lp;
+};
+
+// dL/d(cage points) from dL/d(level-D points): V_D x 3 in, V_0 x 3 out.
+// bufA holds V_D x 3 and bufB V_{D-1} x 3 scalars of scratch.
+template <typename T>
+void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA, T *bufB, int creased,
+                    hipStream_t st);
+
+}  // namespace cch

@@ -1,0 +1,334 @@
+// Adjoint of the point refinement.
+//
+// At fixed topology and sharpness every refined point is a fixed linear
+// combination of the points one level up (the weights of the scatter rules,
+// CatmullClark.h:1437-1991 and :2032-2615, depend only on valence, crease
+// count, boundary and sharpness).  The adjoint applies the transposes level
+// by level, finest first: for level d -> d+1 the gradient of the level-d+1
+// points [vertex | face | edge] flows back through the vertex rule, then the
+// edge rule, then the face rule into the gradient of the level-d points.
+// Only the stored topology and creases are read; nothing of the forward
+// points is kept or recomputed.  Kernels are templated on the scalar type so
+// the same code runs in float and, for gradient checks, in double.
+#include "cc_adjoint.h"
+#include "cc_fan.h"
+
+namespace cch {
+
+namespace {
+
+template <typename T>
+struct V3 { T x, y, z; };
+
+template <typename T>
+__device__ __forceinline__ void atomic_add3(V3<T> *p, T a, const V3<T> &g) {
+    unsafeAtomicAdd(&p->x, a * g.x);
+    unsafeAtomicAdd(&p->y, a * g.y);
+    unsafeAtomicAdd(&p->z, a * g.z);
+}
+
+// weights (alpha, beta, gamma, delta) of one halfedge's vertex contribution
+// alpha*V + beta*E(edge h) + gamma*F(face h) + delta*E(edge prev h)
+template <typename T>
+struct VW { T a, b, g, dl; };
+
+// level d >= 1 scatter rule (creased :2568-2607, corner test first; smooth
+// :2339-2348)
+template <typename T>
+__device__ __forceinline__ VW<T> level_vertex_weights(const RingParams &rp, float ts, float ps, int32_t edge,
+                                                      bool creased) {
+    VW<T> w{0, 0, 0, 0};
+    const T n = (T)rp.n;
+    if (!creased) {
+        const T iw = (T)1 / n, s = rp.boundary ? (T)0 : (T)1;
+        w.a = iw - (T)3 * iw * iw * s;
+        w.b = (T)4 * iw * iw * s;
+        w.g = -iw * iw * s;
+        return w;
+    }
+    const T nn = rp.boundary ? n + (T)1 : n;
+    if (rp.cc >= 3.0f || nn == (T)2) {
+        w.a = (T)1 / n;
+        return w;
+    }
+    if (rp.cc <= 1.0f) {
+        w.a = (n - (T)3) / (n * n);
+        w.b = (T)4 / (n * n);
+        w.g = (T)-1 / (n * n);
+        return w;
+    }
+    float avg = rp.sall;
+    if (rp.boundary) avg = edge == rp.e1 ? rp.s2 : (edge == rp.e2 ? rp.s1 : rp.s1 + rp.s2);
+    const T t = (T)sat(avg * 0.5f);
+    const T cw = (T)sgn(ts), pcw = rp.boundary ? (T)sgn(ps) : (T)0;
+    w.a = ((T)1 - t) / n + t * (T)0.25 * (cw + pcw);
+    w.b = t * (T)0.25 * cw;
+    w.dl = t * (T)0.25 * pcw;
+    return w;
+}
+
+}  // namespace
+
+// ---- level d >= 1 -------------------------------------------------------------
+// vertex rule: the owner of each vertex takes its fan, writes alpha * g into
+// the level-d gradient of the vertex and sends beta, gamma, delta shares to
+// the level-(d+1) edge and face gradients
+template <typename T, int CREASED>
+__global__ __launch_bounds__(256) void k_adj_vertices(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
+    const int32_t h = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (h >= a.H) return;
+    const He *he = a.he;
+    const int32_t v = he[h].vert;
+    Fan f;
+    if (!owner_fan(h, v, d, c, t, lp.he, f)) return;
+    const int n = f.n;
+    RingParams rp;
+    rp.n = (float)n; rp.cc = 0.0f; rp.boundary = f.boundary; rp.s1 = rp.s2 = 0.0f; rp.e1 = rp.e2 = -1; rp.sall = 0.0f;
+    if (CREASED) {
+        for (int i = (f.boundary ? -1 : 0); i < n; ++i) {
+            const int32_t e = i < 0 ? he[fan_at(f, c, 0)].edge : he[qprev(fan_at(f, c, i))].edge;
+            const float se = lsharp(a.cr, a.C, e);
+            rp.cc += sgn(se);
+            if (se != 0.0f) {
+                if (rp.e1 < 0) { rp.s1 = se; rp.e1 = e; }
+                else if (rp.e2 < 0) { rp.s2 = se; rp.e2 = e; }
+            }
+            rp.sall += se;
+        }
+    }
+    const V3<T> gv = reinterpret_cast<const V3<T> *>(a.gV)[v];
+    T alpha = 0;
+    V3<T> *gE = reinterpret_cast<V3<T> *>(a.gE);
+    V3<T> *gF = reinterpret_cast<V3<T> *>(a.gF);
+    for (int i = 0; i < n; ++i) {
+        const int32_t r = fan_at(f, c, i);
+        const int32_t re = he[r].edge, pe = he[qprev(r)].edge;
+        const float ts = CREASED ? lsharp(a.cr, a.C, re) : 0.0f;
+        const float ps = CREASED ? lsharp(a.cr, a.C, pe) : 0.0f;
+        const VW<T> w = level_vertex_weights<T>(rp, ts, ps, re, CREASED);
+        alpha += w.a;
+        if (w.b != (T)0) atomic_add3(&gE[re], w.b, gv);
+        if (w.g != (T)0) atomic_add3(&gF[r >> 2], w.g, gv);
+        if (w.dl != (T)0) atomic_add3(&gE[pe], w.dl, gv);
+    }
+    V3<T> *gP = reinterpret_cast<V3<T> *>(a.gP);
+    gP[v] = {alpha * gv.x, alpha * gv.y, alpha * gv.z};
+}
+
+// edge rule (creased :2221-2233, smooth :2125-2131): each halfedge's share of
+// its edge point flows to its two endpoints and its face point
+template <typename T, int CREASED>
+__global__ __launch_bounds__(256) void k_adj_edges(AdjArgs<T> a) {
+    const int32_t h = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (h >= a.H) return;
+    const He x = a.he[h];
+    const V3<T> ge = reinterpret_cast<const V3<T> *>(a.gE)[x.edge];
+    const bool boundary = x.twin < 0;
+    T c0, c1, cf;
+    if (CREASED) {
+        const T w = (T)sat(lsharp(a.cr, a.C, x.edge));
+        const T k = boundary ? (T)1 : (T)0.5;
+        c0 = (T)0.25 * ((T)1 - w) + (T)0.5 * k * w;
+        c1 = (T)0.5 * k * w;
+        cf = (T)0.25 * ((T)1 - w);
+    } else {
+        const T w = boundary ? (T)1 : (T)0.5;
+        c0 = c1 = (T)0.5 * w * w;
+        cf = (T)0.5 * ((T)1 - w);
+    }
+    V3<T> *gP = reinterpret_cast<V3<T> *>(a.gP);
+    atomic_add3(&gP[x.vert], c0, ge);
+    atomic_add3(&gP[a.he[qnext(h)].vert], c1, ge);
+    if (cf != (T)0) atomic_add3(reinterpret_cast<V3<T> *>(a.gF) + (h >> 2), cf, ge);
+}
+
+// face rule (:2048): each corner receives a quarter of its face's gradient
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_faces(AdjArgs<T> a) {
+    const int32_t h = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (h >= a.H) return;
+    const V3<T> gf = reinterpret_cast<const V3<T> *>(a.gF)[h >> 2];
+    atomic_add3(reinterpret_cast<V3<T> *>(a.gP) + a.he[h].vert, (T)0.25, gf);
+}
+
+// ---- the cage step (level 0 -> 1) -----------------------------------------------
+// vertex rule, one thread per cage halfedge with the reference's own walk
+// (creased :1884-1988, whose smooth test comes first; smooth :1724-1753)
+template <typename T, int CREASED>
+__global__ __launch_bounds__(256) void k_adj_cage_vertices(AdjArgs<T> a, DCage c) {
+    const int32_t h = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (h >= c.H) return;
+    const CageHe x = c.he[h];
+    const V3<T> gv = reinterpret_cast<const V3<T> *>(a.gV)[x.vert];
+    V3<T> *gE = reinterpret_cast<V3<T> *>(a.gE);
+    V3<T> *gF = reinterpret_cast<V3<T> *>(a.gF);
+    V3<T> *gP = reinterpret_cast<V3<T> *>(a.gP);
+    if (!CREASED) {
+        int32_t n = 1, fw, bw;
+        for (fw = c.he[x.prev].twin; fw >= 0 && fw != h && n < kWalkCap; fw = c.he[c.he[fw].prev].twin) ++n;
+        for (bw = x.twin >= 0 ? c.he[x.twin].next : -1; fw < 0 && bw >= 0 && bw != h && n < kWalkCap;
+             bw = c.he[bw].twin >= 0 ? c.he[c.he[bw].twin].next : -1)
+            ++n;
+        const T iw = (T)1 / (T)n, s = fw < 0 ? (T)0 : (T)1;
+        atomic_add3(&gP[x.vert], iw - (T)3 * iw * iw * s, gv);
+        if (s != (T)0) {
+            atomic_add3(&gE[x.edge], (T)4 * iw * iw * s, gv);
+            atomic_add3(&gF[x.face], -iw * iw * s, gv);
+        }
+        return;
+    }
+    const int32_t p = x.prev;
+    const float ts = c.cr[x.edge].sharp, ps = c.cr[c.he[p].edge].sharp;
+    float avgS = ps, cc = sgn(ps), n = 1.0f;
+    int32_t fw, bw;
+    for (fw = c.he[p].twin; fw >= 0 && fw != h && n < (float)kWalkCap; fw = c.he[fw].twin) {
+        const int32_t q = c.he[fw].prev;
+        const float qs = c.cr[c.he[q].edge].sharp;
+        n = n + 1.0f;
+        avgS = avgS + qs;
+        cc = cc + sgn(qs);
+        fw = q;
+    }
+    for (bw = x.twin; fw < 0 && bw >= 0 && bw != h && n < (float)kWalkCap; bw = c.he[bw].twin) {
+        const int32_t q = c.he[bw].next;
+        const float qs = c.cr[c.he[q].edge].sharp;
+        n = n + 1.0f;
+        avgS = avgS + qs;
+        cc = cc + sgn(qs);
+        bw = q;
+    }
+    const bool boundary = fw < 0;
+    const T nn = (T)n;
+    const float ccAll = boundary ? cc + sgn(ts) : cc;
+    const T nPost = boundary ? nn + (T)1 : nn;
+    if (ccAll <= 1.0f) {
+        atomic_add3(&gP[x.vert], (nn - (T)3) / (nn * nn), gv);
+        atomic_add3(&gE[x.edge], (T)4 / (nn * nn), gv);
+        atomic_add3(&gF[x.face], (T)-1 / (nn * nn), gv);
+    } else if (ccAll >= 3.0f || nPost == (T)2) {
+        atomic_add3(&gP[x.vert], (T)1 / nn, gv);
+    } else {
+        const T tt = (T)sat(avgS * 0.5f);
+        const T cw = (T)sgn(ts), pcw = boundary ? (T)sgn(ps) : (T)0;
+        atomic_add3(&gP[x.vert], ((T)1 - tt) / nn + tt * (T)0.25 * (cw + pcw), gv);
+        if (cw != (T)0) atomic_add3(&gE[x.edge], tt * (T)0.25 * cw, gv);
+        if (pcw != (T)0) atomic_add3(&gE[c.he[p].edge], tt * (T)0.25 * pcw, gv);
+    }
+}
+
+// edge rule (creased :1612-1645, smooth :1527-1545)
+template <typename T, int CREASED>
+__global__ __launch_bounds__(256) void k_adj_cage_edges(AdjArgs<T> a, DCage c) {
+    const int32_t h = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (h >= c.H) return;
+    const CageHe x = c.he[h];
+    const V3<T> ge = reinterpret_cast<const V3<T> *>(a.gE)[x.edge];
+    const bool boundary = x.twin < 0;
+    T c0, c1, cf;
+    if (CREASED) {
+        const T w = (T)sat(c.cr[x.edge].sharp);
+        const T k = boundary ? (T)1 : (T)0.5;
+        c0 = (T)0.25 * ((T)1 - w) + (T)0.5 * k * w;
+        c1 = (T)0.5 * k * w;
+        cf = (T)0.25 * ((T)1 - w);
+    } else {
+        const T w = boundary ? (T)1 : (T)0.5;
+        c0 = c1 = (T)0.5 * w * w;
+        cf = (T)0.5 * ((T)1 - w);
+    }
+    V3<T> *gP = reinterpret_cast<V3<T> *>(a.gP);
+    atomic_add3(&gP[x.vert], c0, ge);
+    atomic_add3(&gP[c.he[x.next].vert], c1, ge);
+    if (cf != (T)0) atomic_add3(reinterpret_cast<V3<T> *>(a.gF) + x.face, cf, ge);
+}
+
+// face rule (:1444-1460): each corner receives 1/m of its face's gradient
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_cage_faces(AdjArgs<T> a, DCage c) {
+    const int32_t h = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (h >= c.H) return;
+    const CageHe x = c.he[h];
+    int32_t m = 1;
+    for (int32_t it = x.next; it != h && m < kWalkCap; it = c.he[it].next) ++m;
+    const V3<T> gf = reinterpret_cast<const V3<T> *>(a.gF)[x.face];
+    atomic_add3(reinterpret_cast<V3<T> *>(a.gP) + x.vert, (T)1 / (T)m, gf);
+}
+
+template <typename T>
+__global__ void k_copy(const T *__restrict__ src, T *__restrict__ dst, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+template <typename T>
+__global__ void k_fill0(T *dst, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = (T)0;
+}
+
+// ---- driver -----------------------------------------------------------------------
+namespace {
+unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+unsigned grid_stride(int64_t n) {
+    const int64_t b = (n + 255) / 256;
+    return (unsigned)(b < 65536 ? b : 65536);
+}
+}  // namespace
+
+template <typename T>
+void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA, T *bufB, int creased,
+                    hipStream_t st) {
+    const Counts &n = *pl.n;
+    const int D = pl.D;
+    // G holds the gradient of level d+1 (modified in place), P receives level d
+    T *G = bufA, *P = bufB;
+    hipLaunchKernelGGL(k_copy<T>, dim3(grid_stride(3 * n.V[D])), dim3(256), 0, st, gradFinest, G, 3 * n.V[D]);
+    for (int d = D - 1; d >= 1; --d) {
+        AdjArgs<T> a;
+        a.he = pl.he + n.offH[d];
+        a.cr = pl.cr + n.offC[d];
+        a.H = (int32_t)n.H[d];
+        a.C = (int32_t)n.C[d];
+        a.gV = G;
+        a.gF = G + 3 * n.V[d];
+        a.gE = G + 3 * (n.V[d] + n.F[d]);
+        a.gP = P;
+        const dim3 g(grid_for(n.H[d])), b(256);
+        if (creased) {
+            hipLaunchKernelGGL((k_adj_vertices<T, 1>), g, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            hipLaunchKernelGGL((k_adj_edges<T, 1>), g, b, 0, st, a);
+        } else {
+            hipLaunchKernelGGL((k_adj_vertices<T, 0>), g, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            hipLaunchKernelGGL((k_adj_edges<T, 0>), g, b, 0, st, a);
+        }
+        hipLaunchKernelGGL(k_adj_faces<T>, g, b, 0, st, a);
+        T *tmp = G; G = P; P = tmp;
+    }
+    AdjArgs<T> a;
+    a.he = nullptr;
+    a.cr = nullptr;
+    a.H = pl.cage.H;
+    a.C = pl.cage.E;
+    a.gV = G;
+    a.gF = G + 3 * (int64_t)pl.cage.V;
+    a.gE = G + 3 * ((int64_t)pl.cage.V + pl.cage.F);
+    a.gP = gradCage;
+    hipLaunchKernelGGL(k_fill0<T>, dim3(grid_stride(3 * (int64_t)pl.cage.V)), dim3(256), 0, st, gradCage,
+                       3 * (int64_t)pl.cage.V);
+    const dim3 g(grid_for(pl.cage.H)), b(256);
+    if (creased) {
+        hipLaunchKernelGGL((k_adj_cage_vertices<T, 1>), g, b, 0, st, a, pl.cage);
+        hipLaunchKernelGGL((k_adj_cage_edges<T, 1>), g, b, 0, st, a, pl.cage);
+    } else {
+        hipLaunchKernelGGL((k_adj_cage_vertices<T, 0>), g, b, 0, st, a, pl.cage);
+        hipLaunchKernelGGL((k_adj_cage_edges<T, 0>), g, b, 0, st, a, pl.cage);
+    }
+    hipLaunchKernelGGL(k_adj_cage_faces<T>, g, b, 0, st, a, pl.cage);
+}
+
+template void adjoint_levels<float>(const AdjPlan &, const float *, float *, float *, float *, int, hipStream_t);
+template void adjoint_levels<double>(const AdjPlan &, const double *, double *, double *, double *, int, hipStream_t);
+
+}  // namespace cch

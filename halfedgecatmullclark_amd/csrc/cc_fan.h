@@ -1,0 +1,263 @@
+// Device helpers shared by the fused forward kernels and the adjoint:
+// launch remapping, quad-lane exchanges, the point rules and the structural
+// enumeration of vertex rings.
+#pragma once
+
+#include "cc_common.h"
+
+namespace cch {
+
+constexpr int kWalkCap = 1 << 16;
+
+// Contiguous block ranges per XCD: hardware hands consecutive workgroups to
+// the 8 XCDs in turn; this returns the logical block so that XCD x sees
+// blocks [x*q + min(x, r) ...) of a launch with nb = 8q + r blocks.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb, int remap) {
+    if (!remap) return b;
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, i = b >> 3;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// value of `v` held by lane (quad base + k) for k = 0..3 (DPP quad_perm)
+template <int K>
+__device__ __forceinline__ int quad_bcast(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, K * 0x55, 0xF, 0xF, false);
+}
+template <int K>
+__device__ __forceinline__ float quad_bcastf(float v) {
+    return __int_as_float(quad_bcast<K>(__float_as_int(v)));
+}
+template <int K>
+__device__ __forceinline__ P3 quad_bcast3(P3 p) {
+    return {quad_bcastf<K>(p.x), quad_bcastf<K>(p.y), quad_bcastf<K>(p.z)};
+}
+
+// value held by the lane of the next (+1), opposite (+2) or previous (+3)
+// halfedge of the same quad (DPP quad_perm [1,2,3,0], [2,3,0,1], [3,0,1,2])
+constexpr int kQuadNext = 0x39, kQuadOpp = 0x4E, kQuadPrev = 0x93;
+template <int CTRL>
+__device__ __forceinline__ int quad_rel(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float quad_relf(float v) {
+    return __int_as_float(quad_rel<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ int4 quad_rel4(int4 v) {
+    return make_int4(quad_rel<CTRL>(v.x), quad_rel<CTRL>(v.y), quad_rel<CTRL>(v.z), quad_rel<CTRL>(v.w));
+}
+template <int CTRL>
+__device__ __forceinline__ P3 quad_rel3(P3 p) {
+    return {quad_relf<CTRL>(p.x), quad_relf<CTRL>(p.y), quad_relf<CTRL>(p.z)};
+}
+template <int CTRL>
+__device__ __forceinline__ UV quad_reluv(UV t) {
+    return {quad_relf<CTRL>(t.u), quad_relf<CTRL>(t.v)};
+}
+
+template <typename T>
+__device__ __forceinline__ T pick4(T a0, T a1, T a2, T a3, int k) {
+    return k == 0 ? a0 : (k == 1 ? a1 : (k == 2 ? a2 : a3));
+}
+
+__device__ __forceinline__ float lsharp(const Crease *cr, int32_t C, int32_t e) {
+    return e < C ? cr[e].sharp : 0.0f;
+}
+
+// face point of a level-d quad from its four corner points in halfedge
+// order: gather form (:2015-2025) or scatter form (v/4 accumulated, :2048)
+__device__ __forceinline__ P3 face_rule(P3 p0, P3 p1, P3 p2, P3 p3, bool gather) {
+    if (gather) return mul(add(add(add(p0, p1), p2), p3), 0.25f);
+    P3 a = zero3();
+    a.x += p0.x / 4.0f; a.y += p0.y / 4.0f; a.z += p0.z / 4.0f;
+    a.x += p1.x / 4.0f; a.y += p1.y / 4.0f; a.z += p1.z / 4.0f;
+    a.x += p2.x / 4.0f; a.y += p2.y / 4.0f; a.z += p2.z / 4.0f;
+    a.x += p3.x / 4.0f; a.y += p3.y / 4.0f; a.z += p3.z / 4.0f;
+    return a;
+}
+
+__device__ __forceinline__ P3 quad_face(const He *he, const P3 *old, int32_t quad_base, bool gather) {
+    const int4 *q4 = reinterpret_cast<const int4 *>(he) + quad_base;
+    return face_rule(old[q4[0].z], old[q4[1].z], old[q4[2].z], old[q4[3].z], gather);
+}
+
+// scatter-form edge contribution (:2118-2136 smooth, :2205-2238 creased)
+__device__ __forceinline__ P3 edge_contrib(P3 v0, P3 v1, P3 f, bool boundary, float sharp, bool creased) {
+    if (creased) {
+        const float w = sat(sharp);
+        P3 sp = mul(lerp(v0, v1, 0.5f), boundary ? 1.0f : 0.5f);
+        P3 sm = mul(lerp(v0, f, 0.5f), 0.5f);
+        return lerp(sm, sp, w);
+    }
+    const float w = boundary ? 1.0f : 0.5f;
+    return lerp(mul(f, 0.5f), lerp(mul(v0, w), mul(v1, w), 0.5f), w);
+}
+
+// gather-form edge point (:2094-2102 smooth, :2180-2188 creased)
+__device__ __forceinline__ P3 edge_gather(P3 v0, P3 v1, P3 f0, P3 f1, float w, bool creased) {
+    P3 t1 = add(v0, v1), t2 = add(f0, f1);
+    P3 sp = mul(t1, 0.5f);
+    P3 sm = mul(add(t1, t2), 0.25f);
+    return creased ? lerp(sm, sp, w) : lerp(sp, sm, w);
+}
+
+__device__ __forceinline__ int4 child_record(int k, int32_t h, int4 x, int4 p, int32_t V, int32_t F, int32_t E) {
+    const int32_t prev = qprev(h);
+    switch (k) {
+    case 0: return make_int4(i4(qnext(x.x)) + 3, 2 * x.y + (h > x.x ? 0 : 1), x.z, 0);
+    case 1: return make_int4(i4(qnext(h)) + 2, 2 * E + h, V + F + x.y, 0);
+    case 2: return make_int4(i4(prev) + 1, 2 * E + prev, V + (h >> 2), 0);
+    default: return make_int4(i4(p.x) + 0, 2 * p.y + (prev > p.x ? 1 : 0), V + F + p.y, 0);
+    }
+}
+
+
+// ---- vertex points ----------------------------------------------------------
+// Per-halfedge scatter contribution (creased :2509-2611 with the corner test
+// first; smooth :2320-2348) from the quantities the ring walk gathered.
+struct RingParams {
+    float n;        // halfedges around the vertex (valence before the boundary correction)
+    float cc;       // crease count including the boundary correction
+    bool boundary;
+    float s1, s2;   // the first two nonzero sharpness values seen ...
+    int32_t e1, e2; // ... and their edges
+    float sall;     // sum of every incident sharpness
+};
+
+__device__ __forceinline__ P3 vertex_scatter_contrib(const RingParams &rp, P3 old, P3 E, P3 PE, P3 Fc, float ts,
+                                                     float ps, int32_t edge, bool creased) {
+    if (!creased) {
+        const float w = 1.0f / rp.n;
+        const float s = rp.boundary ? 0.0f : 1.0f;
+        P3 a;
+        a.x = w * (old.x + w * s * (4.0f * E.x - Fc.x - 3.0f * old.x));
+        a.y = w * (old.y + w * s * (4.0f * E.y - Fc.y - 3.0f * old.y));
+        a.z = w * (old.z + w * s * (4.0f * E.z - Fc.z - 3.0f * old.z));
+        return a;
+    }
+    const float cw = sgn(ts), pcw = sgn(ps);
+    const float n = rp.n;
+    const P3 corner = mul(old, 1.0f / n);
+    P3 cr = add(mul(old, 0.25f * cw), mul(E, 0.25f * cw));
+    P3 sm = add(mul(Fc, -1.0f), mul(E, 4.0f));
+    sm = add(sm, mul(old, n - 3.0f));
+    sm = mul(sm, 1.0f / (n * n));
+    const float nn = rp.boundary ? n + 1.0f : n;
+    if (rp.boundary) cr = add(cr, add(mul(old, 0.25f * pcw), mul(PE, 0.25f * pcw)));
+    if (rp.cc >= 3.0f || nn == 2.0f) return corner;
+    if (rp.cc <= 1.0f) return sm;
+    // two creased edges: the sum of the incident sharpness this halfedge's
+    // walk visits (every edge but its own on a boundary)
+    float avg = rp.sall;
+    if (rp.boundary) avg = edge == rp.e1 ? rp.s2 : (edge == rp.e2 ? rp.s1 : rp.s1 + rp.s2);
+    return lerp(corner, cr, sat(avg * 0.5f));
+}
+
+// ---- vertex points with structurally enumerated rings -------------------------
+// The halfedges around a vertex of level d follow from the level c at which
+// the vertex appeared (CatmullClark.h:2770-2793): children 0 of a parent keep
+// the parent's vertex, so a ring at level d is the ring at level c shifted
+// by 2(d - c) bits; a face vertex of a quad face f' (level c-1 >= 1) has the
+// ring 16f'+2, +6, +10, +14; an edge vertex of edge(p) with twin t has
+// 4p+1, 4next(t)+3, 4t+1, 4next(p)+3 (or 4next(p)+3, 4p+1 on a boundary);
+// cage vertices and faces of the cage take their ring from the cage.  The
+// ring is listed in the reference's forward walk order (twin(prev(x))),
+// starting at the vertex's first boundary halfedge or at its representative
+// (CatmullClark.h:1336), so no step waits on the previous one.
+struct Fan {
+    int kind;       // 0 cage vertex, 1 face vertex of a quad, 2 face vertex of a cage face, 3 edge vertex
+    int shift;      // level-d halfedge = level-c halfedge << shift
+    int n;          // halfedges around the vertex
+    int ri;         // index of the representative in the fan
+    bool boundary;
+    int32_t a0, a1, a2, a3;
+};
+
+__device__ __forceinline__ int32_t fan_at(const Fan &f, const DCage &c, int i) {
+    switch (f.kind) {
+    case 1: return (f.a0 + 4 * i) << f.shift;
+    case 3: return pick4(f.a0, f.a1, f.a2, f.a3, i) << f.shift;
+    case 2: {  // the face's cage halfedges from f2h, children 2
+        int32_t p = f.a0;
+        for (int k = 0; k < i && k < kWalkCap; ++k) p = c.he[p].next;
+        return (4 * p + 2) << f.shift;
+    }
+    default: {  // the vertex's cage halfedges in forward order, children 0
+        int32_t g = f.a0;
+        for (int k = 0; k < i && k < kWalkCap; ++k) g = c.he[c.he[g].prev].twin;
+        return (4 * g) << f.shift;
+    }
+    }
+}
+
+// Is h the representative of its vertex?  If so, describe the fan.
+__device__ __forceinline__ bool owner_fan(int32_t h, int32_t vert, int d, const DCage &c, const LevelTab &t,
+                                          const He *const *levels, Fan &f) {
+    int k = 0;
+    while (k < d - 1 && ((h >> (2 * k)) & 3) == 0) ++k;
+    const int32_t q = h >> (2 * k);
+    const int lvl = d - k;            // level at which the vertex appeared
+    const int digit = q & 3;
+    const int32_t p = q >> 2;         // parent halfedge, level lvl-1
+    f.shift = 2 * k;
+    f.boundary = false;
+    f.ri = 0;
+    if (digit == 0) {                 // lvl == 1: a cage vertex
+        if (c.v2h[vert] != p) return false;
+        f.kind = 0;
+        f.a0 = p;
+        int n = 1;
+        int32_t g = c.he[c.he[p].prev].twin;
+        for (; g >= 0 && g != p && n < kWalkCap; g = c.he[c.he[g].prev].twin) ++n;
+        f.n = n;
+        f.boundary = g < 0;
+        return true;
+    }
+    if (digit == 3) return false;
+    if (digit == 2) {
+        if (lvl - 1 >= 1) {
+            if ((p & 3) != 0) return false;
+            f.kind = 1;
+            f.a0 = 4 * p + 2;
+            f.n = 4;
+            return true;
+        }
+        const int32_t face = c.he[p].face;
+        if (c.f2h[face] != p) return false;
+        f.kind = 2;
+        f.a0 = p;
+        int n = 1;
+        for (int32_t it = c.he[p].next; it != p && n < kWalkCap; it = c.he[it].next) ++n;
+        f.n = n;
+        return true;
+    }
+    // digit 1: an edge vertex of edge(p)
+    const int32_t e = vert - t.V[lvl - 1] - t.F[lvl - 1];
+    int32_t tw, np, nt;
+    if (lvl - 1 >= 1) {
+        if (edge_rep(c, t, e, lvl - 1) != p) return false;
+        tw = levels[lvl - 1][p].twin;
+        np = qnext(p);
+        nt = qnext(tw);
+    } else {
+        if (c.e2h[e] != p) return false;
+        tw = c.he[p].twin;
+        np = c.he[p].next;
+        nt = tw >= 0 ? c.he[tw].next : -1;
+    }
+    f.kind = 3;
+    if (tw >= 0) {
+        f.n = 4;
+        f.a0 = 4 * p + 1; f.a1 = 4 * nt + 3; f.a2 = 4 * tw + 1; f.a3 = 4 * np + 3;
+    } else {
+        f.n = 2;
+        f.boundary = true;
+        f.ri = 1;
+        f.a0 = 4 * np + 3; f.a1 = 4 * p + 1; f.a2 = f.a3 = 0;
+    }
+    return true;
+}
+
+
+}  // namespace cch

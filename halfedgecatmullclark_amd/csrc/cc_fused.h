@@ -1,0 +1,35 @@
+// Fused per-level kernels (levels d >= 1) and their launchers.
+#pragma once
+
+#include "cc_common.h"
+
+namespace cch {
+
+enum { kStoreDirect = 0, kStoreLds = 1 };
+
+// Level-d inputs and level-(d+1) outputs of one fused launch.
+struct FusedArgs {
+    const He *he;       // level d halfedges
+    He *out_he;         // level d+1 halfedges
+    const Crease *cr;   // level d creases
+    const P3 *old;      // level d points
+    P3 *vp, *fp, *ep;   // level d+1 points: vertex | face | edge slices
+    int32_t H, V, F, E, C;
+    int xcd_remap;
+};
+
+// topo: write child twin/edge/vertex IDs; uvMode: 0 leave uvIDs, 1 refine
+// them, 2 write 0; points: face and edge points of level d+1.
+void launch_level_fused(const FusedArgs &a, int gather, int creased, int topo, int uvMode, int points, int store,
+                        hipStream_t st);
+// base pointers of the halfedges of every level (index 1..D)
+struct LevelPtrs {
+    const He *he[kMaxDepth + 1];
+};
+
+// vertex points of level d+1 (after launch_level_fused with points); fan
+// selects the structurally enumerated rings over the ring walk
+void launch_level_vertices(const FusedArgs &a, const DCage &c, const LevelTab &t, const LevelPtrs &lp, int d,
+                           int gather, int creased, int fan, hipStream_t st);
+
+}  // namespace cch

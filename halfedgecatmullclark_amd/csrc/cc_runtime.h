@@ -6,6 +6,8 @@
 #include <vector>
 
 #include "cc_internal.h"
+#include "cc_adjoint.h"
+#include "cc_fused.h"
 #include "cc_kernels.h"
 
 struct ccd_Subd {
@@ -29,7 +31,11 @@ struct ccd_Subd {
     cch::Crease *cr = nullptr;
     cch::P3 *pts = nullptr;
     hipStream_t stream = nullptr;
+    // adjoint scratch (allocated on first use, per scalar size)
+    void *adj_a = nullptr, *adj_b = nullptr, *adj_in = nullptr, *adj_out = nullptr;
+    size_t adj_scalar = 0;
     bool timing = false;
+    int opt_fused = 1, opt_store = cch::kStoreLds, opt_remap = 1, opt_fan = 1;
     std::vector<Timed> timed;
 
     bool alloc(const cc_Mesh *cage, int32_t depth);
@@ -37,11 +43,17 @@ struct ccd_Subd {
     bool upload_cage(const cc_Mesh *cage, hipStream_t st);
     cch::DCage cage_view(const cch::P3 *points) const;
     cch::Lvl level_view(int d, const cch::P3 *levelPts, cch::P3 *nextPts) const;
+    cch::FusedArgs fused_args(int d, const cch::P3 *points) const;
+    cch::LevelPtrs level_ptrs() const;
+    void cage_points(int32_t steps, const cch::P3 *cagePts, cch::P3 *out, hipStream_t st);
     bool refine(int32_t steps, int dmax, hipStream_t st);
     bool refine_topology(int32_t steps, int dmax, hipStream_t st);
     bool refine_points(int32_t steps, int dmax, const cch::P3 *cagePts, cch::P3 *out, hipStream_t st);
     void begin_timing();
+    bool adjoint(const void *gradFinest, void *gradCage, int32_t steps, hipStream_t st);
     void zero(float *p, int64_t count, hipStream_t st);
     template <typename K, typename... Args>
     void launch(const char *name, K kernel, int64_t threads, hipStream_t st, Args... args);
+    template <typename Fn>
+    void launch_timed(const char *name, int64_t threads, hipStream_t st, Fn &&fn);
 };

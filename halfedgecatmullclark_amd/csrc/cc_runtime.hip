@@ -92,7 +92,7 @@ bool ccd_Subd::alloc(const cc_Mesh *cage, int32_t depth) {
 
 void ccd_Subd::free_all() {
     DeviceGuard g(device);
-    void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts};
+    void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts, adj_a, adj_b, adj_in, adj_out};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : timed) {
@@ -165,6 +165,22 @@ void ccd_Subd::launch(const char *name, K kernel, int64_t threads, hipStream_t s
     }
 }
 
+template <typename Fn>
+void ccd_Subd::launch_timed(const char *name, int64_t threads, hipStream_t st, Fn &&fn) {
+    if (threads <= 0) return;
+    Timed t{name, nullptr, nullptr};
+    if (timing) {
+        (void)hipEventCreate(&t.start);
+        (void)hipEventCreate(&t.stop);
+        (void)hipEventRecord(t.start, st);
+    }
+    fn();
+    if (timing) {
+        (void)hipEventRecord(t.stop, st);
+        timed.push_back(t);
+    }
+}
+
 void ccd_Subd::zero(float *p, int64_t count, hipStream_t st) {
     if (count <= 0) return;
     const int64_t blocks = (count + kBlock - 1) / kBlock;
@@ -172,8 +188,29 @@ void ccd_Subd::zero(float *p, int64_t count, hipStream_t st) {
     hipLaunchKernelGGL(k_zero, dim3(grid), dim3(kBlock), 0, st, p, count);
 }
 
-// Topology, creases and uvs of levels 1..dmax (CatmullClark.h:2803-2812,
-// :2928-2939, :3033-3042).  uvMode: 0 leave uvIDs, 1 refine, 2 write zero.
+FusedArgs ccd_Subd::fused_args(int d, const P3 *points) const {
+    FusedArgs a;
+    a.he = he + n.offH[d];
+    a.out_he = he + n.offH[d + 1];
+    a.cr = cr + n.offC[d];
+    a.old = points + n.offV[d];
+    a.vp = const_cast<P3 *>(points) + n.offV[d + 1];
+    a.fp = a.vp + n.V[d];
+    a.ep = a.fp + n.F[d];
+    a.H = (int32_t)n.H[d]; a.V = (int32_t)n.V[d]; a.F = (int32_t)n.F[d]; a.E = (int32_t)n.E[d]; a.C = (int32_t)n.C[d];
+    a.xcd_remap = opt_remap;
+    return a;
+}
+
+LevelPtrs ccd_Subd::level_ptrs() const {
+    LevelPtrs lp;
+    for (int d = 0; d <= kMaxDepth; ++d) lp.he[d] = (d >= 1 && d <= D) ? he + n.offH[d] : nullptr;
+    return lp;
+}
+
+// Topology, creases and uvs of levels 1..dmax with one kernel per output
+// kind (CatmullClark.h:2803-2812, :2928-2939, :3033-3042).  uvMode: 0 leave
+// uvIDs, 1 refine, 2 write zero.
 bool ccd_Subd::refine_topology(int32_t steps, int dmax, hipStream_t st) {
     const bool topo = (steps & CCD_HALFEDGES) != 0;
     int uvMode = ((steps & CCD_UVS) && U0 > 0) ? 1 : 0;
@@ -194,24 +231,38 @@ bool ccd_Subd::refine_topology(int32_t steps, int dmax, hipStream_t st) {
     return true;
 }
 
+// the cage step of the points (CatmullClark.h:2633-2636, :2661-2663)
+void ccd_Subd::cage_points(int32_t steps, const P3 *cagePts, P3 *out, hipStream_t st) {
+    const bool gather = (steps & CCD_GATHER) != 0;
+    const int creased = (steps & CCD_NOCREASES) ? 0 : 1;
+    const DCage c = cage_view(cagePts);
+    P3 *vp = out, *fp = out + V0, *ep = out + V0 + F0;
+    launch("cage_faces", k_cage_faces, F0, st, c, fp, (int)gather);
+    launch("cage_edges", k_cage_edges, E0, st, c, (const P3 *)fp, ep, (int)gather, creased);
+    if (gather) {
+        launch("cage_vertices", k_cage_vertices_gather, V0, st, c, (const P3 *)fp, (const P3 *)ep, vp, creased);
+    } else {
+        zero(&vp->x, 3 * (int64_t)V0, st);
+        launch("cage_vertices", k_cage_vertices_scatter, H0, st, c, (const P3 *)fp, (const P3 *)ep, vp, creased);
+    }
+}
+
 // Points of levels 1..dmax (CatmullClark.h:2631-2683): the cage step, then
 // face, edge and vertex points per level.
 bool ccd_Subd::refine_points(int32_t steps, int dmax, const P3 *cagePts, P3 *out, hipStream_t st) {
     const bool gather = (steps & CCD_GATHER) != 0;
     const int creased = (steps & CCD_NOCREASES) ? 0 : 1;
     const DCage c = cage_view(cagePts);
-    {
-        P3 *vp = out, *fp = out + V0, *ep = out + V0 + F0;
-        launch("cage_faces", k_cage_faces, F0, st, c, fp, (int)gather);
-        launch("cage_edges", k_cage_edges, E0, st, c, (const P3 *)fp, ep, (int)gather, creased);
-        if (gather) {
-            launch("cage_vertices", k_cage_vertices_gather, V0, st, c, (const P3 *)fp, (const P3 *)ep, vp, creased);
-        } else {
-            zero(&vp->x, 3 * (int64_t)V0, st);
-            launch("cage_vertices", k_cage_vertices_scatter, H0, st, c, (const P3 *)fp, (const P3 *)ep, vp, creased);
-        }
-    }
+    cage_points(steps, cagePts, out, st);
     for (int d = 1; d < dmax; ++d) {
+        if (opt_fused) {
+            const FusedArgs a = fused_args(d, out);
+            launch_timed("level_points", n.H[d], st,
+                         [&] { launch_level_fused(a, gather, creased, 0, 0, 1, opt_store, st); });
+            launch_timed("level_vertices", n.H[d], st,
+                         [&] { launch_level_vertices(a, c, tab, level_ptrs(), d, gather, creased, opt_fan, st); });
+            continue;
+        }
         const Lvl L = level_view(d, out + n.offV[d], out + n.offV[d + 1]);
         launch("faces", k_faces, n.F[d], st, L, (int)gather);
         if (gather) {
@@ -228,11 +279,70 @@ bool ccd_Subd::refine_points(int32_t steps, int dmax, const P3 *cagePts, P3 *out
 
 bool ccd_Subd::refine(int32_t steps, int dmax, hipStream_t st) {
     if (timing) begin_timing();
-    if (!refine_topology(steps, dmax, st)) return false;
-    if (steps & CCD_POINTS) {
-        if (!refine_points(steps, dmax, nullptr, pts, st)) return false;
+    if (!opt_fused) {
+        if (!refine_topology(steps, dmax, st)) return false;
+        if ((steps & CCD_POINTS) && !refine_points(steps, dmax, nullptr, pts, st)) return false;
+        CC_TRY(hipGetLastError(), "kernel launch");
+        return true;
+    }
+    // fused: creases of every level first (the point rules read them), then
+    // per level one kernel for children + face/edge points and one for
+    // vertex points
+    const bool topo = (steps & CCD_HALFEDGES) != 0;
+    const bool points = (steps & CCD_POINTS) != 0;
+    const bool gather = (steps & CCD_GATHER) != 0;
+    const int creased = (steps & CCD_NOCREASES) ? 0 : 1;
+    int uvMode = ((steps & CCD_UVS) && U0 > 0) ? 1 : 0;
+    if (topo && uvMode == 0 && U0 == 0) uvMode = 2;
+    if (steps & CCD_CREASES) {
+        launch("cage_creases", k_creases, E0, st, (const Crease *)c_cr, E0, cr);
+        for (int d = 1; d < dmax; ++d)
+            launch("level_creases", k_creases, n.C[d], st, (const Crease *)(cr + n.offC[d]), (int32_t)n.C[d], cr + n.offC[d + 1]);
+    }
+    const DCage c = cage_view(nullptr);
+    if (topo || uvMode == 1) launch("cage_halfedges", k_cage_halfedges, H0, st, c, he, (int)topo, uvMode);
+    if (points) cage_points(steps, nullptr, pts, st);
+    for (int d = 1; d < dmax; ++d) {
+        const FusedArgs a = fused_args(d, pts);
+        if (topo || uvMode == 1 || points)
+            launch_timed("level_fused", n.H[d], st,
+                         [&] { launch_level_fused(a, gather, creased, topo, uvMode, points, opt_store, st); });
+        if (points)
+            launch_timed("level_vertices", n.H[d], st,
+                         [&] { launch_level_vertices(a, c, tab, level_ptrs(), d, gather, creased, opt_fan, st); });
     }
     CC_TRY(hipGetLastError(), "kernel launch");
+    return true;
+}
+
+// Adjoint of the point refinement for the current topology and creases.
+bool ccd_Subd::adjoint(const void *gradFinest, void *gradCage, int32_t steps, hipStream_t st) {
+    const size_t scalar = (steps & CCD_FP64) ? sizeof(double) : sizeof(float);
+    if (adj_scalar != scalar) {
+        if (adj_a) (void)hipFree(adj_a);
+        if (adj_b) (void)hipFree(adj_b);
+        adj_a = adj_b = nullptr;
+        adj_scalar = 0;
+        CC_TRY(hipMalloc(&adj_a, 3 * scalar * (size_t)n.V[D]), "hipMalloc(adjoint scratch)");
+        CC_TRY(hipMalloc(&adj_b, 3 * scalar * (size_t)(D >= 1 ? n.V[D - 1] : 1)), "hipMalloc(adjoint scratch)");
+        adj_scalar = scalar;
+    }
+    AdjPlan pl;
+    pl.n = &n;
+    pl.D = D;
+    pl.he = he;
+    pl.cr = cr;
+    pl.cage = cage_view(nullptr);
+    pl.tab = tab;
+    pl.lp = level_ptrs();
+    const int creased = (steps & CCD_NOCREASES) ? 0 : 1;
+    if (scalar == sizeof(double))
+        adjoint_levels<double>(pl, (const double *)gradFinest, (double *)gradCage, (double *)adj_a, (double *)adj_b,
+                               creased, st);
+    else
+        adjoint_levels<float>(pl, (const float *)gradFinest, (float *)gradCage, (float *)adj_a, (float *)adj_b, creased,
+                              st);
+    CC_TRY(hipGetLastError(), "adjoint launch");
     return true;
 }
 
@@ -390,6 +500,49 @@ CC_HOST_ENTRY(ccs_RefineCageVertexUvs, CCD_UVS, 1)
 CC_HOST_ENTRY(ccs_RefineCageVertexPoints_Scatter, CCD_POINTS, 1)
 CC_HOST_ENTRY(ccs_RefineCageVertexPoints_Gather, CCD_POINTS | CCD_GATHER, 1)
 
+// The adjoint on a host cc_Subd: the topology and creases it reads are the
+// subd's host arrays (levels 1..D-1), uploaded with the cage and gradient.
+static void host_adjoint(const cc_Subd *subd, const void *dL_dFinest, void *dL_dCage, size_t scalar) {
+    clear_error();
+    if (!subd || !subd->cage || !dL_dFinest || !dL_dCage) {
+        set_error(11, "ccs_RefineVertexPoints_Adjoint: null argument");
+        fprintf(stderr, "cc: ccs_RefineVertexPoints_Adjoint: null argument\n");
+        return;
+    }
+    const char *why = nullptr;
+    if (!ref_counts_fit_int32(subd->cage, subd->maxDepth, &why)) {
+        set_error(12, why);
+        return;
+    }
+    ccd_Subd *s = mirror_for(subd);
+    if (!s) return;
+    DeviceGuard g(s->device);
+    hipStream_t st = s->stream;
+    const Counts &n = s->n;
+    const int D = s->D;
+    if (!s->upload_cage(subd->cage, st)) return;
+    // gradient staging, sized for doubles so both precisions fit
+    if (!s->adj_in && hip_failed(hipMalloc(&s->adj_in, 3 * sizeof(double) * (size_t)n.V[D]), "hipMalloc(adjoint input)")) return;
+    if (!s->adj_out && hip_failed(hipMalloc(&s->adj_out, 3 * sizeof(double) * (size_t)n.V[0]), "hipMalloc(adjoint output)")) return;
+    if (D > 1 && (hip_failed(hipMemcpyAsync(s->he, subd->halfedges, sizeof(He) * (size_t)n.cumH(D - 1), hipMemcpyHostToDevice, st), "upload halfedges") ||
+                  hip_failed(hipMemcpyAsync(s->cr, subd->creases, sizeof(Crease) * (size_t)n.cumC(D - 1), hipMemcpyHostToDevice, st), "upload creases")))
+        return;
+    if (hip_failed(hipMemcpyAsync(s->adj_in, dL_dFinest, 3 * scalar * (size_t)n.V[D], hipMemcpyHostToDevice, st), "upload gradient"))
+        return;
+    if (!s->adjoint(s->adj_in, s->adj_out, scalar == sizeof(double) ? CCD_FP64 : 0, st)) return;
+    if (hip_failed(hipMemcpyAsync(dL_dCage, s->adj_out, 3 * scalar * (size_t)n.V[0], hipMemcpyDeviceToHost, st), "download gradient") ||
+        hip_failed(hipStreamSynchronize(st), "ccs_RefineVertexPoints_Adjoint"))
+        return;
+}
+
+void ccs_RefineVertexPoints_Adjoint(const cc_Subd *subd, const cc_VertexPoint *dL_dFinest, cc_VertexPoint *dL_dCage) {
+    host_adjoint(subd, dL_dFinest, dL_dCage, sizeof(float));
+}
+
+void ccs_RefineVertexPoints_Adjoint_f64(const cc_Subd *subd, const double *dL_dFinest, double *dL_dCage) {
+    host_adjoint(subd, dL_dFinest, dL_dCage, sizeof(double));
+}
+
 // no-op without cage uvs, like the reference (CatmullClark.h:2930)
 void ccs_RefineVertexUvs(cc_Subd *subd) {
     if (subd && subd->cage && subd->cage->uvCount <= 0) { clear_error(); return; }
@@ -489,6 +642,16 @@ int32_t ccd_RefinePoints(ccd_Subd *s, const float *devCagePoints, float *devPoin
     return 0;
 }
 
+int32_t ccd_RefinePointsAdjoint(ccd_Subd *s, const float *devGradFinest, float *devGradCage, int32_t steps,
+                                void *stream) {
+    clear_error();
+    if (!s || !devGradFinest || !devGradCage) { set_error(11, "ccd_RefinePointsAdjoint: null argument"); return t_err; }
+    DeviceGuard g(s->device);
+    hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    if (!s->adjoint(devGradFinest, devGradCage, steps, st)) return t_err ? t_err : 1;
+    return 0;
+}
+
 int32_t ccd_Download(const ccd_Subd *s, cc_Subd *dst, int32_t steps) {
     clear_error();
     if (!s || !dst) { set_error(11, "ccd_Download: null argument"); return t_err; }
@@ -514,6 +677,17 @@ int32_t ccd_Synchronize(ccd_Subd *s) {
     DeviceGuard g(s->device);
     if (hip_failed(hipDeviceSynchronize(), "ccd_Synchronize")) return t_err;
     return 0;
+}
+
+int32_t ccd_SetOption(ccd_Subd *s, int32_t key, int32_t value) {
+    if (!s) return 11;
+    switch (key) {
+    case CCD_OPT_FUSED: s->opt_fused = value != 0; return 0;
+    case CCD_OPT_STORE: s->opt_store = value ? kStoreLds : kStoreDirect; return 0;
+    case CCD_OPT_XCD_REMAP: s->opt_remap = value != 0; return 0;
+    case CCD_OPT_FAN: s->opt_fan = value != 0; return 0;
+    default: return 14;
+    }
 }
 
 int32_t ccd_SetTiming(ccd_Subd *s, int32_t enable) {

@@ -215,6 +215,11 @@ void ccs_RefineCageVertexPoints_Gather(cc_Subd *subd);
 void ccs_RefineVertexPoints_Adjoint(const cc_Subd *subd,
                                     const cc_VertexPoint *dL_dFinest,
                                     cc_VertexPoint *dL_dCage);
+/* the same in double precision (V_D x 3 and V_0 x 3 doubles), for gradient
+ * checks */
+void ccs_RefineVertexPoints_Adjoint_f64(const cc_Subd *subd,
+                                        const double *dL_dFinest,
+                                        double *dL_dCage);
 
 /* ---- extension: device-resident refinement ------------------------------
  * A ccd_Subd owns device copies of a cage and of levels 1..maxDepth laid out
@@ -230,6 +235,7 @@ enum {
     CCD_POINTS = 8,       /* vertex points                                 */
     CCD_GATHER = 16,      /* gather point rules instead of scatter rules   */
     CCD_NOCREASES = 32,   /* plain Catmull-Clark rules (ignore sharpness)  */
+    CCD_FP64 = 64,        /* ccd_RefinePointsAdjoint: gradients are doubles */
     CCD_ALL = 15
 };
 
@@ -250,11 +256,24 @@ int32_t ccd_Refine(ccd_Subd *subd, int32_t steps, void *stream);
 int32_t ccd_RefinePoints(ccd_Subd *subd, const float *devCagePoints, float *devPoints,
                          int32_t steps, void *stream);
 /* adjoint on the device: devGradFinest holds V_D x 3 floats, devGradCage
- * receives V_0 x 3 floats. */
+ * receives V_0 x 3 floats (doubles for both with CCD_FP64).  The transpose
+ * of the scatter rules, creased unless CCD_NOCREASES; topology and creases
+ * must be refined already.  devGradFinest is not modified. */
 int32_t ccd_RefinePointsAdjoint(ccd_Subd *subd, const float *devGradFinest, float *devGradCage,
                                 int32_t steps, void *stream);
 int32_t ccd_Download(const ccd_Subd *subd, cc_Subd *dst, int32_t steps);
 int32_t ccd_Synchronize(ccd_Subd *subd);
+/* kernel-selection options (all have working defaults):
+ *   CCD_OPT_FUSED      1: one fused kernel per level for children, face and
+ *                      edge points plus one for vertex points (default);
+ *                      0: one kernel per output kind, vertex scatter with
+ *                      float atomics (the reference's own structure)
+ *   CCD_OPT_STORE      child-record stores: 0 direct, 1 staged through LDS
+ *   CCD_OPT_XCD_REMAP  1: each XCD processes one contiguous range of a level
+ *   CCD_OPT_FAN        1: vertex rings enumerated from the level each vertex
+ *                      appeared at (default); 0: ring walks over twins */
+enum { CCD_OPT_FUSED = 0, CCD_OPT_STORE = 1, CCD_OPT_XCD_REMAP = 2, CCD_OPT_FAN = 3 };
+int32_t ccd_SetOption(ccd_Subd *subd, int32_t key, int32_t value);
 /* per-kernel timing with HIP events on the launch stream: enable, run, then
  * read the kernel names and mean milliseconds of the last ccd_Refine. */
 int32_t ccd_SetTiming(ccd_Subd *subd, int32_t enable);

@@ -52,8 +52,9 @@ def refine(cage: Cage, depth: int, flags: int = ALL, real=np.float32, threads: i
     real = np.dtype(real)
     c = cage.contiguous()
     n = level_counts(c.V, c.H, c.E, c.F, depth)
-    pts = np.ascontiguousarray(c.vertex_points, dtype=real)
-    uvs = np.ascontiguousarray(c.uvs, dtype=real)
+    # points and uvs at the requested precision straight from the caller
+    pts = np.ascontiguousarray(np.asarray(cage.vertex_points).reshape(-1, 3), dtype=real)
+    uvs = np.ascontiguousarray(np.asarray(cage.uvs).reshape(-1, 2), dtype=real)
     cr = np.zeros(c.E, dtype=crease_dtype(real))
     for k in ("nextID", "prevID", "sharpness"):
         cr[k] = c.creases[k]

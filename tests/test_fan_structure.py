@@ -1,0 +1,135 @@
+"""The structural vertex rings used by the GPU vertex-point kernel.
+
+The kernel never walks twin pointers to find the halfedges around a vertex:
+it derives them from the level at which the vertex appeared (children 0 keep
+their parent's vertex; face and edge vertices have fixed child patterns,
+CatmullClark.h:2770-2793).  This restates that derivation in Python and
+checks, on oracle-refined topology, that for every vertex exactly one
+halfedge claims to be its representative (the reference's
+ccs_VertexPointToHalfedgeID, CatmullClark.h:1336), that the claimed fan is
+exactly the ring found by walking twins, in forward order, and that every
+fan halfedge is in range.
+"""
+import numpy as np
+import pytest
+
+from halfedgecatmullclark_amd import cage as C
+from oracle import oracle as O
+from tests.helpers import small_cages
+
+
+def qnext(h):
+    return (h & ~3) | ((h + 1) & 3)
+
+
+def qprev(h):
+    return (h & ~3) | ((h - 1) & 3)
+
+
+def owner_fan(h, vert, d, cage, counts, levels):
+    """Mirror of owner_fan() in cc_fused.hip; returns None or (fan, ri, boundary)."""
+    he = cage.halfedges
+    k = 0
+    while k < d - 1 and ((h >> (2 * k)) & 3) == 0:
+        k += 1
+    q = h >> (2 * k)
+    lvl = d - k
+    digit = q & 3
+    p = q >> 2
+    shift = 2 * k
+    if digit == 0:
+        if cage.v2h[vert] != p:
+            return None
+        ring, g = [p], he[he[p, 2], 0]
+        while g >= 0 and g != p:
+            ring.append(int(g))
+            g = he[he[g, 2], 0]
+        return [(4 * x) << shift for x in ring], 0, bool(g < 0)
+    if digit == 3:
+        return None
+    if digit == 2:
+        if lvl - 1 >= 1:
+            if p & 3:
+                return None
+            return [((4 * p + 2) + 4 * i) << shift for i in range(4)], 0, False
+        face = he[p, 3]
+        if cage.f2h[face] != p:
+            return None
+        ring, it = [p], he[p, 1]
+        while it != p:
+            ring.append(int(it))
+            it = he[it, 1]
+        return [(4 * x + 2) << shift for x in ring], 0, False
+    e = vert - counts["V"][lvl - 1] - counts["F"][lvl - 1]
+    if lvl - 1 >= 1:
+        if O.edge_to_halfedge(cage, int(e), lvl - 1) != p:
+            return None
+        tw = int(levels[lvl - 1]["twinID"][p])
+        np_, nt = qnext(p), qnext(tw)
+    else:
+        if cage.e2h[e] != p:
+            return None
+        tw = int(he[p, 0])
+        np_ = int(he[p, 1])
+        nt = int(he[tw, 1]) if tw >= 0 else -1
+    if tw >= 0:
+        return [x << shift for x in (4 * p + 1, 4 * nt + 3, 4 * tw + 1, 4 * np_ + 3)], 0, False
+    return [x << shift for x in (4 * np_ + 3, 4 * p + 1)], 1, True
+
+
+def walked_fan(first_candidates, lv):
+    """Ring of the vertex by walking twins: from the halfedge whose twin is
+    negative (boundary) or from the given start, forward via twin(prev(x))."""
+    twin = lv["twinID"]
+    h = first_candidates
+    it, start = h, h
+    # back up to the boundary, if any
+    for _ in range(10000):
+        t = twin[it]
+        if t < 0:
+            start = it
+            break
+        nx = qnext(int(t))
+        if nx == h:
+            start = h
+            break
+        it = nx
+    ring, x = [start], start
+    for _ in range(10000):
+        nxt = int(twin[qprev(x)])
+        if nxt < 0 or nxt == start:
+            return ring, nxt < 0
+        ring.append(nxt)
+        x = nxt
+
+
+@pytest.mark.parametrize("name,cage,depth", small_cages()[:6], ids=[c[0] for c in small_cages()[:6]])
+def test_structural_fans(name, cage, depth):
+    depth = min(depth, 4)
+    out = O.refine(cage, depth, O.HALFEDGES)
+    counts = out["counts"]
+    levels = {d: O.level(out, "halfedges", d) for d in range(1, depth + 1)}
+    for d in range(1, depth + 1):
+        lv = levels[d]
+        H = counts["H"][d]
+        owners = {}
+        for h in range(H):
+            v = int(lv["vertexID"][h])
+            r = owner_fan(h, v, d, cage, counts, levels)
+            if r is None:
+                continue
+            assert v not in owners, f"level {d}: vertex {v} has two owners"
+            owners[v] = h
+            fan, ri, boundary = r
+            assert fan[ri] == h
+            assert all(0 <= x < H for x in fan), (d, h, fan)
+            ring, is_boundary = walked_fan(h, lv)
+            assert boundary == is_boundary, (d, h)
+            if boundary:
+                assert fan == ring, (d, h, fan, ring)
+            else:  # same cycle, starting at the representative
+                i = ring.index(h)
+                assert fan == ring[i:] + ring[:i], (d, h, fan, ring)
+            assert all(int(lv["vertexID"][x]) == v for x in fan)
+            assert O.vertex_to_halfedge(cage, v, d) == h
+        assert len(owners) == counts["V"][d], (d, len(owners), counts["V"][d])

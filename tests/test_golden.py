@@ -1,0 +1,81 @@
+"""The oracle (CPU) and the HIP path (GPU) against golden fixtures written
+by tests/golden/make_golden.py from the reference's own CatmullClark.h."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from halfedgecatmullclark_amd import cage as C
+from oracle import oracle as O
+from tests.golden.make_golden import CASES, DRIVERS
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = np.load(os.path.join(HERE, "golden", "golden.npz"))
+FLAGS = {
+    "ccs_Refine_Scatter": O.ALL,
+    "ccs_Refine_Gather": O.ALL | O.GATHER,
+    "ccs_Refine_NoCreases_Scatter": O.ALL | O.NOCREASES,
+    "ccs_Refine_NoCreases_Gather": O.ALL | O.GATHER | O.NOCREASES,
+}
+POINT_TOL = 1e-5
+
+
+def load(name):
+    return C.load_ccm(os.path.join(HERE, "golden", "cages", name + ".ccm"))
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def rel_err(p, ref):
+    scale = max(float(np.abs(ref).max()), 1e-30)
+    return float(np.abs(np.asarray(p, np.float64) - ref).max()) / scale
+
+
+@pytest.mark.parametrize("drv", DRIVERS)
+@pytest.mark.parametrize("name,depth", CASES, ids=[c[0] for c in CASES])
+def test_oracle_matches_golden(name, depth, drv):
+    cg = load(name)
+    key = f"{name}/{drv}"
+    got = O.refine(cg, depth, FLAGS[drv])
+    assert digest(got["halfedges"]) == str(GOLD[key + "/halfedges"])
+    assert digest(got["creases"]) == str(GOLD[key + "/creases"])
+    assert digest(got["points"]) == str(GOLD[key + "/points"])
+    if key + "/points64" in GOLD:
+        got64 = O.refine(cg, depth, FLAGS[drv], real=np.float64)
+        assert np.array_equal(got64["points"], GOLD[key + "/points64"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("drv", DRIVERS)
+@pytest.mark.parametrize("name,depth", CASES, ids=[c[0] for c in CASES])
+def test_gpu_matches_golden(name, depth, drv):
+    from halfedgecatmullclark_amd import capi
+    cg = load(name)
+    key = f"{name}/{drv}"
+    got = capi.refine_host(cg, depth, drv)
+    assert digest(got["halfedges"]) == str(GOLD[key + "/halfedges"])
+    assert digest(got["creases"]) == str(GOLD[key + "/creases"])
+    if "Gather" in drv:
+        assert digest(got["points"]) == str(GOLD[key + "/points"])
+    else:
+        ref64 = GOLD[key + "/points64"] if key + "/points64" in GOLD else \
+            O.refine(cg, depth, FLAGS[drv], real=np.float64)["points"]
+        assert rel_err(got["points"], ref64) <= POINT_TOL
+
+
+def test_jacobian_fixture_matches_oracle():
+    """The reference's basis-vector Jacobian (cube, depth 3) against the
+    double-precision oracle run on the same basis vectors."""
+    J = GOLD["jacobian/cube_d3_scatter"]
+    cube = load("cube")
+    for i in range(cube.V):
+        basis = cube.copy()
+        basis.vertex_points = np.zeros((cube.V, 3), np.float64)
+        basis.vertex_points[i, 0] = 1.0
+        out = O.refine(basis, 3, O.ALL, real=np.float64)
+        assert np.array_equal(O.level(out, "points", 3)[:, 0], J[:, i])
+    # every row of a Catmull-Clark map sums to one (partition of unity)
+    np.testing.assert_allclose(J.sum(axis=1), 1.0, atol=1e-12)

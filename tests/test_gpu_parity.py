@@ -58,3 +58,39 @@ def test_refine_drivers(capi, name, cage, depth, driver, flags):
         # and the float oracle (sequential scatter) is as close
         err32, _ = point_error(got["points"], want["points"])
         assert err32 <= POINT_TOL, err32
+
+
+PATHS = [  # (fused, store, xcd remap, structural fans)
+    (1, 1, 1, 1),
+    (1, 0, 0, 0),
+    (0, 1, 1, 1),
+]
+STEPS = [
+    ("scatter", 15, O.ALL),
+    ("gather", 15 | 16, O.ALL | O.GATHER),
+    ("nocreases_scatter", 15 | 32, O.ALL | O.NOCREASES),
+    ("nocreases_gather", 15 | 16 | 32, O.ALL | O.GATHER | O.NOCREASES),
+]
+
+
+@pytest.mark.parametrize("fused,store,remap,fan", PATHS, ids=["fused_lds_remap_fan", "fused_direct_walk", "per_element"])
+@pytest.mark.parametrize("name,cage,depth", small_cages(), ids=[c[0] for c in small_cages()])
+def test_device_paths(capi, name, cage, depth, fused, store, remap, fan):
+    ds = capi.DeviceSubd(cage, depth)
+    ds.set_option(capi.OPT_FUSED, fused)
+    ds.set_option(capi.OPT_STORE, store)
+    ds.set_option(capi.OPT_XCD_REMAP, remap)
+    ds.set_option(capi.OPT_FAN, fan)
+    for label, steps, flags in STEPS:
+        ds.refine(steps)
+        got = ds.download()
+        want = O.refine(cage, depth, flags)
+        assert_bytes_equal(got["halfedges"], want["halfedges"], f"{name}/{label} halfedges")
+        assert_bytes_equal(got["creases"], want["creases"], f"{name}/{label} creases")
+        if flags & O.GATHER:
+            assert_bytes_equal(got["points"], want["points"], f"{name}/{label} points")
+        else:
+            ref64 = O.refine(cage, depth, flags, real=np.float64)["points"]
+            err, _ = point_error(got["points"], ref64)
+            assert err <= POINT_TOL, (label, err)
+    ds.close()
