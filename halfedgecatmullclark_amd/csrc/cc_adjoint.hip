@@ -32,13 +32,34 @@ __device__ __forceinline__ void atomic_add3(V3<T> *p, T a, const V3<T> &g) {
 template <typename T>
 struct VW { T a, b, g, dl; };
 
+template <typename T>
+__device__ __forceinline__ T satT(T x) { return x > (T)1 ? (T)1 : (x < (T)0 ? (T)0 : x); }
+template <typename T>
+__device__ __forceinline__ T sgnT(T x) { return x < (T)0 ? (T)-1 : (x > (T)0 ? (T)1 : (T)0); }
+
+// sharpness of level-d edge e in the adjoint's precision: the crease records
+// in float, a separately refined array in double
+template <typename T>
+__device__ __forceinline__ T sharp_at(const AdjArgs<T> &a, int32_t e) {
+    if (e >= a.C) return (T)0;
+    return a.sd ? a.sd[e] : (T)a.cr[e].sharp;
+}
+
+template <typename T>
+struct RingT {
+    T n, cc;
+    bool boundary;
+    T s1, s2;
+    int32_t e1, e2;
+    T sall;
+};
+
 // level d >= 1 scatter rule (creased :2568-2607, corner test first; smooth
 // :2339-2348)
 template <typename T>
-__device__ __forceinline__ VW<T> level_vertex_weights(const RingParams &rp, float ts, float ps, int32_t edge,
-                                                      bool creased) {
+__device__ __forceinline__ VW<T> level_vertex_weights(const RingT<T> &rp, T ts, T ps, int32_t edge, bool creased) {
     VW<T> w{0, 0, 0, 0};
-    const T n = (T)rp.n;
+    const T n = rp.n;
     if (!creased) {
         const T iw = (T)1 / n, s = rp.boundary ? (T)0 : (T)1;
         w.a = iw - (T)3 * iw * iw * s;
@@ -47,20 +68,20 @@ __device__ __forceinline__ VW<T> level_vertex_weights(const RingParams &rp, floa
         return w;
     }
     const T nn = rp.boundary ? n + (T)1 : n;
-    if (rp.cc >= 3.0f || nn == (T)2) {
+    if (rp.cc >= (T)3 || nn == (T)2) {
         w.a = (T)1 / n;
         return w;
     }
-    if (rp.cc <= 1.0f) {
+    if (rp.cc <= (T)1) {
         w.a = (n - (T)3) / (n * n);
         w.b = (T)4 / (n * n);
         w.g = (T)-1 / (n * n);
         return w;
     }
-    float avg = rp.sall;
+    T avg = rp.sall;
     if (rp.boundary) avg = edge == rp.e1 ? rp.s2 : (edge == rp.e2 ? rp.s1 : rp.s1 + rp.s2);
-    const T t = (T)sat(avg * 0.5f);
-    const T cw = (T)sgn(ts), pcw = rp.boundary ? (T)sgn(ps) : (T)0;
+    const T t = satT(avg * (T)0.5);
+    const T cw = sgnT(ts), pcw = rp.boundary ? sgnT(ps) : (T)0;
     w.a = ((T)1 - t) / n + t * (T)0.25 * (cw + pcw);
     w.b = t * (T)0.25 * cw;
     w.dl = t * (T)0.25 * pcw;
@@ -82,14 +103,14 @@ __global__ __launch_bounds__(256) void k_adj_vertices(AdjArgs<T> a, DCage c, Lev
     Fan f;
     if (!owner_fan(h, v, d, c, t, lp.he, f)) return;
     const int n = f.n;
-    RingParams rp;
-    rp.n = (float)n; rp.cc = 0.0f; rp.boundary = f.boundary; rp.s1 = rp.s2 = 0.0f; rp.e1 = rp.e2 = -1; rp.sall = 0.0f;
+    RingT<T> rp;
+    rp.n = (T)n; rp.cc = 0; rp.boundary = f.boundary; rp.s1 = rp.s2 = 0; rp.e1 = rp.e2 = -1; rp.sall = 0;
     if (CREASED) {
         for (int i = (f.boundary ? -1 : 0); i < n; ++i) {
             const int32_t e = i < 0 ? he[fan_at(f, c, 0)].edge : he[qprev(fan_at(f, c, i))].edge;
-            const float se = lsharp(a.cr, a.C, e);
-            rp.cc += sgn(se);
-            if (se != 0.0f) {
+            const T se = sharp_at(a, e);
+            rp.cc += sgnT(se);
+            if (se != (T)0) {
                 if (rp.e1 < 0) { rp.s1 = se; rp.e1 = e; }
                 else if (rp.e2 < 0) { rp.s2 = se; rp.e2 = e; }
             }
@@ -103,8 +124,8 @@ __global__ __launch_bounds__(256) void k_adj_vertices(AdjArgs<T> a, DCage c, Lev
     for (int i = 0; i < n; ++i) {
         const int32_t r = fan_at(f, c, i);
         const int32_t re = he[r].edge, pe = he[qprev(r)].edge;
-        const float ts = CREASED ? lsharp(a.cr, a.C, re) : 0.0f;
-        const float ps = CREASED ? lsharp(a.cr, a.C, pe) : 0.0f;
+        const T ts = CREASED ? sharp_at(a, re) : (T)0;
+        const T ps = CREASED ? sharp_at(a, pe) : (T)0;
         const VW<T> w = level_vertex_weights<T>(rp, ts, ps, re, CREASED);
         alpha += w.a;
         if (w.b != (T)0) atomic_add3(&gE[re], w.b, gv);
@@ -126,7 +147,7 @@ __global__ __launch_bounds__(256) void k_adj_edges(AdjArgs<T> a) {
     const bool boundary = x.twin < 0;
     T c0, c1, cf;
     if (CREASED) {
-        const T w = (T)sat(lsharp(a.cr, a.C, x.edge));
+        const T w = satT(sharp_at(a, x.edge));
         const T k = boundary ? (T)1 : (T)0.5;
         c0 = (T)0.25 * ((T)1 - w) + (T)0.5 * k * w;
         c1 = (T)0.5 * k * w;
@@ -178,38 +199,38 @@ __global__ __launch_bounds__(256) void k_adj_cage_vertices(AdjArgs<T> a, DCage c
         return;
     }
     const int32_t p = x.prev;
-    const float ts = c.cr[x.edge].sharp, ps = c.cr[c.he[p].edge].sharp;
-    float avgS = ps, cc = sgn(ps), n = 1.0f;
+    const T ts = (T)c.cr[x.edge].sharp, ps = (T)c.cr[c.he[p].edge].sharp;
+    T avgS = ps, cc = sgnT(ps), n = 1;
     int32_t fw, bw;
-    for (fw = c.he[p].twin; fw >= 0 && fw != h && n < (float)kWalkCap; fw = c.he[fw].twin) {
+    for (fw = c.he[p].twin; fw >= 0 && fw != h && n < (T)kWalkCap; fw = c.he[fw].twin) {
         const int32_t q = c.he[fw].prev;
-        const float qs = c.cr[c.he[q].edge].sharp;
-        n = n + 1.0f;
+        const T qs = (T)c.cr[c.he[q].edge].sharp;
+        n = n + (T)1;
         avgS = avgS + qs;
-        cc = cc + sgn(qs);
+        cc = cc + sgnT(qs);
         fw = q;
     }
-    for (bw = x.twin; fw < 0 && bw >= 0 && bw != h && n < (float)kWalkCap; bw = c.he[bw].twin) {
+    for (bw = x.twin; fw < 0 && bw >= 0 && bw != h && n < (T)kWalkCap; bw = c.he[bw].twin) {
         const int32_t q = c.he[bw].next;
-        const float qs = c.cr[c.he[q].edge].sharp;
-        n = n + 1.0f;
+        const T qs = (T)c.cr[c.he[q].edge].sharp;
+        n = n + (T)1;
         avgS = avgS + qs;
-        cc = cc + sgn(qs);
+        cc = cc + sgnT(qs);
         bw = q;
     }
     const bool boundary = fw < 0;
-    const T nn = (T)n;
-    const float ccAll = boundary ? cc + sgn(ts) : cc;
+    const T nn = n;
+    const T ccAll = boundary ? cc + sgnT(ts) : cc;
     const T nPost = boundary ? nn + (T)1 : nn;
-    if (ccAll <= 1.0f) {
+    if (ccAll <= (T)1) {
         atomic_add3(&gP[x.vert], (nn - (T)3) / (nn * nn), gv);
         atomic_add3(&gE[x.edge], (T)4 / (nn * nn), gv);
         atomic_add3(&gF[x.face], (T)-1 / (nn * nn), gv);
-    } else if (ccAll >= 3.0f || nPost == (T)2) {
+    } else if (ccAll >= (T)3 || nPost == (T)2) {
         atomic_add3(&gP[x.vert], (T)1 / nn, gv);
     } else {
-        const T tt = (T)sat(avgS * 0.5f);
-        const T cw = (T)sgn(ts), pcw = boundary ? (T)sgn(ps) : (T)0;
+        const T tt = satT(avgS * (T)0.5);
+        const T cw = sgnT(ts), pcw = boundary ? sgnT(ps) : (T)0;
         atomic_add3(&gP[x.vert], ((T)1 - tt) / nn + tt * (T)0.25 * (cw + pcw), gv);
         if (cw != (T)0) atomic_add3(&gE[x.edge], tt * (T)0.25 * cw, gv);
         if (pcw != (T)0) atomic_add3(&gE[c.he[p].edge], tt * (T)0.25 * pcw, gv);
@@ -226,7 +247,7 @@ __global__ __launch_bounds__(256) void k_adj_cage_edges(AdjArgs<T> a, DCage c) {
     const bool boundary = x.twin < 0;
     T c0, c1, cf;
     if (CREASED) {
-        const T w = (T)sat(c.cr[x.edge].sharp);
+        const T w = satT((T)c.cr[x.edge].sharp);
         const T k = boundary ? (T)1 : (T)0.5;
         c0 = (T)0.25 * ((T)1 - w) + (T)0.5 * k * w;
         c1 = (T)0.5 * k * w;
@@ -254,6 +275,27 @@ __global__ __launch_bounds__(256) void k_adj_cage_faces(AdjArgs<T> a, DCage c) {
     atomic_add3(reinterpret_cast<V3<T> *>(a.gP) + x.vert, (T)1 / (T)m, gf);
 }
 
+// crease sharpness of the next level in T (CatmullClark.h:2962-2980), for
+// the double-precision adjoint; links come from the float crease records
+template <typename T>
+__global__ void k_sharp_next(const Crease *__restrict__ links, const T *__restrict__ sin, int32_t C,
+                             T *__restrict__ sout) {
+    const int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= C) return;
+    const Crease x = links[e];
+    const T s3 = (T)3 * sin[e];
+    const T ns = sin[x.next], ps = sin[x.prev];
+    const T c0 = (ps + s3) / (T)4 - (T)1, c1 = (s3 + ns) / (T)4 - (T)1;
+    sout[2 * (int64_t)e + 0] = c0 > (T)0 ? c0 : (T)0;
+    sout[2 * (int64_t)e + 1] = c1 > (T)0 ? c1 : (T)0;
+}
+
+template <typename T>
+__global__ void k_sharp_cage(const Crease *__restrict__ cr, int32_t E, T *__restrict__ s) {
+    const int32_t e = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e < E) s[e] = (T)cr[e].sharp;
+}
+
 template <typename T>
 __global__ void k_copy(const T *__restrict__ src, T *__restrict__ dst, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -278,10 +320,21 @@ unsigned grid_stride(int64_t n) {
 }  // namespace
 
 template <typename T>
-void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA, T *bufB, int creased,
+void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA, T *bufB, T *sharp, int creased,
                     hipStream_t st) {
     const Counts &n = *pl.n;
     const int D = pl.D;
+    // sharpness of levels 0..D-1 in T when a buffer is given (cumulative,
+    // level d at offset offC[d] + E0 with level 0 first)
+    if (sharp && creased) {
+        hipLaunchKernelGGL(k_sharp_cage<T>, dim3(grid_for(pl.cage.E)), dim3(256), 0, st, pl.cage.cr, pl.cage.E, sharp);
+        for (int d = 0; d + 1 < D; ++d) {
+            const Crease *links = d == 0 ? pl.cage.cr : pl.cr + n.offC[d];
+            const T *sin = sharp + (d == 0 ? 0 : pl.cage.E + n.offC[d]);
+            hipLaunchKernelGGL(k_sharp_next<T>, dim3(grid_for(n.C[d])), dim3(256), 0, st, links, sin, (int32_t)n.C[d],
+                               sharp + pl.cage.E + n.offC[d + 1]);
+        }
+    }
     // G holds the gradient of level d+1 (modified in place), P receives level d
     T *G = bufA, *P = bufB;
     hipLaunchKernelGGL(k_copy<T>, dim3(grid_stride(3 * n.V[D])), dim3(256), 0, st, gradFinest, G, 3 * n.V[D]);
@@ -291,6 +344,7 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
         a.cr = pl.cr + n.offC[d];
         a.H = (int32_t)n.H[d];
         a.C = (int32_t)n.C[d];
+        a.sd = (sharp && creased) ? sharp + pl.cage.E + n.offC[d] : nullptr;
         a.gV = G;
         a.gF = G + 3 * n.V[d];
         a.gE = G + 3 * (n.V[d] + n.F[d]);
@@ -311,6 +365,7 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
     a.cr = nullptr;
     a.H = pl.cage.H;
     a.C = pl.cage.E;
+    a.sd = nullptr;
     a.gV = G;
     a.gF = G + 3 * (int64_t)pl.cage.V;
     a.gE = G + 3 * ((int64_t)pl.cage.V + pl.cage.F);
@@ -328,7 +383,9 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
     hipLaunchKernelGGL(k_adj_cage_faces<T>, g, b, 0, st, a, pl.cage);
 }
 
-template void adjoint_levels<float>(const AdjPlan &, const float *, float *, float *, float *, int, hipStream_t);
-template void adjoint_levels<double>(const AdjPlan &, const double *, double *, double *, double *, int, hipStream_t);
+template void adjoint_levels<float>(const AdjPlan &, const float *, float *, float *, float *, float *, int,
+                                    hipStream_t);
+template void adjoint_levels<double>(const AdjPlan &, const double *, double *, double *, double *, double *, int,
+                                     hipStream_t);
 
 }  // namespace cch

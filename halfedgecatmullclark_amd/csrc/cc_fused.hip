@@ -27,12 +27,31 @@ namespace cch {
 
 template <int GATHER, int CREASED, int TOPO, int UVMODE, int POINTS, int STORE>
 __global__ __launch_bounds__(256) void k_level_fused(FusedArgs a) {
-    __shared__ int4 stage[STORE == kStoreLds ? 4 * 256 : 1];
+    constexpr bool kStaged = STORE == kStoreLds && TOPO && UVMODE != 0;
+    __shared__ int4 stage[kStaged ? 4 * 256 : 1];
+    __shared__ P3 spts[POINTS ? 256 : 1];
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, a.xcd_remap);
-    const int32_t h = (int32_t)(lb * blockDim.x + threadIdx.x);
+    const int32_t base = (int32_t)(lb * blockDim.x);
+    const int32_t h = base + (int32_t)threadIdx.x;
     const bool live = h < a.H;   // H is a multiple of 4: quads are whole
     const int j = h & 3;
-    int4 x = live ? reinterpret_cast<const int4 *>(a.he)[h] : make_int4(0, 0, 0, 0);
+    const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
+    const int4 x = live ? he4[h] : make_int4(0, 0, 0, 0);
+    const int32_t twin = x.x;
+    const bool owner = live && h > twin;   // this halfedge writes its edge point
+    // twin quads inside this block come from LDS, the others from memory
+    const bool twin_here = owner && twin >= base && twin < base + (int32_t)blockDim.x;
+    const bool twin_far = owner && twin >= 0 && !twin_here;
+
+    // issue the loads of the point work first: this halfedge's vertex point
+    // and the records of a twin quad outside the block
+    P3 P = zero3();
+    int4 tr0, tr1, tr2, tr3;
+    if (POINTS) {
+        if (live) P = a.old[x.z];
+        const int32_t tq = twin_far ? (twin & ~3) : (h & ~3);
+        tr0 = he4[tq]; tr1 = he4[tq + 1]; tr2 = he4[tq + 2]; tr3 = he4[tq + 3];
+    }
 
     // record of the previous halfedge of the quad, by DPP
     const int4 p = quad_rel4<kQuadPrev>(x);
@@ -58,15 +77,16 @@ __global__ __launch_bounds__(256) void k_level_fused(FusedArgs a) {
             c[3].w = uv_encode(uv_mid(uv, puv));
         }
         int4 *out4 = reinterpret_cast<int4 *>(a.out_he);
-        if (STORE == kStoreLds && TOPO && UVMODE != 0) {
+        if (kStaged) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) stage[4 * threadIdx.x + k] = c[k];
+            if (POINTS) spts[threadIdx.x] = P;
             __syncthreads();
-            const int64_t base = 4 * (int64_t)(lb * blockDim.x);
+            const int64_t obase = 4 * (int64_t)base;
             const int64_t limit = 4 * (int64_t)a.H;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const int64_t o = base + threadIdx.x + (int64_t)k * blockDim.x;
+                const int64_t o = obase + threadIdx.x + (int64_t)k * blockDim.x;
                 if (o < limit) out4[o] = stage[threadIdx.x + k * blockDim.x];
             }
         } else if (live) {
@@ -86,21 +106,27 @@ __global__ __launch_bounds__(256) void k_level_fused(FusedArgs a) {
     }
 
     if (!POINTS) return;
-    const P3 P = live ? a.old[x.z] : zero3();
+    if (!kStaged) {
+        spts[threadIdx.x] = P;
+        __syncthreads();
+    }
     const P3 q0 = quad_bcast3<0>(P), q1 = quad_bcast3<1>(P), q2 = quad_bcast3<2>(P), q3 = quad_bcast3<3>(P);
     if (!live) return;
     const P3 F = face_rule(q0, q1, q2, q3, GATHER);
     if (j == 0) a.fp[h >> 2] = F;
-    const int32_t twin = x.x;
-    if (!(h > twin)) return;
+    if (!owner) return;
     // this halfedge owns its edge: both contributions
     const P3 v1 = pick4(q0, q1, q2, q3, (j + 1) & 3);
     const float s = CREASED ? lsharp(a.cr, a.C, x.y) : 0.0f;
     P3 Ev;
     if (twin >= 0) {
-        const int32_t tq = twin & ~3;
-        const int4 *t4 = reinterpret_cast<const int4 *>(a.he) + tq;
-        const P3 t0 = a.old[t4[0].z], t1 = a.old[t4[1].z], t2 = a.old[t4[2].z], t3 = a.old[t4[3].z];
+        P3 t0, t1, t2, t3;
+        if (twin_here) {
+            const int tb = (twin & ~3) - base;
+            t0 = spts[tb]; t1 = spts[tb + 1]; t2 = spts[tb + 2]; t3 = spts[tb + 3];
+        } else {
+            t0 = a.old[tr0.z]; t1 = a.old[tr1.z]; t2 = a.old[tr2.z]; t3 = a.old[tr3.z];
+        }
         const P3 Ft = face_rule(t0, t1, t2, t3, GATHER);
         if (GATHER) {
             const float w = CREASED ? sat(s) : 1.0f;
@@ -255,80 +281,105 @@ __global__ __launch_bounds__(256) void k_level_vertices(FusedArgs a, DCage c, Le
     a.vp[v] = acc;
 }
 
-template <int GATHER, int CREASED>
-__global__ __launch_bounds__(256) void k_level_vertices_fan(FusedArgs a, DCage c, LevelTab t, LevelPtrs lp, int d) {
-    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, a.xcd_remap);
-    const int32_t h = (int32_t)(lb * blockDim.x + threadIdx.x);
-    if (h >= a.H) return;
-    const He *he = a.he;
-    const int32_t v = he[h].vert;
-    Fan f;
-    if (!owner_fan(h, v, d, c, t, lp.he, f)) return;
-    const P3 old = a.old[v];
+// Everything the vertex rules need about one fan halfedge r: the points of
+// its own edge and of its previous edge, its face point, both sharpnesses.
+struct FanEntry {
+    P3 E, PE, F;
+    float ts, ps;
+    int32_t re, pe;
+};
+
+__device__ __forceinline__ FanEntry fan_entry(const FusedArgs &a, int32_t r, int4 rec, int4 prv) {
+    FanEntry e;
+    e.re = rec.y;
+    e.pe = prv.y;
+    e.E = a.ep[e.re];
+    e.PE = a.ep[e.pe];
+    e.F = a.fp[r >> 2];
+    e.ts = lsharp(a.cr, a.C, e.re);
+    e.ps = lsharp(a.cr, a.C, e.pe);
+    return e;
+}
+
+// fans of up to four halfedges (face and edge vertices) with every load
+// issued before any is used
+struct FanPreloaded {
+    FanEntry e0, e1, e2, e3;
+    __device__ __forceinline__ FanEntry get(int i) const { return pick4(e0, e1, e2, e3, i); }
+};
+
+// any fan, entries loaded when asked for
+struct FanOnDemand {
+    const FusedArgs *a;
+    const Fan *f;
+    const DCage *c;
+    __device__ __forceinline__ FanEntry get(int i) const {
+        const int32_t r = fan_at(*f, *c, i);
+        const int4 *he4 = reinterpret_cast<const int4 *>(a->he);
+        return fan_entry(*a, r, he4[r], he4[qprev(r)]);
+    }
+};
+
+// The vertex rules over a fan (see k_level_vertices for the reference walk
+// they reproduce).  MAXN > 0 bounds the fan size at compile time so the loops
+// unroll.
+template <int GATHER, int CREASED, int MAXN, class Acc>
+__device__ __forceinline__ P3 fan_vertex_point(const Fan &f, const Acc &acc, P3 old) {
     const int n = f.n;
+    const int ri = f.ri;
+    const int lim = MAXN > 0 ? MAXN : n;
     if (GATHER) {
-        const int ri = f.ri;
         if (!CREASED) {
-            // S terms (F(face r), E(edge r)) forward from the representative
-            // (:2277-2294); a boundary stops the walk and keeps the old point
-            P3 acc = zero3();
+            P3 sum = zero3();
             float nv = 1.0f;
-            for (int i = ri; i < n; ++i) {
-                const int32_t r = fan_at(f, c, i);
-                const P3 Fi = a.fp[r >> 2], Ei = a.ep[he[r].edge];
+            for (int i = 0; i < lim; ++i) {
+                if (i < ri || i >= n) continue;
+                const FanEntry e = acc.get(i);
                 if (i == ri) {
-                    acc = add(mul(Fi, -1.0f), mul(Ei, 4.0f));
+                    sum = add(mul(e.F, -1.0f), mul(e.E, 4.0f));
                 } else {
-                    acc = add(acc, mul(Fi, -1.0f));
-                    acc = add(acc, mul(Ei, 4.0f));
+                    sum = add(sum, mul(e.F, -1.0f));
+                    sum = add(sum, mul(e.E, 4.0f));
                     nv = nv + 1.0f;
                 }
             }
-            P3 sm = add(mul(acc, 1.0f / (nv * nv)), mul(old, 1.0f - 3.0f / nv));
-            a.vp[v] = lerp(old, sm, f.boundary ? 0.0f : 1.0f);
-            return;
+            P3 sm = add(mul(sum, 1.0f / (nv * nv)), mul(old, 1.0f - 3.0f / nv));
+            return lerp(old, sm, f.boundary ? 0.0f : 1.0f);
         }
-        // T terms (E(edge prev r), F(face r)) forward from the representative,
-        // then B terms (E(edge r), F(face r)) backward (:2385-2460)
-        P3 sm = zero3(), cr = zero3();
-        float avgS = 0.0f, cc = 0.0f, nv = 0.0f;
-        float cwRep = 0.0f;
-        P3 Erep = zero3();
-        for (int i = ri; i < n; ++i) {
-            const int32_t r = fan_at(f, c, i);
-            const int32_t pe = he[qprev(r)].edge;
-            const P3 Fi = a.fp[r >> 2], PEi = a.ep[pe];
-            const float ps = lsharp(a.cr, a.C, pe), pw = sgn(ps);
+        P3 sm = zero3(), cr = zero3(), Erep = zero3();
+        float avgS = 0.0f, cc = 0.0f, nv = 0.0f, cwRep = 0.0f;
+        for (int i = 0; i < lim; ++i) {
+            if (i < ri || i >= n) continue;
+            const FanEntry e = acc.get(i);
+            const float pw = sgn(e.ps);
             if (i == ri) {
-                const int32_t re = he[r].edge;
-                const float ts = lsharp(a.cr, a.C, re);
-                cwRep = sgn(ts);
-                Erep = a.ep[re];
-                sm = add(mul(Fi, -1.0f), mul(PEi, 4.0f));
-                cr = add(zero3(), mul(PEi, pw));
-                avgS = ps;
+                cwRep = sgn(e.ts);
+                Erep = e.E;
+                sm = add(mul(e.F, -1.0f), mul(e.PE, 4.0f));
+                cr = add(zero3(), mul(e.PE, pw));
+                avgS = e.ps;
                 cc = pw;
                 nv = 1.0f;
             } else {
-                sm = add(sm, mul(Fi, -1.0f));
-                sm = add(sm, mul(PEi, 4.0f));
+                sm = add(sm, mul(e.F, -1.0f));
+                sm = add(sm, mul(e.PE, 4.0f));
                 nv = nv + 1.0f;
-                cr = add(cr, mul(PEi, pw));
-                avgS = avgS + ps;
+                cr = add(cr, mul(e.PE, pw));
+                avgS = avgS + e.ps;
                 cc = cc + pw;
             }
         }
         if (f.boundary) {
-            for (int i = ri - 1; i >= 0; --i) {
-                const int32_t r = fan_at(f, c, i);
-                const int32_t re = he[r].edge;
-                const P3 Fi = a.fp[r >> 2], Ei = a.ep[re];
-                const float qs = lsharp(a.cr, a.C, re), qw = sgn(qs);
-                sm = add(sm, mul(Fi, -1.0f));
-                sm = add(sm, mul(Ei, 4.0f));
+            for (int k = 0; k < lim; ++k) {
+                const int i = ri - 1 - k;
+                if (i < 0) break;
+                const FanEntry e = acc.get(i);
+                const float qw = sgn(e.ts);
+                sm = add(sm, mul(e.F, -1.0f));
+                sm = add(sm, mul(e.E, 4.0f));
                 nv = nv + 1.0f;
-                cr = add(cr, mul(Ei, qw));
-                avgS = avgS + qs;
+                cr = add(cr, mul(e.E, qw));
+                avgS = avgS + e.ts;
                 cc = cc + qw;
             }
             cr = add(cr, mul(Erep, cwRep));
@@ -337,39 +388,129 @@ __global__ __launch_bounds__(256) void k_level_vertices_fan(FusedArgs a, DCage c
         }
         sm = add(mul(sm, 1.0f / (nv * nv)), mul(old, 1.0f - 3.0f / nv));
         cr = add(mul(cr, 0.5f / cc), mul(old, 0.5f));
-        if (cc <= 1.0f) a.vp[v] = sm;
-        else if (cc >= 3.0f || nv == 2.0f) a.vp[v] = old;
-        else a.vp[v] = lerp(old, cr, sat(avgS * 0.5f));
-        return;
+        if (cc <= 1.0f) return sm;
+        if (cc >= 3.0f || nv == 2.0f) return old;
+        return lerp(old, cr, sat(avgS * 0.5f));
     }
-    // scatter rules: parameters of the fan, then per-halfedge contributions
-    // summed in fan order
+    // scatter rules: fan parameters, then per-halfedge contributions in fan
+    // order (incident edges: the own edge of the first fan halfedge on a
+    // boundary, and the previous edge of every fan halfedge)
     RingParams rp;
     rp.n = (float)n; rp.cc = 0.0f; rp.boundary = f.boundary; rp.s1 = rp.s2 = 0.0f; rp.e1 = rp.e2 = -1; rp.sall = 0.0f;
     if (CREASED) {
-        // incident edges: the own edge of the first fan halfedge on a
-        // boundary, and the previous edge of every fan halfedge
-        for (int i = (f.boundary ? -1 : 0); i < n; ++i) {
-            const int32_t e = i < 0 ? he[fan_at(f, c, 0)].edge : he[qprev(fan_at(f, c, i))].edge;
-            const float se = lsharp(a.cr, a.C, e);
+        for (int i = -1; i < lim; ++i) {
+            if (i >= n || (i < 0 && !f.boundary)) continue;
+            const FanEntry e = acc.get(i < 0 ? 0 : i);
+            const float se = i < 0 ? e.ts : e.ps;
+            const int32_t ed = i < 0 ? e.re : e.pe;
             rp.cc += sgn(se);
             if (se != 0.0f) {
-                if (rp.e1 < 0) { rp.s1 = se; rp.e1 = e; }
-                else if (rp.e2 < 0) { rp.s2 = se; rp.e2 = e; }
+                if (rp.e1 < 0) { rp.s1 = se; rp.e1 = ed; }
+                else if (rp.e2 < 0) { rp.s2 = se; rp.e2 = ed; }
             }
             rp.sall += se;
         }
     }
-    P3 acc = zero3();
-    for (int i = 0; i < n; ++i) {
-        const int32_t r = fan_at(f, c, i);
-        const int32_t re = he[r].edge, pe = he[qprev(r)].edge;
-        const P3 E = a.ep[re], PE = a.ep[pe], Fc = a.fp[r >> 2];
-        const float ts = CREASED ? lsharp(a.cr, a.C, re) : 0.0f;
-        const float ps = CREASED ? lsharp(a.cr, a.C, pe) : 0.0f;
-        acc = add(acc, vertex_scatter_contrib(rp, old, E, PE, Fc, ts, ps, re, CREASED));
+    P3 sum = zero3();
+    for (int i = 0; i < lim; ++i) {
+        if (i >= n) break;
+        const FanEntry e = acc.get(i);
+        sum = add(sum, vertex_scatter_contrib(rp, old, e.E, e.PE, e.F, CREASED ? e.ts : 0.0f, CREASED ? e.ps : 0.0f,
+                                              e.re, CREASED));
     }
-    a.vp[v] = acc;
+    return sum;
+}
+
+template <int GATHER, int CREASED>
+__global__ __launch_bounds__(256) void k_level_vertices_fan(FusedArgs a, DCage c, LevelTab t, LevelPtrs lp, int d) {
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, a.xcd_remap);
+    const int32_t h = (int32_t)(lb * blockDim.x + threadIdx.x);
+    if (h >= a.H) return;
+    const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
+    const int32_t v = he4[h].z;
+    Fan f;
+    if (!owner_fan(h, v, d, c, t, lp.he, f)) return;
+    const P3 old = a.old[v];
+    if (f.kind == 1 || f.kind == 3) {
+        // fans of at most four halfedges known by arithmetic: load all
+        // records, then all points, then evaluate
+        const int n = f.n;
+        int32_t r[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r[i] = fan_at(f, c, i < n ? i : 0);
+        int4 rec[4], prv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            rec[i] = he4[r[i]];
+            prv[i] = he4[qprev(r[i])];
+        }
+        FanPreloaded pre;
+        pre.e0 = fan_entry(a, r[0], rec[0], prv[0]);
+        pre.e1 = fan_entry(a, r[1], rec[1], prv[1]);
+        pre.e2 = fan_entry(a, r[2], rec[2], prv[2]);
+        pre.e3 = fan_entry(a, r[3], rec[3], prv[3]);
+        a.vp[v] = fan_vertex_point<GATHER, CREASED, 4>(f, pre, old);
+        return;
+    }
+    FanOnDemand od{&a, &f, &c};
+    a.vp[v] = fan_vertex_point<GATHER, CREASED, 0>(f, od, old);
+}
+
+// Vertex points with owners compacted per block: a block scans
+// 256 * SCAN halfedges, collects the representatives in LDS, then every
+// thread evaluates owners, so all lanes do ring work.
+template <int GATHER, int CREASED, int SCAN>
+__global__ __launch_bounds__(256) void k_level_vertices_compact(FusedArgs a, DCage c, LevelTab t, LevelPtrs lp,
+                                                                int d) {
+    __shared__ int32_t owners[256 * SCAN];
+    __shared__ int32_t nown;
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, a.xcd_remap);
+    const int32_t base = (int32_t)(lb * 256 * SCAN);
+    const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
+    if (threadIdx.x == 0) nown = 0;
+    __syncthreads();
+    int32_t vs[SCAN];
+#pragma unroll
+    for (int k = 0; k < SCAN; ++k) {
+        const int32_t h = base + k * 256 + (int32_t)threadIdx.x;
+        vs[k] = h < a.H ? he4[h].z : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < SCAN; ++k) {
+        const int32_t h = base + k * 256 + (int32_t)threadIdx.x;
+        Fan f;
+        if (vs[k] >= 0 && owner_fan(h, vs[k], d, c, t, lp.he, f)) owners[atomicAdd(&nown, 1)] = h;
+    }
+    __syncthreads();
+    const int32_t count = nown;
+    for (int32_t i = (int32_t)threadIdx.x; i < count; i += 256) {
+        const int32_t h = owners[i];
+        const int32_t v = he4[h].z;
+        Fan f;
+        owner_fan(h, v, d, c, t, lp.he, f);
+        const P3 old = a.old[v];
+        if (f.kind == 1 || f.kind == 3) {
+            const int n = f.n;
+            int32_t r[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) r[q] = fan_at(f, c, q < n ? q : 0);
+            int4 rec[4], prv[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                rec[q] = he4[r[q]];
+                prv[q] = he4[qprev(r[q])];
+            }
+            FanPreloaded pre;
+            pre.e0 = fan_entry(a, r[0], rec[0], prv[0]);
+            pre.e1 = fan_entry(a, r[1], rec[1], prv[1]);
+            pre.e2 = fan_entry(a, r[2], rec[2], prv[2]);
+            pre.e3 = fan_entry(a, r[3], rec[3], prv[3]);
+            a.vp[v] = fan_vertex_point<GATHER, CREASED, 4>(f, pre, old);
+        } else {
+            FanOnDemand od{&a, &f, &c};
+            a.vp[v] = fan_vertex_point<GATHER, CREASED, 0>(f, od, old);
+        }
+    }
 }
 
 // ---- launcher ----------------------------------------------------------------
@@ -399,6 +540,15 @@ void launch_level_vertices(const FusedArgs &a, const DCage &c, const LevelTab &t
                            int gather, int creased, int fan, hipStream_t st) {
     const dim3 block(256);
     const dim3 grid((unsigned)((a.H + 255) / 256));
+    if (fan == 2) {
+        constexpr int kScan = 4;
+        const dim3 g2((unsigned)((a.H + 256 * kScan - 1) / (256 * kScan)));
+        if (gather && creased) hipLaunchKernelGGL((k_level_vertices_compact<1, 1, kScan>), g2, block, 0, st, a, c, t, lp, d);
+        else if (gather) hipLaunchKernelGGL((k_level_vertices_compact<1, 0, kScan>), g2, block, 0, st, a, c, t, lp, d);
+        else if (creased) hipLaunchKernelGGL((k_level_vertices_compact<0, 1, kScan>), g2, block, 0, st, a, c, t, lp, d);
+        else hipLaunchKernelGGL((k_level_vertices_compact<0, 0, kScan>), g2, block, 0, st, a, c, t, lp, d);
+        return;
+    }
     if (fan) {
         if (gather && creased) hipLaunchKernelGGL((k_level_vertices_fan<1, 1>), grid, block, 0, st, a, c, t, lp, d);
         else if (gather) hipLaunchKernelGGL((k_level_vertices_fan<1, 0>), grid, block, 0, st, a, c, t, lp, d);

@@ -32,10 +32,10 @@ struct ccd_Subd {
     cch::P3 *pts = nullptr;
     hipStream_t stream = nullptr;
     // adjoint scratch (allocated on first use, per scalar size)
-    void *adj_a = nullptr, *adj_b = nullptr, *adj_in = nullptr, *adj_out = nullptr;
+    void *adj_a = nullptr, *adj_b = nullptr, *adj_in = nullptr, *adj_out = nullptr, *adj_sharp = nullptr;
     size_t adj_scalar = 0;
     bool timing = false;
-    int opt_fused = 1, opt_store = cch::kStoreLds, opt_remap = 1, opt_fan = 1;
+    int opt_fused = 1, opt_store = cch::kStoreLds, opt_remap = 1, opt_fan = 2;
     std::vector<Timed> timed;
 
     bool alloc(const cc_Mesh *cage, int32_t depth);

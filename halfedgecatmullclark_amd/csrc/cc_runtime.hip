@@ -92,7 +92,7 @@ bool ccd_Subd::alloc(const cc_Mesh *cage, int32_t depth) {
 
 void ccd_Subd::free_all() {
     DeviceGuard g(device);
-    void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts, adj_a, adj_b, adj_in, adj_out};
+    void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts, adj_a, adj_b, adj_in, adj_out, adj_sharp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : timed) {
@@ -336,12 +336,17 @@ bool ccd_Subd::adjoint(const void *gradFinest, void *gradCage, int32_t steps, hi
     pl.tab = tab;
     pl.lp = level_ptrs();
     const int creased = (steps & CCD_NOCREASES) ? 0 : 1;
-    if (scalar == sizeof(double))
+    if (scalar == sizeof(double)) {
+        // the double adjoint refines sharpness in double too, as the double
+        // build of the reference does
+        if (!adj_sharp)
+            CC_TRY(hipMalloc(&adj_sharp, sizeof(double) * (size_t)(E0 + n.cumC(D))), "hipMalloc(adjoint sharpness)");
         adjoint_levels<double>(pl, (const double *)gradFinest, (double *)gradCage, (double *)adj_a, (double *)adj_b,
-                               creased, st);
-    else
-        adjoint_levels<float>(pl, (const float *)gradFinest, (float *)gradCage, (float *)adj_a, (float *)adj_b, creased,
-                              st);
+                               (double *)adj_sharp, creased, st);
+    } else {
+        adjoint_levels<float>(pl, (const float *)gradFinest, (float *)gradCage, (float *)adj_a, (float *)adj_b, nullptr,
+                              creased, st);
+    }
     CC_TRY(hipGetLastError(), "adjoint launch");
     return true;
 }
@@ -685,7 +690,7 @@ int32_t ccd_SetOption(ccd_Subd *s, int32_t key, int32_t value) {
     case CCD_OPT_FUSED: s->opt_fused = value != 0; return 0;
     case CCD_OPT_STORE: s->opt_store = value ? kStoreLds : kStoreDirect; return 0;
     case CCD_OPT_XCD_REMAP: s->opt_remap = value != 0; return 0;
-    case CCD_OPT_FAN: s->opt_fan = value != 0; return 0;
+    case CCD_OPT_FAN: s->opt_fan = value < 0 ? 0 : (value > 2 ? 2 : value); return 0;
     default: return 14;
     }
 }

@@ -270,8 +270,9 @@ int32_t ccd_Synchronize(ccd_Subd *subd);
  *                      float atomics (the reference's own structure)
  *   CCD_OPT_STORE      child-record stores: 0 direct, 1 staged through LDS
  *   CCD_OPT_XCD_REMAP  1: each XCD processes one contiguous range of a level
- *   CCD_OPT_FAN        1: vertex rings enumerated from the level each vertex
- *                      appeared at (default); 0: ring walks over twins */
+ *   CCD_OPT_FAN        vertex rings: 2 enumerated from the level each vertex
+ *                      appeared at, owners compacted per block (default);
+ *                      1 enumerated, one thread per halfedge; 0 ring walks */
 enum { CCD_OPT_FUSED = 0, CCD_OPT_STORE = 1, CCD_OPT_XCD_REMAP = 2, CCD_OPT_FAN = 3 };
 int32_t ccd_SetOption(ccd_Subd *subd, int32_t key, int32_t value);
 /* per-kernel timing with HIP events on the launch stream: enable, run, then

@@ -60,7 +60,8 @@ def test_refine_drivers(capi, name, cage, depth, driver, flags):
         assert err32 <= POINT_TOL, err32
 
 
-PATHS = [  # (fused, store, xcd remap, structural fans)
+PATHS = [  # (fused, store, xcd remap, vertex rings)
+    (1, 1, 1, 2),
     (1, 1, 1, 1),
     (1, 0, 0, 0),
     (0, 1, 1, 1),
@@ -73,7 +74,7 @@ STEPS = [
 ]
 
 
-@pytest.mark.parametrize("fused,store,remap,fan", PATHS, ids=["fused_lds_remap_fan", "fused_direct_walk", "per_element"])
+@pytest.mark.parametrize("fused,store,remap,fan", PATHS, ids=["fused_compact", "fused_fan", "fused_direct_walk", "per_element"])
 @pytest.mark.parametrize("name,cage,depth", small_cages(), ids=[c[0] for c in small_cages()])
 def test_device_paths(capi, name, cage, depth, fused, store, remap, fan):
     ds = capi.DeviceSubd(cage, depth)
