@@ -16,6 +16,9 @@ struct FusedArgs {
     P3 *vp, *fp, *ep;   // level d+1 points: vertex | face | edge slices
     int32_t H, V, F, E, C;
     int xcd_remap;
+    // vertices the patch-local kernel leaves to the list kernel
+    int32_t *defer;        // representative halfedges, V entries at most
+    int32_t *defer_count;  // one counter, zero before the launch
 };
 
 // topo: write child twin/edge/vertex IDs; uvMode: 0 leave uvIDs, 1 refine
@@ -26,6 +29,14 @@ void launch_level_fused(const FusedArgs &a, int gather, int creased, int topo, i
 struct LevelPtrs {
     const He *he[kMaxDepth + 1];
 };
+
+// Patch-local fused kernel: children, face, edge and vertex points of level
+// d+1; vertices whose ring leaves the block are appended to a.defer and
+// finished by launch_level_deferred.
+void launch_level_patch(const FusedArgs &a, const DCage &c, const LevelTab &t, const LevelPtrs &lp, int d, int gather,
+                        int creased, int topo, int uvMode, int store, hipStream_t st);
+void launch_level_deferred(const FusedArgs &a, const DCage &c, const LevelTab &t, const LevelPtrs &lp, int d,
+                           int gather, int creased, int64_t max_count, hipStream_t st);
 
 // vertex points of level d+1 (after launch_level_fused with points); fan
 // selects the structurally enumerated rings over the ring walk

@@ -281,146 +281,6 @@ __global__ __launch_bounds__(256) void k_level_vertices(FusedArgs a, DCage c, Le
     a.vp[v] = acc;
 }
 
-// Everything the vertex rules need about one fan halfedge r: the points of
-// its own edge and of its previous edge, its face point, both sharpnesses.
-struct FanEntry {
-    P3 E, PE, F;
-    float ts, ps;
-    int32_t re, pe;
-};
-
-__device__ __forceinline__ FanEntry fan_entry(const FusedArgs &a, int32_t r, int4 rec, int4 prv) {
-    FanEntry e;
-    e.re = rec.y;
-    e.pe = prv.y;
-    e.E = a.ep[e.re];
-    e.PE = a.ep[e.pe];
-    e.F = a.fp[r >> 2];
-    e.ts = lsharp(a.cr, a.C, e.re);
-    e.ps = lsharp(a.cr, a.C, e.pe);
-    return e;
-}
-
-// fans of up to four halfedges (face and edge vertices) with every load
-// issued before any is used
-struct FanPreloaded {
-    FanEntry e0, e1, e2, e3;
-    __device__ __forceinline__ FanEntry get(int i) const { return pick4(e0, e1, e2, e3, i); }
-};
-
-// any fan, entries loaded when asked for
-struct FanOnDemand {
-    const FusedArgs *a;
-    const Fan *f;
-    const DCage *c;
-    __device__ __forceinline__ FanEntry get(int i) const {
-        const int32_t r = fan_at(*f, *c, i);
-        const int4 *he4 = reinterpret_cast<const int4 *>(a->he);
-        return fan_entry(*a, r, he4[r], he4[qprev(r)]);
-    }
-};
-
-// The vertex rules over a fan (see k_level_vertices for the reference walk
-// they reproduce).  MAXN > 0 bounds the fan size at compile time so the loops
-// unroll.
-template <int GATHER, int CREASED, int MAXN, class Acc>
-__device__ __forceinline__ P3 fan_vertex_point(const Fan &f, const Acc &acc, P3 old) {
-    const int n = f.n;
-    const int ri = f.ri;
-    const int lim = MAXN > 0 ? MAXN : n;
-    if (GATHER) {
-        if (!CREASED) {
-            P3 sum = zero3();
-            float nv = 1.0f;
-            for (int i = 0; i < lim; ++i) {
-                if (i < ri || i >= n) continue;
-                const FanEntry e = acc.get(i);
-                if (i == ri) {
-                    sum = add(mul(e.F, -1.0f), mul(e.E, 4.0f));
-                } else {
-                    sum = add(sum, mul(e.F, -1.0f));
-                    sum = add(sum, mul(e.E, 4.0f));
-                    nv = nv + 1.0f;
-                }
-            }
-            P3 sm = add(mul(sum, 1.0f / (nv * nv)), mul(old, 1.0f - 3.0f / nv));
-            return lerp(old, sm, f.boundary ? 0.0f : 1.0f);
-        }
-        P3 sm = zero3(), cr = zero3(), Erep = zero3();
-        float avgS = 0.0f, cc = 0.0f, nv = 0.0f, cwRep = 0.0f;
-        for (int i = 0; i < lim; ++i) {
-            if (i < ri || i >= n) continue;
-            const FanEntry e = acc.get(i);
-            const float pw = sgn(e.ps);
-            if (i == ri) {
-                cwRep = sgn(e.ts);
-                Erep = e.E;
-                sm = add(mul(e.F, -1.0f), mul(e.PE, 4.0f));
-                cr = add(zero3(), mul(e.PE, pw));
-                avgS = e.ps;
-                cc = pw;
-                nv = 1.0f;
-            } else {
-                sm = add(sm, mul(e.F, -1.0f));
-                sm = add(sm, mul(e.PE, 4.0f));
-                nv = nv + 1.0f;
-                cr = add(cr, mul(e.PE, pw));
-                avgS = avgS + e.ps;
-                cc = cc + pw;
-            }
-        }
-        if (f.boundary) {
-            for (int k = 0; k < lim; ++k) {
-                const int i = ri - 1 - k;
-                if (i < 0) break;
-                const FanEntry e = acc.get(i);
-                const float qw = sgn(e.ts);
-                sm = add(sm, mul(e.F, -1.0f));
-                sm = add(sm, mul(e.E, 4.0f));
-                nv = nv + 1.0f;
-                cr = add(cr, mul(e.E, qw));
-                avgS = avgS + e.ts;
-                cc = cc + qw;
-            }
-            cr = add(cr, mul(Erep, cwRep));
-            cc = cc + cwRep;
-            nv = nv + 1.0f;
-        }
-        sm = add(mul(sm, 1.0f / (nv * nv)), mul(old, 1.0f - 3.0f / nv));
-        cr = add(mul(cr, 0.5f / cc), mul(old, 0.5f));
-        if (cc <= 1.0f) return sm;
-        if (cc >= 3.0f || nv == 2.0f) return old;
-        return lerp(old, cr, sat(avgS * 0.5f));
-    }
-    // scatter rules: fan parameters, then per-halfedge contributions in fan
-    // order (incident edges: the own edge of the first fan halfedge on a
-    // boundary, and the previous edge of every fan halfedge)
-    RingParams rp;
-    rp.n = (float)n; rp.cc = 0.0f; rp.boundary = f.boundary; rp.s1 = rp.s2 = 0.0f; rp.e1 = rp.e2 = -1; rp.sall = 0.0f;
-    if (CREASED) {
-        for (int i = -1; i < lim; ++i) {
-            if (i >= n || (i < 0 && !f.boundary)) continue;
-            const FanEntry e = acc.get(i < 0 ? 0 : i);
-            const float se = i < 0 ? e.ts : e.ps;
-            const int32_t ed = i < 0 ? e.re : e.pe;
-            rp.cc += sgn(se);
-            if (se != 0.0f) {
-                if (rp.e1 < 0) { rp.s1 = se; rp.e1 = ed; }
-                else if (rp.e2 < 0) { rp.s2 = se; rp.e2 = ed; }
-            }
-            rp.sall += se;
-        }
-    }
-    P3 sum = zero3();
-    for (int i = 0; i < lim; ++i) {
-        if (i >= n) break;
-        const FanEntry e = acc.get(i);
-        sum = add(sum, vertex_scatter_contrib(rp, old, e.E, e.PE, e.F, CREASED ? e.ts : 0.0f, CREASED ? e.ps : 0.0f,
-                                              e.re, CREASED));
-    }
-    return sum;
-}
-
 template <int GATHER, int CREASED>
 __global__ __launch_bounds__(256) void k_level_vertices_fan(FusedArgs a, DCage c, LevelTab t, LevelPtrs lp, int d) {
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, a.xcd_remap);

@@ -33,9 +33,11 @@ struct ccd_Subd {
     hipStream_t stream = nullptr;
     // adjoint scratch (allocated on first use, per scalar size)
     void *adj_a = nullptr, *adj_b = nullptr, *adj_in = nullptr, *adj_out = nullptr, *adj_sharp = nullptr;
+    // deferred vertex list of the patch kernels and one counter per level
+    int32_t *defer = nullptr, *defer_counts = nullptr;
     size_t adj_scalar = 0;
     bool timing = false;
-    int opt_fused = 1, opt_store = cch::kStoreLds, opt_remap = 1, opt_fan = 2;
+    int opt_fused = 2, opt_store = cch::kStoreLds, opt_remap = 1, opt_fan = 2;
     std::vector<Timed> timed;
 
     bool alloc(const cc_Mesh *cage, int32_t depth);
@@ -49,6 +51,7 @@ struct ccd_Subd {
     bool refine(int32_t steps, int dmax, hipStream_t st);
     bool refine_topology(int32_t steps, int dmax, hipStream_t st);
     bool refine_points(int32_t steps, int dmax, const cch::P3 *cagePts, cch::P3 *out, hipStream_t st);
+    void level_patch(int d, int32_t steps, int topo, int uvMode, cch::P3 *points, hipStream_t st);
     void begin_timing();
     bool adjoint(const void *gradFinest, void *gradCage, int32_t steps, hipStream_t st);
     void zero(float *p, int64_t count, hipStream_t st);

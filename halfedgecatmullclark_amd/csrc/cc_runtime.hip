@@ -87,12 +87,15 @@ bool ccd_Subd::alloc(const cc_Mesh *cage, int32_t depth) {
     CC_TRY(hipMalloc(&he, sizeof(He) * (size_t)n.cumH(D)), "hipMalloc(halfedges)");
     CC_TRY(hipMalloc(&cr, sizeof(Crease) * (size_t)n.cumC(D)), "hipMalloc(creases)");
     CC_TRY(hipMalloc(&pts, sizeof(P3) * (size_t)n.cumV(D)), "hipMalloc(points)");
+    CC_TRY(hipMalloc(&defer, sizeof(int32_t) * (size_t)(D >= 2 ? n.V[D - 1] : 1)), "hipMalloc(deferred vertices)");
+    CC_TRY(hipMalloc(&defer_counts, sizeof(int32_t) * (kMaxDepth + 1)), "hipMalloc(deferred counters)");
     return true;
 }
 
 void ccd_Subd::free_all() {
     DeviceGuard g(device);
-    void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts, adj_a, adj_b, adj_in, adj_out, adj_sharp};
+    void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts, adj_a, adj_b, adj_in, adj_out, adj_sharp,
+                    defer, defer_counts};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : timed) {
@@ -231,6 +234,22 @@ bool ccd_Subd::refine_topology(int32_t steps, int dmax, hipStream_t st) {
     return true;
 }
 
+// one level of the patch-local kernels: children, face, edge and most vertex
+// points, then the vertices whose rings leave their block
+void ccd_Subd::level_patch(int d, int32_t steps, int topo, int uvMode, P3 *points, hipStream_t st) {
+    const bool gather = (steps & CCD_GATHER) != 0;
+    const int creased = (steps & CCD_NOCREASES) ? 0 : 1;
+    FusedArgs a = fused_args(d, points);
+    a.defer = defer;
+    a.defer_count = defer_counts + d;
+    const DCage c = cage_view(nullptr);
+    const LevelPtrs lp = level_ptrs();
+    launch_timed("level_patch", n.H[d], st,
+                 [&] { launch_level_patch(a, c, tab, lp, d, gather, creased, topo, uvMode, opt_store, st); });
+    launch_timed("level_deferred", n.V[d], st,
+                 [&] { launch_level_deferred(a, c, tab, lp, d, gather, creased, n.V[d], st); });
+}
+
 // the cage step of the points (CatmullClark.h:2633-2636, :2661-2663)
 void ccd_Subd::cage_points(int32_t steps, const P3 *cagePts, P3 *out, hipStream_t st) {
     const bool gather = (steps & CCD_GATHER) != 0;
@@ -254,7 +273,12 @@ bool ccd_Subd::refine_points(int32_t steps, int dmax, const P3 *cagePts, P3 *out
     const int creased = (steps & CCD_NOCREASES) ? 0 : 1;
     const DCage c = cage_view(cagePts);
     cage_points(steps, cagePts, out, st);
+    if (opt_fused == 2) zero(reinterpret_cast<float *>(defer_counts), kMaxDepth + 1, st);
     for (int d = 1; d < dmax; ++d) {
+        if (opt_fused == 2) {
+            level_patch(d, steps, 0, 0, out, st);
+            continue;
+        }
         if (opt_fused) {
             const FusedArgs a = fused_args(d, out);
             launch_timed("level_points", n.H[d], st,
@@ -302,7 +326,12 @@ bool ccd_Subd::refine(int32_t steps, int dmax, hipStream_t st) {
     const DCage c = cage_view(nullptr);
     if (topo || uvMode == 1) launch("cage_halfedges", k_cage_halfedges, H0, st, c, he, (int)topo, uvMode);
     if (points) cage_points(steps, nullptr, pts, st);
+    if (points && opt_fused == 2) zero(reinterpret_cast<float *>(defer_counts), kMaxDepth + 1, st);
     for (int d = 1; d < dmax; ++d) {
+        if (points && opt_fused == 2) {
+            level_patch(d, steps, topo, uvMode, pts, st);
+            continue;
+        }
         const FusedArgs a = fused_args(d, pts);
         if (topo || uvMode == 1 || points)
             launch_timed("level_fused", n.H[d], st,
@@ -687,7 +716,7 @@ int32_t ccd_Synchronize(ccd_Subd *s) {
 int32_t ccd_SetOption(ccd_Subd *s, int32_t key, int32_t value) {
     if (!s) return 11;
     switch (key) {
-    case CCD_OPT_FUSED: s->opt_fused = value != 0; return 0;
+    case CCD_OPT_FUSED: s->opt_fused = value < 0 ? 0 : (value > 2 ? 2 : value); return 0;
     case CCD_OPT_STORE: s->opt_store = value ? kStoreLds : kStoreDirect; return 0;
     case CCD_OPT_XCD_REMAP: s->opt_remap = value != 0; return 0;
     case CCD_OPT_FAN: s->opt_fan = value < 0 ? 0 : (value > 2 ? 2 : value); return 0;

@@ -264,8 +264,11 @@ int32_t ccd_RefinePointsAdjoint(ccd_Subd *subd, const float *devGradFinest, floa
 int32_t ccd_Download(const ccd_Subd *subd, cc_Subd *dst, int32_t steps);
 int32_t ccd_Synchronize(ccd_Subd *subd);
 /* kernel-selection options (all have working defaults):
- *   CCD_OPT_FUSED      1: one fused kernel per level for children, face and
- *                      edge points plus one for vertex points (default);
+ *   CCD_OPT_FUSED      2: one patch-local kernel per level for children,
+ *                      face, edge and vertex points, plus a small kernel for
+ *                      the vertices whose rings leave their block (default);
+ *                      1: one fused kernel for children, face and edge
+ *                      points plus one for vertex points;
  *                      0: one kernel per output kind, vertex scatter with
  *                      float atomics (the reference's own structure)
  *   CCD_OPT_STORE      child-record stores: 0 direct, 1 staged through LDS
