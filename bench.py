@@ -7,8 +7,17 @@ N(0, 0.01) jitter, one longitudinal crease ring of sharpness 1.5 and one of
 refinement of all 8 levels from the cage resident in HBM; value is refined
 halfedges (levels 1..8, 715,816,960) per second.
 
+Before timing, ccd_Autotune picks each level's kernel configuration (the
+fused kernels with 2, 4 or 8 halfedges tested per vertex-kernel thread, or
+the patch-local kernel with 128/256/512-halfedge tiles) by timing them.
+
 Also reported on the same line:
   - the gather point rules on the same workload;
+  - batched refinement: 256 cages of uneven size (halfedges log-uniform in
+    [24, 60000], depths 4-6, 5% of edges creased U[0,4]) refined in one
+    launch per kernel kind and level from a captured hipGraph, forward and
+    adjoint, against the same 256 cages refined one after another through
+    the single-cage device API (ccd_Refine / ccd_RefinePointsAdjoint);
   - forward + adjoint against forward alone on the 32x16 torus at depth 5
     (the cage-fitting workload);
   - roofline of the dominant kernel (per-kernel HIP events on the launch
@@ -16,7 +25,7 @@ Also reported on the same line:
   - the reference's own OpenMP CPU path (CatmullClark.h built by
     oracle/Makefile) timed on this host's cores on the same torus at depth 6.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
 """
 from __future__ import annotations
 
@@ -47,6 +56,12 @@ def vertex_kernel_bytes(n, d):
     records (16 B), points and creases (12 B) and the level-(d+1) face and
     edge points it reads once, the level-(d+1) vertex points it writes."""
     return 16 * n["H"][d] + 12 * n["V"][d] + 12 * n["C"][d] + 12 * (n["F"][d] + n["E"][d]) + 12 * n["V"][d]
+
+
+def patch_bytes(n, d):
+    """Algorithmic bytes of the patch-local kernel d -> d+1: the fused
+    kernel's plus the level-(d+1) vertex points it writes."""
+    return level_bytes(n, d) + 12 * n["V"][d]
 
 
 def refinement_bytes(n, D):
@@ -93,12 +108,76 @@ def cpu_baseline(seconds_budget: float = 20.0):
                       f"({hc} refined halfedges), median of {len(times)} runs: {med * 1e3:.1f} ms"}
 
 
+def batched(torch, capi, C, steps: int, stream):
+    """Batched forward + adjoint of 256 uneven cages against the same cages
+    refined one by one through the single-cage device API."""
+    sptr = stream.cuda_stream
+    t0 = time.perf_counter()
+    cages, depths = C.batch_cages(256, seed=0)
+    gen_s = time.perf_counter() - t0
+    b = capi.Batch(cages, depths)
+    total = b.count(capi.COUNT_HALFEDGES)
+    nf = b.count(capi.COUNT_FINEST_POINTS)
+    b.refine(capi.ALL, sptr)        # topology for the adjoint, and graph capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.autotune(capi.ALL, sptr)
+    tune_s = time.perf_counter() - t0
+    g = torch.randn((nf, 3), device="cuda", dtype=torch.float32)
+    gc = torch.empty((b.count(capi.COUNT_CAGE_POINTS), 3), device="cuda", dtype=torch.float32)
+
+    def run(f, k):
+        f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(k):
+            f()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    k = max(3, steps // 4)
+    b_fwd = run(lambda: b.refine(capi.ALL, sptr), k)
+    b_adj = run(lambda: b.refine_points_adjoint(g.data_ptr(), gc.data_ptr(), 0, sptr), k)
+    offs = [b.offset(capi.COUNT_FINEST_POINTS, i) for i in range(len(cages))]
+    b.close()
+    torch.cuda.empty_cache()
+    subs = [capi.DeviceSubd(c, d) for c, d in zip(cages, depths)]
+    for s_ in subs:
+        s_.refine(capi.ALL, sptr)
+    gslices = [g[o:] for o in offs]
+    gcs = [torch.empty((c.V, 3), device="cuda", dtype=torch.float32) for c in cages]
+
+    def plain_fwd():
+        for s_ in subs:
+            s_.refine(capi.ALL, sptr)
+
+    def plain_adj():
+        for s_, gi, go in zip(subs, gslices, gcs):
+            s_.refine_points_adjoint(gi.data_ptr(), go.data_ptr(), 0, sptr)
+
+    p_fwd = run(plain_fwd, k)
+    p_adj = run(plain_adj, k)
+    for s_ in subs:
+        s_.close()
+    return {"workload": "256 cages (cube/torus/irregular generators, halfedges log-uniform in [24, 60000], "
+                        "depths uniform in {4,5,6}, 5% of edges creased U[0,4], seed 0)",
+            "refined_halfedges": total, "cage_generation_s": gen_s, "autotune_s": tune_s,
+            "batched_fwd_ms": b_fwd, "plain_fwd_ms": p_fwd, "fwd_speedup": p_fwd / b_fwd,
+            "batched_fwd_value": total / (b_fwd * 1e-3),
+            "batched_adj_ms": b_adj, "plain_adj_ms": p_adj, "adj_speedup": p_adj / b_adj,
+            "batched_fwd_adj_ms": b_fwd + b_adj, "plain_fwd_adj_ms": p_fwd + p_adj,
+            "fwd_adj_speedup": (p_fwd + p_adj) / (b_fwd + b_adj)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched comparison")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the refinement of one cage runs on one device; "
@@ -138,15 +217,20 @@ def main():
         ds.set_timing(False)
         return e0.elapsed_time(e1) / k, kern
 
+    ds.refine(capi.ALL, sptr)
+    ds.autotune(capi.ALL, sptr)
+    level_cfg = ds.level_options()
     ms_scatter, kern = timed(capi.ALL, args.steps, args.warmup, per_kernel=True)
     ms_gather, _ = timed(capi.ALL | capi.GATHER, max(3, args.steps // 2), 1)
 
     # dominant kernel: largest mean duration among the launches of one step;
-    # the per-level kernels appear once per level, in level order
-    seen, labelled = {}, []
-    for (i, name) in sorted(kern):
-        seen[name] = seen.get(name, 0) + 1
-        labelled.append(((i, name), seen[name]))
+    # the per-level kernels run in level order, each level starting with its
+    # fused or patch-local kernel
+    labelled, d = [], 0
+    for (i, nm) in sorted(kern):
+        if nm in ("level_fused", "level_points", "level_patch"):
+            d += 1
+        labelled.append(((i, nm), d))
     ((idx, name), level) = max(labelled, key=lambda kv: sum(kern[kv[0]]) / len(kern[kv[0]]))
     durs = kern[(idx, name)]
     mean_ms = sum(durs) / len(durs)
@@ -154,6 +238,8 @@ def main():
         nbytes = level_bytes(n, level)
     elif name == "level_vertices":
         nbytes = vertex_kernel_bytes(n, level)
+    elif name == "level_patch":
+        nbytes = patch_bytes(n, level)
     else:
         nbytes = None
     achieved = nbytes / (mean_ms * 1e-3) / 1e9 if nbytes else None
@@ -197,8 +283,9 @@ def main():
         res[label] = e0.elapsed_time(e1) / K3
     sub.close()
 
-    cpu = None if args.no_cpu else cpu_baseline()
     ds.close()
+    batch = None if args.no_batch else batched(torch, capi, C, args.steps, stream)
+    cpu = None if args.no_cpu else cpu_baseline()
 
     line = {
         "metric": METRIC,
@@ -221,6 +308,8 @@ def main():
         "fwd_adjoint": {"workload": "torus 32x16 cage, depth 5, loss 0.5*||S(V)-T||^2",
                         "fwd_ms": res["fwd"], "fwd_adj_ms": res["fwd_adj"],
                         "ratio": res["fwd_adj"] / res["fwd"]},
+        "batched": batch,
+        "level_kernels": level_cfg,
         "roofline": {"bound": "hbm", "kernel": f"{name} (level {level} -> {level + 1})",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,

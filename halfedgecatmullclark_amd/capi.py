@@ -18,7 +18,10 @@ LIB_PATH = os.path.join(HERE, "libcchip.so")
 HALFEDGES, CREASES, UVS, POINTS, GATHER, NOCREASES, FP64 = 1, 2, 4, 8, 16, 32, 64
 ALL = HALFEDGES | CREASES | UVS | POINTS
 PTR_HALFEDGES, PTR_CREASES, PTR_POINTS, PTR_CAGE_POINTS = 0, 1, 2, 3
-OPT_FUSED, OPT_STORE, OPT_XCD_REMAP, OPT_FAN = 0, 1, 2, 3
+OPT_FUSED, OPT_STORE, OPT_XCD_REMAP, OPT_FAN, OPT_PATCH_TILE, OPT_PATCH_HPT, OPT_SCAN = 0, 1, 2, 3, 4, 5, 6
+OPT_GRAPH = 100
+(COUNT_HALFEDGES, COUNT_CREASES, COUNT_POINTS, COUNT_CAGE_POINTS, COUNT_FINEST_POINTS, COUNT_DEPTH,
+ COUNT_CAGES) = range(7)
 
 HOST_DRIVERS = ("ccs_Refine_Scatter", "ccs_Refine_Gather", "ccs_Refine_NoCreases_Scatter",
                 "ccs_Refine_NoCreases_Gather", "ccs_RefineVertexPoints_Scatter",
@@ -68,6 +71,27 @@ def lib() -> ctypes.CDLL:
         "ccd_TimingCount": (i32, [vp]),
         "ccd_TimingName": (ctypes.c_char_p, [vp, i32]),
         "ccd_TimingMs": (ctypes.c_float, [vp, i32]),
+        "ccd_Autotune": (i32, [vp, i32, vp]),
+        "ccd_TileOption": (i32, [vp, i32, i32]),
+        "ccb_Create": (vp, [P(P(cc_Mesh)), P(i32), i32]),
+        "ccb_Release": (None, [vp]),
+        "ccb_Count": (i64, [vp, i32, i32]),
+        "ccb_Offset": (i64, [vp, i32, i32]),
+        "ccb_DevicePointer": (vp, [vp, i32]),
+        "ccb_Refine": (i32, [vp, i32, vp]),
+        "ccb_RefinePoints": (i32, [vp, vp, i32, vp]),
+        "ccb_FinestPoints": (i32, [vp, vp, vp]),
+        "ccb_RefinePointsAdjoint": (i32, [vp, vp, vp, i32, vp]),
+        "ccb_Download": (i32, [vp, i32, P(cc_Subd), i32]),
+        "ccb_Synchronize": (i32, [vp]),
+        "ccb_SetOption": (i32, [vp, i32, i32]),
+        "ccb_Autotune": (i32, [vp, i32, vp]),
+        "ccb_TileOption": (i32, [vp, i32, i32]),
+        "ccb_SetTiming": (i32, [vp, i32]),
+        "ccb_TimingCount": (i32, [vp]),
+        "ccb_TimingName": (ctypes.c_char_p, [vp, i32]),
+        "ccb_TimingLevel": (i32, [vp, i32]),
+        "ccb_TimingMs": (ctypes.c_float, [vp, i32]),
         "ccm_Load": (P(cc_Mesh), [ctypes.c_char_p]),
         "ccm_Save": (ctypes.c_bool, [P(cc_Mesh), ctypes.c_char_p]),
         "ccm_Release": (None, [P(cc_Mesh)]),
@@ -213,6 +237,14 @@ class DeviceSubd:
     def set_option(self, key: int, value: int):
         _check(self.L.ccd_SetOption(self.handle, key, value), "ccd_SetOption")
 
+    def autotune(self, steps: int = ALL, stream: int | None = None):
+        _check(self.L.ccd_Autotune(self.handle, steps, stream), "ccd_Autotune")
+
+    def level_options(self) -> list[dict]:
+        keys = {"kind": OPT_FUSED, "scan": OPT_SCAN, "tile": OPT_PATCH_TILE, "hpt": OPT_PATCH_HPT}
+        return [{k: int(self.L.ccd_TileOption(self.handle, v, d)) for k, v in keys.items()}
+                for d in range(1, self.depth)]
+
     def set_timing(self, on: bool):
         self.L.ccd_SetTiming(self.handle, int(on))
 
@@ -224,6 +256,92 @@ class DeviceSubd:
     def close(self):
         if getattr(self, "handle", None):
             self.L.ccd_Release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """A ``ccb_Batch``: many cages, each with its own depth, refined together
+    (one launch per kernel kind and level, replayed as a hipGraph)."""
+
+    def __init__(self, cages: list[Cage], depths: list[int]):
+        self.L = lib()
+        self.cages = [c.contiguous() for c in cages]
+        self.meshes = [MeshView(c) for c in self.cages]
+        self.depths = [int(d) for d in depths]
+        arr = (ctypes.POINTER(cc_Mesh) * len(self.meshes))(*[m.pointer for m in self.meshes])
+        dep = (ctypes.c_int32 * len(self.depths))(*self.depths)
+        self.handle = self.L.ccb_Create(arr, dep, len(self.meshes))
+        if not self.handle:
+            raise CCError(f"ccb_Create failed: {last_error()[1]}")
+
+    def __len__(self) -> int:
+        return len(self.cages)
+
+    def count(self, what: int, cage: int = -1) -> int:
+        return int(self.L.ccb_Count(self.handle, what, cage))
+
+    def offset(self, what: int, cage: int) -> int:
+        return int(self.L.ccb_Offset(self.handle, what, cage))
+
+    def pointer(self, which: int) -> int:
+        return int(self.L.ccb_DevicePointer(self.handle, which) or 0)
+
+    def refine(self, steps: int = ALL, stream: int | None = None):
+        _check(self.L.ccb_Refine(self.handle, steps, stream), "ccb_Refine")
+
+    def refine_points(self, cage_points_ptr: int | None = None, steps: int = 0, stream: int | None = None):
+        _check(self.L.ccb_RefinePoints(self.handle, cage_points_ptr, steps, stream), "ccb_RefinePoints")
+
+    def finest_points(self, out_ptr: int, stream: int | None = None):
+        _check(self.L.ccb_FinestPoints(self.handle, out_ptr, stream), "ccb_FinestPoints")
+
+    def refine_points_adjoint(self, grad_finest_ptr: int, grad_cage_ptr: int, steps: int = 0,
+                              stream: int | None = None):
+        _check(self.L.ccb_RefinePointsAdjoint(self.handle, grad_finest_ptr, grad_cage_ptr, steps, stream),
+               "ccb_RefinePointsAdjoint")
+
+    def synchronize(self):
+        _check(self.L.ccb_Synchronize(self.handle), "ccb_Synchronize")
+
+    def set_option(self, key: int, value: int):
+        _check(self.L.ccb_SetOption(self.handle, key, value), "ccb_SetOption")
+
+    def autotune(self, steps: int = ALL, stream: int | None = None):
+        _check(self.L.ccb_Autotune(self.handle, steps, stream), "ccb_Autotune")
+
+    def level_options(self) -> list[dict]:
+        keys = {"kind": OPT_FUSED, "scan": OPT_SCAN, "tile": OPT_PATCH_TILE, "hpt": OPT_PATCH_HPT}
+        dmax = self.count(COUNT_DEPTH)
+        return [{k: int(self.L.ccb_TileOption(self.handle, v, d)) for k, v in keys.items()} for d in range(1, dmax)]
+
+    def set_timing(self, on: bool):
+        self.L.ccb_SetTiming(self.handle, int(on))
+
+    def timings(self) -> list[tuple[str, int, float]]:
+        n = self.L.ccb_TimingCount(self.handle)
+        return [(self.L.ccb_TimingName(self.handle, i).decode(), int(self.L.ccb_TimingLevel(self.handle, i)),
+                 float(self.L.ccb_TimingMs(self.handle, i))) for i in range(n)]
+
+    def download(self, i: int, steps: int = ALL) -> dict:
+        """Copy cage i's levels 1..D_i to host numpy arrays."""
+        c, d = self.cages[i], self.depths[i]
+        n = level_counts(c.V, c.H, c.E, c.F, d)
+        he = np.zeros(n["Hcum"][d], HALFEDGE_DTYPE)
+        cr = np.zeros(n["Ccum"][d], CREASE_DTYPE)
+        pt = np.zeros((n["Vcum"][d], 3), np.float32)
+        sub = cc_Subd(self.meshes[i].pointer, pt.ctypes.data, he.ctypes.data, cr.ctypes.data, d)
+        _check(self.L.ccb_Download(self.handle, i, ctypes.byref(sub), steps), "ccb_Download")
+        return {"halfedges": he, "creases": cr, "points": pt, "counts": n}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.L.ccb_Release(self.handle)
             self.handle = None
 
     def __del__(self):

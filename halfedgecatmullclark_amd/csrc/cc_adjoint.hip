@@ -11,6 +11,7 @@
 // points is kept or recomputed.  Kernels are templated on the scalar type so
 // the same code runs in float and, for gradient checks, in double.
 #include "cc_adjoint.h"
+#include "cc_batch.h"
 #include "cc_fan.h"
 
 namespace cch {
@@ -95,8 +96,7 @@ __device__ __forceinline__ VW<T> level_vertex_weights(const RingT<T> &rp, T ts, 
 // the level-d gradient of the vertex and sends beta, gamma, delta shares to
 // the level-(d+1) edge and face gradients
 template <typename T, int CREASED>
-__global__ __launch_bounds__(256) void k_adj_vertices(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
-    const int32_t h = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+__device__ __forceinline__ void adj_vertices_at(const AdjArgs<T> &a, const DCage &c, const LevelTab &t, const LevelPtrs &lp, int d, int32_t h) {
     if (h >= a.H) return;
     const He *he = a.he;
     const int32_t v = he[h].vert;
@@ -139,8 +139,7 @@ __global__ __launch_bounds__(256) void k_adj_vertices(AdjArgs<T> a, DCage c, Lev
 // edge rule (creased :2221-2233, smooth :2125-2131): each halfedge's share of
 // its edge point flows to its two endpoints and its face point
 template <typename T, int CREASED>
-__global__ __launch_bounds__(256) void k_adj_edges(AdjArgs<T> a) {
-    const int32_t h = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+__device__ __forceinline__ void adj_edges_at(const AdjArgs<T> &a, int32_t h) {
     if (h >= a.H) return;
     const He x = a.he[h];
     const V3<T> ge = reinterpret_cast<const V3<T> *>(a.gE)[x.edge];
@@ -165,8 +164,7 @@ __global__ __launch_bounds__(256) void k_adj_edges(AdjArgs<T> a) {
 
 // face rule (:2048): each corner receives a quarter of its face's gradient
 template <typename T>
-__global__ __launch_bounds__(256) void k_adj_faces(AdjArgs<T> a) {
-    const int32_t h = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+__device__ __forceinline__ void adj_faces_at(const AdjArgs<T> &a, int32_t h) {
     if (h >= a.H) return;
     const V3<T> gf = reinterpret_cast<const V3<T> *>(a.gF)[h >> 2];
     atomic_add3(reinterpret_cast<V3<T> *>(a.gP) + a.he[h].vert, (T)0.25, gf);
@@ -176,8 +174,7 @@ __global__ __launch_bounds__(256) void k_adj_faces(AdjArgs<T> a) {
 // vertex rule, one thread per cage halfedge with the reference's own walk
 // (creased :1884-1988, whose smooth test comes first; smooth :1724-1753)
 template <typename T, int CREASED>
-__global__ __launch_bounds__(256) void k_adj_cage_vertices(AdjArgs<T> a, DCage c) {
-    const int32_t h = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+__device__ __forceinline__ void adj_cage_vertices_at(const AdjArgs<T> &a, const DCage &c, int32_t h) {
     if (h >= c.H) return;
     const CageHe x = c.he[h];
     const V3<T> gv = reinterpret_cast<const V3<T> *>(a.gV)[x.vert];
@@ -239,8 +236,7 @@ __global__ __launch_bounds__(256) void k_adj_cage_vertices(AdjArgs<T> a, DCage c
 
 // edge rule (creased :1612-1645, smooth :1527-1545)
 template <typename T, int CREASED>
-__global__ __launch_bounds__(256) void k_adj_cage_edges(AdjArgs<T> a, DCage c) {
-    const int32_t h = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+__device__ __forceinline__ void adj_cage_edges_at(const AdjArgs<T> &a, const DCage &c, int32_t h) {
     if (h >= c.H) return;
     const CageHe x = c.he[h];
     const V3<T> ge = reinterpret_cast<const V3<T> *>(a.gE)[x.edge];
@@ -265,14 +261,44 @@ __global__ __launch_bounds__(256) void k_adj_cage_edges(AdjArgs<T> a, DCage c) {
 
 // face rule (:1444-1460): each corner receives 1/m of its face's gradient
 template <typename T>
-__global__ __launch_bounds__(256) void k_adj_cage_faces(AdjArgs<T> a, DCage c) {
-    const int32_t h = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+__device__ __forceinline__ void adj_cage_faces_at(const AdjArgs<T> &a, const DCage &c, int32_t h) {
     if (h >= c.H) return;
     const CageHe x = c.he[h];
     int32_t m = 1;
     for (int32_t it = x.next; it != h && m < kWalkCap; it = c.he[it].next) ++m;
     const V3<T> gf = reinterpret_cast<const V3<T> *>(a.gF)[x.face];
     atomic_add3(reinterpret_cast<V3<T> *>(a.gP) + x.vert, (T)1 / (T)m, gf);
+}
+
+// ---- kernels: one thread per halfedge ------------------------------------------
+template <typename T, int CREASED>
+__global__ __launch_bounds__(256) void k_adj_vertices(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
+    adj_vertices_at<T, CREASED>(a, c, t, lp, d, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+template <typename T, int CREASED>
+__global__ __launch_bounds__(256) void k_adj_edges(AdjArgs<T> a) {
+    adj_edges_at<T, CREASED>(a, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_faces(AdjArgs<T> a) {
+    adj_faces_at<T>(a, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+template <typename T, int CREASED>
+__global__ __launch_bounds__(256) void k_adj_cage_vertices(AdjArgs<T> a, DCage c) {
+    adj_cage_vertices_at<T, CREASED>(a, c, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+template <typename T, int CREASED>
+__global__ __launch_bounds__(256) void k_adj_cage_edges(AdjArgs<T> a, DCage c) {
+    adj_cage_edges_at<T, CREASED>(a, c, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_cage_faces(AdjArgs<T> a, DCage c) {
+    adj_cage_faces_at<T>(a, c, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 // crease sharpness of the next level in T (CatmullClark.h:2962-2980), for
@@ -381,6 +407,64 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
         hipLaunchKernelGGL((k_adj_cage_edges<T, 0>), g, b, 0, st, a, pl.cage);
     }
     hipLaunchKernelGGL(k_adj_cage_faces<T>, g, b, 0, st, a, pl.cage);
+}
+
+// ---- batched (float): one launch per rule and level over every cage ------------
+namespace {
+
+template <int CREASED>
+struct OpAdjVertices {
+    int d;
+    __device__ void operator()(const AdjJob &J, int64_t i) const {
+        adj_vertices_at<float, CREASED>(J.a, J.c, J.t, J.lp, d, (int32_t)i);
+    }
+};
+template <int CREASED>
+struct OpAdjEdges {
+    __device__ void operator()(const AdjJob &J, int64_t i) const { adj_edges_at<float, CREASED>(J.a, (int32_t)i); }
+};
+struct OpAdjFaces {
+    __device__ void operator()(const AdjJob &J, int64_t i) const { adj_faces_at<float>(J.a, (int32_t)i); }
+};
+template <int CREASED>
+struct OpAdjCageVertices {
+    __device__ void operator()(const AdjJob &J, int64_t i) const {
+        adj_cage_vertices_at<float, CREASED>(J.a, J.c, (int32_t)i);
+    }
+};
+template <int CREASED>
+struct OpAdjCageEdges {
+    __device__ void operator()(const AdjJob &J, int64_t i) const {
+        adj_cage_edges_at<float, CREASED>(J.a, J.c, (int32_t)i);
+    }
+};
+struct OpAdjCageFaces {
+    __device__ void operator()(const AdjJob &J, int64_t i) const { adj_cage_faces_at<float>(J.a, J.c, (int32_t)i); }
+};
+
+}  // namespace
+
+// level d of every job: the vertex, edge and face transposes in turn
+void batch_adj_level(const BatchTable &t, int d, int creased, hipStream_t st) {
+    if (creased) {
+        launch_batch<AdjJob>(t, OpAdjVertices<1>{d}, st);
+        launch_batch<AdjJob>(t, OpAdjEdges<1>{}, st);
+    } else {
+        launch_batch<AdjJob>(t, OpAdjVertices<0>{d}, st);
+        launch_batch<AdjJob>(t, OpAdjEdges<0>{}, st);
+    }
+    launch_batch<AdjJob>(t, OpAdjFaces{}, st);
+}
+
+void batch_adj_cage(const BatchTable &t, int creased, hipStream_t st) {
+    if (creased) {
+        launch_batch<AdjJob>(t, OpAdjCageVertices<1>{}, st);
+        launch_batch<AdjJob>(t, OpAdjCageEdges<1>{}, st);
+    } else {
+        launch_batch<AdjJob>(t, OpAdjCageVertices<0>{}, st);
+        launch_batch<AdjJob>(t, OpAdjCageEdges<0>{}, st);
+    }
+    launch_batch<AdjJob>(t, OpAdjCageFaces{}, st);
 }
 
 template void adjoint_levels<float>(const AdjPlan &, const float *, float *, float *, float *, float *, int,

@@ -63,6 +63,11 @@ struct LevelTab {
     int32_t E[kMaxDepth + 1], V[kMaxDepth + 1], F[kMaxDepth + 1];
 };
 
+// Upper bound on the steps of any face or vertex walk.  Valid meshes never
+// reach it; it only guarantees that corrupt topology cannot keep a wave
+// running forever.
+constexpr int kWalkCap = 1 << 16;
+
 // Device view of a cage (kernel argument).
 struct DCage {
     const CageHe *he;

@@ -8,8 +8,6 @@
 
 namespace cch {
 
-constexpr int kWalkCap = 1 << 16;
-
 // Contiguous block ranges per XCD: hardware hands consecutive workgroups to
 // the 8 XCDs in turn; this returns the logical block so that XCD x sees
 // blocks [x*q + min(x, r) ...) of a launch with nb = 8q + r blocks.
@@ -261,6 +259,73 @@ __device__ __forceinline__ bool owner_fan(int32_t h, int32_t vert, int d, const 
 }
 
 
+// The fan of vertex v of level d >= 1, equal to what owner_fan gives its
+// representative (CatmullClark.h:1336): v was born at the level b with
+// V[b-1] <= v < V[b] as the face vertex of face v - V[b-1] or the edge
+// vertex of edge v - V[b-1] - F[b-1] of level b-1 (a cage vertex when
+// v < V[0]); its ring at level d is the ring at level b shifted by 2(d-b).
+__device__ __forceinline__ void vertex_fan(int32_t v, int d, const DCage &c, const LevelTab &t,
+                                           const He *const *levels, Fan &f) {
+    int b = d;
+    while (b > 0 && v < t.V[b - 1]) --b;
+    f.boundary = false;
+    f.ri = 0;
+    if (b == 0) {   // a cage vertex
+        const int32_t p = c.v2h[v];
+        f.kind = 0;
+        f.shift = 2 * (d - 1);
+        f.a0 = p;
+        int n = 1;
+        int32_t g = c.he[c.he[p].prev].twin;
+        for (; g >= 0 && g != p && n < kWalkCap; g = c.he[c.he[g].prev].twin) ++n;
+        f.n = n;
+        f.boundary = g < 0;
+        return;
+    }
+    f.shift = 2 * (d - b);
+    const int32_t Vp = b - 1 >= 1 ? t.V[b - 1] : c.V;
+    const int32_t Fp = b - 1 >= 1 ? t.F[b - 1] : c.F;
+    if (v < Vp + Fp) {   // a face vertex
+        const int32_t face = v - Vp;
+        if (b - 1 >= 1) {
+            f.kind = 1;
+            f.a0 = 16 * face + 2;
+            f.n = 4;
+            return;
+        }
+        const int32_t p = c.f2h[face];
+        f.kind = 2;
+        f.a0 = p;
+        int n = 1;
+        for (int32_t it = c.he[p].next; it != p && n < kWalkCap; it = c.he[it].next) ++n;
+        f.n = n;
+        return;
+    }
+    const int32_t e = v - Vp - Fp;   // an edge vertex
+    int32_t p, tw, np, nt;
+    if (b - 1 >= 1) {
+        p = edge_rep(c, t, e, b - 1);
+        tw = levels[b - 1][p].twin;
+        np = qnext(p);
+        nt = qnext(tw);
+    } else {
+        p = c.e2h[e];
+        tw = c.he[p].twin;
+        np = c.he[p].next;
+        nt = tw >= 0 ? c.he[tw].next : -1;
+    }
+    f.kind = 3;
+    if (tw >= 0) {
+        f.n = 4;
+        f.a0 = 4 * p + 1; f.a1 = 4 * nt + 3; f.a2 = 4 * tw + 1; f.a3 = 4 * np + 3;
+    } else {
+        f.n = 2;
+        f.boundary = true;
+        f.ri = 1;
+        f.a0 = 4 * np + 3; f.a1 = 4 * p + 1; f.a2 = f.a3 = 0;
+    }
+}
+
 // Everything the vertex rules need about one fan halfedge r: the points of
 // its own edge and of its previous edge, its face point, both sharpnesses.
 struct FanEntry {
@@ -399,6 +464,34 @@ __device__ __forceinline__ P3 fan_vertex_point(const Fan &f, const Acc &acc, P3 
                                               e.re, CREASED));
     }
     return sum;
+}
+
+// vertex point of vertex v of level d with fan f (kinds 1 and 3 load their
+// at most four entries up front, cage fans are walked)
+template <int GATHER, int CREASED>
+__device__ __forceinline__ void fan_vertex_store(const FusedArgs &a, const DCage &c, const Fan &f, int32_t v) {
+    const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
+    const P3 old = a.old[v];
+    if (f.kind == 1 || f.kind == 3) {
+        int32_t r[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[q] = fan_at(f, c, q < f.n ? q : 0);
+        int4 rec[4], prv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            rec[q] = he4[r[q]];
+            prv[q] = he4[qprev(r[q])];
+        }
+        FanPreloaded pre;
+        pre.e0 = fan_entry(a, r[0], rec[0], prv[0]);
+        pre.e1 = fan_entry(a, r[1], rec[1], prv[1]);
+        pre.e2 = fan_entry(a, r[2], rec[2], prv[2]);
+        pre.e3 = fan_entry(a, r[3], rec[3], prv[3]);
+        a.vp[v] = fan_vertex_point<GATHER, CREASED, 4>(f, pre, old);
+    } else {
+        FanOnDemand od{&a, &f, &c};
+        a.vp[v] = fan_vertex_point<GATHER, CREASED, 0>(f, od, old);
+    }
 }
 
 }  // namespace cch

@@ -16,9 +16,11 @@ struct FusedArgs {
     P3 *vp, *fp, *ep;   // level d+1 points: vertex | face | edge slices
     int32_t H, V, F, E, C;
     int xcd_remap;
-    // vertices the patch-local kernel leaves to the list kernel
-    int32_t *defer;        // representative halfedges, V entries at most
-    int32_t *defer_count;  // one counter, zero before the launch
+    int uv_refine;      // patch kernel with uvMode 3: 1 refine uvIDs, 0 write 0
+    // vertices the patch-local kernel leaves to the deferred kernel: bit l of
+    // word w marks halfedge 64w + l as the representative of such a vertex;
+    // the patch kernel writes every one of the ceil(H / 64) words
+    uint64_t *defer_mask;
 };
 
 // topo: write child twin/edge/vertex IDs; uvMode: 0 leave uvIDs, 1 refine
@@ -31,16 +33,18 @@ struct LevelPtrs {
 };
 
 // Patch-local fused kernel: children, face, edge and vertex points of level
-// d+1; vertices whose ring leaves the block are appended to a.defer and
+// d+1; vertices whose ring leaves the tile are marked in a.defer_mask and
 // finished by launch_level_deferred.
+// nt: halfedges per tile (128, 256, 512); hpt: tiles per workgroup
 void launch_level_patch(const FusedArgs &a, const DCage &c, const LevelTab &t, const LevelPtrs &lp, int d, int gather,
-                        int creased, int topo, int uvMode, int store, hipStream_t st);
+                        int creased, int topo, int uvMode, int nt, int hpt, hipStream_t st);
 void launch_level_deferred(const FusedArgs &a, const DCage &c, const LevelTab &t, const LevelPtrs &lp, int d,
-                           int gather, int creased, int64_t max_count, hipStream_t st);
+                           int gather, int creased, hipStream_t st);
 
 // vertex points of level d+1 (after launch_level_fused with points); fan
 // selects the structurally enumerated rings over the ring walk
+// (scan: halfedges each thread of the compacted kernel tests, 2, 4 or 8)
 void launch_level_vertices(const FusedArgs &a, const DCage &c, const LevelTab &t, const LevelPtrs &lp, int d,
-                           int gather, int creased, int fan, hipStream_t st);
+                           int gather, int creased, int fan, int scan, hipStream_t st);
 
 }  // namespace cch

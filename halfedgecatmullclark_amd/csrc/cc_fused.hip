@@ -19,134 +19,14 @@
 //
 // Blocks are remapped so that each XCD processes one contiguous range of
 // the level, keeping neighbouring quads in the same L2.
-#include "cc_fan.h"
-#include "cc_fused.h"
+#include "cc_level.h"
 
 namespace cch {
 
 
 template <int GATHER, int CREASED, int TOPO, int UVMODE, int POINTS, int STORE>
 __global__ __launch_bounds__(256) void k_level_fused(FusedArgs a) {
-    constexpr bool kStaged = STORE == kStoreLds && TOPO && UVMODE != 0;
-    __shared__ int4 stage[kStaged ? 4 * 256 : 1];
-    __shared__ P3 spts[POINTS ? 256 : 1];
-    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, a.xcd_remap);
-    const int32_t base = (int32_t)(lb * blockDim.x);
-    const int32_t h = base + (int32_t)threadIdx.x;
-    const bool live = h < a.H;   // H is a multiple of 4: quads are whole
-    const int j = h & 3;
-    const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
-    const int4 x = live ? he4[h] : make_int4(0, 0, 0, 0);
-    const int32_t twin = x.x;
-    const bool owner = live && h > twin;   // this halfedge writes its edge point
-    // twin quads inside this block come from LDS, the others from memory
-    const bool twin_here = owner && twin >= base && twin < base + (int32_t)blockDim.x;
-    const bool twin_far = owner && twin >= 0 && !twin_here;
-
-    // issue the loads of the point work first: this halfedge's vertex point
-    // and the records of a twin quad outside the block
-    P3 P = zero3();
-    int4 tr0, tr1, tr2, tr3;
-    if (POINTS) {
-        if (live) P = a.old[x.z];
-        const int32_t tq = twin_far ? (twin & ~3) : (h & ~3);
-        tr0 = he4[tq]; tr1 = he4[tq + 1]; tr2 = he4[tq + 2]; tr3 = he4[tq + 3];
-    }
-
-    // record of the previous halfedge of the quad, by DPP
-    const int4 p = quad_rel4<kQuadPrev>(x);
-
-    if (TOPO || UVMODE) {
-        int4 c[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) c[k] = child_record(k, h, x, p, a.V, a.F, a.E);
-        if (UVMODE == 1) {
-            // uvs of this, the next, the opposite and the previous halfedge;
-            // the face average sums them in that order (:2904-2913)
-            const UV uv = uv_decode(x.w);
-            const UV nuv = quad_reluv<kQuadNext>(uv), ouv = quad_reluv<kQuadOpp>(uv), puv = quad_reluv<kQuadPrev>(uv);
-            UV fuv = uv;
-            fuv.u += nuv.u; fuv.v += nuv.v;
-            fuv.u += ouv.u; fuv.v += ouv.v;
-            fuv.u += puv.u; fuv.v += puv.v;
-            fuv.u /= 4.0f;
-            fuv.v /= 4.0f;
-            c[0].w = x.w;
-            c[1].w = uv_encode(uv_mid(uv, nuv));
-            c[2].w = uv_encode(fuv);
-            c[3].w = uv_encode(uv_mid(uv, puv));
-        }
-        int4 *out4 = reinterpret_cast<int4 *>(a.out_he);
-        if (kStaged) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) stage[4 * threadIdx.x + k] = c[k];
-            if (POINTS) spts[threadIdx.x] = P;
-            __syncthreads();
-            const int64_t obase = 4 * (int64_t)base;
-            const int64_t limit = 4 * (int64_t)a.H;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int64_t o = obase + threadIdx.x + (int64_t)k * blockDim.x;
-                if (o < limit) out4[o] = stage[threadIdx.x + k * blockDim.x];
-            }
-        } else if (live) {
-            if (TOPO && (UVMODE == 1 || UVMODE == 2)) {
-#pragma unroll
-                for (int k = 0; k < 4; ++k) out4[4 * (int64_t)h + k] = c[k];
-            } else if (TOPO) {
-                He *o = a.out_he + 4 * (int64_t)h;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) { o[k].twin = c[k].x; o[k].edge = c[k].y; o[k].vert = c[k].z; }
-            } else {
-                He *o = a.out_he + 4 * (int64_t)h;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) o[k].uv = c[k].w;
-            }
-        }
-    }
-
-    if (!POINTS) return;
-    if (!kStaged) {
-        spts[threadIdx.x] = P;
-        __syncthreads();
-    }
-    const P3 q0 = quad_bcast3<0>(P), q1 = quad_bcast3<1>(P), q2 = quad_bcast3<2>(P), q3 = quad_bcast3<3>(P);
-    if (!live) return;
-    const P3 F = face_rule(q0, q1, q2, q3, GATHER);
-    if (j == 0) a.fp[h >> 2] = F;
-    if (!owner) return;
-    // this halfedge owns its edge: both contributions
-    const P3 v1 = pick4(q0, q1, q2, q3, (j + 1) & 3);
-    const float s = CREASED ? lsharp(a.cr, a.C, x.y) : 0.0f;
-    P3 Ev;
-    if (twin >= 0) {
-        P3 t0, t1, t2, t3;
-        if (twin_here) {
-            const int tb = (twin & ~3) - base;
-            t0 = spts[tb]; t1 = spts[tb + 1]; t2 = spts[tb + 2]; t3 = spts[tb + 3];
-        } else {
-            t0 = a.old[tr0.z]; t1 = a.old[tr1.z]; t2 = a.old[tr2.z]; t3 = a.old[tr3.z];
-        }
-        const P3 Ft = face_rule(t0, t1, t2, t3, GATHER);
-        if (GATHER) {
-            const float w = CREASED ? sat(s) : 1.0f;
-            Ev = edge_gather(P, v1, F, Ft, w, CREASED);
-        } else {
-            const int tj = twin & 3;
-            const P3 ct = edge_contrib(pick4(t0, t1, t2, t3, tj), pick4(t0, t1, t2, t3, (tj + 1) & 3), Ft, false, s,
-                                       CREASED);
-            const P3 ch = edge_contrib(P, v1, F, false, s, CREASED);
-            Ev = add(add(zero3(), ct), ch);
-        }
-    } else {
-        if (GATHER) {
-            const float w = CREASED ? sat(s) : 0.0f;
-            Ev = edge_gather(P, v1, F, quad_face(a.he, a.old, 0, true), w, CREASED);
-        } else {
-            Ev = add(zero3(), edge_contrib(P, v1, F, true, s, CREASED));
-        }
-    }
-    a.ep[x.y] = Ev;
+    level_fused_block<GATHER, CREASED, TOPO, UVMODE, POINTS, STORE>(a, xcd_block(blockIdx.x, gridDim.x, a.xcd_remap));
 }
 
 template <int GATHER, int CREASED>
@@ -316,61 +196,23 @@ __global__ __launch_bounds__(256) void k_level_vertices_fan(FusedArgs a, DCage c
     a.vp[v] = fan_vertex_point<GATHER, CREASED, 0>(f, od, old);
 }
 
-// Vertex points with owners compacted per block: a block scans
-// 256 * SCAN halfedges, collects the representatives in LDS, then every
-// thread evaluates owners, so all lanes do ring work.
 template <int GATHER, int CREASED, int SCAN>
 __global__ __launch_bounds__(256) void k_level_vertices_compact(FusedArgs a, DCage c, LevelTab t, LevelPtrs lp,
                                                                 int d) {
-    __shared__ int32_t owners[256 * SCAN];
-    __shared__ int32_t nown;
+    level_vertices_compact_block<GATHER, CREASED, SCAN>(a, c, t, lp, d, xcd_block(blockIdx.x, gridDim.x, a.xcd_remap));
+}
+
+// Vertex points, one thread per vertex of level d with the fan derived from
+// the level the vertex was born at (vertex_fan): no ownership test, vertex
+// points written in order.
+template <int GATHER, int CREASED>
+__global__ __launch_bounds__(256) void k_level_vertices_direct(FusedArgs a, DCage c, LevelTab t, LevelPtrs lp, int d) {
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, a.xcd_remap);
-    const int32_t base = (int32_t)(lb * 256 * SCAN);
-    const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
-    if (threadIdx.x == 0) nown = 0;
-    __syncthreads();
-    int32_t vs[SCAN];
-#pragma unroll
-    for (int k = 0; k < SCAN; ++k) {
-        const int32_t h = base + k * 256 + (int32_t)threadIdx.x;
-        vs[k] = h < a.H ? he4[h].z : -1;
-    }
-#pragma unroll
-    for (int k = 0; k < SCAN; ++k) {
-        const int32_t h = base + k * 256 + (int32_t)threadIdx.x;
-        Fan f;
-        if (vs[k] >= 0 && owner_fan(h, vs[k], d, c, t, lp.he, f)) owners[atomicAdd(&nown, 1)] = h;
-    }
-    __syncthreads();
-    const int32_t count = nown;
-    for (int32_t i = (int32_t)threadIdx.x; i < count; i += 256) {
-        const int32_t h = owners[i];
-        const int32_t v = he4[h].z;
-        Fan f;
-        owner_fan(h, v, d, c, t, lp.he, f);
-        const P3 old = a.old[v];
-        if (f.kind == 1 || f.kind == 3) {
-            const int n = f.n;
-            int32_t r[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) r[q] = fan_at(f, c, q < n ? q : 0);
-            int4 rec[4], prv[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                rec[q] = he4[r[q]];
-                prv[q] = he4[qprev(r[q])];
-            }
-            FanPreloaded pre;
-            pre.e0 = fan_entry(a, r[0], rec[0], prv[0]);
-            pre.e1 = fan_entry(a, r[1], rec[1], prv[1]);
-            pre.e2 = fan_entry(a, r[2], rec[2], prv[2]);
-            pre.e3 = fan_entry(a, r[3], rec[3], prv[3]);
-            a.vp[v] = fan_vertex_point<GATHER, CREASED, 4>(f, pre, old);
-        } else {
-            FanOnDemand od{&a, &f, &c};
-            a.vp[v] = fan_vertex_point<GATHER, CREASED, 0>(f, od, old);
-        }
-    }
+    const int32_t v = (int32_t)(lb * 256 + threadIdx.x);
+    if (v >= a.V) return;
+    Fan f;
+    vertex_fan(v, d, c, t, lp.he, f);
+    fan_vertex_store<GATHER, CREASED>(a, c, f, v);
 }
 
 // ---- launcher ----------------------------------------------------------------
@@ -396,17 +238,32 @@ void launch_level_fused(const FusedArgs &a, int gather, int creased, int topo, i
     launch_fused_impl(a, gather ? 1 : 0, creased ? 1 : 0, topo ? 1 : 0, uvMode, points ? 1 : 0, store, grid, block, st);
 }
 
+template <int SCAN>
+static void launch_compact(const FusedArgs &a, const DCage &c, const LevelTab &t, const LevelPtrs &lp, int d, int gather,
+                           int creased, hipStream_t st) {
+    const dim3 grid((unsigned)((a.H + 256 * SCAN - 1) / (256 * SCAN))), block(256);
+    if (gather && creased) hipLaunchKernelGGL((k_level_vertices_compact<1, 1, SCAN>), grid, block, 0, st, a, c, t, lp, d);
+    else if (gather) hipLaunchKernelGGL((k_level_vertices_compact<1, 0, SCAN>), grid, block, 0, st, a, c, t, lp, d);
+    else if (creased) hipLaunchKernelGGL((k_level_vertices_compact<0, 1, SCAN>), grid, block, 0, st, a, c, t, lp, d);
+    else hipLaunchKernelGGL((k_level_vertices_compact<0, 0, SCAN>), grid, block, 0, st, a, c, t, lp, d);
+}
+
 void launch_level_vertices(const FusedArgs &a, const DCage &c, const LevelTab &t, const LevelPtrs &lp, int d,
-                           int gather, int creased, int fan, hipStream_t st) {
+                           int gather, int creased, int fan, int scan, hipStream_t st) {
     const dim3 block(256);
     const dim3 grid((unsigned)((a.H + 255) / 256));
+    if (fan == 3) {
+        const dim3 g3((unsigned)((a.V + 255) / 256));
+        if (gather && creased) hipLaunchKernelGGL((k_level_vertices_direct<1, 1>), g3, block, 0, st, a, c, t, lp, d);
+        else if (gather) hipLaunchKernelGGL((k_level_vertices_direct<1, 0>), g3, block, 0, st, a, c, t, lp, d);
+        else if (creased) hipLaunchKernelGGL((k_level_vertices_direct<0, 1>), g3, block, 0, st, a, c, t, lp, d);
+        else hipLaunchKernelGGL((k_level_vertices_direct<0, 0>), g3, block, 0, st, a, c, t, lp, d);
+        return;
+    }
     if (fan == 2) {
-        constexpr int kScan = 4;
-        const dim3 g2((unsigned)((a.H + 256 * kScan - 1) / (256 * kScan)));
-        if (gather && creased) hipLaunchKernelGGL((k_level_vertices_compact<1, 1, kScan>), g2, block, 0, st, a, c, t, lp, d);
-        else if (gather) hipLaunchKernelGGL((k_level_vertices_compact<1, 0, kScan>), g2, block, 0, st, a, c, t, lp, d);
-        else if (creased) hipLaunchKernelGGL((k_level_vertices_compact<0, 1, kScan>), g2, block, 0, st, a, c, t, lp, d);
-        else hipLaunchKernelGGL((k_level_vertices_compact<0, 0, kScan>), g2, block, 0, st, a, c, t, lp, d);
+        if (scan == 2) launch_compact<2>(a, c, t, lp, d, gather, creased, st);
+        else if (scan == 8) launch_compact<8>(a, c, t, lp, d, gather, creased, st);
+        else launch_compact<4>(a, c, t, lp, d, gather, creased, st);
         return;
     }
     if (fan) {

@@ -1,6 +1,8 @@
 // Library-internal helpers shared by the host and runtime translation units.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include "../../include/CatmullClarkHIP.h"
 #include "cc_common.h"
 
@@ -9,6 +11,22 @@ namespace cch {
 // thread-local error channel behind ccs_LastError / ccs_LastErrorString
 void set_error(int32_t code, const char *msg);
 void clear_error();
+int32_t last_error();
+// reports a HIP failure on the error channel and stderr; true on error
+bool hip_failed(hipError_t e, const char *what);
+
+// makes `dev` current for the scope
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
 
 // true when every cumulative count the reference evaluates in int32 fits
 bool ref_counts_fit_int32(const cc_Mesh *cage, int32_t D, const char **why);

@@ -33,14 +33,23 @@ struct ccd_Subd {
     hipStream_t stream = nullptr;
     // adjoint scratch (allocated on first use, per scalar size)
     void *adj_a = nullptr, *adj_b = nullptr, *adj_in = nullptr, *adj_out = nullptr, *adj_sharp = nullptr;
-    // deferred vertex list of the patch kernels and one counter per level
-    int32_t *defer = nullptr, *defer_counts = nullptr;
+    // deferred-vertex masks of the patch kernels, one word per 64 halfedges
+    uint64_t *defer_mask = nullptr;
     size_t adj_scalar = 0;
     bool timing = false;
-    int opt_fused = 2, opt_store = cch::kStoreLds, opt_remap = 1, opt_fan = 2;
+    int opt_fused = 1, opt_store = cch::kStoreLds, opt_remap = 1, opt_fan = 2;
+    // per-level kernels of the fused paths (options or ccd_Autotune): kind 1
+    // fused children + face/edge points, then vertex points (scan: halfedges
+    // each thread of the compacted kernel tests); kind 2 patch-local (nt
+    // halfedges per tile, hpt tiles per workgroup) plus deferred vertices
+    struct LevelCfg {
+        int kind = 1, scan = 4, nt = 256, hpt = 1;
+    };
+    LevelCfg cfg[cch::kMaxDepth + 1];
     std::vector<Timed> timed;
 
     bool alloc(const cc_Mesh *cage, int32_t depth);
+    int32_t autotune(int32_t steps, hipStream_t st);
     void free_all();
     bool upload_cage(const cc_Mesh *cage, hipStream_t st);
     cch::DCage cage_view(const cch::P3 *points) const;
@@ -52,6 +61,7 @@ struct ccd_Subd {
     bool refine_topology(int32_t steps, int dmax, hipStream_t st);
     bool refine_points(int32_t steps, int dmax, const cch::P3 *cagePts, cch::P3 *out, hipStream_t st);
     void level_patch(int d, int32_t steps, int topo, int uvMode, cch::P3 *points, hipStream_t st);
+    void level_fused(int d, int32_t steps, int topo, int uvMode, cch::P3 *points, hipStream_t st);
     void begin_timing();
     bool adjoint(const void *gradFinest, void *gradCage, int32_t steps, hipStream_t st);
     void zero(float *p, int64_t count, hipStream_t st);

@@ -38,8 +38,7 @@ void clear_error() {
 extern "C" int32_t ccs_LastError(void) { return t_err; }
 extern "C" const char *ccs_LastErrorString(void) { return t_msg; }
 
-namespace {
-
+namespace cch {
 // reports a HIP failure on the error channel and stderr; returns true on error
 bool hip_failed(hipError_t e, const char *what) {
     if (e == hipSuccess) return false;
@@ -49,21 +48,13 @@ bool hip_failed(hipError_t e, const char *what) {
     set_error(100 + (int32_t)e, buf);
     return true;
 }
+int32_t last_error() { return t_err; }
+}  // namespace cch
+
+namespace {
 
 #define CC_TRY(expr, what) \
     do { if (hip_failed((expr), (what))) return false; } while (0)
-
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev) {
-        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        if (prev != dev) (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard() {
-        int cur = -1;
-        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-    }
-};
 
 constexpr int kBlock = 256;
 
@@ -87,15 +78,15 @@ bool ccd_Subd::alloc(const cc_Mesh *cage, int32_t depth) {
     CC_TRY(hipMalloc(&he, sizeof(He) * (size_t)n.cumH(D)), "hipMalloc(halfedges)");
     CC_TRY(hipMalloc(&cr, sizeof(Crease) * (size_t)n.cumC(D)), "hipMalloc(creases)");
     CC_TRY(hipMalloc(&pts, sizeof(P3) * (size_t)n.cumV(D)), "hipMalloc(points)");
-    CC_TRY(hipMalloc(&defer, sizeof(int32_t) * (size_t)(D >= 2 ? n.V[D - 1] : 1)), "hipMalloc(deferred vertices)");
-    CC_TRY(hipMalloc(&defer_counts, sizeof(int32_t) * (kMaxDepth + 1)), "hipMalloc(deferred counters)");
+    CC_TRY(hipMalloc(&defer_mask, sizeof(uint64_t) * (size_t)(D >= 2 ? (n.H[D - 1] + 63) / 64 : 1)),
+           "hipMalloc(deferred-vertex masks)");
     return true;
 }
 
 void ccd_Subd::free_all() {
     DeviceGuard g(device);
     void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts, adj_a, adj_b, adj_in, adj_out, adj_sharp,
-                    defer, defer_counts};
+                    defer_mask};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : timed) {
@@ -202,6 +193,8 @@ FusedArgs ccd_Subd::fused_args(int d, const P3 *points) const {
     a.ep = a.fp + n.F[d];
     a.H = (int32_t)n.H[d]; a.V = (int32_t)n.V[d]; a.F = (int32_t)n.F[d]; a.E = (int32_t)n.E[d]; a.C = (int32_t)n.C[d];
     a.xcd_remap = opt_remap;
+    a.uv_refine = 0;
+    a.defer_mask = nullptr;
     return a;
 }
 
@@ -240,14 +233,30 @@ void ccd_Subd::level_patch(int d, int32_t steps, int topo, int uvMode, P3 *point
     const bool gather = (steps & CCD_GATHER) != 0;
     const int creased = (steps & CCD_NOCREASES) ? 0 : 1;
     FusedArgs a = fused_args(d, points);
-    a.defer = defer;
-    a.defer_count = defer_counts + d;
+    a.defer_mask = defer_mask;
     const DCage c = cage_view(nullptr);
     const LevelPtrs lp = level_ptrs();
     launch_timed("level_patch", n.H[d], st,
-                 [&] { launch_level_patch(a, c, tab, lp, d, gather, creased, topo, uvMode, opt_store, st); });
-    launch_timed("level_deferred", n.V[d], st,
-                 [&] { launch_level_deferred(a, c, tab, lp, d, gather, creased, n.V[d], st); });
+                 [&] { launch_level_patch(a, c, tab, lp, d, gather, creased, topo, uvMode, cfg[d].nt, cfg[d].hpt, st); });
+    launch_timed("level_deferred", n.V[d], st, [&] { launch_level_deferred(a, c, tab, lp, d, gather, creased, st); });
+}
+
+// one level of the fused kernels: children with face and edge points, then
+// vertex points
+void ccd_Subd::level_fused(int d, int32_t steps, int topo, int uvMode, P3 *points, hipStream_t st) {
+    const bool gather = (steps & CCD_GATHER) != 0;
+    const int creased = (steps & CCD_NOCREASES) ? 0 : 1;
+    const bool pts_ = (steps & CCD_POINTS) != 0;
+    const FusedArgs a = fused_args(d, points);
+    if (topo || uvMode == 1 || pts_)
+        launch_timed(pts_ ? (topo ? "level_fused" : "level_points") : "level_fused", n.H[d], st,
+                     [&] { launch_level_fused(a, gather, creased, topo, uvMode, pts_, opt_store, st); });
+    if (pts_) {
+        const DCage c = cage_view(nullptr);
+        launch_timed("level_vertices", n.H[d], st, [&] {
+            launch_level_vertices(a, c, tab, level_ptrs(), d, gather, creased, opt_fan, cfg[d].scan, st);
+        });
+    }
 }
 
 // the cage step of the points (CatmullClark.h:2633-2636, :2661-2663)
@@ -273,18 +282,11 @@ bool ccd_Subd::refine_points(int32_t steps, int dmax, const P3 *cagePts, P3 *out
     const int creased = (steps & CCD_NOCREASES) ? 0 : 1;
     const DCage c = cage_view(cagePts);
     cage_points(steps, cagePts, out, st);
-    if (opt_fused == 2) zero(reinterpret_cast<float *>(defer_counts), kMaxDepth + 1, st);
     for (int d = 1; d < dmax; ++d) {
-        if (opt_fused == 2) {
-            level_patch(d, steps, 0, 0, out, st);
-            continue;
-        }
         if (opt_fused) {
-            const FusedArgs a = fused_args(d, out);
-            launch_timed("level_points", n.H[d], st,
-                         [&] { launch_level_fused(a, gather, creased, 0, 0, 1, opt_store, st); });
-            launch_timed("level_vertices", n.H[d], st,
-                         [&] { launch_level_vertices(a, c, tab, level_ptrs(), d, gather, creased, opt_fan, st); });
+            const int32_t s = CCD_POINTS | (steps & (CCD_GATHER | CCD_NOCREASES));
+            if (cfg[d].kind == 2) level_patch(d, s, 0, 0, out, st);
+            else level_fused(d, s, 0, 0, out, st);
             continue;
         }
         const Lvl L = level_view(d, out + n.offV[d], out + n.offV[d + 1]);
@@ -326,19 +328,9 @@ bool ccd_Subd::refine(int32_t steps, int dmax, hipStream_t st) {
     const DCage c = cage_view(nullptr);
     if (topo || uvMode == 1) launch("cage_halfedges", k_cage_halfedges, H0, st, c, he, (int)topo, uvMode);
     if (points) cage_points(steps, nullptr, pts, st);
-    if (points && opt_fused == 2) zero(reinterpret_cast<float *>(defer_counts), kMaxDepth + 1, st);
     for (int d = 1; d < dmax; ++d) {
-        if (points && opt_fused == 2) {
-            level_patch(d, steps, topo, uvMode, pts, st);
-            continue;
-        }
-        const FusedArgs a = fused_args(d, pts);
-        if (topo || uvMode == 1 || points)
-            launch_timed("level_fused", n.H[d], st,
-                         [&] { launch_level_fused(a, gather, creased, topo, uvMode, points, opt_store, st); });
-        if (points)
-            launch_timed("level_vertices", n.H[d], st,
-                         [&] { launch_level_vertices(a, c, tab, level_ptrs(), d, gather, creased, opt_fan, st); });
+        if (points && cfg[d].kind == 2) level_patch(d, steps, topo, uvMode, pts, st);
+        else level_fused(d, steps, topo, uvMode, pts, st);
     }
     CC_TRY(hipGetLastError(), "kernel launch");
     return true;
@@ -716,12 +708,90 @@ int32_t ccd_Synchronize(ccd_Subd *s) {
 int32_t ccd_SetOption(ccd_Subd *s, int32_t key, int32_t value) {
     if (!s) return 11;
     switch (key) {
-    case CCD_OPT_FUSED: s->opt_fused = value < 0 ? 0 : (value > 2 ? 2 : value); return 0;
+    case CCD_OPT_FUSED:
+        s->opt_fused = value < 0 ? 0 : (value > 2 ? 2 : value);
+        if (s->opt_fused)
+            for (auto &c : s->cfg) c.kind = s->opt_fused;
+        return 0;
     case CCD_OPT_STORE: s->opt_store = value ? kStoreLds : kStoreDirect; return 0;
     case CCD_OPT_XCD_REMAP: s->opt_remap = value != 0; return 0;
-    case CCD_OPT_FAN: s->opt_fan = value < 0 ? 0 : (value > 2 ? 2 : value); return 0;
+    case CCD_OPT_FAN: s->opt_fan = value < 0 ? 0 : (value > 3 ? 3 : value); return 0;
+    case CCD_OPT_PATCH_TILE:
+        if (value != 128 && value != 256 && value != 512) return 14;
+        for (auto &c : s->cfg) c.nt = value;
+        return 0;
+    case CCD_OPT_PATCH_HPT:
+        if (value < 1 || value > 64) return 14;
+        for (auto &c : s->cfg) c.hpt = value;
+        return 0;
+    case CCD_OPT_SCAN:
+        if (value != 2 && value != 4 && value != 8) return 14;
+        for (auto &c : s->cfg) c.scan = value;
+        return 0;
     default: return 14;
     }
+}
+
+// Times every candidate kernel configuration of the fused paths on this
+// subd with HIP events and keeps, per level, the fastest: the fused kernels
+// with 2, 4 or 8 halfedges tested per thread of the vertex kernel, and the
+// patch-local kernel with tiles of 128, 256 or 512 halfedges.
+int32_t ccd_Subd::autotune(int32_t steps, hipStream_t st) {
+    if (!(steps & CCD_POINTS) || opt_fused == 0 || D < 2) return 0;
+    std::vector<LevelCfg> cands;
+    for (int scan : {2, 4, 8}) cands.push_back({1, scan, 256, 1});
+    for (int nt : {128, 256, 512}) cands.push_back({2, 4, nt, 1});
+    float best[kMaxDepth + 1];
+    LevelCfg pick[kMaxDepth + 1];
+    for (int d = 0; d <= kMaxDepth; ++d) { best[d] = 1e30f; pick[d] = cfg[d]; }
+    const bool was_timing = timing;
+    timing = true;
+    for (const LevelCfg &cand : cands) {
+        for (auto &c : cfg) c = cand;
+        float ms[kMaxDepth + 1] = {0};
+        for (int rep = 0; rep < 3; ++rep) {
+            begin_timing();
+            if (!refine(steps, D, st)) { timing = was_timing; return t_err ? t_err : 1; }
+            if (hip_failed(hipStreamSynchronize(st), "autotune")) { timing = was_timing; return t_err; }
+            if (rep == 0) continue;  // warm-up
+            // level d's kernels are the timed launches between its first
+            // kernel and the next level's; they come in level order
+            int d = 0;
+            for (auto &t : timed) {
+                const bool first = !strcmp(t.name, "level_fused") || !strcmp(t.name, "level_points") ||
+                                   !strcmp(t.name, "level_patch");
+                const bool level_kernel = first || !strcmp(t.name, "level_vertices") || !strcmp(t.name, "level_deferred");
+                if (!level_kernel) continue;
+                if (first) ++d;
+                float x = 0.0f;
+                (void)hipEventElapsedTime(&x, t.start, t.stop);
+                if (d >= 1 && d <= kMaxDepth) ms[d] += x;
+            }
+        }
+        for (int d = 1; d < D; ++d)
+            if (ms[d] < best[d]) { best[d] = ms[d]; pick[d] = cand; }
+    }
+    begin_timing();
+    timing = was_timing;
+    for (int d = 0; d <= kMaxDepth; ++d) cfg[d] = pick[d];
+    return 0;
+}
+
+int32_t ccd_TileOption(const ccd_Subd *s, int32_t key, int32_t level) {
+    if (!s || level < 0 || level > kMaxDepth) return -1;
+    switch (key) {
+    case CCD_OPT_FUSED: return s->opt_fused ? s->cfg[level].kind : 0;
+    case CCD_OPT_PATCH_TILE: return s->cfg[level].nt;
+    case CCD_OPT_PATCH_HPT: return s->cfg[level].hpt;
+    case CCD_OPT_SCAN: return s->cfg[level].scan;
+    default: return -1;
+    }
+}
+
+int32_t ccd_Autotune(ccd_Subd *s, int32_t steps, void *stream) {
+    if (!s) return 11;
+    DeviceGuard g(s->device);
+    return s->autotune(steps, stream ? (hipStream_t)stream : s->stream);
 }
 
 int32_t ccd_SetTiming(ccd_Subd *s, int32_t enable) {

@@ -264,26 +264,96 @@ int32_t ccd_RefinePointsAdjoint(ccd_Subd *subd, const float *devGradFinest, floa
 int32_t ccd_Download(const ccd_Subd *subd, cc_Subd *dst, int32_t steps);
 int32_t ccd_Synchronize(ccd_Subd *subd);
 /* kernel-selection options (all have working defaults):
- *   CCD_OPT_FUSED      2: one patch-local kernel per level for children,
- *                      face, edge and vertex points, plus a small kernel for
- *                      the vertices whose rings leave their block (default);
- *                      1: one fused kernel for children, face and edge
- *                      points plus one for vertex points;
+ *   CCD_OPT_FUSED      1: per level one fused kernel for children, face and
+ *                      edge points plus one for vertex points (default);
+ *                      2: one patch-local kernel per level for children,
+ *                      face, edge and the vertex points whose rings stay in
+ *                      the tile, plus one for the remaining vertices;
  *                      0: one kernel per output kind, vertex scatter with
  *                      float atomics (the reference's own structure)
  *   CCD_OPT_STORE      child-record stores: 0 direct, 1 staged through LDS
  *   CCD_OPT_XCD_REMAP  1: each XCD processes one contiguous range of a level
  *   CCD_OPT_FAN        vertex rings: 2 enumerated from the level each vertex
  *                      appeared at, owners compacted per block (default);
- *                      1 enumerated, one thread per halfedge; 0 ring walks */
-enum { CCD_OPT_FUSED = 0, CCD_OPT_STORE = 1, CCD_OPT_XCD_REMAP = 2, CCD_OPT_FAN = 3 };
+ *                      3 one thread per vertex; 1 enumerated, one thread per
+ *                      halfedge; 0 ring walks
+ *   CCD_OPT_PATCH_TILE halfedges per tile of the patch-local kernel: 128,
+ *                      256 (default) or 512, every level
+ *   CCD_OPT_PATCH_HPT  tiles (halfedges per thread) each workgroup of the
+ *                      patch-local kernel processes in turn: 1 (default)..64
+ *   CCD_OPT_SCAN       halfedges each thread of the compacted vertex kernel
+ *                      tests: 2, 4 (default) or 8 */
+enum {
+    CCD_OPT_FUSED = 0, CCD_OPT_STORE = 1, CCD_OPT_XCD_REMAP = 2, CCD_OPT_FAN = 3,
+    CCD_OPT_PATCH_TILE = 4, CCD_OPT_PATCH_HPT = 5, CCD_OPT_SCAN = 6
+};
 int32_t ccd_SetOption(ccd_Subd *subd, int32_t key, int32_t value);
+/* Picks, per level, the kernel configuration of the fused paths by timing
+ * every candidate on this subd: the fused kernels with 2, 4 or 8 halfedges
+ * tested per thread of the vertex kernel, and the patch-local kernel with
+ * tiles of 128, 256 or 512 halfedges.  Runs the given steps a few times per
+ * candidate on the stream, synchronizing it; the choice is kept until the
+ * next ccd_SetOption of these keys.  Returns 0 or an error code. */
+int32_t ccd_Autotune(ccd_Subd *subd, int32_t steps, void *stream);
+/* the value of CCD_OPT_FUSED (1 or 2), _PATCH_TILE, _PATCH_HPT or _SCAN used
+ * at one level, or -1 */
+int32_t ccd_TileOption(const ccd_Subd *subd, int32_t key, int32_t level);
 /* per-kernel timing with HIP events on the launch stream: enable, run, then
  * read the kernel names and mean milliseconds of the last ccd_Refine. */
 int32_t ccd_SetTiming(ccd_Subd *subd, int32_t enable);
 int32_t ccd_TimingCount(const ccd_Subd *subd);
 const char *ccd_TimingName(const ccd_Subd *subd, int32_t i);
 float ccd_TimingMs(const ccd_Subd *subd, int32_t i);
+
+/* ---- batched refinement (extension; no counterpart in the reference) ------
+ * Many cages, each with its own depth, refined together: every kernel kind
+ * runs once per level over all cages (64-bit offsets between cages), and the
+ * whole sequence is captured once per set of steps into a hipGraph that later
+ * calls replay.  Cage i's levels 1..D_i keep the cc_Subd layout inside the
+ * shared device arrays (ccb_DevicePointer) at ccb_Offset(b, what, i).
+ * A batched refinement always computes points (CCD_POINTS required);
+ * CCD_HALFEDGES refines uvIDs too (written 0 for cages without uvs). */
+typedef struct ccb_Batch ccb_Batch;
+enum {
+    CCB_COUNT_HALFEDGES = 0,      /* levels 1..D, cumulative */
+    CCB_COUNT_CREASES = 1,
+    CCB_COUNT_POINTS = 2,
+    CCB_COUNT_CAGE_POINTS = 3,
+    CCB_COUNT_FINEST_POINTS = 4,  /* points of level D; the layout of gradients */
+    CCB_COUNT_DEPTH = 5,
+    CCB_COUNT_CAGES = 6
+};
+enum { CCB_OPT_GRAPH = 100 };  /* 1: capture and replay hipGraphs (default) */
+ccb_Batch *ccb_Create(const cc_Mesh *const *cages, const int32_t *depths, int32_t count);
+void ccb_Release(ccb_Batch *batch);
+/* counts of cage i, or totals over the batch for cage = -1 (depth: maximum) */
+int64_t ccb_Count(const ccb_Batch *batch, int32_t what, int32_t cage);
+/* element offset of cage i's slice in the array `what` names */
+int64_t ccb_Offset(const ccb_Batch *batch, int32_t what, int32_t cage);
+void *ccb_DevicePointer(const ccb_Batch *batch, int32_t which);   /* CCD_PTR_* */
+int32_t ccb_Refine(ccb_Batch *batch, int32_t steps, void *stream);
+/* points only, from new cage points (device, ccb_Count(CAGE_POINTS) x 3
+ * floats in cage order) or, with NULL, the current ones */
+int32_t ccb_RefinePoints(ccb_Batch *batch, const float *devCagePoints, int32_t steps, void *stream);
+/* copies every cage's finest points into devOut (FINEST_POINTS x 3 floats) */
+int32_t ccb_FinestPoints(ccb_Batch *batch, float *devOut, void *stream);
+/* adjoint of the points: FINEST_POINTS x 3 gradients in, CAGE_POINTS x 3 out */
+int32_t ccb_RefinePointsAdjoint(ccb_Batch *batch, const float *devGradFinest, float *devGradCage,
+                                int32_t steps, void *stream);
+int32_t ccb_Download(const ccb_Batch *batch, int32_t cage, cc_Subd *dst, int32_t steps);
+int32_t ccb_Synchronize(ccb_Batch *batch);
+/* CCB_OPT_GRAPH, CCD_OPT_XCD_REMAP, CCD_OPT_PATCH_TILE, CCD_OPT_PATCH_HPT */
+int32_t ccb_SetOption(ccb_Batch *batch, int32_t key, int32_t value);
+/* per-level choice of tile size and tiles per workgroup by timing every
+ * candidate on this batch (see ccd_Autotune) */
+int32_t ccb_Autotune(ccb_Batch *batch, int32_t steps, void *stream);
+int32_t ccb_TileOption(const ccb_Batch *batch, int32_t key, int32_t level);
+/* per-launch timing of eager (non-graph) runs */
+int32_t ccb_SetTiming(ccb_Batch *batch, int32_t enable);
+int32_t ccb_TimingCount(const ccb_Batch *batch);
+const char *ccb_TimingName(const ccb_Batch *batch, int32_t i);
+int32_t ccb_TimingLevel(const ccb_Batch *batch, int32_t i);
+float ccb_TimingMs(const ccb_Batch *batch, int32_t i);
 
 #ifdef __cplusplus
 } /* extern "C" */

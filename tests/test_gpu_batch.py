@@ -1,0 +1,272 @@
+"""Batched refinement (ccb_*): many cages of uneven size and depth in one
+launch per kernel kind and level, replayed as a hipGraph.
+
+Every cage of a batch must come out exactly as the same cage refined alone
+through ccd_* (same kernels, same operation order: bytes equal, scatter
+points included), and so match the oracle as the single-cage path does
+(topology, uvIDs, creases and gather points bit-exact; scatter points within
+1e-5 of the mesh scale of the double-precision oracle).  The batched adjoint
+uses float atomics, so it is compared with the single-cage adjoint to a
+relative 1e-5.
+"""
+import numpy as np
+import pytest
+
+from halfedgecatmullclark_amd import cage as C
+from oracle import oracle as O
+from tests.helpers import assert_bytes_equal, point_error, small_cages
+
+pytestmark = pytest.mark.gpu
+
+POINT_TOL = 1e-5
+ADJ_TOL = 1e-5
+
+
+def tiny_cages():
+    """One quad and one triangle: levels much shorter than a workgroup."""
+    quad = C.from_faces(np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0.2], [0, 1, 0]], np.float32), [[0, 1, 2, 3]])
+    tri = C.from_faces(np.array([[0, 0, 0], [1, 0, 0.1], [0, 1, 0]], np.float32), [[0, 1, 2]])
+    return [(quad, 2), (tri, 2), (quad, 3)]
+
+
+def batch_cases():
+    cases = [(c, d) for _, c, d in small_cages()]
+    cases += [(C.cube(0), 1), (C.torus(6, 5, seed=2), 2), (C.irregular(2, seed=8), 1)]
+    return cases + tiny_cages()
+
+
+@pytest.mark.parametrize("k", range(3))
+def test_tiny_cages_single(capi, k):
+    cage, depth = tiny_cages()[k]
+    for steps, flags in ((15, O.ALL), (15 | 16, O.ALL | O.GATHER)):
+        ds = capi.DeviceSubd(cage, depth)
+        ds.refine(steps)
+        got = ds.download()
+        want = O.refine(cage, depth, flags)
+        assert_bytes_equal(got["halfedges"], want["halfedges"], "halfedges")
+        assert_bytes_equal(got["creases"], want["creases"], "creases")
+        if flags & O.GATHER:
+            assert_bytes_equal(got["points"], want["points"], "points")
+        else:
+            ref64 = O.refine(cage, depth, flags, real=np.float64)["points"]
+            assert point_error(got["points"], ref64)[0] <= POINT_TOL
+        ds.close()
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from halfedgecatmullclark_amd import capi as api
+    api.lib()
+    return api
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert t.cuda.is_available()
+    return t
+
+
+STEPS = [
+    ("scatter", 15, O.ALL),
+    ("gather", 15 | 16, O.ALL | O.GATHER),
+    ("nocreases_scatter", 15 | 32, O.ALL | O.NOCREASES),
+    ("nocreases_gather", 15 | 16 | 32, O.ALL | O.GATHER | O.NOCREASES),
+]
+
+
+def download_all(b):
+    return [b.download(i) for i in range(len(b))]
+
+
+@pytest.mark.parametrize("label,steps,flags", STEPS, ids=[s[0] for s in STEPS])
+def test_batch_matches_oracle(capi, label, steps, flags):
+    cases = batch_cases()
+    b = capi.Batch([c for c, _ in cases], [d for _, d in cases])
+    b.refine(steps)
+    b.synchronize()
+    for i, (cage, depth) in enumerate(cases):
+        got = b.download(i)
+        want = O.refine(cage, depth, flags)
+        assert_bytes_equal(got["halfedges"], want["halfedges"], f"cage {i}/{label} halfedges")
+        assert_bytes_equal(got["creases"], want["creases"], f"cage {i}/{label} creases")
+        if flags & O.GATHER:
+            assert_bytes_equal(got["points"], want["points"], f"cage {i}/{label} points")
+        else:
+            ref64 = O.refine(cage, depth, flags, real=np.float64)["points"]
+            assert point_error(got["points"], ref64)[0] <= POINT_TOL, (i, label)
+    b.close()
+
+
+@pytest.mark.parametrize("label,steps,flags", STEPS[:2], ids=[s[0] for s in STEPS[:2]])
+def test_batch_equals_single_cage(capi, label, steps, flags):
+    cases = batch_cases()
+    b = capi.Batch([c for c, _ in cases], [d for _, d in cases])
+    b.refine(steps)
+    got = download_all(b)
+    for i, (cage, depth) in enumerate(cases):
+        ds = capi.DeviceSubd(cage, depth)
+        ds.refine(steps)
+        want = ds.download()
+        for k in ("halfedges", "creases", "points"):
+            assert_bytes_equal(got[i][k], want[k], f"cage {i}/{label} {k}")
+        ds.close()
+    b.close()
+
+
+@pytest.mark.parametrize("key,value", [("kind", 2), ("scan", 2), ("scan", 8), ("tile", 128), ("tile", 512),
+                                       ("remap", 0)])
+def test_batch_kernel_options_agree(capi, key, value):
+    cases = batch_cases()
+    b = capi.Batch([c for c, _ in cases], [d for _, d in cases])
+    b.refine(capi.ALL)
+    want = download_all(b)
+    if key == "kind":
+        b.set_option(capi.OPT_FUSED, value)
+    elif key == "scan":
+        b.set_option(capi.OPT_SCAN, value)
+    elif key == "tile":
+        b.set_option(capi.OPT_FUSED, 2)
+        b.set_option(capi.OPT_PATCH_TILE, value)
+    else:
+        b.set_option(capi.OPT_XCD_REMAP, value)
+    b.refine(capi.ALL)
+    got = download_all(b)
+    for i in range(len(cases)):
+        for k in ("halfedges", "creases", "points"):
+            assert_bytes_equal(got[i][k], want[i][k], f"{key}={value} cage {i} {k}")
+    b.close()
+
+
+def test_graph_replays_match_eager(capi, torch):
+    cases = batch_cases()
+    cages = [c for c, _ in cases]
+    depths = [d for _, d in cases]
+    eager = capi.Batch(cages, depths)
+    eager.set_option(capi.OPT_GRAPH, 0)
+    eager.refine(capi.ALL)
+    want = download_all(eager)
+    eager.close()
+    b = capi.Batch(cages, depths)
+    other = capi.DeviceSubd(C.torus(8, 4), 3)
+    stream = torch.cuda.current_stream().cuda_stream
+    for rep in range(4):
+        # unrelated eager work and synchronizations between the replays
+        x = torch.randn(1 << 16, device="cuda")
+        _ = (x * 2).sum().item()
+        other.refine(capi.ALL, stream)
+        b.refine(capi.ALL, stream)
+        torch.cuda.synchronize()
+        got = download_all(b)
+        for i in range(len(cases)):
+            for k in ("halfedges", "creases", "points"):
+                assert_bytes_equal(got[i][k], want[i][k], f"replay {rep} cage {i} {k}")
+    other.close()
+    b.close()
+
+
+def test_batch_refine_points_and_finest(capi, torch):
+    cases = batch_cases()
+    cages = [c for c, _ in cases]
+    depths = [d for _, d in cases]
+    b = capi.Batch(cages, depths)
+    b.refine(capi.ALL)
+    rng = np.random.default_rng(7)
+    moved = [c.vertex_points + rng.normal(0.0, 0.02, c.vertex_points.shape).astype(np.float32) for c in cages]
+    dev = torch.from_numpy(np.concatenate(moved).astype(np.float32)).cuda()
+    assert dev.shape[0] == b.count(capi.COUNT_CAGE_POINTS)
+    b.refine_points(dev.data_ptr(), 0)
+    fin = torch.empty((b.count(capi.COUNT_FINEST_POINTS), 3), dtype=torch.float32, device="cuda")
+    b.finest_points(fin.data_ptr())
+    b.synchronize()
+    fin = fin.cpu().numpy()
+    for i, (cage, depth) in enumerate(cases):
+        ds = capi.DeviceSubd(cage, depth)
+        ds.refine(capi.ALL)
+        ds.upload_cage_points(moved[i])
+        ds.refine(capi.POINTS)
+        want = ds.download()
+        got = b.download(i)
+        assert_bytes_equal(got["points"], want["points"], f"cage {i} points")
+        n = want["counts"]
+        o = b.offset(capi.COUNT_FINEST_POINTS, i)
+        nf = n["V"][depth]
+        assert_bytes_equal(fin[o:o + nf], want["points"][-nf:], f"cage {i} finest points")
+        ds.close()
+    b.close()
+
+
+def test_batch_adjoint_matches_single_cage(capi, torch):
+    cases = batch_cases()
+    cages = [c for c, _ in cases]
+    depths = [d for _, d in cases]
+    b = capi.Batch(cages, depths)
+    b.refine(capi.ALL)
+    nf = b.count(capi.COUNT_FINEST_POINTS)
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    g = torch.randn((nf, 3), generator=gen, device="cuda", dtype=torch.float32)
+    out = torch.empty((b.count(capi.COUNT_CAGE_POINTS), 3), device="cuda", dtype=torch.float32)
+    b.refine_points_adjoint(g.data_ptr(), out.data_ptr(), 0)
+    b.synchronize()
+    out = out.cpu().numpy()
+    g_host = g.cpu().numpy()
+    for i, (cage, depth) in enumerate(cases):
+        ds = capi.DeviceSubd(cage, depth)
+        ds.refine(capi.ALL)
+        o = b.offset(capi.COUNT_FINEST_POINTS, i)
+        n = b.count(capi.COUNT_FINEST_POINTS, i)
+        gi = torch.from_numpy(np.ascontiguousarray(g_host[o:o + n])).cuda()
+        oi = torch.empty((cage.V, 3), device="cuda", dtype=torch.float32)
+        ds.refine_points_adjoint(gi.data_ptr(), oi.data_ptr(), 0)
+        ds.synchronize()
+        want = oi.cpu().numpy()
+        oc = b.offset(capi.COUNT_CAGE_POINTS, i)
+        got = out[oc:oc + cage.V]
+        scale = max(float(np.abs(want).max()), 1e-30)
+        assert float(np.abs(got - want).max()) / scale <= ADJ_TOL, i
+        ds.close()
+    b.close()
+
+
+def test_batch_autotune_keeps_results(capi):
+    cases = batch_cases()
+    b = capi.Batch([c for c, _ in cases], [d for _, d in cases])
+    b.refine(capi.ALL)
+    want = download_all(b)
+    b.autotune(capi.ALL)
+    opts = b.level_options()
+    assert len(opts) == max(d for _, d in cases) - 1
+    for o in opts:
+        assert o["kind"] in (1, 2) and o["scan"] in (2, 4, 8) and o["tile"] in (128, 256, 512)
+    b.refine(capi.ALL)
+    got = download_all(b)
+    for i in range(len(cases)):
+        for k in ("halfedges", "creases", "points"):
+            assert_bytes_equal(got[i][k], want[i][k], f"autotuned cage {i} {k}")
+    b.close()
+
+
+def test_single_cage_autotune_keeps_results(capi):
+    cage, depth = C.torus(16, 8, seed=1), 5
+    ds = capi.DeviceSubd(cage, depth)
+    ds.refine(capi.ALL)
+    want = ds.download()
+    ds.autotune(capi.ALL)
+    assert len(ds.level_options()) == depth - 1
+    ds.refine(capi.ALL)
+    got = ds.download()
+    for k in ("halfedges", "creases", "points"):
+        assert_bytes_equal(got[k], want[k], f"autotuned {k}")
+    ds.close()
+
+
+def test_batch_errors(capi):
+    cases = batch_cases()[:2]
+    b = capi.Batch([c for c, _ in cases], [d for _, d in cases])
+    with pytest.raises(capi.CCError):
+        b.refine(capi.HALFEDGES)            # points are part of every batched refinement
+    with pytest.raises(capi.CCError):
+        b.refine_points(None, 0)            # topology not refined yet
+    with pytest.raises(capi.CCError):
+        b.set_option(capi.OPT_PATCH_TILE, 100)
+    b.close()

@@ -61,6 +61,7 @@ def test_refine_drivers(capi, name, cage, depth, driver, flags):
 
 
 PATHS = [  # (fused, store, xcd remap, vertex rings)
+    (1, 1, 1, 3),
     (2, 1, 1, 2),
     (2, 0, 0, 2),
     (1, 1, 1, 2),
@@ -76,7 +77,7 @@ STEPS = [
 ]
 
 
-@pytest.mark.parametrize("fused,store,remap,fan", PATHS, ids=["patch_lds", "patch_direct", "fused_compact", "fused_fan", "fused_direct_walk", "per_element"])
+@pytest.mark.parametrize("fused,store,remap,fan", PATHS, ids=["fused_vertexdirect", "patch_lds", "patch_direct", "fused_compact", "fused_fan", "fused_direct_walk", "per_element"])
 @pytest.mark.parametrize("name,cage,depth", small_cages(), ids=[c[0] for c in small_cages()])
 def test_device_paths(capi, name, cage, depth, fused, store, remap, fan):
     ds = capi.DeviceSubd(cage, depth)
