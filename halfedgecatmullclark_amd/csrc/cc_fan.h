@@ -190,6 +190,22 @@ __device__ __forceinline__ int32_t fan_at(const Fan &f, const DCage &c, int i) {
     }
 }
 
+// Is h the representative of its vertex (CatmullClark.h:1336)?  The test of
+// owner_fan without building the fan (no twin record is read).
+__device__ __forceinline__ bool owner_test(int32_t h, int32_t vert, int d, const DCage &c, const LevelTab &t) {
+    int k = 0;
+    while (k < d - 1 && ((h >> (2 * k)) & 3) == 0) ++k;
+    const int32_t q = h >> (2 * k);
+    const int lvl = d - k;
+    const int digit = q & 3;
+    const int32_t p = q >> 2;
+    if (digit == 0) return c.v2h[vert] == p;
+    if (digit == 3) return false;
+    if (digit == 2) return lvl - 1 >= 1 ? (p & 3) == 0 : c.f2h[c.he[p].face] == p;
+    const int32_t e = vert - t.V[lvl - 1] - t.F[lvl - 1];
+    return lvl - 1 >= 1 ? edge_rep(c, t, e, lvl - 1) == p : c.e2h[e] == p;
+}
+
 // Is h the representative of its vertex?  If so, describe the fan.
 __device__ __forceinline__ bool owner_fan(int32_t h, int32_t vert, int d, const DCage &c, const LevelTab &t,
                                           const He *const *levels, Fan &f) {
@@ -472,7 +488,39 @@ template <int GATHER, int CREASED>
 __device__ __forceinline__ void fan_vertex_store(const FusedArgs &a, const DCage &c, const Fan &f, int32_t v) {
     const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
     const P3 old = a.old[v];
-    if (f.kind == 1 || f.kind == 3) {
+    if (f.kind == 1 || (f.kind == 3 && !f.boundary)) {
+        // a closed fan of four in forward order: r[q+1] = twin(prev(r[q])),
+        // so the previous edge of r[q] is the edge of r[q+1] and only the
+        // four fan records are read
+        int32_t r[4];
+        int4 rec[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[q] = fan_at(f, c, q);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) rec[q] = he4[r[q]];
+        P3 E[4], F[4];
+        float s[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            E[q] = a.ep[rec[q].y];
+            F[q] = a.fp[r[q] >> 2];
+            s[q] = lsharp(a.cr, a.C, rec[q].y);
+        }
+        FanEntry e[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int n1 = (q + 1) & 3;
+            e[q].re = rec[q].y;
+            e[q].pe = rec[n1].y;
+            e[q].E = E[q];
+            e[q].PE = E[n1];
+            e[q].F = F[q];
+            e[q].ts = s[q];
+            e[q].ps = s[n1];
+        }
+        FanPreloaded pre{e[0], e[1], e[2], e[3]};
+        a.vp[v] = fan_vertex_point<GATHER, CREASED, 4>(f, pre, old);
+    } else if (f.kind == 3) {
         int32_t r[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) r[q] = fan_at(f, c, q < f.n ? q : 0);

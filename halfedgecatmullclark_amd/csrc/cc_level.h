@@ -158,8 +158,7 @@ __device__ __forceinline__ void level_vertices_compact_block(const FusedArgs &a,
 #pragma unroll
     for (int k = 0; k < SCAN; ++k) {
         const int32_t h = base + k * 256 + (int32_t)threadIdx.x;
-        Fan f;
-        if (vs[k] >= 0 && owner_fan(h, vs[k], d, c, t, lp.he, f)) owners[atomicAdd(&nown, 1)] = h;
+        if (vs[k] >= 0 && owner_test(h, vs[k], d, c, t)) owners[atomicAdd(&nown, 1)] = h;
     }
     __syncthreads();
     const int32_t count = nown;

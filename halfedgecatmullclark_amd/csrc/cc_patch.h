@@ -208,27 +208,7 @@ __device__ __forceinline__ void deferred_vertex(const FusedArgs &a, const DCage 
     const int32_t v = he4[h].z;
     Fan f;
     owner_fan(h, v, d, c, t, lp.he, f);
-    const P3 old = a.old[v];
-    if (f.kind == 1 || f.kind == 3) {
-        int32_t r[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) r[q] = fan_at(f, c, q < f.n ? q : 0);
-        int4 rec[4], prv[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            rec[q] = he4[r[q]];
-            prv[q] = he4[qprev(r[q])];
-        }
-        FanPreloaded pre;
-        pre.e0 = fan_entry(a, r[0], rec[0], prv[0]);
-        pre.e1 = fan_entry(a, r[1], rec[1], prv[1]);
-        pre.e2 = fan_entry(a, r[2], rec[2], prv[2]);
-        pre.e3 = fan_entry(a, r[3], rec[3], prv[3]);
-        a.vp[v] = fan_vertex_point<GATHER, CREASED, 4>(f, pre, old);
-    } else {
-        FanOnDemand od{&a, &f, &c};
-        a.vp[v] = fan_vertex_point<GATHER, CREASED, 0>(f, od, old);
-    }
+    fan_vertex_store<GATHER, CREASED>(a, c, f, v);
 }
 
 // Source mask words per wave of the deferred kernel: about a quarter of the
