@@ -178,6 +178,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched comparison")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the timed region, copy 1 GiB between two device buffers twice (a kernel of "
+                         "known traffic for scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the refinement of one cage runs on one device; "
@@ -243,11 +246,13 @@ def main():
     else:
         nbytes = None
     achieved = nbytes / (mean_ms * 1e-3) / 1e9 if nbytes else None
+    # HBM bytes per launch of this kernel at this level, from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by a 1 GiB copy)
     traffic = None
-    prof = os.path.join(ROOT, "profiles", "pmc_level_fused_torus64x32_d8.json")
+    prof = os.path.join(ROOT, "profiles", "traffic_torus64x32_d8.json")
     if os.path.exists(prof):
         try:
-            traffic = json.load(open(prof)).get("traffic_bytes_per_launch")
+            traffic = json.load(open(prof)).get("kernels", {}).get(f"{name}@{level}", {}).get("traffic_bytes")
         except Exception:
             traffic = None
 
@@ -284,6 +289,13 @@ def main():
     sub.close()
 
     ds.close()
+    if args.copy_calibration:
+        a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(2):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     batch = None if args.no_batch else batched(torch, capi, C, args.steps, stream)
     cpu = None if args.no_cpu else cpu_baseline()
 

@@ -45,6 +45,14 @@ def lib() -> ctypes.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise CCError(f"{LIB_PATH} is missing: build it with `python -m halfedgecatmullclark_amd.build`")
+    # One HIP runtime per process: when torch is installed, load it first so
+    # that the library binds to the libamdhip64.so.7 torch brings instead of
+    # a second copy from /opt/rocm (two runtimes in one process leave torch
+    # without devices).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     P = ctypes.POINTER
     vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64

@@ -50,7 +50,7 @@ struct OpCageVerticesScatter {
     int creased;
     __device__ void operator()(const CageJob &J, int64_t i) const {
         const P3 *fp = J.pts1 + J.c.V, *ep = fp + J.c.F;
-        if (i < J.c.H) cage_vertices_scatter_at(J.c, fp, ep, J.pts1, creased, (int32_t)i);
+        if (i < J.c.V) cage_vertices_scatter_at(J.c, fp, ep, J.pts1, creased, (int32_t)i);
     }
 };
 

@@ -401,12 +401,7 @@ void ccb_Batch::record_forward(int32_t steps, hipStream_t st) {
     timed_launch("cage_points", 0, st, [&] {
         batch_cage_faces(t_cage_f, gather, st);
         batch_cage_edges(t_cage_e, gather, creased, st);
-        if (gather) {
-            batch_cage_vertices(t_cage_v, 1, creased, st);
-        } else {
-            batch_zero_cage_vertices(t_cage_zero, st);
-            batch_cage_vertices(t_cage_h3, 0, creased, st);
-        }
+        batch_cage_vertices(t_cage_v, gather, creased, st);
     });
     for (int d = 1; d < Dmax; ++d) {
         if (cfg[d].kind == 2) {
@@ -595,7 +590,7 @@ int32_t ccb_Refine(ccb_Batch *b, int32_t steps, void *stream) {
     steps &= CCD_HALFEDGES | CCD_CREASES | CCD_POINTS | CCD_GATHER | CCD_NOCREASES;
     if (steps & CCD_HALFEDGES) steps |= CCD_UVS;
     DeviceGuard g(b->device);
-    hipStream_t st = stream ? (hipStream_t)stream : b->stream;
+    hipStream_t st = (hipStream_t)stream;
     if (!b->run(steps, false, steps, st)) return last_error();
     if (steps & CCD_HALFEDGES) b->topo_done = true;
     if (steps & CCD_CREASES) b->creases_done = true;
@@ -609,7 +604,7 @@ int32_t ccb_RefinePoints(ccb_Batch *b, const float *devCagePoints, int32_t steps
     const bool creased = !(steps & CCD_NOCREASES);
     if (creased && !b->creases_done) return fail(16, "ccb_RefinePoints: creases not refined yet");
     DeviceGuard g(b->device);
-    hipStream_t st = stream ? (hipStream_t)stream : b->stream;
+    hipStream_t st = (hipStream_t)stream;
     if (devCagePoints &&
         hip_failed(hipMemcpyAsync(b->c_pts, devCagePoints, sizeof(P3) * (size_t)b->tot_v0, hipMemcpyDeviceToDevice, st),
                    "ccb_RefinePoints"))
@@ -622,7 +617,7 @@ int32_t ccb_FinestPoints(ccb_Batch *b, float *devOut, void *stream) {
     clear_error();
     if (!b || !devOut) return fail(11, "ccb_FinestPoints: null argument");
     DeviceGuard g(b->device);
-    hipStream_t st = stream ? (hipStream_t)stream : b->stream;
+    hipStream_t st = (hipStream_t)stream;
     batch_copy(b->t_fin, reinterpret_cast<const float *>(b->pts), devOut, st);
     return hip_failed(hipGetLastError(), "ccb_FinestPoints") ? last_error() : 0;
 }
@@ -636,7 +631,7 @@ int32_t ccb_RefinePointsAdjoint(ccb_Batch *b, const float *devGradFinest, float 
     if (creased && !b->creases_done) return fail(16, "ccb_RefinePointsAdjoint: creases not refined yet");
     DeviceGuard g(b->device);
     if (!b->alloc_adjoint()) return last_error();
-    hipStream_t st = stream ? (hipStream_t)stream : b->stream;
+    hipStream_t st = (hipStream_t)stream;
     if (hip_failed(hipMemcpyAsync(b->adjA, devGradFinest, sizeof(float) * 3 * (size_t)b->tot_fin,
                                   hipMemcpyDeviceToDevice, st), "ccb_RefinePointsAdjoint"))
         return last_error();
@@ -653,6 +648,8 @@ int32_t ccb_Download(const ccb_Batch *b, int32_t cage, cc_Subd *dst, int32_t ste
     const ccb_Batch::Cage &g = b->cages[cage];
     if (dst->maxDepth != g.D) return fail(13, "ccb_Download: depth mismatch");
     DeviceGuard gd(b->device);
+    // the refinement may have been queued on any stream
+    if (hip_failed(hipDeviceSynchronize(), "ccb_Download")) return last_error();
     hipStream_t st = b->stream;
     if ((steps & (CCD_HALFEDGES | CCD_UVS)) &&
         hip_failed(hipMemcpyAsync(dst->halfedges, b->he + g.o_he, sizeof(He) * (size_t)g.n.cumH(g.D),
@@ -707,7 +704,7 @@ int32_t ccb_Autotune(ccb_Batch *b, int32_t steps, void *stream) {
     if (!b) return fail(11, "ccb_Autotune: null batch");
     if (!(steps & CCD_POINTS)) return fail(15, "ccb_Autotune: add CCD_POINTS");
     DeviceGuard g(b->device);
-    hipStream_t st = stream ? (hipStream_t)stream : b->stream;
+    hipStream_t st = (hipStream_t)stream;
     steps &= CCD_HALFEDGES | CCD_CREASES | CCD_POINTS | CCD_GATHER | CCD_NOCREASES;
     if (!(steps & CCD_HALFEDGES) && !b->topo_done) return fail(16, "ccb_Autotune: halfedges not refined yet");
     const int32_t r = b->autotune(steps, st);

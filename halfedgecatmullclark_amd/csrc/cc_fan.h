@@ -348,10 +348,12 @@ struct FanEntry {
     P3 E, PE, F;
     float ts, ps;
     int32_t re, pe;
+    int32_t r;   // the fan halfedge itself (orders the scatter sum)
 };
 
 __device__ __forceinline__ FanEntry fan_entry(const FusedArgs &a, int32_t r, int4 rec, int4 prv) {
     FanEntry e;
+    e.r = r;
     e.re = rec.y;
     e.pe = prv.y;
     e.E = a.ep[e.re];
@@ -367,6 +369,7 @@ __device__ __forceinline__ FanEntry fan_entry(const FusedArgs &a, int32_t r, int
 struct FanPreloaded {
     FanEntry e0, e1, e2, e3;
     __device__ __forceinline__ FanEntry get(int i) const { return pick4(e0, e1, e2, e3, i); }
+    __device__ __forceinline__ int32_t key(int i) const { return pick4(e0.r, e1.r, e2.r, e3.r, i); }
 };
 
 // any fan, entries loaded when asked for
@@ -379,6 +382,7 @@ struct FanOnDemand {
         const int4 *he4 = reinterpret_cast<const int4 *>(a->he);
         return fan_entry(*a, r, he4[r], he4[qprev(r)]);
     }
+    __device__ __forceinline__ int32_t key(int i) const { return fan_at(*f, *c, i); }
 };
 
 // The vertex rules over a fan (see k_level_vertices for the reference walk
@@ -472,12 +476,48 @@ __device__ __forceinline__ P3 fan_vertex_point(const Fan &f, const Acc &acc, P3 
             rp.sall += se;
         }
     }
+    // contributions added from zero in ascending halfedge order, the order
+    // of the reference's sequential scatter (CatmullClark.h:2345-2347,
+    // :2610-2611)
     P3 sum = zero3();
-    for (int i = 0; i < lim; ++i) {
-        if (i >= n) break;
-        const FanEntry e = acc.get(i);
+    if (MAXN == 4) {
+        // fans of at most four: every contribution in fan order, the rank of
+        // each fan halfedge among the fan, then the sum by rank
+        P3 con[4];
+        int32_t key[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const FanEntry e = acc.get(i);
+            con[i] = vertex_scatter_contrib(rp, old, e.E, e.PE, e.F, CREASED ? e.ts : 0.0f, CREASED ? e.ps : 0.0f,
+                                            e.re, CREASED);
+            key[i] = i < n ? e.r : INT32_MAX;
+        }
+        int rank[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            rank[i] = (key[0] < key[i]) + (key[1] < key[i]) + (key[2] < key[i]) + (key[3] < key[i]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k >= n) break;
+            const P3 ck = rank[0] == k ? con[0] : (rank[1] == k ? con[1] : (rank[2] == k ? con[2] : con[3]));
+            sum = add(sum, ck);
+        }
+        return sum;
+    }
+    int32_t last = INT32_MIN;
+    for (int k = 0; k < lim; ++k) {
+        if (k >= n) break;
+        int32_t bk = INT32_MAX;
+        int bi = 0;
+        for (int i = 0; i < lim; ++i) {
+            if (i >= n) break;
+            const int32_t r = acc.key(i);
+            if (r > last && r < bk) { bk = r; bi = i; }
+        }
+        const FanEntry e = acc.get(bi);
         sum = add(sum, vertex_scatter_contrib(rp, old, e.E, e.PE, e.F, CREASED ? e.ts : 0.0f, CREASED ? e.ps : 0.0f,
                                               e.re, CREASED));
+        last = bk;
     }
     return sum;
 }
@@ -510,6 +550,7 @@ __device__ __forceinline__ void fan_vertex_store(const FusedArgs &a, const DCage
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int n1 = (q + 1) & 3;
+            e[q].r = r[q];
             e[q].re = rec[q].y;
             e[q].pe = rec[n1].y;
             e[q].E = E[q];

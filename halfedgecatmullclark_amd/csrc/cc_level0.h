@@ -228,10 +228,9 @@ __device__ inline void cage_vertices_gather_at(DCage c, const P3 *fp, const P3 *
     else vp[v] = lerp(old, cr, sat(avgS * 0.5f));
 }
 
-// cage vertex points, scatter: per-halfedge contributions (smooth
-// :1713-1756, creased :1872-1991) added with float atomics
-__device__ inline void cage_vertices_scatter_at(DCage c, const P3 *fp, const P3 *ep, P3 *vp, int creased, int32_t h) {
-    if (h >= c.H) return;
+// scatter-form contribution of cage halfedge h to its vertex point (smooth
+// :1713-1756, creased :1872-1991)
+__device__ inline P3 cage_vertex_contrib(const DCage &c, const P3 *fp, const P3 *ep, int creased, int32_t h) {
     const CageHe x = c.he[h];
     const P3 old = c.pts[x.vert];
     P3 a;
@@ -284,9 +283,39 @@ __device__ inline void cage_vertices_scatter_at(DCage c, const P3 *fp, const P3 
         else if (cc >= 3.0f || n == 2.0f) a = corner;
         else a = lerp(corner, cr, sat(avgS * 0.5f));
     }
-    unsafeAtomicAdd(&vp[x.vert].x, a.x);
-    unsafeAtomicAdd(&vp[x.vert].y, a.y);
-    unsafeAtomicAdd(&vp[x.vert].z, a.z);
+    return a;
+}
+
+// cage vertex points, scatter: one thread per vertex adds the contributions
+// of the halfedges leaving it from zero in ascending halfedge order, the
+// order of the reference's sequential scatter (:1750, :1986-1987), so the
+// result does not depend on scheduling
+__device__ inline void cage_vertices_scatter_at(DCage c, const P3 *fp, const P3 *ep, P3 *vp, int creased, int32_t v) {
+    if (v >= c.V) return;
+    const int32_t h0 = c.v2h[v];
+    P3 acc = zero3();
+    int32_t last = -1;
+    for (int k = 0; k < kWalkCap; ++k) {
+        // the smallest halfedge leaving v above `last`: walk forward
+        // (twin(prev(h))) and, past a boundary, backward (next(twin(h)))
+        int32_t best = INT32_MAX;
+        int32_t h = h0, g = 0;
+        for (; h >= 0 && g < kWalkCap; ++g) {
+            if (h > last && h < best) best = h;
+            h = c.he[c.he[h].prev].twin;
+            if (h == h0) break;
+        }
+        if (h < 0) {
+            for (h = c.he[h0].twin >= 0 ? c.he[c.he[h0].twin].next : -1, g = 0; h >= 0 && g < kWalkCap; ++g) {
+                if (h > last && h < best) best = h;
+                h = c.he[h].twin >= 0 ? c.he[c.he[h].twin].next : -1;
+            }
+        }
+        if (best == INT32_MAX) break;
+        acc = add(acc, cage_vertex_contrib(c, fp, ep, creased, best));
+        last = best;
+    }
+    vp[v] = acc;
 }
 
 }  // namespace cch

@@ -188,6 +188,7 @@ __device__ __forceinline__ void patch_tile(const FusedArgs &a, const DCage &c, c
     for (int q = 0; q < 4; ++q) {
         const int32_t rb = r[q] - base, pb = qprev(r[q]) - base;
         const int4 rr = L.srec[rb], pr = L.srec[pb];
+        e[q].r = r[q];
         e[q].re = rr.y;
         e[q].pe = pr.y;
         e[q].E = L.sE[edge_owner(r[q], rr.x) - base];

@@ -270,8 +270,7 @@ void ccd_Subd::cage_points(int32_t steps, const P3 *cagePts, P3 *out, hipStream_
     if (gather) {
         launch("cage_vertices", k_cage_vertices_gather, V0, st, c, (const P3 *)fp, (const P3 *)ep, vp, creased);
     } else {
-        zero(&vp->x, 3 * (int64_t)V0, st);
-        launch("cage_vertices", k_cage_vertices_scatter, H0, st, c, (const P3 *)fp, (const P3 *)ep, vp, creased);
+        launch("cage_vertices", k_cage_vertices_scatter, V0, st, c, (const P3 *)fp, (const P3 *)ep, vp, creased);
     }
 }
 
@@ -640,7 +639,9 @@ int32_t ccd_UploadCagePoints(ccd_Subd *s, const cc_VertexPoint *hostPoints) {
     clear_error();
     if (!s || !hostPoints) { set_error(11, "ccd_UploadCagePoints: null argument"); return t_err; }
     DeviceGuard g(s->device);
-    if (hip_failed(hipMemcpyAsync(s->c_pts, hostPoints, sizeof(P3) * (size_t)s->V0, hipMemcpyHostToDevice, s->stream),
+    // work queued on any stream may still read the cage points
+    if (hip_failed(hipDeviceSynchronize(), "ccd_UploadCagePoints") ||
+        hip_failed(hipMemcpyAsync(s->c_pts, hostPoints, sizeof(P3) * (size_t)s->V0, hipMemcpyHostToDevice, s->stream),
                    "ccd_UploadCagePoints") ||
         hip_failed(hipStreamSynchronize(s->stream), "ccd_UploadCagePoints"))
         return t_err;
@@ -651,7 +652,7 @@ int32_t ccd_Refine(ccd_Subd *s, int32_t steps, void *stream) {
     clear_error();
     if (!s) { set_error(11, "ccd_Refine: null subd"); return t_err; }
     DeviceGuard g(s->device);
-    hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    hipStream_t st = (hipStream_t)stream;
     if (!s->refine(steps, s->D, st)) return t_err ? t_err : 1;
     return 0;
 }
@@ -660,7 +661,7 @@ int32_t ccd_RefinePoints(ccd_Subd *s, const float *devCagePoints, float *devPoin
     clear_error();
     if (!s) { set_error(11, "ccd_RefinePoints: null subd"); return t_err; }
     DeviceGuard g(s->device);
-    hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    hipStream_t st = (hipStream_t)stream;
     if (s->timing) s->begin_timing();
     P3 *out = devPoints ? reinterpret_cast<P3 *>(devPoints) : s->pts;
     if (!s->refine_points(steps | CCD_POINTS, s->D, reinterpret_cast<const P3 *>(devCagePoints), out, st)) return t_err ? t_err : 1;
@@ -673,7 +674,7 @@ int32_t ccd_RefinePointsAdjoint(ccd_Subd *s, const float *devGradFinest, float *
     clear_error();
     if (!s || !devGradFinest || !devGradCage) { set_error(11, "ccd_RefinePointsAdjoint: null argument"); return t_err; }
     DeviceGuard g(s->device);
-    hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    hipStream_t st = (hipStream_t)stream;
     if (!s->adjoint(devGradFinest, devGradCage, steps, st)) return t_err ? t_err : 1;
     return 0;
 }
@@ -683,6 +684,8 @@ int32_t ccd_Download(const ccd_Subd *s, cc_Subd *dst, int32_t steps) {
     if (!s || !dst) { set_error(11, "ccd_Download: null argument"); return t_err; }
     if (dst->maxDepth != s->D) { set_error(13, "ccd_Download: depth mismatch"); return t_err; }
     DeviceGuard g(s->device);
+    // the refinement may have been queued on any stream
+    if (hip_failed(hipDeviceSynchronize(), "ccd_Download")) return t_err;
     hipStream_t st = s->stream;
     if ((steps & (CCD_HALFEDGES | CCD_UVS)) &&
         hip_failed(hipMemcpyAsync(dst->halfedges, s->he, sizeof(He) * (size_t)s->n.cumH(s->D), hipMemcpyDeviceToHost, st), "download"))
@@ -791,7 +794,7 @@ int32_t ccd_TileOption(const ccd_Subd *s, int32_t key, int32_t level) {
 int32_t ccd_Autotune(ccd_Subd *s, int32_t steps, void *stream) {
     if (!s) return 11;
     DeviceGuard g(s->device);
-    return s->autotune(steps, stream ? (hipStream_t)stream : s->stream);
+    return s->autotune(steps, (hipStream_t)stream);
 }
 
 int32_t ccd_SetTiming(ccd_Subd *s, int32_t enable) {

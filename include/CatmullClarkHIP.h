@@ -224,8 +224,10 @@ void ccs_RefineVertexPoints_Adjoint_f64(const cc_Subd *subd,
 /* ---- extension: device-resident refinement ------------------------------
  * A ccd_Subd owns device copies of a cage and of levels 1..maxDepth laid out
  * exactly as cc_Subd's arrays.  Calls are asynchronous on `stream` (a
- * hipStream_t; NULL selects the ccd_Subd's own stream).  Return 0 on
- * success, a nonzero code otherwise (also readable via ccs_LastError). */
+ * hipStream_t; NULL is the HIP null stream, which is also torch's default
+ * stream).  ccd_Download and ccd_UploadCagePoints synchronize the device
+ * first.  Return 0 on success, a nonzero code otherwise (also readable via
+ * ccs_LastError). */
 typedef struct ccd_Subd ccd_Subd;
 
 enum {

@@ -1,10 +1,13 @@
 """GPU parity: the HIP path against the CPU oracle on the same cages.
 
-Topology (twin/edge/vertex IDs), uvIDs and creases must match bit for bit;
-points of the gather rules too (deterministic, same operation order).  Points
-of the scatter rules are summed in another order than a sequential scatter,
-so they are held to a tolerance relative to the mesh scale against the
-double-precision oracle:
+Topology (twin/edge/vertex IDs), uvIDs and creases must match bit for bit,
+and so must the points of every rule on the default paths: gather rules keep
+the reference's operation order, and scatter sums add the per-halfedge
+contributions from zero in ascending halfedge order, as the reference's
+sequential scatter does.  The optional per-element path keeps the
+reference's float atomics and the optional ring-walk vertex kernel sums in
+ring order; their scatter points are held to a tolerance relative to the
+mesh scale against the double-precision oracle:
 
     max |p_gpu - p_fp64| / max|p_fp64| <= 1e-5
 """
@@ -49,9 +52,8 @@ def test_refine_drivers(capi, name, cage, depth, driver, flags):
     want = O.refine(cage, depth, flags)
     assert_bytes_equal(got["halfedges"], want["halfedges"], f"{name} halfedges")
     assert_bytes_equal(got["creases"], want["creases"], f"{name} creases")
-    if flags & O.GATHER:
-        assert_bytes_equal(got["points"], want["points"], f"{name} gather points")
-    else:
+    assert_bytes_equal(got["points"], want["points"], f"{name} points")
+    if not flags & O.GATHER:
         ref64 = O.refine(cage, depth, flags, real=np.float64)["points"]
         err_max, err_rms = point_error(got["points"], ref64)
         assert err_max <= POINT_TOL, (err_max, err_rms)
@@ -97,4 +99,31 @@ def test_device_paths(capi, name, cage, depth, fused, store, remap, fan):
             ref64 = O.refine(cage, depth, flags, real=np.float64)["points"]
             err, _ = point_error(got["points"], ref64)
             assert err <= POINT_TOL, (label, err)
+    ds.close()
+
+
+DETERMINISTIC = [  # (fused, fan): paths whose scatter sums run in ascending halfedge order
+    (1, 2),
+    (1, 1),
+    (1, 3),
+    (2, 2),
+]
+
+
+@pytest.mark.parametrize("fused,fan", DETERMINISTIC, ids=["fused_compact", "fused_fan", "fused_vertexdirect", "patch"])
+@pytest.mark.parametrize("name,cage,depth", small_cages(), ids=[c[0] for c in small_cages()])
+def test_scatter_points_bitexact(capi, name, cage, depth, fused, fan):
+    """Scatter vertex points are sums of per-halfedge contributions; the fused
+    and patch paths add them from zero in ascending halfedge order, as the
+    reference's sequential scatter does, so they equal the float oracle bit
+    for bit (the per-element path and the ring-walk vertex kernel are held to
+    the tolerance above)."""
+    ds = capi.DeviceSubd(cage, depth)
+    ds.set_option(capi.OPT_FUSED, fused)
+    ds.set_option(capi.OPT_FAN, fan)
+    for label, steps, flags in (STEPS[0], STEPS[2]):
+        ds.refine(steps)
+        got = ds.download()
+        want = O.refine(cage, depth, flags)
+        assert_bytes_equal(got["points"], want["points"], f"{name}/{label} scatter points")
     ds.close()
