@@ -17,6 +17,7 @@ struct AdjArgs {
     int32_t H, C;
     T *gV, *gF, *gE;
     T *gP;
+    T *W;          // per-halfedge vertex-rule weights (a, b, g, dl): 4 H scalars
 };
 
 struct AdjPlan {
@@ -30,11 +31,11 @@ struct AdjPlan {
 };
 
 // dL/d(cage points) from dL/d(level-D points): V_D x 3 in, V_0 x 3 out.
-// bufA holds V_D x 3 and bufB V_{D-1} x 3 scalars of scratch.  When sharp
-// is given (E_0 + Ccum(D-1) scalars) the sharpness of every level is refined
-// again in T and used instead of the float crease records.
+// bufA holds V_D x 3 and bufB V_{D-1} x 3 scalars of scratch, W 4 H_{D-1}.
+// When sharp is given (E_0 + Ccum(D-1) scalars) the sharpness of every level
+// is refined again in T and used instead of the float crease records.
 template <typename T>
-void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA, T *bufB, T *sharp, int creased,
-                    hipStream_t st);
+void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA, T *bufB, T *W, T *sharp,
+                    int creased, hipStream_t st);
 
 }  // namespace cch

@@ -4,12 +4,20 @@
 // combination of the points one level up (the weights of the scatter rules,
 // CatmullClark.h:1437-1991 and :2032-2615, depend only on valence, crease
 // count, boundary and sharpness).  The adjoint applies the transposes level
-// by level, finest first: for level d -> d+1 the gradient of the level-d+1
-// points [vertex | face | edge] flows back through the vertex rule, then the
-// edge rule, then the face rule into the gradient of the level-d points.
-// Only the stored topology and creases are read; nothing of the forward
-// points is kept or recomputed.  Kernels are templated on the scalar type so
-// the same code runs in float and, for gradient checks, in double.
+// by level, finest first.  For a level step d -> d+1 (d >= 1) it runs in
+// gather form, without atomics, in four passes:
+//   1. weights: the owner of each level-d vertex evaluates the vertex rule's
+//      weights (a, b, g, dl) of every halfedge of its ring;
+//   2. edges: the owner of each edge adds the vertex rule's shares to the
+//      edge's gradient;
+//   3. faces: each quad adds the vertex and edge rules' shares to its face's
+//      gradient;
+//   4. vertices: the owner of each vertex gathers its level-d gradient from
+//      the vertex, edge and face rules.
+// The cage step (small, polygons of any size) keeps per-halfedge float
+// atomics.  Only the stored topology and creases are read; nothing of the
+// forward points is kept or recomputed.  Kernels are templated on the scalar
+// type so the same code runs in float and, for gradient checks, in double.
 #include "cc_adjoint.h"
 #include "cc_batch.h"
 #include "cc_fan.h"
@@ -91,22 +99,35 @@ __device__ __forceinline__ VW<T> level_vertex_weights(const RingT<T> &rp, T ts, 
 
 }  // namespace
 
-// ---- level d >= 1 -------------------------------------------------------------
-// vertex rule: the owner of each vertex takes its fan, writes alpha * g into
-// the level-d gradient of the vertex and sends beta, gamma, delta shares to
-// the level-(d+1) edge and face gradients
+// ---- level d >= 1, gather form -------------------------------------------------
+template <typename T>
+struct W4 { T a, b, g, dl; };
+
+// weights of one halfedge h in the edge rule (creased :2221-2233, smooth
+// :2125-2131): c0 on its vertex, c1 on the next vertex, cf on its face point
 template <typename T, int CREASED>
-__device__ __forceinline__ void adj_vertices_at(const AdjArgs<T> &a, const DCage &c, const LevelTab &t, const LevelPtrs &lp, int d, int32_t h) {
-    if (h >= a.H) return;
-    const He *he = a.he;
-    const int32_t v = he[h].vert;
-    Fan f;
-    if (!owner_fan(h, v, d, c, t, lp.he, f)) return;
-    const int n = f.n;
-    RingT<T> rp;
-    rp.n = (T)n; rp.cc = 0; rp.boundary = f.boundary; rp.s1 = rp.s2 = 0; rp.e1 = rp.e2 = -1; rp.sall = 0;
+__device__ __forceinline__ void edge_weights(const AdjArgs<T> &a, int32_t edge, bool boundary, T &c0, T &c1, T &cf) {
     if (CREASED) {
-        for (int i = (f.boundary ? -1 : 0); i < n; ++i) {
+        const T w = satT(sharp_at(a, edge));
+        const T k = boundary ? (T)1 : (T)0.5;
+        c0 = (T)0.25 * ((T)1 - w) + (T)0.5 * k * w;
+        c1 = (T)0.5 * k * w;
+        cf = (T)0.25 * ((T)1 - w);
+    } else {
+        const T w = boundary ? (T)1 : (T)0.5;
+        c0 = c1 = (T)0.5 * w * w;
+        cf = (T)0.5 * ((T)1 - w);
+    }
+}
+
+// ring parameters of a vertex from its fan (valence, creases, boundary)
+template <typename T, int CREASED>
+__device__ __forceinline__ RingT<T> ring_params(const AdjArgs<T> &a, const DCage &c, const Fan &f) {
+    const He *he = a.he;
+    RingT<T> rp;
+    rp.n = (T)f.n; rp.cc = 0; rp.boundary = f.boundary; rp.s1 = rp.s2 = 0; rp.e1 = rp.e2 = -1; rp.sall = 0;
+    if (CREASED) {
+        for (int i = (f.boundary ? -1 : 0); i < f.n; ++i) {
             const int32_t e = i < 0 ? he[fan_at(f, c, 0)].edge : he[qprev(fan_at(f, c, i))].edge;
             const T se = sharp_at(a, e);
             rp.cc += sgnT(se);
@@ -117,57 +138,172 @@ __device__ __forceinline__ void adj_vertices_at(const AdjArgs<T> &a, const DCage
             rp.sall += se;
         }
     }
-    const V3<T> gv = reinterpret_cast<const V3<T> *>(a.gV)[v];
-    T alpha = 0;
-    V3<T> *gE = reinterpret_cast<V3<T> *>(a.gE);
-    V3<T> *gF = reinterpret_cast<V3<T> *>(a.gF);
-    for (int i = 0; i < n; ++i) {
+    return rp;
+}
+
+// pass 1: the vertex rule's weights of every halfedge of a vertex's ring
+template <typename T, int CREASED>
+__device__ __forceinline__ void adj_ring_weights(const AdjArgs<T> &a, const DCage &c, const Fan &f) {
+    const He *he = a.he;
+    const RingT<T> rp = ring_params<T, CREASED>(a, c, f);
+    W4<T> *W = reinterpret_cast<W4<T> *>(a.W);
+    for (int i = 0; i < f.n; ++i) {
         const int32_t r = fan_at(f, c, i);
         const int32_t re = he[r].edge, pe = he[qprev(r)].edge;
         const T ts = CREASED ? sharp_at(a, re) : (T)0;
         const T ps = CREASED ? sharp_at(a, pe) : (T)0;
         const VW<T> w = level_vertex_weights<T>(rp, ts, ps, re, CREASED);
-        alpha += w.a;
-        if (w.b != (T)0) atomic_add3(&gE[re], w.b, gv);
-        if (w.g != (T)0) atomic_add3(&gF[r >> 2], w.g, gv);
-        if (w.dl != (T)0) atomic_add3(&gE[pe], w.dl, gv);
+        W[r] = {w.a, w.b, w.g, w.dl};
     }
-    V3<T> *gP = reinterpret_cast<V3<T> *>(a.gP);
-    gP[v] = {alpha * gv.x, alpha * gv.y, alpha * gv.z};
 }
 
-// edge rule (creased :2221-2233, smooth :2125-2131): each halfedge's share of
-// its edge point flows to its two endpoints and its face point
-template <typename T, int CREASED>
-__device__ __forceinline__ void adj_edges_at(const AdjArgs<T> &a, int32_t h) {
+// pass 2: edge(h) for its owner h (the larger halfedge, or a boundary one):
+// the b shares of the halfedges on the edge and the dl shares of the
+// halfedges whose previous edge it is
+template <typename T>
+__device__ __forceinline__ void adj_edge_at(const AdjArgs<T> &a, int32_t h) {
     if (h >= a.H) return;
     const He x = a.he[h];
-    const V3<T> ge = reinterpret_cast<const V3<T> *>(a.gE)[x.edge];
-    const bool boundary = x.twin < 0;
-    T c0, c1, cf;
-    if (CREASED) {
-        const T w = satT(sharp_at(a, x.edge));
-        const T k = boundary ? (T)1 : (T)0.5;
-        c0 = (T)0.25 * ((T)1 - w) + (T)0.5 * k * w;
-        c1 = (T)0.5 * k * w;
-        cf = (T)0.25 * ((T)1 - w);
-    } else {
-        const T w = boundary ? (T)1 : (T)0.5;
-        c0 = c1 = (T)0.5 * w * w;
-        cf = (T)0.5 * ((T)1 - w);
+    if (x.twin >= 0 && h < x.twin) return;
+    const W4<T> *W = reinterpret_cast<const W4<T> *>(a.W);
+    const V3<T> *gV = reinterpret_cast<const V3<T> *>(a.gV);
+    V3<T> *gE = reinterpret_cast<V3<T> *>(a.gE);
+    V3<T> ge = gE[x.edge];
+    const int32_t nh = qnext(h);
+    const T bh = W[h].b, dn = W[nh].dl;
+    const V3<T> vh = gV[x.vert], vn = gV[a.he[nh].vert];
+    ge.x += bh * vh.x; ge.y += bh * vh.y; ge.z += bh * vh.z;
+    ge.x += dn * vn.x; ge.y += dn * vn.y; ge.z += dn * vn.z;
+    if (x.twin >= 0) {
+        const int32_t t = x.twin, nt = qnext(t);
+        const T bt = W[t].b, dt = W[nt].dl;
+        const V3<T> vt = gV[a.he[t].vert], vnt = gV[a.he[nt].vert];
+        ge.x += bt * vt.x; ge.y += bt * vt.y; ge.z += bt * vt.z;
+        ge.x += dt * vnt.x; ge.y += dt * vnt.y; ge.z += dt * vnt.z;
     }
-    V3<T> *gP = reinterpret_cast<V3<T> *>(a.gP);
-    atomic_add3(&gP[x.vert], c0, ge);
-    atomic_add3(&gP[a.he[qnext(h)].vert], c1, ge);
-    if (cf != (T)0) atomic_add3(reinterpret_cast<V3<T> *>(a.gF) + (h >> 2), cf, ge);
+    gE[x.edge] = ge;
 }
 
-// face rule (:2048): each corner receives a quarter of its face's gradient
-template <typename T>
-__device__ __forceinline__ void adj_faces_at(const AdjArgs<T> &a, int32_t h) {
-    if (h >= a.H) return;
-    const V3<T> gf = reinterpret_cast<const V3<T> *>(a.gF)[h >> 2];
-    atomic_add3(reinterpret_cast<V3<T> *>(a.gP) + a.he[h].vert, (T)0.25, gf);
+// pass 3: face f (halfedges 4f..4f+3): the g shares of the vertex rule and
+// the cf shares of the edge rule
+template <typename T, int CREASED>
+__device__ __forceinline__ void adj_face_at(const AdjArgs<T> &a, int32_t f) {
+    if (f >= (a.H >> 2)) return;
+    const W4<T> *W = reinterpret_cast<const W4<T> *>(a.W);
+    const V3<T> *gV = reinterpret_cast<const V3<T> *>(a.gV);
+    const V3<T> *gE = reinterpret_cast<const V3<T> *>(a.gE);
+    V3<T> *gF = reinterpret_cast<V3<T> *>(a.gF);
+    V3<T> gf = gF[f];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int32_t h = 4 * f + j;
+        const He x = a.he[h];
+        T c0, c1, cf;
+        edge_weights<T, CREASED>(a, x.edge, x.twin < 0, c0, c1, cf);
+        const T g = W[h].g;
+        const V3<T> v = gV[x.vert], e = gE[x.edge];
+        gf.x += g * v.x; gf.y += g * v.y; gf.z += g * v.z;
+        gf.x += cf * e.x; gf.y += cf * e.y; gf.z += cf * e.z;
+    }
+    gF[f] = gf;
+}
+
+// pass 4: vertex v with fan f gathers a * gV[v] from the vertex rule, c0
+// and c1 shares of its incident edges from the edge rule and a quarter of
+// each incident face from the face rule
+template <typename T, int CREASED>
+__device__ __forceinline__ void adj_vertex_gather(const AdjArgs<T> &a, const DCage &c, const Fan &f, int32_t v) {
+    const He *he = a.he;
+    const W4<T> *W = reinterpret_cast<const W4<T> *>(a.W);
+    const V3<T> *gE = reinterpret_cast<const V3<T> *>(a.gE);
+    const V3<T> *gF = reinterpret_cast<const V3<T> *>(a.gF);
+    const V3<T> gv = reinterpret_cast<const V3<T> *>(a.gV)[v];
+    T alpha = 0;
+    V3<T> acc{0, 0, 0};
+    for (int i = 0; i < f.n; ++i) {
+        const int32_t r = fan_at(f, c, i);
+        const int32_t p = qprev(r);
+        const He x = he[r], xp = he[p];
+        alpha += W[r].a;
+        T c0, c1, cf, d0, d1, df;
+        edge_weights<T, CREASED>(a, x.edge, x.twin < 0, c0, c1, cf);     // r leaves v
+        edge_weights<T, CREASED>(a, xp.edge, xp.twin < 0, d0, d1, df);  // prev(r) enters v
+        const V3<T> e = gE[x.edge], ep = gE[xp.edge], g = gF[r >> 2];
+        acc.x += c0 * e.x; acc.y += c0 * e.y; acc.z += c0 * e.z;
+        acc.x += d1 * ep.x; acc.y += d1 * ep.y; acc.z += d1 * ep.z;
+        acc.x += (T)0.25 * g.x; acc.y += (T)0.25 * g.y; acc.z += (T)0.25 * g.z;
+    }
+    reinterpret_cast<V3<T> *>(a.gP)[v] = {alpha * gv.x + acc.x, alpha * gv.y + acc.y, alpha * gv.z + acc.z};
+}
+
+// the owners of level-d vertices among 256 * SCAN halfedges, collected in
+// LDS so that every thread then works on one (as the forward vertex kernel)
+template <int SCAN, class Fn>
+__device__ __forceinline__ void owners_block(const He *he, int32_t H, int d, const DCage &c, const LevelTab &t,
+                                             uint32_t lb, const Fn &fn) {
+    __shared__ int32_t owners[256 * SCAN];
+    __shared__ int32_t nown;
+    const int32_t base = (int32_t)(lb * 256 * SCAN);
+    const int4 *he4 = reinterpret_cast<const int4 *>(he);
+    if (threadIdx.x == 0) nown = 0;
+    __syncthreads();
+    int32_t vs[SCAN];
+#pragma unroll
+    for (int k = 0; k < SCAN; ++k) {
+        const int32_t h = base + k * 256 + (int32_t)threadIdx.x;
+        vs[k] = h < H ? he4[h].z : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < SCAN; ++k) {
+        const int32_t h = base + k * 256 + (int32_t)threadIdx.x;
+        if (vs[k] >= 0 && owner_test(h, vs[k], d, c, t)) owners[atomicAdd(&nown, 1)] = h;
+    }
+    __syncthreads();
+    const int32_t count = nown;
+    for (int32_t i = (int32_t)threadIdx.x; i < count; i += 256) fn(owners[i]);
+}
+
+constexpr int kAdjScan = 4;
+
+template <typename T, int CREASED>
+struct RingWeightsFn {
+    const AdjArgs<T> &a;
+    const DCage &c;
+    const LevelTab &t;
+    const LevelPtrs &lp;
+    int d;
+    __device__ void operator()(int32_t h) const {
+        Fan f;
+        owner_fan(h, a.he[h].vert, d, c, t, lp.he, f);
+        adj_ring_weights<T, CREASED>(a, c, f);
+    }
+};
+
+template <typename T, int CREASED>
+struct VertexGatherFn {
+    const AdjArgs<T> &a;
+    const DCage &c;
+    const LevelTab &t;
+    const LevelPtrs &lp;
+    int d;
+    __device__ void operator()(int32_t h) const {
+        const int32_t v = a.he[h].vert;
+        Fan f;
+        owner_fan(h, v, d, c, t, lp.he, f);
+        adj_vertex_gather<T, CREASED>(a, c, f, v);
+    }
+};
+
+template <typename T, int CREASED>
+__device__ __forceinline__ void adj_weights_block(const AdjArgs<T> &a, const DCage &c, const LevelTab &t,
+                                                  const LevelPtrs &lp, int d, uint32_t lb) {
+    owners_block<kAdjScan>(a.he, a.H, d, c, t, lb, RingWeightsFn<T, CREASED>{a, c, t, lp, d});
+}
+
+template <typename T, int CREASED>
+__device__ __forceinline__ void adj_vertices_block(const AdjArgs<T> &a, const DCage &c, const LevelTab &t,
+                                                   const LevelPtrs &lp, int d, uint32_t lb) {
+    owners_block<kAdjScan>(a.he, a.H, d, c, t, lb, VertexGatherFn<T, CREASED>{a, c, t, lp, d});
 }
 
 // ---- the cage step (level 0 -> 1) -----------------------------------------------
@@ -272,18 +408,23 @@ __device__ __forceinline__ void adj_cage_faces_at(const AdjArgs<T> &a, const DCa
 
 // ---- kernels: one thread per halfedge ------------------------------------------
 template <typename T, int CREASED>
-__global__ __launch_bounds__(256) void k_adj_vertices(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
-    adj_vertices_at<T, CREASED>(a, c, t, lp, d, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
-}
-
-template <typename T, int CREASED>
-__global__ __launch_bounds__(256) void k_adj_edges(AdjArgs<T> a) {
-    adj_edges_at<T, CREASED>(a, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
+__global__ __launch_bounds__(256) void k_adj_weights(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
+    adj_weights_block<T, CREASED>(a, c, t, lp, d, blockIdx.x);
 }
 
 template <typename T>
+__global__ __launch_bounds__(256) void k_adj_edges(AdjArgs<T> a) {
+    adj_edge_at<T>(a, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+template <typename T, int CREASED>
 __global__ __launch_bounds__(256) void k_adj_faces(AdjArgs<T> a) {
-    adj_faces_at<T>(a, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
+    adj_face_at<T, CREASED>(a, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+template <typename T, int CREASED>
+__global__ __launch_bounds__(256) void k_adj_vertices(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
+    adj_vertices_block<T, CREASED>(a, c, t, lp, d, blockIdx.x);
 }
 
 template <typename T, int CREASED>
@@ -346,8 +487,8 @@ unsigned grid_stride(int64_t n) {
 }  // namespace
 
 template <typename T>
-void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA, T *bufB, T *sharp, int creased,
-                    hipStream_t st) {
+void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA, T *bufB, T *W, T *sharp,
+                    int creased, hipStream_t st) {
     const Counts &n = *pl.n;
     const int D = pl.D;
     // sharpness of levels 0..D-1 in T when a buffer is given (cumulative,
@@ -375,15 +516,19 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
         a.gF = G + 3 * n.V[d];
         a.gE = G + 3 * (n.V[d] + n.F[d]);
         a.gP = P;
-        const dim3 g(grid_for(n.H[d])), b(256);
+        a.W = W;
+        const dim3 g(grid_for(n.H[d])), gq(grid_for(n.H[d] / 4)), b(256);
         if (creased) {
-            hipLaunchKernelGGL((k_adj_vertices<T, 1>), g, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
-            hipLaunchKernelGGL((k_adj_edges<T, 1>), g, b, 0, st, a);
+            hipLaunchKernelGGL((k_adj_weights<T, 1>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            hipLaunchKernelGGL(k_adj_edges<T>, g, b, 0, st, a);
+            hipLaunchKernelGGL((k_adj_faces<T, 1>), gq, b, 0, st, a);
+            hipLaunchKernelGGL((k_adj_vertices<T, 1>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
         } else {
-            hipLaunchKernelGGL((k_adj_vertices<T, 0>), g, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
-            hipLaunchKernelGGL((k_adj_edges<T, 0>), g, b, 0, st, a);
+            hipLaunchKernelGGL((k_adj_weights<T, 0>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            hipLaunchKernelGGL(k_adj_edges<T>, g, b, 0, st, a);
+            hipLaunchKernelGGL((k_adj_faces<T, 0>), gq, b, 0, st, a);
+            hipLaunchKernelGGL((k_adj_vertices<T, 0>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
         }
-        hipLaunchKernelGGL(k_adj_faces<T>, g, b, 0, st, a);
         T *tmp = G; G = P; P = tmp;
     }
     AdjArgs<T> a;
@@ -396,6 +541,7 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
     a.gF = G + 3 * (int64_t)pl.cage.V;
     a.gE = G + 3 * ((int64_t)pl.cage.V + pl.cage.F);
     a.gP = gradCage;
+    a.W = nullptr;
     hipLaunchKernelGGL(k_fill0<T>, dim3(grid_stride(3 * (int64_t)pl.cage.V)), dim3(256), 0, st, gradCage,
                        3 * (int64_t)pl.cage.V);
     const dim3 g(grid_for(pl.cage.H)), b(256);
@@ -413,18 +559,25 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
 namespace {
 
 template <int CREASED>
+struct OpAdjWeights {
+    int d;
+    __device__ void operator()(const AdjJob &J, int64_t i) const {
+        adj_weights_block<float, CREASED>(J.a, J.c, J.t, J.lp, d, (uint32_t)(i >> 8));
+    }
+};
+struct OpAdjEdges {
+    __device__ void operator()(const AdjJob &J, int64_t i) const { adj_edge_at<float>(J.a, (int32_t)i); }
+};
+template <int CREASED>
+struct OpAdjFaces {
+    __device__ void operator()(const AdjJob &J, int64_t i) const { adj_face_at<float, CREASED>(J.a, (int32_t)i); }
+};
+template <int CREASED>
 struct OpAdjVertices {
     int d;
     __device__ void operator()(const AdjJob &J, int64_t i) const {
-        adj_vertices_at<float, CREASED>(J.a, J.c, J.t, J.lp, d, (int32_t)i);
+        adj_vertices_block<float, CREASED>(J.a, J.c, J.t, J.lp, d, (uint32_t)(i >> 8));
     }
-};
-template <int CREASED>
-struct OpAdjEdges {
-    __device__ void operator()(const AdjJob &J, int64_t i) const { adj_edges_at<float, CREASED>(J.a, (int32_t)i); }
-};
-struct OpAdjFaces {
-    __device__ void operator()(const AdjJob &J, int64_t i) const { adj_faces_at<float>(J.a, (int32_t)i); }
 };
 template <int CREASED>
 struct OpAdjCageVertices {
@@ -444,16 +597,19 @@ struct OpAdjCageFaces {
 
 }  // namespace
 
-// level d of every job: the vertex, edge and face transposes in turn
-void batch_adj_level(const BatchTable &t, int d, int creased, hipStream_t st) {
+// level d of every job: weights, edges, faces, vertices in turn; `quads`
+// has one workgroup per 1024 halfedges of each job, `halfedges` one per 256
+void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, int d, int creased, hipStream_t st) {
+    if (creased) launch_batch<AdjJob>(quads, OpAdjWeights<1>{d}, st);
+    else launch_batch<AdjJob>(quads, OpAdjWeights<0>{d}, st);
+    launch_batch<AdjJob>(halfedges, OpAdjEdges{}, st);
     if (creased) {
-        launch_batch<AdjJob>(t, OpAdjVertices<1>{d}, st);
-        launch_batch<AdjJob>(t, OpAdjEdges<1>{}, st);
+        launch_batch<AdjJob>(quads, OpAdjFaces<1>{}, st);
+        launch_batch<AdjJob>(quads, OpAdjVertices<1>{d}, st);
     } else {
-        launch_batch<AdjJob>(t, OpAdjVertices<0>{d}, st);
-        launch_batch<AdjJob>(t, OpAdjEdges<0>{}, st);
+        launch_batch<AdjJob>(quads, OpAdjFaces<0>{}, st);
+        launch_batch<AdjJob>(quads, OpAdjVertices<0>{d}, st);
     }
-    launch_batch<AdjJob>(t, OpAdjFaces{}, st);
 }
 
 void batch_adj_cage(const BatchTable &t, int creased, hipStream_t st) {
@@ -467,9 +623,9 @@ void batch_adj_cage(const BatchTable &t, int creased, hipStream_t st) {
     launch_batch<AdjJob>(t, OpAdjCageFaces{}, st);
 }
 
-template void adjoint_levels<float>(const AdjPlan &, const float *, float *, float *, float *, float *, int,
-                                    hipStream_t);
-template void adjoint_levels<double>(const AdjPlan &, const double *, double *, double *, double *, double *, int,
-                                     hipStream_t);
+template void adjoint_levels<float>(const AdjPlan &, const float *, float *, float *, float *, float *, float *,
+                                    int, hipStream_t);
+template void adjoint_levels<double>(const AdjPlan &, const double *, double *, double *, double *, double *,
+                                     double *, int, hipStream_t);
 
 }  // namespace cch

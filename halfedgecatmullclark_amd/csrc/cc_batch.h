@@ -106,7 +106,7 @@ void batch_level_fused(const BatchTable &t, int gather, int creased, int topo, i
 void batch_level_vertices(const BatchTable &t, int d, int gather, int creased, int scan, int remap, hipStream_t st);
 
 // adjoint launches (cc_adjoint.hip)
-void batch_adj_level(const BatchTable &t, int d, int creased, hipStream_t st);
+void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, int d, int creased, hipStream_t st);
 void batch_adj_cage(const BatchTable &t, int creased, hipStream_t st);
 
 }  // namespace cch

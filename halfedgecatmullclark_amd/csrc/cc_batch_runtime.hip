@@ -26,7 +26,7 @@ struct ccb_Batch {
         LevelTab tab;
         // element offsets: cage arrays, then levels 1..D, then per-cage scratch
         int64_t o_he0 = 0, o_v0 = 0, o_u0 = 0, o_e0 = 0, o_f0 = 0;
-        int64_t o_he = 0, o_cr = 0, o_pts = 0, o_fin = 0, o_def = 0, o_adjB = 0;
+        int64_t o_he = 0, o_cr = 0, o_pts = 0, o_fin = 0, o_def = 0, o_adjB = 0, o_adjW = 0;
     };
     struct Timed {
         const char *name;
@@ -39,7 +39,7 @@ struct ccb_Batch {
     int32_t N = 0, Dmax = 0;
     std::vector<Cage> cages;
     int64_t tot_he0 = 0, tot_v0 = 0, tot_u0 = 0, tot_e0 = 0, tot_f0 = 0;
-    int64_t tot_he = 0, tot_cr = 0, tot_pts = 0, tot_fin = 0, tot_def = 0, tot_adjB = 0;
+    int64_t tot_he = 0, tot_cr = 0, tot_pts = 0, tot_fin = 0, tot_def = 0, tot_adjB = 0, tot_adjW = 0;
     // device memory
     CageHe *c_he = nullptr;
     int32_t *c_v2h = nullptr, *c_e2h = nullptr, *c_f2h = nullptr;
@@ -50,13 +50,13 @@ struct ccb_Batch {
     Crease *cr = nullptr;
     P3 *pts = nullptr;
     uint64_t *defer_mask = nullptr;   // deferred-vertex masks, per cage one word per 64 halfedges
-    float *adjA = nullptr, *adjB = nullptr, *adjCage = nullptr;
+    float *adjA = nullptr, *adjB = nullptr, *adjW = nullptr, *adjCage = nullptr;
     void *tables = nullptr;
     size_t tables_cap = 0;
     // launch tables
     BatchTable t_cage_he, t_cage_f, t_cage_e, t_cage_v, t_cage_h3, t_cage_zero, t_fin;
     BatchTable t_cr[kMaxDepth + 1], t_patch[kMaxDepth + 1], t_def[kMaxDepth + 1], t_adj[kMaxDepth + 1];
-    BatchTable t_fused[kMaxDepth + 1], t_vert[kMaxDepth + 1];
+    BatchTable t_fused[kMaxDepth + 1], t_vert[kMaxDepth + 1], t_adjq[kMaxDepth + 1];
     // per-level kernels: kind 1 fused + compacted vertex points (scan
     // halfedges tested per thread), kind 2 patch-local (tiles of nt, hpt
     // tiles per workgroup) + deferred vertices
@@ -170,6 +170,7 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
         g.o_fin = tot_fin; tot_fin += g.n.V[g.D];
         g.o_def = tot_def; tot_def += g.D >= 2 ? (g.n.H[g.D - 1] + 63) / 64 : 1;
         g.o_adjB = tot_adjB; tot_adjB += g.n.V[g.D - 1];
+        g.o_adjW = tot_adjW; tot_adjW += g.D >= 2 ? g.n.H[g.D - 1] : 1;
         Dmax = std::max(Dmax, g.D);
     }
     CC_TRY(hipGetDevice(&device), "hipGetDevice");
@@ -206,7 +207,7 @@ void ccb_Batch::free_all() {
     drop_graphs();
     begin_timing();
     void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts, defer_mask,
-                    adjA, adjB, adjCage, tables};
+                    adjA, adjB, adjW, adjCage, tables};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
@@ -221,6 +222,7 @@ bool ccb_Batch::alloc_adjoint() {
     if (adjA) return true;
     CC_TRY(hipMalloc(&adjA, sizeof(float) * 3 * (size_t)tot_fin), "hipMalloc(batch adjoint)");
     CC_TRY(hipMalloc(&adjB, sizeof(float) * 3 * (size_t)tot_adjB), "hipMalloc(batch adjoint)");
+    CC_TRY(hipMalloc(&adjW, sizeof(float) * 4 * (size_t)tot_adjW), "hipMalloc(batch adjoint weights)");
     CC_TRY(hipMalloc(&adjCage, sizeof(float) * 3 * (size_t)tot_v0), "hipMalloc(batch adjoint)");
     return build_tables();
 }
@@ -315,7 +317,7 @@ bool ccb_Batch::build_tables() {
     if (adjA) {   // adjoint steps d+1 -> d, d = 0 the cage step
         for (int d = 0; d < Dmax; ++d) {
             std::vector<AdjJob> jobs;
-            std::vector<int64_t> b;
+            std::vector<int64_t> b, bq;
             for (int i = 0; i < N; ++i) {
                 const Cage &g = cages[i];
                 if (g.D <= d) continue;
@@ -340,6 +342,7 @@ bool ccb_Batch::build_tables() {
                     a.gP = buf(d);
                 }
                 a.sd = nullptr;
+                a.W = d == 0 ? nullptr : adjW + 4 * g.o_adjW;
                 a.gV = G;
                 a.gF = G + 3 * g.n.V[d];
                 a.gE = G + 3 * (g.n.V[d] + g.n.F[d]);
@@ -348,8 +351,10 @@ bool ccb_Batch::build_tables() {
                 for (int k = 0; k <= kMaxDepth; ++k) j.lp.he[k] = (k >= 1 && k <= g.D) ? hb + g.n.offH[k] : nullptr;
                 jobs.push_back(j);
                 b.push_back(blocks_of(a.H));
+                bq.push_back(blocks_of(a.H, 1024));
             }
             tb.add(t_adj[d], jobs, b);
+            tb.add(t_adjq[d], jobs, bq);
         }
     }
     if (tb.blob.size() > tables_cap) {
@@ -423,7 +428,7 @@ void ccb_Batch::record_adjoint(int creased, hipStream_t st) {
     const int64_t nc = 3 * tot_v0;
     hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<int64_t>((nc + 255) / 256, 65536)), dim3(256), 0, st, adjCage, nc);
     for (int d = Dmax - 1; d >= 1; --d)
-        timed_launch("adj_level", d, st, [&] { batch_adj_level(t_adj[d], d, creased, st); });
+        timed_launch("adj_level", d, st, [&] { batch_adj_level(t_adjq[d], t_adj[d], d, creased, st); });
     timed_launch("adj_cage", 0, st, [&] { batch_adj_cage(t_adj[0], creased, st); });
 }
 

@@ -32,7 +32,8 @@ struct ccd_Subd {
     cch::P3 *pts = nullptr;
     hipStream_t stream = nullptr;
     // adjoint scratch (allocated on first use, per scalar size)
-    void *adj_a = nullptr, *adj_b = nullptr, *adj_in = nullptr, *adj_out = nullptr, *adj_sharp = nullptr;
+    void *adj_a = nullptr, *adj_b = nullptr, *adj_w = nullptr, *adj_in = nullptr, *adj_out = nullptr,
+         *adj_sharp = nullptr;
     // deferred-vertex masks of the patch kernels, one word per 64 halfedges
     uint64_t *defer_mask = nullptr;
     size_t adj_scalar = 0;

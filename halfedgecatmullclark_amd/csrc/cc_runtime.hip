@@ -85,7 +85,7 @@ bool ccd_Subd::alloc(const cc_Mesh *cage, int32_t depth) {
 
 void ccd_Subd::free_all() {
     DeviceGuard g(device);
-    void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts, adj_a, adj_b, adj_in, adj_out, adj_sharp,
+    void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts, adj_a, adj_b, adj_w, adj_in, adj_out, adj_sharp,
                     defer_mask};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -341,10 +341,12 @@ bool ccd_Subd::adjoint(const void *gradFinest, void *gradCage, int32_t steps, hi
     if (adj_scalar != scalar) {
         if (adj_a) (void)hipFree(adj_a);
         if (adj_b) (void)hipFree(adj_b);
-        adj_a = adj_b = nullptr;
+        if (adj_w) (void)hipFree(adj_w);
+        adj_a = adj_b = adj_w = nullptr;
         adj_scalar = 0;
         CC_TRY(hipMalloc(&adj_a, 3 * scalar * (size_t)n.V[D]), "hipMalloc(adjoint scratch)");
         CC_TRY(hipMalloc(&adj_b, 3 * scalar * (size_t)(D >= 1 ? n.V[D - 1] : 1)), "hipMalloc(adjoint scratch)");
+        CC_TRY(hipMalloc(&adj_w, 4 * scalar * (size_t)(D >= 2 ? n.H[D - 1] : 1)), "hipMalloc(adjoint weights)");
         adj_scalar = scalar;
     }
     AdjPlan pl;
@@ -362,10 +364,10 @@ bool ccd_Subd::adjoint(const void *gradFinest, void *gradCage, int32_t steps, hi
         if (!adj_sharp)
             CC_TRY(hipMalloc(&adj_sharp, sizeof(double) * (size_t)(E0 + n.cumC(D))), "hipMalloc(adjoint sharpness)");
         adjoint_levels<double>(pl, (const double *)gradFinest, (double *)gradCage, (double *)adj_a, (double *)adj_b,
-                               (double *)adj_sharp, creased, st);
+                               (double *)adj_w, (double *)adj_sharp, creased, st);
     } else {
-        adjoint_levels<float>(pl, (const float *)gradFinest, (float *)gradCage, (float *)adj_a, (float *)adj_b, nullptr,
-                              creased, st);
+        adjoint_levels<float>(pl, (const float *)gradFinest, (float *)gradCage, (float *)adj_a, (float *)adj_b,
+                              (float *)adj_w, nullptr, creased, st);
     }
     CC_TRY(hipGetLastError(), "adjoint launch");
     return true;
