@@ -101,8 +101,9 @@ void batch_level_patch(const BatchTable &t, int d, int gather, int creased, int 
                        hipStream_t st);
 void batch_level_deferred(const BatchTable &t, int d, int gather, int creased, hipStream_t st);
 // fused children + face/edge points (one workgroup per 256 halfedges), then
-// vertex points (one workgroup per 256 * scan halfedges)
-void batch_level_fused(const BatchTable &t, int gather, int creased, int topo, int remap, hipStream_t st);
+// vertex points (one workgroup per 256 * scan halfedges; scan 0: the owners
+// the fused kernel marked, one workgroup per 1024 mask words)
+void batch_level_fused(const BatchTable &t, int d, int gather, int creased, int topo, int remap, hipStream_t st);
 void batch_level_vertices(const BatchTable &t, int d, int gather, int creased, int scan, int remap, hipStream_t st);
 
 // adjoint launches (cc_adjoint.hip)

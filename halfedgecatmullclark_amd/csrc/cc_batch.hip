@@ -89,10 +89,24 @@ __global__ __launch_bounds__(NT) void k_batch_patch(const LevelJob *__restrict__
 // halfedges)
 template <int GATHER, int CREASED, int TOPO>
 __global__ __launch_bounds__(256) void k_batch_fused(const LevelJob *__restrict__ jobs, const int32_t *__restrict__ start,
-                                                     int n, int remap) {
+                                                     int n, int d, int remap) {
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, remap);
     const int k = find_job(start, n, lb);
-    level_fused_block<GATHER, CREASED, TOPO, TOPO ? 3 : 0, 1, kStoreLds>(jobs[k].a, lb - (uint32_t)start[k]);
+    const LevelJob &J = jobs[k];
+    level_fused_block<GATHER, CREASED, TOPO, TOPO ? 3 : 0, 1, kStoreLds>(J.a, J.c, J.t, d, lb - (uint32_t)start[k]);
+}
+
+// vertex points of the owners k_batch_fused marked (kOwnerWords mask words
+// per wave, four waves per workgroup)
+template <int GATHER, int CREASED>
+__global__ __launch_bounds__(256) void k_batch_vertices_masked(const LevelJob *__restrict__ jobs,
+                                                               const int32_t *__restrict__ start, int n, int d,
+                                                               int remap) {
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, remap);
+    const int k = find_job(start, n, lb);
+    const LevelJob &J = jobs[k];
+    const int32_t wave = (int32_t)(lb - (uint32_t)start[k]) * 4 + (int32_t)(threadIdx.x >> 6);
+    level_vertices_masked<GATHER, CREASED>(J.a, J.c, J.t, J.lp, d, wave);
 }
 
 // vertex points of job k's level (one block per 256 * SCAN halfedges)
@@ -113,16 +127,23 @@ void vertices_scan(const BatchTable &t, int d, int remap, hipStream_t st) {
 
 template <int G, int C>
 void vertices_gc(const BatchTable &t, int d, int scan, int remap, hipStream_t st) {
+    if (scan == 0) {
+        hipLaunchKernelGGL((k_batch_vertices_masked<G, C>), dim3(t.blocks), dim3(256), 0, st,
+                           static_cast<const LevelJob *>(t.jobs), t.start, (int)t.n, d, remap);
+        return;
+    }
     if (scan == 2) vertices_scan<G, C, 2>(t, d, remap, st);
     else if (scan == 8) vertices_scan<G, C, 8>(t, d, remap, st);
     else vertices_scan<G, C, 4>(t, d, remap, st);
 }
 
 template <int G, int C>
-void fused_gc(const BatchTable &t, int topo, int remap, hipStream_t st) {
+void fused_gc(const BatchTable &t, int d, int topo, int remap, hipStream_t st) {
     const LevelJob *jobs = static_cast<const LevelJob *>(t.jobs);
-    if (topo) hipLaunchKernelGGL((k_batch_fused<G, C, 1>), dim3(t.blocks), dim3(256), 0, st, jobs, t.start, (int)t.n, remap);
-    else hipLaunchKernelGGL((k_batch_fused<G, C, 0>), dim3(t.blocks), dim3(256), 0, st, jobs, t.start, (int)t.n, remap);
+    if (topo)
+        hipLaunchKernelGGL((k_batch_fused<G, C, 1>), dim3(t.blocks), dim3(256), 0, st, jobs, t.start, (int)t.n, d, remap);
+    else
+        hipLaunchKernelGGL((k_batch_fused<G, C, 0>), dim3(t.blocks), dim3(256), 0, st, jobs, t.start, (int)t.n, d, remap);
 }
 
 // job k's deferred vertices: kDeferGroup mask words per wave
@@ -194,12 +215,12 @@ void batch_level_patch(const BatchTable &t, int d, int gather, int creased, int 
     else patch_gc<0, 0>(t, d, topo, nt, hpt, remap, st);
 }
 
-void batch_level_fused(const BatchTable &t, int gather, int creased, int topo, int remap, hipStream_t st) {
+void batch_level_fused(const BatchTable &t, int d, int gather, int creased, int topo, int remap, hipStream_t st) {
     if (t.blocks == 0 || t.n == 0) return;
-    if (gather && creased) fused_gc<1, 1>(t, topo, remap, st);
-    else if (gather) fused_gc<1, 0>(t, topo, remap, st);
-    else if (creased) fused_gc<0, 1>(t, topo, remap, st);
-    else fused_gc<0, 0>(t, topo, remap, st);
+    if (gather && creased) fused_gc<1, 1>(t, d, topo, remap, st);
+    else if (gather) fused_gc<1, 0>(t, d, topo, remap, st);
+    else if (creased) fused_gc<0, 1>(t, d, topo, remap, st);
+    else fused_gc<0, 0>(t, d, topo, remap, st);
 }
 
 void batch_level_vertices(const BatchTable &t, int d, int gather, int creased, int scan, int remap, hipStream_t st) {

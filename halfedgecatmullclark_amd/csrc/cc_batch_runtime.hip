@@ -15,6 +15,7 @@
 #include "cc_batch.h"
 #include "cc_internal.h"
 #include "cc_kernels.h"
+#include "cc_level.h"
 #include "cc_patch.h"
 
 using namespace cch;
@@ -61,7 +62,7 @@ struct ccb_Batch {
     // halfedges tested per thread), kind 2 patch-local (tiles of nt, hpt
     // tiles per workgroup) + deferred vertices
     struct LevelCfg {
-        int kind = 1, scan = 4, nt = 256, hpt = 1;
+        int kind = 1, scan = 0, nt = 256, hpt = 1;
     };
     LevelCfg cfg[kMaxDepth + 1];
     int remap = 1, use_graph = 1;
@@ -299,6 +300,7 @@ bool ccb_Batch::build_tables() {
             a.xcd_remap = remap;
             a.uv_refine = g.U0 > 0 ? 1 : 0;
             a.defer_mask = defer_mask + g.o_def;
+            a.owner_mask = cfg[d].kind == 1 && cfg[d].scan == 0 ? defer_mask + g.o_def : nullptr;
             j.c = cage_view(i);
             j.t = g.tab;
             for (int k = 0; k <= kMaxDepth; ++k) j.lp.he[k] = (k >= 1 && k <= g.D) ? hb + g.n.offH[k] : nullptr;
@@ -307,7 +309,8 @@ bool ccb_Batch::build_tables() {
             bp.push_back((tiles + cfg[d].hpt - 1) / cfg[d].hpt);
             bd.push_back(blocks_of(blocks_of(blocks_of(g.n.H[d], 64), kDeferGroup), 4));
             bf.push_back(blocks_of(g.n.H[d]));
-            bv.push_back(blocks_of(g.n.H[d], 256 * cfg[d].scan));
+            bv.push_back(cfg[d].scan == 0 ? blocks_of(blocks_of(blocks_of(g.n.H[d], 64), kOwnerWords), 4)
+                                          : blocks_of(g.n.H[d], 256 * cfg[d].scan));
         }
         tb.add(t_patch[d], jobs, bp);
         tb.add(t_def[d], jobs, bd);
@@ -415,7 +418,8 @@ void ccb_Batch::record_forward(int32_t steps, hipStream_t st) {
             });
             timed_launch("level_deferred", d, st, [&] { batch_level_deferred(t_def[d], d, gather, creased, st); });
         } else {
-            timed_launch("level_fused", d, st, [&] { batch_level_fused(t_fused[d], gather, creased, topo, remap, st); });
+            timed_launch("level_fused", d, st,
+                         [&] { batch_level_fused(t_fused[d], d, gather, creased, topo, remap, st); });
             timed_launch("level_vertices", d, st,
                          [&] { batch_level_vertices(t_vert[d], d, gather, creased, cfg[d].scan, remap, st); });
         }
@@ -461,7 +465,7 @@ bool ccb_Batch::run(int32_t key, bool adjoint, int32_t steps, hipStream_t st) {
 // tiles of 128, 256 or 512 halfedges, timed with HIP events on this batch.
 int32_t ccb_Batch::autotune(int32_t steps, hipStream_t st) {
     std::vector<LevelCfg> cands;
-    for (int scan : {2, 4, 8}) cands.push_back({1, scan, 256, 1});
+    for (int scan : {0, 2, 4, 8}) cands.push_back({1, scan, 256, 1});
     for (int t : {128, 256, 512}) cands.push_back({2, 4, t, 1});
     float best[kMaxDepth + 1];
     LevelCfg pick[kMaxDepth + 1];
@@ -697,7 +701,7 @@ int32_t ccb_SetOption(ccb_Batch *b, int32_t key, int32_t value) {
         for (auto &c : b->cfg) c.hpt = value;
         return b->build_tables() ? 0 : last_error();
     case CCD_OPT_SCAN:
-        if (value != 2 && value != 4 && value != 8) return 14;
+        if (value != 0 && value != 2 && value != 4 && value != 8) return 14;
         for (auto &c : b->cfg) c.scan = value;
         return b->build_tables() ? 0 : last_error();
     default: return 14;

@@ -148,6 +148,31 @@ __host__ __device__ inline int32_t edge_rep(const DCage &c, const LevelTab &t, i
     return rep;
 }
 
+// Is level-d halfedge p (d >= 1) the representative edge_rep(e, d) of its
+// edge e?  Compares level by level from fine to coarse and stops at the
+// first mismatch, so most calls end after a few integer steps: child 0 of
+// rep(e') represents the even child edge 2e', child 3 of next(rep(e')) the
+// odd one 2e' + 1.
+__host__ __device__ inline bool is_edge_rep(const DCage &c, const LevelTab &t, int32_t e, int d, int32_t p) {
+    int k = d;
+    for (; k > 1; --k) {
+        const int32_t Ep = t.E[k - 1];
+        if (e >= 2 * Ep) {
+            const int32_t h = e - 2 * Ep;
+            return p == imax(4 * h + 1, 4 * qnext(h) + 2);
+        }
+        if (e & 1) {
+            if ((p & 3) != 3) return false;
+            p = qprev(p >> 2);
+        } else {
+            if ((p & 3) != 0) return false;
+            p >>= 2;
+        }
+        e >>= 1;
+    }
+    return edge_rep_first(c, e) == p;
+}
+
 __host__ __device__ inline int32_t vertex_rep(const DCage &c, const LevelTab &t, int32_t v, int d) {
     int shift = 0;
     int32_t rep = 0;

@@ -203,7 +203,7 @@ __device__ __forceinline__ bool owner_test(int32_t h, int32_t vert, int d, const
     if (digit == 3) return false;
     if (digit == 2) return lvl - 1 >= 1 ? (p & 3) == 0 : c.f2h[c.he[p].face] == p;
     const int32_t e = vert - t.V[lvl - 1] - t.F[lvl - 1];
-    return lvl - 1 >= 1 ? edge_rep(c, t, e, lvl - 1) == p : c.e2h[e] == p;
+    return lvl - 1 >= 1 ? is_edge_rep(c, t, e, lvl - 1, p) : c.e2h[e] == p;
 }
 
 // Is h the representative of its vertex?  If so, describe the fan.
@@ -251,10 +251,19 @@ __device__ __forceinline__ bool owner_fan(int32_t h, int32_t vert, int d, const 
     const int32_t e = vert - t.V[lvl - 1] - t.F[lvl - 1];
     int32_t tw, np, nt;
     if (lvl - 1 >= 1) {
-        if (edge_rep(c, t, e, lvl - 1) != p) return false;
-        tw = levels[lvl - 1][p].twin;
+        if (!is_edge_rep(c, t, e, lvl - 1, p)) return false;
         np = qnext(p);
-        nt = qnext(tw);
+        if (k == 0) {
+            // born at this level: child 0 of p (halfedge 4p of level d, next
+            // to h = 4p + 1) has the twin 4 next(twin(p)) + 3, negative on a
+            // boundary (CatmullClark.h:2777), so the parent level is not read
+            const int32_t tc = levels[d][4 * p].twin;
+            nt = tc >> 2;
+            tw = tc < 0 ? -1 : qprev(nt);
+        } else {
+            tw = levels[lvl - 1][p].twin;
+            nt = qnext(tw);
+        }
     } else {
         if (c.e2h[e] != p) return false;
         tw = c.he[p].twin;

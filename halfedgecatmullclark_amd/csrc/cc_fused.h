@@ -21,12 +21,17 @@ struct FusedArgs {
     // word w marks halfedge 64w + l as the representative of such a vertex;
     // the patch kernel writes every one of the ceil(H / 64) words
     uint64_t *defer_mask;
+    // representatives of the level-d vertices (CatmullClark.h:1336), one bit
+    // per halfedge, written by the fused kernel when set, read by the masked
+    // vertex kernel
+    uint64_t *owner_mask;
 };
 
 // topo: write child twin/edge/vertex IDs; uvMode: 0 leave uvIDs, 1 refine
 // them, 2 write 0; points: face and edge points of level d+1.
-void launch_level_fused(const FusedArgs &a, int gather, int creased, int topo, int uvMode, int points, int store,
-                        hipStream_t st);
+// (c, t, d: the cage and counts for the owner bits when a.owner_mask is set)
+void launch_level_fused(const FusedArgs &a, const DCage &c, const LevelTab &t, int d, int gather, int creased, int topo,
+                        int uvMode, int points, int store, hipStream_t st);
 // base pointers of the halfedges of every level (index 1..D)
 struct LevelPtrs {
     const He *he[kMaxDepth + 1];
@@ -43,7 +48,8 @@ void launch_level_deferred(const FusedArgs &a, const DCage &c, const LevelTab &t
 
 // vertex points of level d+1 (after launch_level_fused with points); fan
 // selects the structurally enumerated rings over the ring walk
-// (scan: halfedges each thread of the compacted kernel tests, 2, 4 or 8)
+// (scan: halfedges each thread of the compacted kernel tests, 2, 4 or 8;
+// 0: the owners marked in a.owner_mask by the fused kernel)
 void launch_level_vertices(const FusedArgs &a, const DCage &c, const LevelTab &t, const LevelPtrs &lp, int d,
                            int gather, int creased, int fan, int scan, hipStream_t st);
 

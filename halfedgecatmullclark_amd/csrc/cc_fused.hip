@@ -25,8 +25,17 @@ namespace cch {
 
 
 template <int GATHER, int CREASED, int TOPO, int UVMODE, int POINTS, int STORE>
-__global__ __launch_bounds__(256) void k_level_fused(FusedArgs a) {
-    level_fused_block<GATHER, CREASED, TOPO, UVMODE, POINTS, STORE>(a, xcd_block(blockIdx.x, gridDim.x, a.xcd_remap));
+__global__ __launch_bounds__(256) void k_level_fused(FusedArgs a, DCage c, LevelTab t, int d) {
+    level_fused_block<GATHER, CREASED, TOPO, UVMODE, POINTS, STORE>(a, c, t, d,
+                                                                    xcd_block(blockIdx.x, gridDim.x, a.xcd_remap));
+}
+
+// vertex points of the owners the fused kernel marked (kOwnerWords mask
+// words per wave)
+template <int GATHER, int CREASED>
+__global__ __launch_bounds__(256) void k_level_vertices_masked(FusedArgs a, DCage c, LevelTab t, LevelPtrs lp, int d) {
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, a.xcd_remap);
+    level_vertices_masked<GATHER, CREASED>(a, c, t, lp, d, (int32_t)(lb * 4 + (threadIdx.x >> 6)));
 }
 
 template <int GATHER, int CREASED>
@@ -218,12 +227,12 @@ __global__ __launch_bounds__(256) void k_level_vertices_direct(FusedArgs a, DCag
 // ---- launcher ----------------------------------------------------------------
 #define CC_FUSED_CASE(G, C, T, U, P, S)                                                      \
     if (gather == G && creased == C && topo == T && uvMode == U && points == P && store == S) { \
-        hipLaunchKernelGGL((k_level_fused<G, C, T, U, P, S>), grid, block, 0, st, a);          \
+        hipLaunchKernelGGL((k_level_fused<G, C, T, U, P, S>), grid, block, 0, st, a, c, t, d);  \
         return;                                                                               \
     }
 
-static void launch_fused_impl(const FusedArgs &a, int gather, int creased, int topo, int uvMode, int points, int store,
-                              dim3 grid, dim3 block, hipStream_t st) {
+static void launch_fused_impl(const FusedArgs &a, const DCage &c, const LevelTab &t, int d, int gather, int creased,
+                              int topo, int uvMode, int points, int store, dim3 grid, dim3 block, hipStream_t st) {
 #define CC_FUSED_STORES(G, C, T, U, P) CC_FUSED_CASE(G, C, T, U, P, kStoreDirect) CC_FUSED_CASE(G, C, T, U, P, kStoreLds)
 #define CC_FUSED_POINTS(G, C, T, U) CC_FUSED_STORES(G, C, T, U, 0) CC_FUSED_STORES(G, C, T, U, 1)
 #define CC_FUSED_UV(G, C, T) CC_FUSED_POINTS(G, C, T, 0) CC_FUSED_POINTS(G, C, T, 1) CC_FUSED_POINTS(G, C, T, 2)
@@ -231,11 +240,12 @@ static void launch_fused_impl(const FusedArgs &a, int gather, int creased, int t
     CC_FUSED_TOPO(0, 0) CC_FUSED_TOPO(0, 1) CC_FUSED_TOPO(1, 0) CC_FUSED_TOPO(1, 1)
 }
 
-void launch_level_fused(const FusedArgs &a, int gather, int creased, int topo, int uvMode, int points, int store,
-                        hipStream_t st) {
+void launch_level_fused(const FusedArgs &a, const DCage &c, const LevelTab &t, int d, int gather, int creased, int topo,
+                        int uvMode, int points, int store, hipStream_t st) {
     const dim3 block(256);
     const dim3 grid((unsigned)((a.H + 255) / 256));
-    launch_fused_impl(a, gather ? 1 : 0, creased ? 1 : 0, topo ? 1 : 0, uvMode, points ? 1 : 0, store, grid, block, st);
+    launch_fused_impl(a, c, t, d, gather ? 1 : 0, creased ? 1 : 0, topo ? 1 : 0, uvMode, points ? 1 : 0, store, grid,
+                      block, st);
 }
 
 template <int SCAN>
@@ -258,6 +268,16 @@ void launch_level_vertices(const FusedArgs &a, const DCage &c, const LevelTab &t
         else if (gather) hipLaunchKernelGGL((k_level_vertices_direct<1, 0>), g3, block, 0, st, a, c, t, lp, d);
         else if (creased) hipLaunchKernelGGL((k_level_vertices_direct<0, 1>), g3, block, 0, st, a, c, t, lp, d);
         else hipLaunchKernelGGL((k_level_vertices_direct<0, 0>), g3, block, 0, st, a, c, t, lp, d);
+        return;
+    }
+    if (fan == 2 && scan == 0) {
+        const int64_t words = ((int64_t)a.H + 63) / 64;
+        const int64_t waves = (words + kOwnerWords - 1) / kOwnerWords;
+        const dim3 g0((unsigned)((waves + 3) / 4));
+        if (gather && creased) hipLaunchKernelGGL((k_level_vertices_masked<1, 1>), g0, block, 0, st, a, c, t, lp, d);
+        else if (gather) hipLaunchKernelGGL((k_level_vertices_masked<1, 0>), g0, block, 0, st, a, c, t, lp, d);
+        else if (creased) hipLaunchKernelGGL((k_level_vertices_masked<0, 1>), g0, block, 0, st, a, c, t, lp, d);
+        else hipLaunchKernelGGL((k_level_vertices_masked<0, 0>), g0, block, 0, st, a, c, t, lp, d);
         return;
     }
     if (fan == 2) {

@@ -15,7 +15,8 @@
 namespace cch {
 
 template <int GATHER, int CREASED, int TOPO, int UVMODE, int POINTS, int STORE>
-__device__ __forceinline__ void level_fused_block(const FusedArgs &a, uint32_t lb) {
+__device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCage &cg, const LevelTab &tb, int d,
+                                                  uint32_t lb) {
     constexpr bool kStaged = STORE == kStoreLds && TOPO && UVMODE != 0;
     __shared__ int4 stage[kStaged ? 4 * 256 : 1];
     __shared__ P3 spts[POINTS ? 256 : 1];
@@ -25,6 +26,11 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, uint32_t l
     const int j = h & 3;
     const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
     const int4 x = live ? he4[h] : make_int4(0, 0, 0, 0);
+    if (POINTS && a.owner_mask) {
+        // one bit per halfedge: is it the representative of its vertex?
+        const uint64_t m = __ballot(live && owner_test(h, x.z, d, cg, tb));
+        if ((threadIdx.x & 63u) == 0 && live) a.owner_mask[h >> 6] = m;
+    }
     const int32_t twin = x.x;
     const bool owner = live && h > twin;   // this halfedge writes its edge point
     // twin quads inside this block come from LDS, the others from memory
@@ -169,6 +175,65 @@ __device__ __forceinline__ void level_vertices_compact_block(const FusedArgs &a,
         owner_fan(h, v, d, c, t, lp.he, f);
         fan_vertex_store<GATHER, CREASED>(a, c, f, v);
     }
+}
+
+// Owners per wave of the masked vertex kernel: kOwnerWords mask words of 64
+// halfedges, about a quarter of which represent a vertex.
+constexpr int kOwnerWords = 4;
+
+// Calls fn(h) once per bit set in mask words [w0, w0 + GW) of `words`, one
+// halfedge per lane: popcounts prefix-summed across lanes 0..GW-1, lane l
+// takes the l-th marked halfedge, in rounds of 64.
+template <int GW, class Fn>
+__device__ __forceinline__ void mask_group(const uint64_t *mask, int32_t w0, int32_t words, const Fn &fn) {
+    const int lane = (int)(threadIdx.x & 63u);
+    const int32_t w = w0 + lane;
+    const uint64_t m = (lane < GW && w < words) ? mask[w] : 0ull;
+    int32_t pre = __popcll(m);
+#pragma unroll
+    for (int off = 1; off < GW; off <<= 1) {
+        const int32_t y = __shfl_up(pre, off);
+        if (lane >= off) pre += y;
+    }
+    const int32_t total = __shfl(pre, GW - 1);
+    for (int32_t base = 0; base < total; base += 64) {
+        const int32_t e = base + lane;
+        int32_t s = 0;
+#pragma unroll
+        for (int j = 0; j < GW; ++j) s += __shfl(pre, j) <= e ? 1 : 0;
+        const int src = s < GW ? s : GW - 1;
+        const int32_t before = __shfl(pre, s > 0 ? s - 1 : 0);
+        uint64_t ms = __shfl(m, src);
+        if (e < total) {
+            const int32_t r = e - (s > 0 ? before : 0);
+            for (int32_t k = 0; k < r; ++k) ms &= ms - 1ull;
+            fn((w0 + s) * 64 + (__ffsll((unsigned long long)ms) - 1));
+        }
+    }
+}
+
+template <int GATHER, int CREASED>
+struct MaskedVertexFn {
+    const FusedArgs &a;
+    const DCage &c;
+    const LevelTab &t;
+    const LevelPtrs &lp;
+    int d;
+    __device__ void operator()(int32_t h) const {
+        const int32_t v = reinterpret_cast<const int4 *>(a.he)[h].z;
+        Fan f;
+        owner_fan(h, v, d, c, t, lp.he, f);
+        fan_vertex_store<GATHER, CREASED>(a, c, f, v);
+    }
+};
+
+// vertex points of the owners marked by the fused kernel; `wave` is the
+// index of this wave among the level's waves (kOwnerWords words each)
+template <int GATHER, int CREASED>
+__device__ __forceinline__ void level_vertices_masked(const FusedArgs &a, const DCage &c, const LevelTab &t,
+                                                      const LevelPtrs &lp, int d, int32_t wave) {
+    const int32_t words = (a.H + 63) >> 6;
+    mask_group<kOwnerWords>(a.owner_mask, wave * kOwnerWords, words, MaskedVertexFn<GATHER, CREASED>{a, c, t, lp, d});
 }
 
 }  // namespace cch

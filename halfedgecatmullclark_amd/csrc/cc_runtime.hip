@@ -195,6 +195,7 @@ FusedArgs ccd_Subd::fused_args(int d, const P3 *points) const {
     a.xcd_remap = opt_remap;
     a.uv_refine = 0;
     a.defer_mask = nullptr;
+    a.owner_mask = nullptr;
     return a;
 }
 
@@ -247,12 +248,14 @@ void ccd_Subd::level_fused(int d, int32_t steps, int topo, int uvMode, P3 *point
     const bool gather = (steps & CCD_GATHER) != 0;
     const int creased = (steps & CCD_NOCREASES) ? 0 : 1;
     const bool pts_ = (steps & CCD_POINTS) != 0;
-    const FusedArgs a = fused_args(d, points);
+    FusedArgs a = fused_args(d, points);
+    const DCage c = cage_view(nullptr);
+    // the fused kernel marks the vertex representatives for the masked kernel
+    if (pts_ && opt_fan == 2 && cfg[d].scan == 0) a.owner_mask = defer_mask;
     if (topo || uvMode == 1 || pts_)
         launch_timed(pts_ ? (topo ? "level_fused" : "level_points") : "level_fused", n.H[d], st,
-                     [&] { launch_level_fused(a, gather, creased, topo, uvMode, pts_, opt_store, st); });
+                     [&] { launch_level_fused(a, c, tab, d, gather, creased, topo, uvMode, pts_, opt_store, st); });
     if (pts_) {
-        const DCage c = cage_view(nullptr);
         launch_timed("level_vertices", n.H[d], st, [&] {
             launch_level_vertices(a, c, tab, level_ptrs(), d, gather, creased, opt_fan, cfg[d].scan, st);
         });
@@ -730,7 +733,7 @@ int32_t ccd_SetOption(ccd_Subd *s, int32_t key, int32_t value) {
         for (auto &c : s->cfg) c.hpt = value;
         return 0;
     case CCD_OPT_SCAN:
-        if (value != 2 && value != 4 && value != 8) return 14;
+        if (value != 0 && value != 2 && value != 4 && value != 8) return 14;
         for (auto &c : s->cfg) c.scan = value;
         return 0;
     default: return 14;
@@ -744,7 +747,7 @@ int32_t ccd_SetOption(ccd_Subd *s, int32_t key, int32_t value) {
 int32_t ccd_Subd::autotune(int32_t steps, hipStream_t st) {
     if (!(steps & CCD_POINTS) || opt_fused == 0 || D < 2) return 0;
     std::vector<LevelCfg> cands;
-    for (int scan : {2, 4, 8}) cands.push_back({1, scan, 256, 1});
+    for (int scan : {0, 2, 4, 8}) cands.push_back({1, scan, 256, 1});
     for (int nt : {128, 256, 512}) cands.push_back({2, 4, nt, 1});
     float best[kMaxDepth + 1];
     LevelCfg pick[kMaxDepth + 1];

@@ -283,17 +283,19 @@ int32_t ccd_Synchronize(ccd_Subd *subd);
  *                      256 (default) or 512, every level
  *   CCD_OPT_PATCH_HPT  tiles (halfedges per thread) each workgroup of the
  *                      patch-local kernel processes in turn: 1 (default)..64
- *   CCD_OPT_SCAN       halfedges each thread of the compacted vertex kernel
- *                      tests: 2, 4 (default) or 8 */
+ *   CCD_OPT_SCAN       vertex kernel after the fused kernel: 0 (default)
+ *                      takes the vertex representatives the fused kernel
+ *                      marked in a bit mask; 2, 4 or 8 scans that many
+ *                      halfedges per thread for them */
 enum {
     CCD_OPT_FUSED = 0, CCD_OPT_STORE = 1, CCD_OPT_XCD_REMAP = 2, CCD_OPT_FAN = 3,
     CCD_OPT_PATCH_TILE = 4, CCD_OPT_PATCH_HPT = 5, CCD_OPT_SCAN = 6
 };
 int32_t ccd_SetOption(ccd_Subd *subd, int32_t key, int32_t value);
 /* Picks, per level, the kernel configuration of the fused paths by timing
- * every candidate on this subd: the fused kernels with 2, 4 or 8 halfedges
- * tested per thread of the vertex kernel, and the patch-local kernel with
- * tiles of 128, 256 or 512 halfedges.  Runs the given steps a few times per
+ * every candidate on this subd: the fused kernels with each CCD_OPT_SCAN
+ * vertex kernel, and the patch-local kernel with tiles of 128, 256 or 512
+ * halfedges.  Runs the given steps a few times per
  * candidate on the stream, synchronizing it; the choice is kept until the
  * next ccd_SetOption of these keys.  Returns 0 or an error code. */
 int32_t ccd_Autotune(ccd_Subd *subd, int32_t steps, void *stream);

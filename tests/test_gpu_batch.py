@@ -112,8 +112,8 @@ def test_batch_equals_single_cage(capi, label, steps, flags):
     b.close()
 
 
-@pytest.mark.parametrize("key,value", [("kind", 2), ("scan", 2), ("scan", 8), ("tile", 128), ("tile", 512),
-                                       ("remap", 0)])
+@pytest.mark.parametrize("key,value", [("kind", 2), ("scan", 2), ("scan", 4), ("scan", 8), ("tile", 128),
+                                       ("tile", 512), ("remap", 0)])
 def test_batch_kernel_options_agree(capi, key, value):
     cases = batch_cases()
     b = capi.Batch([c for c, _ in cases], [d for _, d in cases])
@@ -235,7 +235,7 @@ def test_batch_autotune_keeps_results(capi):
     opts = b.level_options()
     assert len(opts) == max(d for _, d in cases) - 1
     for o in opts:
-        assert o["kind"] in (1, 2) and o["scan"] in (2, 4, 8) and o["tile"] in (128, 256, 512)
+        assert o["kind"] in (1, 2) and o["scan"] in (0, 2, 4, 8) and o["tile"] in (128, 256, 512)
     b.refine(capi.ALL)
     got = download_all(b)
     for i in range(len(cases)):

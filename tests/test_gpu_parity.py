@@ -62,7 +62,7 @@ def test_refine_drivers(capi, name, cage, depth, driver, flags):
         assert err32 <= POINT_TOL, err32
 
 
-PATHS = [  # (fused, store, xcd remap, vertex rings)
+PATHS = [  # (fused, store, xcd remap, vertex rings); vertex kernel: the default (owner bits)
     (1, 1, 1, 3),
     (2, 1, 1, 2),
     (2, 0, 0, 2),
@@ -102,17 +102,19 @@ def test_device_paths(capi, name, cage, depth, fused, store, remap, fan):
     ds.close()
 
 
-DETERMINISTIC = [  # (fused, fan): paths whose scatter sums run in ascending halfedge order
-    (1, 2),
-    (1, 1),
-    (1, 3),
-    (2, 2),
+DETERMINISTIC = [  # (fused, fan, scan): paths whose scatter sums run in ascending halfedge order
+    (1, 2, 0),
+    (1, 2, 4),
+    (1, 1, 0),
+    (1, 3, 0),
+    (2, 2, 0),
 ]
 
 
-@pytest.mark.parametrize("fused,fan", DETERMINISTIC, ids=["fused_compact", "fused_fan", "fused_vertexdirect", "patch"])
+@pytest.mark.parametrize("fused,fan,scan", DETERMINISTIC,
+                         ids=["fused_masked", "fused_compact", "fused_fan", "fused_vertexdirect", "patch"])
 @pytest.mark.parametrize("name,cage,depth", small_cages(), ids=[c[0] for c in small_cages()])
-def test_scatter_points_bitexact(capi, name, cage, depth, fused, fan):
+def test_scatter_points_bitexact(capi, name, cage, depth, fused, fan, scan):
     """Scatter vertex points are sums of per-halfedge contributions; the fused
     and patch paths add them from zero in ascending halfedge order, as the
     reference's sequential scatter does, so they equal the float oracle bit
@@ -121,6 +123,7 @@ def test_scatter_points_bitexact(capi, name, cage, depth, fused, fan):
     ds = capi.DeviceSubd(cage, depth)
     ds.set_option(capi.OPT_FUSED, fused)
     ds.set_option(capi.OPT_FAN, fan)
+    ds.set_option(capi.OPT_SCAN, scan)
     for label, steps, flags in (STEPS[0], STEPS[2]):
         ds.refine(steps)
         got = ds.download()
