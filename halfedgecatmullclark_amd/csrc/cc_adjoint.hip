@@ -294,16 +294,26 @@ struct VertexGatherFn {
     }
 };
 
+// (the level tables go to LDS first: owners_block's first barrier orders
+// the copy before any read)
 template <typename T, int CREASED>
 __device__ __forceinline__ void adj_weights_block(const AdjArgs<T> &a, const DCage &c, const LevelTab &t,
                                                   const LevelPtrs &lp, int d, uint32_t lb) {
-    owners_block<kAdjScan>(a.he, a.H, d, c, t, lb, RingWeightsFn<T, CREASED>{a, c, t, lp, d});
+    __shared__ LevelTab st;
+    __shared__ LevelPtrs slp;
+    lds_fill(st, t);
+    lds_fill(slp, lp);
+    owners_block<kAdjScan>(a.he, a.H, d, c, st, lb, RingWeightsFn<T, CREASED>{a, c, st, slp, d});
 }
 
 template <typename T, int CREASED>
 __device__ __forceinline__ void adj_vertices_block(const AdjArgs<T> &a, const DCage &c, const LevelTab &t,
                                                    const LevelPtrs &lp, int d, uint32_t lb) {
-    owners_block<kAdjScan>(a.he, a.H, d, c, t, lb, VertexGatherFn<T, CREASED>{a, c, t, lp, d});
+    __shared__ LevelTab st;
+    __shared__ LevelPtrs slp;
+    lds_fill(st, t);
+    lds_fill(slp, lp);
+    owners_block<kAdjScan>(a.he, a.H, d, c, st, lb, VertexGatherFn<T, CREASED>{a, c, st, slp, d});
 }
 
 // ---- the cage step (level 0 -> 1) -----------------------------------------------

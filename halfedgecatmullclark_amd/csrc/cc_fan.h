@@ -17,6 +17,18 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb, int remap
     return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// Copies a small table (kernel argument or job field) into LDS, one dword
+// per thread.  Reads of LevelTab / LevelPtrs at a per-lane index become
+// vector memory loads when the table is a kernel argument, with one memory
+// round trip per step of the level loops; from LDS they are cheap.  The
+// caller synchronizes the block before the first read.
+template <class T>
+__device__ __forceinline__ void lds_fill(T &dst, const T &src) {
+    static_assert(sizeof(T) % 4 == 0, "tables are whole dwords");
+    for (int i = (int)threadIdx.x; i < (int)(sizeof(T) / 4); i += (int)blockDim.x)
+        reinterpret_cast<int32_t *>(&dst)[i] = reinterpret_cast<const int32_t *>(&src)[i];
+}
+
 // value of `v` held by lane (quad base + k) for k = 0..3 (DPP quad_perm)
 template <int K>
 __device__ __forceinline__ int quad_bcast(int v) {

@@ -14,23 +14,64 @@
 
 namespace cch {
 
+// face point of h's quad (lane 0 of the quad) and, for the larger halfedge
+// of each pair, the edge point
+template <int GATHER, int CREASED>
+__device__ __forceinline__ void level_fused_points(const FusedArgs &a, int32_t h, int j, int4 x, P3 P, P3 q0, P3 q1,
+                                                   P3 q2, P3 q3, bool owner, bool twin_here, int32_t base,
+                                                   const P3 *spts, int4 tr0, int4 tr1, int4 tr2, int4 tr3) {
+    const int32_t twin = x.x;
+    const P3 F = face_rule(q0, q1, q2, q3, GATHER);
+    if (j == 0) a.fp[h >> 2] = F;
+    if (!owner) return;
+    // this halfedge owns its edge: both contributions
+    const P3 v1 = pick4(q0, q1, q2, q3, (j + 1) & 3);
+    const float s = CREASED ? lsharp(a.cr, a.C, x.y) : 0.0f;
+    P3 Ev;
+    if (twin >= 0) {
+        P3 t0, t1, t2, t3;
+        if (twin_here) {
+            const int tb = (twin & ~3) - base;
+            t0 = spts[tb]; t1 = spts[tb + 1]; t2 = spts[tb + 2]; t3 = spts[tb + 3];
+        } else {
+            t0 = a.old[tr0.z]; t1 = a.old[tr1.z]; t2 = a.old[tr2.z]; t3 = a.old[tr3.z];
+        }
+        const P3 Ft = face_rule(t0, t1, t2, t3, GATHER);
+        if (GATHER) {
+            const float w = CREASED ? sat(s) : 1.0f;
+            Ev = edge_gather(P, v1, F, Ft, w, CREASED);
+        } else {
+            const int tj = twin & 3;
+            const P3 ct = edge_contrib(pick4(t0, t1, t2, t3, tj), pick4(t0, t1, t2, t3, (tj + 1) & 3), Ft, false, s,
+                                       CREASED);
+            const P3 ch = edge_contrib(P, v1, F, false, s, CREASED);
+            Ev = add(add(zero3(), ct), ch);
+        }
+    } else {
+        if (GATHER) {
+            const float w = CREASED ? sat(s) : 0.0f;
+            Ev = edge_gather(P, v1, F, quad_face(a.he, a.old, 0, true), w, CREASED);
+        } else {
+            Ev = add(zero3(), edge_contrib(P, v1, F, true, s, CREASED));
+        }
+    }
+    a.ep[x.y] = Ev;
+}
+
 template <int GATHER, int CREASED, int TOPO, int UVMODE, int POINTS, int STORE>
 __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCage &cg, const LevelTab &tb, int d,
                                                   uint32_t lb) {
     constexpr bool kStaged = STORE == kStoreLds && TOPO && UVMODE != 0;
     __shared__ int4 stage[kStaged ? 4 * 256 : 1];
     __shared__ P3 spts[POINTS ? 256 : 1];
+    __shared__ LevelTab stab;   // for the owner test (read after the block's barrier)
+    if (POINTS && a.owner_mask) lds_fill(stab, tb);
     const int32_t base = (int32_t)(lb * blockDim.x);
     const int32_t h = base + (int32_t)threadIdx.x;
     const bool live = h < a.H;   // H is a multiple of 4: quads are whole
     const int j = h & 3;
     const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
     const int4 x = live ? he4[h] : make_int4(0, 0, 0, 0);
-    if (POINTS && a.owner_mask) {
-        // one bit per halfedge: is it the representative of its vertex?
-        const uint64_t m = __ballot(live && owner_test(h, x.z, d, cg, tb));
-        if ((threadIdx.x & 63u) == 0 && live) a.owner_mask[h >> 6] = m;
-    }
     const int32_t twin = x.x;
     const bool owner = live && h > twin;   // this halfedge writes its edge point
     // twin quads inside this block come from LDS, the others from memory
@@ -105,43 +146,16 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
         __syncthreads();
     }
     const P3 q0 = quad_bcast3<0>(P), q1 = quad_bcast3<1>(P), q2 = quad_bcast3<2>(P), q3 = quad_bcast3<3>(P);
-    if (!live) return;
-    const P3 F = face_rule(q0, q1, q2, q3, GATHER);
-    if (j == 0) a.fp[h >> 2] = F;
-    if (!owner) return;
-    // this halfedge owns its edge: both contributions
-    const P3 v1 = pick4(q0, q1, q2, q3, (j + 1) & 3);
-    const float s = CREASED ? lsharp(a.cr, a.C, x.y) : 0.0f;
-    P3 Ev;
-    if (twin >= 0) {
-        P3 t0, t1, t2, t3;
-        if (twin_here) {
-            const int tb = (twin & ~3) - base;
-            t0 = spts[tb]; t1 = spts[tb + 1]; t2 = spts[tb + 2]; t3 = spts[tb + 3];
-        } else {
-            t0 = a.old[tr0.z]; t1 = a.old[tr1.z]; t2 = a.old[tr2.z]; t3 = a.old[tr3.z];
-        }
-        const P3 Ft = face_rule(t0, t1, t2, t3, GATHER);
-        if (GATHER) {
-            const float w = CREASED ? sat(s) : 1.0f;
-            Ev = edge_gather(P, v1, F, Ft, w, CREASED);
-        } else {
-            const int tj = twin & 3;
-            const P3 ct = edge_contrib(pick4(t0, t1, t2, t3, tj), pick4(t0, t1, t2, t3, (tj + 1) & 3), Ft, false, s,
-                                       CREASED);
-            const P3 ch = edge_contrib(P, v1, F, false, s, CREASED);
-            Ev = add(add(zero3(), ct), ch);
-        }
-    } else {
-        if (GATHER) {
-            const float w = CREASED ? sat(s) : 0.0f;
-            Ev = edge_gather(P, v1, F, quad_face(a.he, a.old, 0, true), w, CREASED);
-        } else {
-            Ev = add(zero3(), edge_contrib(P, v1, F, true, s, CREASED));
-        }
+    if (live) level_fused_points<GATHER, CREASED>(a, h, j, x, P, q0, q1, q2, q3, owner, twin_here, base, spts, tr0, tr1,
+                                                   tr2, tr3);
+    if (a.owner_mask) {
+        // one bit per halfedge: is it the representative of its vertex?  Last,
+        // so that the cage lookups of a few lanes overlap the wave's stores
+        const uint64_t m = __ballot(live && owner_test(h, x.z, d, cg, stab));
+        if ((threadIdx.x & 63u) == 0 && live) a.owner_mask[h >> 6] = m;
     }
-    a.ep[x.y] = Ev;
 }
+
 
 // Vertex points with owners compacted per block: a block scans
 // 256 * SCAN halfedges, collects the representatives in LDS, then every
@@ -151,6 +165,10 @@ __device__ __forceinline__ void level_vertices_compact_block(const FusedArgs &a,
                                                              const LevelPtrs &lp, int d, uint32_t lb) {
     __shared__ int32_t owners[256 * SCAN];
     __shared__ int32_t nown;
+    __shared__ LevelTab st;
+    __shared__ LevelPtrs slp;
+    lds_fill(st, t);
+    lds_fill(slp, lp);
     const int32_t base = (int32_t)(lb * 256 * SCAN);
     const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
     if (threadIdx.x == 0) nown = 0;
@@ -164,7 +182,7 @@ __device__ __forceinline__ void level_vertices_compact_block(const FusedArgs &a,
 #pragma unroll
     for (int k = 0; k < SCAN; ++k) {
         const int32_t h = base + k * 256 + (int32_t)threadIdx.x;
-        if (vs[k] >= 0 && owner_test(h, vs[k], d, c, t)) owners[atomicAdd(&nown, 1)] = h;
+        if (vs[k] >= 0 && owner_test(h, vs[k], d, c, st)) owners[atomicAdd(&nown, 1)] = h;
     }
     __syncthreads();
     const int32_t count = nown;
@@ -172,7 +190,7 @@ __device__ __forceinline__ void level_vertices_compact_block(const FusedArgs &a,
         const int32_t h = owners[i];
         const int32_t v = he4[h].z;
         Fan f;
-        owner_fan(h, v, d, c, t, lp.he, f);
+        owner_fan(h, v, d, c, st, slp.he, f);
         fan_vertex_store<GATHER, CREASED>(a, c, f, v);
     }
 }
@@ -232,8 +250,14 @@ struct MaskedVertexFn {
 template <int GATHER, int CREASED>
 __device__ __forceinline__ void level_vertices_masked(const FusedArgs &a, const DCage &c, const LevelTab &t,
                                                       const LevelPtrs &lp, int d, int32_t wave) {
+    __shared__ LevelTab st;
+    __shared__ LevelPtrs slp;
+    lds_fill(st, t);
+    lds_fill(slp, lp);
+    __syncthreads();
     const int32_t words = (a.H + 63) >> 6;
-    mask_group<kOwnerWords>(a.owner_mask, wave * kOwnerWords, words, MaskedVertexFn<GATHER, CREASED>{a, c, t, lp, d});
+    mask_group<kOwnerWords>(a.owner_mask, wave * kOwnerWords, words,
+                            MaskedVertexFn<GATHER, CREASED>{a, c, st, slp, d});
 }
 
 }  // namespace cch

@@ -60,6 +60,10 @@ __device__ __forceinline__ void patch_tile(const FusedArgs &a, const DCage &c, c
                                            int d, uint32_t tile) {
     constexpr bool kStaged = TOPO && UVMODE != 0;
     __shared__ PatchLds<NT, kStaged> L;
+    __shared__ LevelTab st;     // read after the tile's first barrier
+    __shared__ LevelPtrs slp;
+    lds_fill(st, t);
+    lds_fill(slp, lp);
     const int32_t base = (int32_t)(tile * NT);
     const int tid = (int)threadIdx.x;
     const int32_t h = base + tid;
@@ -168,7 +172,7 @@ __device__ __forceinline__ void patch_tile(const FusedArgs &a, const DCage &c, c
 
     // vertex point of the vertex this halfedge represents
     Fan f;
-    const bool owner = live && owner_fan(h, x.z, d, c, t, lp.he, f);
+    const bool owner = live && owner_fan(h, x.z, d, c, st, slp.he, f);
     bool local = owner && (f.kind == 1 || f.kind == 3);
     int32_t r[4];
 #pragma unroll
@@ -221,8 +225,13 @@ constexpr int kDeferGroup = 16;
 // per lane: the masks' popcounts are prefix-summed across lanes 0..15 and
 // lane l takes the l-th marked halfedge, in rounds of 64.
 template <int GATHER, int CREASED>
-__device__ __forceinline__ void deferred_group(const FusedArgs &a, const DCage &c, const LevelTab &t,
-                                               const LevelPtrs &lp, int d, int32_t w0, int32_t words) {
+__device__ __forceinline__ void deferred_group(const FusedArgs &a, const DCage &c, const LevelTab &tg,
+                                               const LevelPtrs &lpg, int d, int32_t w0, int32_t words) {
+    __shared__ LevelTab t;
+    __shared__ LevelPtrs lp;
+    lds_fill(t, tg);
+    lds_fill(lp, lpg);
+    __syncthreads();
     const int lane = (int)(threadIdx.x & 63u);
     const int32_t w = w0 + lane;
     const uint64_t m = (lane < kDeferGroup && w < words) ? a.defer_mask[w] : 0ull;
