@@ -78,6 +78,28 @@ struct DCage {
     int32_t V, U, H, E, F;
 };
 
+// Loads through explicit address spaces.  A pointer that may point to LDS
+// or global memory compiles to flat instructions, which also count against
+// the LDS wait counter; these keep global loads global and LDS loads LDS.
+template <class T>
+__device__ __forceinline__ T gld(const T *p) {
+    static_assert(sizeof(T) % 4 == 0, "whole dwords");
+    T r;
+    const __attribute__((address_space(1))) uint32_t *q = (const __attribute__((address_space(1))) uint32_t *)(p);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) reinterpret_cast<uint32_t *>(&r)[i] = q[i];
+    return r;
+}
+template <class T>
+__device__ __forceinline__ T sld(const T *p) {
+    static_assert(sizeof(T) % 4 == 0, "whole dwords");
+    T r;
+    const __attribute__((address_space(3))) uint32_t *q = (const __attribute__((address_space(3))) uint32_t *)(p);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) reinterpret_cast<uint32_t *>(&r)[i] = q[i];
+    return r;
+}
+
 // ---- scalar rules (CatmullClark.h:296-392, :667-689) ----------------------
 __host__ __device__ __forceinline__ int32_t qnext(int32_t h) {
     return (int32_t)(((uint32_t)h & ~3u) | (((uint32_t)h + 1u) & 3u));

@@ -29,13 +29,15 @@ __device__ __forceinline__ void level_fused_points(const FusedArgs &a, int32_t h
     const float s = CREASED ? lsharp(a.cr, a.C, x.y) : 0.0f;
     P3 Ev;
     if (twin >= 0) {
-        P3 t0, t1, t2, t3;
-        if (twin_here) {
-            const int tb = (twin & ~3) - base;
-            t0 = spts[tb]; t1 = spts[tb + 1]; t2 = spts[tb + 2]; t3 = spts[tb + 3];
-        } else {
-            t0 = a.old[tr0.z]; t1 = a.old[tr1.z]; t2 = a.old[tr2.z]; t3 = a.old[tr3.z];
-        }
+        // the twin quad's points from LDS when it is in this block, else from
+        // memory; both reads are valid for every lane (tr* are the records of
+        // quad 0 for an in-block twin), and selecting afterwards keeps LDS
+        // reads and global reads in their own instructions
+        const int tb = twin_here ? (twin & ~3) - base : 0;
+        const P3 l0 = sld(spts + tb), l1 = sld(spts + tb + 1), l2 = sld(spts + tb + 2), l3 = sld(spts + tb + 3);
+        const P3 g0 = gld(a.old + tr0.z), g1 = gld(a.old + tr1.z), g2 = gld(a.old + tr2.z), g3 = gld(a.old + tr3.z);
+        const P3 t0 = twin_here ? l0 : g0, t1 = twin_here ? l1 : g1, t2 = twin_here ? l2 : g2,
+                 t3 = twin_here ? l3 : g3;
         const P3 Ft = face_rule(t0, t1, t2, t3, GATHER);
         if (GATHER) {
             const float w = CREASED ? sat(s) : 1.0f;
