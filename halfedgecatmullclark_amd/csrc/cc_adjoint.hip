@@ -296,9 +296,9 @@ struct VertexGatherFn {
 
 // (the level tables go to LDS first: owners_block's first barrier orders
 // the copy before any read)
-template <typename T, int CREASED>
-__device__ __forceinline__ void adj_weights_block(const AdjArgs<T> &a, const DCage &c, const LevelTab &t,
-                                                  const LevelPtrs &lp, int d, uint32_t lb) {
+template <typename T, int CREASED, class LP>
+__device__ __forceinline__ void adj_weights_block(const AdjArgs<T> &a, const DCage &c, const LevelTab &t, const LP &lp,
+                                                  int d, uint32_t lb) {
     __shared__ LevelTab st;
     __shared__ LevelPtrs slp;
     lds_fill(st, t);
@@ -306,9 +306,9 @@ __device__ __forceinline__ void adj_weights_block(const AdjArgs<T> &a, const DCa
     owners_block<kAdjScan>(a.he, a.H, d, c, st, lb, RingWeightsFn<T, CREASED>{a, c, st, slp, d});
 }
 
-template <typename T, int CREASED>
+template <typename T, int CREASED, class LP>
 __device__ __forceinline__ void adj_vertices_block(const AdjArgs<T> &a, const DCage &c, const LevelTab &t,
-                                                   const LevelPtrs &lp, int d, uint32_t lb) {
+                                                   const LP &lp, int d, uint32_t lb) {
     __shared__ LevelTab st;
     __shared__ LevelPtrs slp;
     lds_fill(st, t);
@@ -571,66 +571,103 @@ namespace {
 template <int CREASED>
 struct OpAdjWeights {
     int d;
+    char *arena, *adj;
     __device__ void operator()(const AdjJob &J, int64_t i) const {
-        adj_weights_block<float, CREASED>(J.a, J.c, J.t, J.lp, d, (uint32_t)(i >> 8));
+        AdjArgs<float> a;
+        DCage c;
+        unpack_adj(J, arena, adj, a, c);
+        const JobLevelPtrs lp{J.lp, arena};
+        adj_weights_block<float, CREASED>(a, c, J.t, lp, d, (uint32_t)(i >> 8));
     }
 };
 struct OpAdjEdges {
-    __device__ void operator()(const AdjJob &J, int64_t i) const { adj_edge_at<float>(J.a, (int32_t)i); }
+    char *arena, *adj;
+    __device__ void operator()(const AdjJob &J, int64_t i) const {
+        AdjArgs<float> a;
+        DCage c;
+        unpack_adj(J, arena, adj, a, c);
+        adj_edge_at<float>(a, (int32_t)i);
+    }
 };
 template <int CREASED>
 struct OpAdjFaces {
-    __device__ void operator()(const AdjJob &J, int64_t i) const { adj_face_at<float, CREASED>(J.a, (int32_t)i); }
+    char *arena, *adj;
+    __device__ void operator()(const AdjJob &J, int64_t i) const {
+        AdjArgs<float> a;
+        DCage c;
+        unpack_adj(J, arena, adj, a, c);
+        adj_face_at<float, CREASED>(a, (int32_t)i);
+    }
 };
 template <int CREASED>
 struct OpAdjVertices {
     int d;
+    char *arena, *adj;
     __device__ void operator()(const AdjJob &J, int64_t i) const {
-        adj_vertices_block<float, CREASED>(J.a, J.c, J.t, J.lp, d, (uint32_t)(i >> 8));
+        AdjArgs<float> a;
+        DCage c;
+        unpack_adj(J, arena, adj, a, c);
+        const JobLevelPtrs lp{J.lp, arena};
+        adj_vertices_block<float, CREASED>(a, c, J.t, lp, d, (uint32_t)(i >> 8));
     }
 };
 template <int CREASED>
 struct OpAdjCageVertices {
+    char *arena, *adj;
     __device__ void operator()(const AdjJob &J, int64_t i) const {
-        adj_cage_vertices_at<float, CREASED>(J.a, J.c, (int32_t)i);
+        AdjArgs<float> a;
+        DCage c;
+        unpack_adj(J, arena, adj, a, c);
+        adj_cage_vertices_at<float, CREASED>(a, c, (int32_t)i);
     }
 };
 template <int CREASED>
 struct OpAdjCageEdges {
+    char *arena, *adj;
     __device__ void operator()(const AdjJob &J, int64_t i) const {
-        adj_cage_edges_at<float, CREASED>(J.a, J.c, (int32_t)i);
+        AdjArgs<float> a;
+        DCage c;
+        unpack_adj(J, arena, adj, a, c);
+        adj_cage_edges_at<float, CREASED>(a, c, (int32_t)i);
     }
 };
 struct OpAdjCageFaces {
-    __device__ void operator()(const AdjJob &J, int64_t i) const { adj_cage_faces_at<float>(J.a, J.c, (int32_t)i); }
+    char *arena, *adj;
+    __device__ void operator()(const AdjJob &J, int64_t i) const {
+        AdjArgs<float> a;
+        DCage c;
+        unpack_adj(J, arena, adj, a, c);
+        adj_cage_faces_at<float>(a, c, (int32_t)i);
+    }
 };
 
 }  // namespace
 
 // level d of every job: weights, edges, faces, vertices in turn; `quads`
 // has one workgroup per 1024 halfedges of each job, `halfedges` one per 256
-void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, int d, int creased, hipStream_t st) {
-    if (creased) launch_batch<AdjJob>(quads, OpAdjWeights<1>{d}, st);
-    else launch_batch<AdjJob>(quads, OpAdjWeights<0>{d}, st);
-    launch_batch<AdjJob>(halfedges, OpAdjEdges{}, st);
+void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, char *arena, char *adj, int d, int creased,
+                     hipStream_t st) {
+    if (creased) launch_batch<AdjJob>(quads, OpAdjWeights<1>{d, arena, adj}, st);
+    else launch_batch<AdjJob>(quads, OpAdjWeights<0>{d, arena, adj}, st);
+    launch_batch<AdjJob>(halfedges, OpAdjEdges{arena, adj}, st);
     if (creased) {
-        launch_batch<AdjJob>(quads, OpAdjFaces<1>{}, st);
-        launch_batch<AdjJob>(quads, OpAdjVertices<1>{d}, st);
+        launch_batch<AdjJob>(quads, OpAdjFaces<1>{arena, adj}, st);
+        launch_batch<AdjJob>(quads, OpAdjVertices<1>{d, arena, adj}, st);
     } else {
-        launch_batch<AdjJob>(quads, OpAdjFaces<0>{}, st);
-        launch_batch<AdjJob>(quads, OpAdjVertices<0>{d}, st);
+        launch_batch<AdjJob>(quads, OpAdjFaces<0>{arena, adj}, st);
+        launch_batch<AdjJob>(quads, OpAdjVertices<0>{d, arena, adj}, st);
     }
 }
 
-void batch_adj_cage(const BatchTable &t, int creased, hipStream_t st) {
+void batch_adj_cage(const BatchTable &t, char *arena, char *adj, int creased, hipStream_t st) {
     if (creased) {
-        launch_batch<AdjJob>(t, OpAdjCageVertices<1>{}, st);
-        launch_batch<AdjJob>(t, OpAdjCageEdges<1>{}, st);
+        launch_batch<AdjJob>(t, OpAdjCageVertices<1>{arena, adj}, st);
+        launch_batch<AdjJob>(t, OpAdjCageEdges<1>{arena, adj}, st);
     } else {
-        launch_batch<AdjJob>(t, OpAdjCageVertices<0>{}, st);
-        launch_batch<AdjJob>(t, OpAdjCageEdges<0>{}, st);
+        launch_batch<AdjJob>(t, OpAdjCageVertices<0>{arena, adj}, st);
+        launch_batch<AdjJob>(t, OpAdjCageEdges<0>{arena, adj}, st);
     }
-    launch_batch<AdjJob>(t, OpAdjCageFaces{}, st);
+    launch_batch<AdjJob>(t, OpAdjCageFaces{arena, adj}, st);
 }
 
 template void adjoint_levels<float>(const AdjPlan &, const float *, float *, float *, float *, float *, float *,

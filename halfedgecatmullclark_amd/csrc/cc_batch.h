@@ -30,12 +30,18 @@ struct CreaseJob {
     int32_t C;
 };
 
-// one cage's level d -> d+1
+// One cage's level d -> d+1.  Buffers are byte offsets into the batch's
+// arena (one allocation whose address is a kernel argument, so that the
+// compiler knows every address derived from it is global memory and emits
+// global rather than flat instructions); -1 marks an absent buffer.
 struct LevelJob {
-    FusedArgs a;
-    DCage c;
+    int64_t he, out_he, cr, old, vp, fp, ep, defer_mask, owner_mask;
+    int64_t c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr;
+    int64_t lp[kMaxDepth + 1];
+    int32_t H, V, F, E, C;
+    int32_t cV, cU, cH, cE, cF;
+    int32_t xcd_remap, uv_refine;
     LevelTab t;
-    LevelPtrs lp;
 };
 
 // a range of n floats copied from src + src_off to dst + dst_off, the base
@@ -44,13 +50,68 @@ struct SpanJob {
     int64_t src_off, dst_off, n;
 };
 
-// one cage's adjoint step d+1 -> d (d = 0 is the cage step)
+// One cage's adjoint step d+1 -> d (d = 0 is the cage step): he, cr and the
+// cage are offsets into the arena, the gradients and weights into the
+// adjoint arena.
 struct AdjJob {
-    AdjArgs<float> a;
-    DCage c;
+    int64_t he, cr, gV, gF, gE, gP, W;
+    int64_t c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr;
+    int64_t lp[kMaxDepth + 1];
+    int32_t H, C;
+    int32_t cV, cU, cH, cE, cF;
     LevelTab t;
-    LevelPtrs lp;
 };
+
+template <class T>
+__host__ __device__ __forceinline__ T *arena_ptr(char *base, int64_t off) {
+    return off < 0 ? nullptr : reinterpret_cast<T *>(base + off);
+}
+
+__host__ __device__ __forceinline__ DCage arena_cage(char *base, int64_t he, int64_t v2h, int64_t e2h, int64_t f2h,
+                                                     int64_t pts, int64_t uvs, int64_t cr, int32_t V, int32_t U,
+                                                     int32_t H, int32_t E, int32_t F) {
+    DCage c;
+    c.he = arena_ptr<const CageHe>(base, he);
+    c.v2h = arena_ptr<const int32_t>(base, v2h);
+    c.e2h = arena_ptr<const int32_t>(base, e2h);
+    c.f2h = arena_ptr<const int32_t>(base, f2h);
+    c.pts = arena_ptr<const P3>(base, pts);
+    c.uvs = arena_ptr<const UV>(base, uvs);
+    c.cr = arena_ptr<const Crease>(base, cr);
+    c.V = V; c.U = U; c.H = H; c.E = E; c.F = F;
+    return c;
+}
+
+// the kernel arguments of one cage's level from its job and the arena
+__host__ __device__ __forceinline__ void unpack_level(const LevelJob &J, char *arena, FusedArgs &a, DCage &c) {
+    a.he = arena_ptr<const He>(arena, J.he);
+    a.out_he = arena_ptr<He>(arena, J.out_he);
+    a.cr = arena_ptr<const Crease>(arena, J.cr);
+    a.old = arena_ptr<const P3>(arena, J.old);
+    a.vp = arena_ptr<P3>(arena, J.vp);
+    a.fp = arena_ptr<P3>(arena, J.fp);
+    a.ep = arena_ptr<P3>(arena, J.ep);
+    a.defer_mask = arena_ptr<uint64_t>(arena, J.defer_mask);
+    a.owner_mask = arena_ptr<uint64_t>(arena, J.owner_mask);
+    a.H = J.H; a.V = J.V; a.F = J.F; a.E = J.E; a.C = J.C;
+    a.xcd_remap = J.xcd_remap;
+    a.uv_refine = J.uv_refine;
+    c = arena_cage(arena, J.c_he, J.c_v2h, J.c_e2h, J.c_f2h, J.c_pts, J.c_uvs, J.c_cr, J.cV, J.cU, J.cH, J.cE, J.cF);
+}
+
+__host__ __device__ __forceinline__ void unpack_adj(const AdjJob &J, char *arena, char *adj, AdjArgs<float> &a,
+                                                    DCage &c) {
+    a.he = arena_ptr<const He>(arena, J.he);
+    a.cr = arena_ptr<const Crease>(arena, J.cr);
+    a.sd = nullptr;
+    a.H = J.H; a.C = J.C;
+    a.gV = arena_ptr<float>(adj, J.gV);
+    a.gF = arena_ptr<float>(adj, J.gF);
+    a.gE = arena_ptr<float>(adj, J.gE);
+    a.gP = arena_ptr<float>(adj, J.gP);
+    a.W = arena_ptr<float>(adj, J.W);
+    c = arena_cage(arena, J.c_he, J.c_v2h, J.c_e2h, J.c_f2h, J.c_pts, J.c_uvs, J.c_cr, J.cV, J.cU, J.cH, J.cE, J.cF);
+}
 
 // largest k in [0, n) with start[k] <= b (jobs without workgroups are skipped)
 __device__ __forceinline__ int find_job(const int32_t *__restrict__ start, int n, uint32_t b) {
@@ -97,17 +158,20 @@ void batch_zero_cage_vertices(const BatchTable &t, hipStream_t st);
 void batch_copy(const BatchTable &t, const float *src, float *dst, hipStream_t st);
 // patch-local level step; `start` counts workgroups of hpt tiles of nt
 // halfedges per job
-void batch_level_patch(const BatchTable &t, int d, int gather, int creased, int topo, int nt, int hpt, int remap,
-                       hipStream_t st);
-void batch_level_deferred(const BatchTable &t, int d, int gather, int creased, hipStream_t st);
+void batch_level_patch(const BatchTable &t, char *arena, int d, int gather, int creased, int topo, int nt, int hpt,
+                       int remap, hipStream_t st);
+void batch_level_deferred(const BatchTable &t, char *arena, int d, int gather, int creased, hipStream_t st);
 // fused children + face/edge points (one workgroup per 256 halfedges), then
 // vertex points (one workgroup per 256 * scan halfedges; scan 0: the owners
 // the fused kernel marked, one workgroup per 1024 mask words)
-void batch_level_fused(const BatchTable &t, int d, int gather, int creased, int topo, int remap, hipStream_t st);
-void batch_level_vertices(const BatchTable &t, int d, int gather, int creased, int scan, int remap, hipStream_t st);
+void batch_level_fused(const BatchTable &t, char *arena, int d, int gather, int creased, int topo, int remap,
+                       hipStream_t st);
+void batch_level_vertices(const BatchTable &t, char *arena, int d, int gather, int creased, int scan, int remap,
+                          hipStream_t st);
 
 // adjoint launches (cc_adjoint.hip)
-void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, int d, int creased, hipStream_t st);
-void batch_adj_cage(const BatchTable &t, int creased, hipStream_t st);
+void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, char *arena, char *adj, int d, int creased,
+                     hipStream_t st);
+void batch_adj_cage(const BatchTable &t, char *arena, char *adj, int creased, hipStream_t st);
 
 }  // namespace cch

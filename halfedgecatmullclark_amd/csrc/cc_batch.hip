@@ -71,17 +71,21 @@ struct OpCopy {
 // tiles [b*hpt, b*hpt + hpt) of job k's level, b the job-local workgroup
 template <int GATHER, int CREASED, int TOPO, int NT>
 __global__ __launch_bounds__(NT) void k_batch_patch(const LevelJob *__restrict__ jobs, const int32_t *__restrict__ start,
-                                                    int n, int d, uint32_t hpt, int remap) {
+                                                    int n, int d, uint32_t hpt, int remap, char *arena) {
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, remap);
     const int k = find_job(start, n, lb);
     const LevelJob &J = jobs[k];
-    const uint32_t tiles = (uint32_t)((J.a.H + NT - 1) / NT);
+    FusedArgs a;
+    DCage c;
+    unpack_level(J, arena, a, c);
+    const JobLevelPtrs lp{J.lp, arena};
+    const uint32_t tiles = (uint32_t)((a.H + NT - 1) / NT);
     const uint32_t b = lb - (uint32_t)start[k];
     for (uint32_t it = 0; it < hpt; ++it) {
         const uint32_t tile = b * hpt + it;
         if (tile >= tiles) break;
         if (it) __syncthreads();   // the previous tile's LDS reads are done
-        patch_tile<GATHER, CREASED, TOPO, TOPO ? 3 : 0, NT>(J.a, J.c, J.t, J.lp, d, tile);
+        patch_tile<GATHER, CREASED, TOPO, TOPO ? 3 : 0, NT>(a, c, J.t, lp, d, tile);
     }
 }
 
@@ -89,11 +93,14 @@ __global__ __launch_bounds__(NT) void k_batch_patch(const LevelJob *__restrict__
 // halfedges)
 template <int GATHER, int CREASED, int TOPO>
 __global__ __launch_bounds__(256) void k_batch_fused(const LevelJob *__restrict__ jobs, const int32_t *__restrict__ start,
-                                                     int n, int d, int remap) {
+                                                     int n, int d, int remap, char *arena) {
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, remap);
     const int k = find_job(start, n, lb);
     const LevelJob &J = jobs[k];
-    level_fused_block<GATHER, CREASED, TOPO, TOPO ? 3 : 0, 1, kStoreLds>(J.a, J.c, J.t, d, lb - (uint32_t)start[k]);
+    FusedArgs a;
+    DCage c;
+    unpack_level(J, arena, a, c);
+    level_fused_block<GATHER, CREASED, TOPO, TOPO ? 3 : 0, 1, kStoreLds>(a, c, J.t, d, lb - (uint32_t)start[k]);
 }
 
 // vertex points of the owners k_batch_fused marked (kOwnerWords mask words
@@ -101,79 +108,94 @@ __global__ __launch_bounds__(256) void k_batch_fused(const LevelJob *__restrict_
 template <int GATHER, int CREASED>
 __global__ __launch_bounds__(256) void k_batch_vertices_masked(const LevelJob *__restrict__ jobs,
                                                                const int32_t *__restrict__ start, int n, int d,
-                                                               int remap) {
+                                                               int remap, char *arena) {
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, remap);
     const int k = find_job(start, n, lb);
     const LevelJob &J = jobs[k];
+    FusedArgs a;
+    DCage c;
+    unpack_level(J, arena, a, c);
+    const JobLevelPtrs lp{J.lp, arena};
     const int32_t wave = (int32_t)(lb - (uint32_t)start[k]) * 4 + (int32_t)(threadIdx.x >> 6);
-    level_vertices_masked<GATHER, CREASED>(J.a, J.c, J.t, J.lp, d, wave);
+    level_vertices_masked<GATHER, CREASED>(a, c, J.t, lp, d, wave);
 }
 
 // vertex points of job k's level (one block per 256 * SCAN halfedges)
 template <int GATHER, int CREASED, int SCAN>
 __global__ __launch_bounds__(256) void k_batch_vertices(const LevelJob *__restrict__ jobs,
-                                                        const int32_t *__restrict__ start, int n, int d, int remap) {
+                                                        const int32_t *__restrict__ start, int n, int d, int remap,
+                                                        char *arena) {
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, remap);
     const int k = find_job(start, n, lb);
     const LevelJob &J = jobs[k];
-    level_vertices_compact_block<GATHER, CREASED, SCAN>(J.a, J.c, J.t, J.lp, d, lb - (uint32_t)start[k]);
+    FusedArgs a;
+    DCage c;
+    unpack_level(J, arena, a, c);
+    const JobLevelPtrs lp{J.lp, arena};
+    level_vertices_compact_block<GATHER, CREASED, SCAN>(a, c, J.t, lp, d, lb - (uint32_t)start[k]);
 }
 
 template <int G, int C, int SCAN>
-void vertices_scan(const BatchTable &t, int d, int remap, hipStream_t st) {
+void vertices_scan(const BatchTable &t, char *arena, int d, int remap, hipStream_t st) {
     hipLaunchKernelGGL((k_batch_vertices<G, C, SCAN>), dim3(t.blocks), dim3(256), 0, st,
-                       static_cast<const LevelJob *>(t.jobs), t.start, (int)t.n, d, remap);
+                       static_cast<const LevelJob *>(t.jobs), t.start, (int)t.n, d, remap, arena);
 }
 
 template <int G, int C>
-void vertices_gc(const BatchTable &t, int d, int scan, int remap, hipStream_t st) {
+void vertices_gc(const BatchTable &t, char *arena, int d, int scan, int remap, hipStream_t st) {
     if (scan == 0) {
         hipLaunchKernelGGL((k_batch_vertices_masked<G, C>), dim3(t.blocks), dim3(256), 0, st,
-                           static_cast<const LevelJob *>(t.jobs), t.start, (int)t.n, d, remap);
+                           static_cast<const LevelJob *>(t.jobs), t.start, (int)t.n, d, remap, arena);
         return;
     }
-    if (scan == 2) vertices_scan<G, C, 2>(t, d, remap, st);
-    else if (scan == 8) vertices_scan<G, C, 8>(t, d, remap, st);
-    else vertices_scan<G, C, 4>(t, d, remap, st);
+    if (scan == 2) vertices_scan<G, C, 2>(t, arena, d, remap, st);
+    else if (scan == 8) vertices_scan<G, C, 8>(t, arena, d, remap, st);
+    else vertices_scan<G, C, 4>(t, arena, d, remap, st);
 }
 
 template <int G, int C>
-void fused_gc(const BatchTable &t, int d, int topo, int remap, hipStream_t st) {
+void fused_gc(const BatchTable &t, char *arena, int d, int topo, int remap, hipStream_t st) {
     const LevelJob *jobs = static_cast<const LevelJob *>(t.jobs);
     if (topo)
-        hipLaunchKernelGGL((k_batch_fused<G, C, 1>), dim3(t.blocks), dim3(256), 0, st, jobs, t.start, (int)t.n, d, remap);
+        hipLaunchKernelGGL((k_batch_fused<G, C, 1>), dim3(t.blocks), dim3(256), 0, st, jobs, t.start, (int)t.n, d, remap,
+                           arena);
     else
-        hipLaunchKernelGGL((k_batch_fused<G, C, 0>), dim3(t.blocks), dim3(256), 0, st, jobs, t.start, (int)t.n, d, remap);
+        hipLaunchKernelGGL((k_batch_fused<G, C, 0>), dim3(t.blocks), dim3(256), 0, st, jobs, t.start, (int)t.n, d, remap,
+                           arena);
 }
 
 // job k's deferred vertices: kDeferGroup mask words per wave
 template <int GATHER, int CREASED>
 __global__ __launch_bounds__(256) void k_batch_deferred(const LevelJob *__restrict__ jobs,
-                                                        const int32_t *__restrict__ start, int n, int d) {
+                                                        const int32_t *__restrict__ start, int n, int d, char *arena) {
     const int k = find_job(start, n, blockIdx.x);
     const LevelJob &J = jobs[k];
-    const int32_t words = (J.a.H + 63) >> 6;
+    FusedArgs a;
+    DCage c;
+    unpack_level(J, arena, a, c);
+    const JobLevelPtrs lp{J.lp, arena};
+    const int32_t words = (a.H + 63) >> 6;
     const int32_t wave = (int32_t)(blockIdx.x - (uint32_t)start[k]) * 4 + (int32_t)(threadIdx.x >> 6);
-    deferred_group<GATHER, CREASED>(J.a, J.c, J.t, J.lp, d, wave * kDeferGroup, words);
+    deferred_group<GATHER, CREASED>(a, c, J.t, lp, d, wave * kDeferGroup, words);
 }
 
 template <int G, int C, int T, int NT>
-void patch_nt(const BatchTable &t, int d, int hpt, int remap, hipStream_t st) {
+void patch_nt(const BatchTable &t, char *arena, int d, int hpt, int remap, hipStream_t st) {
     hipLaunchKernelGGL((k_batch_patch<G, C, T, NT>), dim3(t.blocks), dim3(NT), 0, st,
-                       static_cast<const LevelJob *>(t.jobs), t.start, (int)t.n, d, (uint32_t)hpt, remap);
+                       static_cast<const LevelJob *>(t.jobs), t.start, (int)t.n, d, (uint32_t)hpt, remap, arena);
 }
 
 template <int G, int C, int T>
-void patch_t(const BatchTable &t, int d, int nt, int hpt, int remap, hipStream_t st) {
-    if (nt == 128) patch_nt<G, C, T, 128>(t, d, hpt, remap, st);
-    else if (nt == 512) patch_nt<G, C, T, 512>(t, d, hpt, remap, st);
-    else patch_nt<G, C, T, 256>(t, d, hpt, remap, st);
+void patch_t(const BatchTable &t, char *arena, int d, int nt, int hpt, int remap, hipStream_t st) {
+    if (nt == 128) patch_nt<G, C, T, 128>(t, arena, d, hpt, remap, st);
+    else if (nt == 512) patch_nt<G, C, T, 512>(t, arena, d, hpt, remap, st);
+    else patch_nt<G, C, T, 256>(t, arena, d, hpt, remap, st);
 }
 
 template <int G, int C>
-void patch_gc(const BatchTable &t, int d, int topo, int nt, int hpt, int remap, hipStream_t st) {
-    if (topo) patch_t<G, C, 1>(t, d, nt, hpt, remap, st);
-    else patch_t<G, C, 0>(t, d, nt, hpt, remap, st);
+void patch_gc(const BatchTable &t, char *arena, int d, int topo, int nt, int hpt, int remap, hipStream_t st) {
+    if (topo) patch_t<G, C, 1>(t, arena, d, nt, hpt, remap, st);
+    else patch_t<G, C, 0>(t, arena, d, nt, hpt, remap, st);
 }
 
 }  // namespace
@@ -205,41 +227,43 @@ void batch_copy(const BatchTable &t, const float *src, float *dst, hipStream_t s
     launch_batch<SpanJob>(t, OpCopy{src, dst}, st);
 }
 
-void batch_level_patch(const BatchTable &t, int d, int gather, int creased, int topo, int nt, int hpt, int remap,
-                       hipStream_t st) {
+void batch_level_patch(const BatchTable &t, char *arena, int d, int gather, int creased, int topo, int nt, int hpt,
+                       int remap, hipStream_t st) {
     if (t.blocks == 0 || t.n == 0) return;
     hpt = hpt < 1 ? 1 : hpt;
-    if (gather && creased) patch_gc<1, 1>(t, d, topo, nt, hpt, remap, st);
-    else if (gather) patch_gc<1, 0>(t, d, topo, nt, hpt, remap, st);
-    else if (creased) patch_gc<0, 1>(t, d, topo, nt, hpt, remap, st);
-    else patch_gc<0, 0>(t, d, topo, nt, hpt, remap, st);
+    if (gather && creased) patch_gc<1, 1>(t, arena, d, topo, nt, hpt, remap, st);
+    else if (gather) patch_gc<1, 0>(t, arena, d, topo, nt, hpt, remap, st);
+    else if (creased) patch_gc<0, 1>(t, arena, d, topo, nt, hpt, remap, st);
+    else patch_gc<0, 0>(t, arena, d, topo, nt, hpt, remap, st);
 }
 
-void batch_level_fused(const BatchTable &t, int d, int gather, int creased, int topo, int remap, hipStream_t st) {
+void batch_level_fused(const BatchTable &t, char *arena, int d, int gather, int creased, int topo, int remap,
+                       hipStream_t st) {
     if (t.blocks == 0 || t.n == 0) return;
-    if (gather && creased) fused_gc<1, 1>(t, d, topo, remap, st);
-    else if (gather) fused_gc<1, 0>(t, d, topo, remap, st);
-    else if (creased) fused_gc<0, 1>(t, d, topo, remap, st);
-    else fused_gc<0, 0>(t, d, topo, remap, st);
+    if (gather && creased) fused_gc<1, 1>(t, arena, d, topo, remap, st);
+    else if (gather) fused_gc<1, 0>(t, arena, d, topo, remap, st);
+    else if (creased) fused_gc<0, 1>(t, arena, d, topo, remap, st);
+    else fused_gc<0, 0>(t, arena, d, topo, remap, st);
 }
 
-void batch_level_vertices(const BatchTable &t, int d, int gather, int creased, int scan, int remap, hipStream_t st) {
+void batch_level_vertices(const BatchTable &t, char *arena, int d, int gather, int creased, int scan, int remap,
+                          hipStream_t st) {
     if (t.blocks == 0 || t.n == 0) return;
-    if (gather && creased) vertices_gc<1, 1>(t, d, scan, remap, st);
-    else if (gather) vertices_gc<1, 0>(t, d, scan, remap, st);
-    else if (creased) vertices_gc<0, 1>(t, d, scan, remap, st);
-    else vertices_gc<0, 0>(t, d, scan, remap, st);
+    if (gather && creased) vertices_gc<1, 1>(t, arena, d, scan, remap, st);
+    else if (gather) vertices_gc<1, 0>(t, arena, d, scan, remap, st);
+    else if (creased) vertices_gc<0, 1>(t, arena, d, scan, remap, st);
+    else vertices_gc<0, 0>(t, arena, d, scan, remap, st);
 }
 
-void batch_level_deferred(const BatchTable &t, int d, int gather, int creased, hipStream_t st) {
+void batch_level_deferred(const BatchTable &t, char *arena, int d, int gather, int creased, hipStream_t st) {
     if (t.blocks == 0 || t.n == 0) return;
     const dim3 grid(t.blocks), block(256);
     const LevelJob *jobs = static_cast<const LevelJob *>(t.jobs);
     const int n = (int)t.n;
-    if (gather && creased) hipLaunchKernelGGL((k_batch_deferred<1, 1>), grid, block, 0, st, jobs, t.start, n, d);
-    else if (gather) hipLaunchKernelGGL((k_batch_deferred<1, 0>), grid, block, 0, st, jobs, t.start, n, d);
-    else if (creased) hipLaunchKernelGGL((k_batch_deferred<0, 1>), grid, block, 0, st, jobs, t.start, n, d);
-    else hipLaunchKernelGGL((k_batch_deferred<0, 0>), grid, block, 0, st, jobs, t.start, n, d);
+    if (gather && creased) hipLaunchKernelGGL((k_batch_deferred<1, 1>), grid, block, 0, st, jobs, t.start, n, d, arena);
+    else if (gather) hipLaunchKernelGGL((k_batch_deferred<1, 0>), grid, block, 0, st, jobs, t.start, n, d, arena);
+    else if (creased) hipLaunchKernelGGL((k_batch_deferred<0, 1>), grid, block, 0, st, jobs, t.start, n, d, arena);
+    else hipLaunchKernelGGL((k_batch_deferred<0, 0>), grid, block, 0, st, jobs, t.start, n, d, arena);
 }
 
 }  // namespace cch

@@ -41,7 +41,10 @@ struct ccb_Batch {
     std::vector<Cage> cages;
     int64_t tot_he0 = 0, tot_v0 = 0, tot_u0 = 0, tot_e0 = 0, tot_f0 = 0;
     int64_t tot_he = 0, tot_cr = 0, tot_pts = 0, tot_fin = 0, tot_def = 0, tot_adjB = 0, tot_adjW = 0;
-    // device memory
+    // device memory: every array of the forward pass is carved out of one
+    // arena, the adjoint's out of a second one (job tables hold byte offsets
+    // into them)
+    char *arena = nullptr, *adj_arena = nullptr;
     CageHe *c_he = nullptr;
     int32_t *c_v2h = nullptr, *c_e2h = nullptr, *c_f2h = nullptr;
     P3 *c_pts = nullptr;
@@ -134,6 +137,16 @@ struct TableBuilder {
 
 int64_t blocks_of(int64_t threads, int64_t per = 256) { return (threads + per - 1) / per; }
 
+// consecutive 256-byte aligned ranges of one allocation
+struct Carve {
+    size_t size = 0;
+    size_t take(size_t bytes) {
+        const size_t off = size;
+        size = (off + bytes + 255) & ~size_t(255);
+        return off;
+    }
+};
+
 }  // namespace
 
 DCage ccb_Batch::cage_view(int i) const {
@@ -176,17 +189,32 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
     }
     CC_TRY(hipGetDevice(&device), "hipGetDevice");
     CC_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
-    CC_TRY(hipMalloc(&c_he, sizeof(CageHe) * (size_t)tot_he0), "hipMalloc(batch cage halfedges)");
-    CC_TRY(hipMalloc(&c_v2h, sizeof(int32_t) * (size_t)tot_v0), "hipMalloc(batch cage v2h)");
-    CC_TRY(hipMalloc(&c_e2h, sizeof(int32_t) * (size_t)tot_e0), "hipMalloc(batch cage e2h)");
-    CC_TRY(hipMalloc(&c_f2h, sizeof(int32_t) * (size_t)tot_f0), "hipMalloc(batch cage f2h)");
-    CC_TRY(hipMalloc(&c_pts, sizeof(P3) * (size_t)tot_v0), "hipMalloc(batch cage points)");
-    CC_TRY(hipMalloc(&c_uvs, sizeof(UV) * (size_t)(tot_u0 + 1)), "hipMalloc(batch cage uvs)");
-    CC_TRY(hipMalloc(&c_cr, sizeof(Crease) * (size_t)tot_e0), "hipMalloc(batch cage creases)");
-    CC_TRY(hipMalloc(&he, sizeof(He) * (size_t)tot_he), "hipMalloc(batch halfedges)");
-    CC_TRY(hipMalloc(&cr, sizeof(Crease) * (size_t)tot_cr), "hipMalloc(batch creases)");
-    CC_TRY(hipMalloc(&pts, sizeof(P3) * (size_t)tot_pts), "hipMalloc(batch points)");
-    CC_TRY(hipMalloc(&defer_mask, sizeof(uint64_t) * (size_t)tot_def), "hipMalloc(batch deferred-vertex masks)");
+    {
+        Carve cv;
+        const size_t o_che = cv.take(sizeof(CageHe) * (size_t)tot_he0);
+        const size_t o_cv2h = cv.take(sizeof(int32_t) * (size_t)tot_v0);
+        const size_t o_ce2h = cv.take(sizeof(int32_t) * (size_t)tot_e0);
+        const size_t o_cf2h = cv.take(sizeof(int32_t) * (size_t)tot_f0);
+        const size_t o_cpts = cv.take(sizeof(P3) * (size_t)tot_v0);
+        const size_t o_cuvs = cv.take(sizeof(UV) * (size_t)(tot_u0 + 1));
+        const size_t o_ccr = cv.take(sizeof(Crease) * (size_t)tot_e0);
+        const size_t o_he = cv.take(sizeof(He) * (size_t)tot_he);
+        const size_t o_cr = cv.take(sizeof(Crease) * (size_t)tot_cr);
+        const size_t o_pts = cv.take(sizeof(P3) * (size_t)tot_pts);
+        const size_t o_def = cv.take(sizeof(uint64_t) * (size_t)tot_def);
+        CC_TRY(hipMalloc(&arena, cv.size), "hipMalloc(batch arena)");
+        c_he = reinterpret_cast<CageHe *>(arena + o_che);
+        c_v2h = reinterpret_cast<int32_t *>(arena + o_cv2h);
+        c_e2h = reinterpret_cast<int32_t *>(arena + o_ce2h);
+        c_f2h = reinterpret_cast<int32_t *>(arena + o_cf2h);
+        c_pts = reinterpret_cast<P3 *>(arena + o_cpts);
+        c_uvs = reinterpret_cast<UV *>(arena + o_cuvs);
+        c_cr = reinterpret_cast<Crease *>(arena + o_ccr);
+        he = reinterpret_cast<He *>(arena + o_he);
+        cr = reinterpret_cast<Crease *>(arena + o_cr);
+        pts = reinterpret_cast<P3 *>(arena + o_pts);
+        defer_mask = reinterpret_cast<uint64_t *>(arena + o_def);
+    }
     for (int i = 0; i < count; ++i) {
         const cc_Mesh *m = meshes[i];
         const Cage &g = cages[i];
@@ -207,8 +235,7 @@ void ccb_Batch::free_all() {
     DeviceGuard g(device);
     drop_graphs();
     begin_timing();
-    void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts, defer_mask,
-                    adjA, adjB, adjW, adjCage, tables};
+    void *ptrs[] = {arena, adj_arena, tables};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
@@ -220,17 +247,36 @@ void ccb_Batch::drop_graphs() {
 }
 
 bool ccb_Batch::alloc_adjoint() {
-    if (adjA) return true;
-    CC_TRY(hipMalloc(&adjA, sizeof(float) * 3 * (size_t)tot_fin), "hipMalloc(batch adjoint)");
-    CC_TRY(hipMalloc(&adjB, sizeof(float) * 3 * (size_t)tot_adjB), "hipMalloc(batch adjoint)");
-    CC_TRY(hipMalloc(&adjW, sizeof(float) * 4 * (size_t)tot_adjW), "hipMalloc(batch adjoint weights)");
-    CC_TRY(hipMalloc(&adjCage, sizeof(float) * 3 * (size_t)tot_v0), "hipMalloc(batch adjoint)");
+    if (adj_arena) return true;
+    Carve cv;
+    const size_t oA = cv.take(sizeof(float) * 3 * (size_t)tot_fin);
+    const size_t oB = cv.take(sizeof(float) * 3 * (size_t)tot_adjB);
+    const size_t oW = cv.take(sizeof(float) * 4 * (size_t)tot_adjW);
+    const size_t oC = cv.take(sizeof(float) * 3 * (size_t)tot_v0);
+    CC_TRY(hipMalloc(&adj_arena, cv.size), "hipMalloc(batch adjoint arena)");
+    adjA = reinterpret_cast<float *>(adj_arena + oA);
+    adjB = reinterpret_cast<float *>(adj_arena + oB);
+    adjW = reinterpret_cast<float *>(adj_arena + oW);
+    adjCage = reinterpret_cast<float *>(adj_arena + oC);
     return build_tables();
 }
 
 // Job tables of every launch, for the current tile options.
 bool ccb_Batch::build_tables() {
     TableBuilder tb;
+    auto off = [&](const void *p) -> int64_t { return static_cast<const char *>(p) - arena; };
+    auto aoff = [&](const void *p) -> int64_t { return static_cast<const char *>(p) - adj_arena; };
+    auto cage_offsets = [&](int i, auto &j) {
+        const Cage &g = cages[i];
+        j.c_he = off(c_he + g.o_he0);
+        j.c_v2h = off(c_v2h + g.o_v0);
+        j.c_e2h = off(c_e2h + g.o_e0);
+        j.c_f2h = off(c_f2h + g.o_f0);
+        j.c_pts = off(c_pts + g.o_v0);
+        j.c_uvs = off(c_uvs + g.o_u0);
+        j.c_cr = off(c_cr + g.o_e0);
+        j.cV = g.V0; j.cU = g.U0; j.cH = g.H0; j.cE = g.E0; j.cF = g.F0;
+    };
     {   // cage step
         std::vector<CageJob> jobs(N);
         std::vector<int64_t> bh(N), bf(N), be(N), bv(N), bz(N);
@@ -286,24 +332,24 @@ bool ccb_Batch::build_tables() {
             if (g.D <= d) continue;
             He *hb = he + g.o_he;
             P3 *pb = pts + g.o_pts;
+            P3 *vp = pb + g.n.offV[d + 1];
             LevelJob j;
-            FusedArgs &a = j.a;
-            a.he = hb + g.n.offH[d];
-            a.out_he = hb + g.n.offH[d + 1];
-            a.cr = cr + g.o_cr + g.n.offC[d];
-            a.old = pb + g.n.offV[d];
-            a.vp = pb + g.n.offV[d + 1];
-            a.fp = a.vp + g.n.V[d];
-            a.ep = a.fp + g.n.F[d];
-            a.H = (int32_t)g.n.H[d]; a.V = (int32_t)g.n.V[d]; a.F = (int32_t)g.n.F[d];
-            a.E = (int32_t)g.n.E[d]; a.C = (int32_t)g.n.C[d];
-            a.xcd_remap = remap;
-            a.uv_refine = g.U0 > 0 ? 1 : 0;
-            a.defer_mask = defer_mask + g.o_def;
-            a.owner_mask = cfg[d].kind == 1 && cfg[d].scan == 0 ? defer_mask + g.o_def : nullptr;
-            j.c = cage_view(i);
+            j.he = off(hb + g.n.offH[d]);
+            j.out_he = off(hb + g.n.offH[d + 1]);
+            j.cr = off(cr + g.o_cr + g.n.offC[d]);
+            j.old = off(pb + g.n.offV[d]);
+            j.vp = off(vp);
+            j.fp = off(vp + g.n.V[d]);
+            j.ep = off(vp + g.n.V[d] + g.n.F[d]);
+            j.defer_mask = off(defer_mask + g.o_def);
+            j.owner_mask = cfg[d].kind == 1 && cfg[d].scan == 0 ? j.defer_mask : -1;
+            j.H = (int32_t)g.n.H[d]; j.V = (int32_t)g.n.V[d]; j.F = (int32_t)g.n.F[d];
+            j.E = (int32_t)g.n.E[d]; j.C = (int32_t)g.n.C[d];
+            j.xcd_remap = remap;
+            j.uv_refine = g.U0 > 0 ? 1 : 0;
+            cage_offsets(i, j);
             j.t = g.tab;
-            for (int k = 0; k <= kMaxDepth; ++k) j.lp.he[k] = (k >= 1 && k <= g.D) ? hb + g.n.offH[k] : nullptr;
+            for (int k = 0; k <= kMaxDepth; ++k) j.lp[k] = (k >= 1 && k <= g.D) ? off(hb + g.n.offH[k]) : -1;
             jobs.push_back(j);
             const int64_t tiles = blocks_of(g.n.H[d], cfg[d].nt);
             bp.push_back((tiles + cfg[d].hpt - 1) / cfg[d].hpt);
@@ -317,7 +363,7 @@ bool ccb_Batch::build_tables() {
         tb.add(t_fused[d], jobs, bf);
         tb.add(t_vert[d], jobs, bv);
     }
-    if (adjA) {   // adjoint steps d+1 -> d, d = 0 the cage step
+    if (adj_arena) {   // adjoint steps d+1 -> d, d = 0 the cage step
         for (int d = 0; d < Dmax; ++d) {
             std::vector<AdjJob> jobs;
             std::vector<int64_t> b, bq;
@@ -329,32 +375,30 @@ bool ccb_Batch::build_tables() {
                 auto buf = [&](int k) { return ((g.D - k) & 1) ? adjB + 3 * g.o_adjB : adjA + 3 * g.o_fin; };
                 He *hb = he + g.o_he;
                 AdjJob j;
-                AdjArgs<float> &a = j.a;
                 float *G = buf(d + 1);
                 if (d == 0) {
-                    a.he = nullptr;
-                    a.cr = nullptr;
-                    a.H = g.H0;
-                    a.C = g.E0;
-                    a.gP = adjCage + 3 * g.o_v0;
+                    j.he = -1;
+                    j.cr = -1;
+                    j.H = g.H0;
+                    j.C = g.E0;
+                    j.gP = aoff(adjCage + 3 * g.o_v0);
                 } else {
-                    a.he = hb + g.n.offH[d];
-                    a.cr = cr + g.o_cr + g.n.offC[d];
-                    a.H = (int32_t)g.n.H[d];
-                    a.C = (int32_t)g.n.C[d];
-                    a.gP = buf(d);
+                    j.he = off(hb + g.n.offH[d]);
+                    j.cr = off(cr + g.o_cr + g.n.offC[d]);
+                    j.H = (int32_t)g.n.H[d];
+                    j.C = (int32_t)g.n.C[d];
+                    j.gP = aoff(buf(d));
                 }
-                a.sd = nullptr;
-                a.W = d == 0 ? nullptr : adjW + 4 * g.o_adjW;
-                a.gV = G;
-                a.gF = G + 3 * g.n.V[d];
-                a.gE = G + 3 * (g.n.V[d] + g.n.F[d]);
-                j.c = cage_view(i);
+                j.W = d == 0 ? -1 : aoff(adjW + 4 * g.o_adjW);
+                j.gV = aoff(G);
+                j.gF = aoff(G + 3 * g.n.V[d]);
+                j.gE = aoff(G + 3 * (g.n.V[d] + g.n.F[d]));
+                cage_offsets(i, j);
                 j.t = g.tab;
-                for (int k = 0; k <= kMaxDepth; ++k) j.lp.he[k] = (k >= 1 && k <= g.D) ? hb + g.n.offH[k] : nullptr;
+                for (int k = 0; k <= kMaxDepth; ++k) j.lp[k] = (k >= 1 && k <= g.D) ? off(hb + g.n.offH[k]) : -1;
                 jobs.push_back(j);
-                b.push_back(blocks_of(a.H));
-                bq.push_back(blocks_of(a.H, 1024));
+                b.push_back(blocks_of(j.H));
+                bq.push_back(blocks_of(j.H, 1024));
             }
             tb.add(t_adj[d], jobs, b);
             tb.add(t_adjq[d], jobs, bq);
@@ -414,14 +458,16 @@ void ccb_Batch::record_forward(int32_t steps, hipStream_t st) {
     for (int d = 1; d < Dmax; ++d) {
         if (cfg[d].kind == 2) {
             timed_launch("level_patch", d, st, [&] {
-                batch_level_patch(t_patch[d], d, gather, creased, topo, cfg[d].nt, cfg[d].hpt, remap, st);
+                batch_level_patch(t_patch[d], arena, d, gather, creased, topo, cfg[d].nt, cfg[d].hpt, remap, st);
             });
-            timed_launch("level_deferred", d, st, [&] { batch_level_deferred(t_def[d], d, gather, creased, st); });
+            timed_launch("level_deferred", d, st,
+                         [&] { batch_level_deferred(t_def[d], arena, d, gather, creased, st); });
         } else {
             timed_launch("level_fused", d, st,
-                         [&] { batch_level_fused(t_fused[d], d, gather, creased, topo, remap, st); });
-            timed_launch("level_vertices", d, st,
-                         [&] { batch_level_vertices(t_vert[d], d, gather, creased, cfg[d].scan, remap, st); });
+                         [&] { batch_level_fused(t_fused[d], arena, d, gather, creased, topo, remap, st); });
+            timed_launch("level_vertices", d, st, [&] {
+                batch_level_vertices(t_vert[d], arena, d, gather, creased, cfg[d].scan, remap, st);
+            });
         }
     }
 }
@@ -432,12 +478,14 @@ void ccb_Batch::record_adjoint(int creased, hipStream_t st) {
     const int64_t nc = 3 * tot_v0;
     hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<int64_t>((nc + 255) / 256, 65536)), dim3(256), 0, st, adjCage, nc);
     for (int d = Dmax - 1; d >= 1; --d)
-        timed_launch("adj_level", d, st, [&] { batch_adj_level(t_adjq[d], t_adj[d], d, creased, st); });
-    timed_launch("adj_cage", 0, st, [&] { batch_adj_cage(t_adj[0], creased, st); });
+        timed_launch("adj_level", d, st,
+                     [&] { batch_adj_level(t_adjq[d], t_adj[d], arena, adj_arena, d, creased, st); });
+    timed_launch("adj_cage", 0, st, [&] { batch_adj_cage(t_adj[0], arena, adj_arena, creased, st); });
 }
 
 bool ccb_Batch::run(int32_t key, bool adjoint, int32_t steps, hipStream_t st) {
     if (!use_graph || timing) {
+        if (timing) begin_timing();   // the timings are those of the last run
         if (adjoint) record_adjoint(steps, st);
         else record_forward(steps, st);
         CC_TRY(hipGetLastError(), "batch launch");

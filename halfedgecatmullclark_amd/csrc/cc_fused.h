@@ -37,6 +37,22 @@ struct LevelPtrs {
     const He *he[kMaxDepth + 1];
 };
 
+// The level pointers of one job of a batch: byte offsets into the batch's
+// arena, -1 for an absent level.  Kernels copy them into a LevelPtrs in LDS
+// lane by lane straight from the job table (a per-lane indexed local copy
+// would live in scratch memory).
+struct JobLevelPtrs {
+    const int64_t *off;
+    char *arena;
+};
+
+__device__ __forceinline__ void lds_fill(LevelPtrs &dst, const JobLevelPtrs &src) {
+    for (int i = (int)threadIdx.x; i <= kMaxDepth; i += (int)blockDim.x) {
+        const int64_t o = src.off[i];
+        dst.he[i] = o < 0 ? nullptr : reinterpret_cast<const He *>(src.arena + o);
+    }
+}
+
 // Patch-local fused kernel: children, face, edge and vertex points of level
 // d+1; vertices whose ring leaves the tile are marked in a.defer_mask and
 // finished by launch_level_deferred.

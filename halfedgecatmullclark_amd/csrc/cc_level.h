@@ -162,9 +162,9 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
 // Vertex points with owners compacted per block: a block scans
 // 256 * SCAN halfedges, collects the representatives in LDS, then every
 // thread evaluates owners, so all lanes do ring work.
-template <int GATHER, int CREASED, int SCAN>
+template <int GATHER, int CREASED, int SCAN, class LP>
 __device__ __forceinline__ void level_vertices_compact_block(const FusedArgs &a, const DCage &c, const LevelTab &t,
-                                                             const LevelPtrs &lp, int d, uint32_t lb) {
+                                                             const LP &lp, int d, uint32_t lb) {
     __shared__ int32_t owners[256 * SCAN];
     __shared__ int32_t nown;
     __shared__ LevelTab st;
@@ -249,9 +249,9 @@ struct MaskedVertexFn {
 
 // vertex points of the owners marked by the fused kernel; `wave` is the
 // index of this wave among the level's waves (kOwnerWords words each)
-template <int GATHER, int CREASED>
+template <int GATHER, int CREASED, class LP>
 __device__ __forceinline__ void level_vertices_masked(const FusedArgs &a, const DCage &c, const LevelTab &t,
-                                                      const LevelPtrs &lp, int d, int32_t wave) {
+                                                      const LP &lp, int d, int32_t wave) {
     __shared__ LevelTab st;
     __shared__ LevelPtrs slp;
     lds_fill(st, t);

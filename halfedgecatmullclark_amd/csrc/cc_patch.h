@@ -55,9 +55,9 @@ struct PatchLds {
     P3 sE[NT];
 };
 
-template <int GATHER, int CREASED, int TOPO, int UVMODE, int NT>
-__device__ __forceinline__ void patch_tile(const FusedArgs &a, const DCage &c, const LevelTab &t, const LevelPtrs &lp,
-                                           int d, uint32_t tile) {
+template <int GATHER, int CREASED, int TOPO, int UVMODE, int NT, class LP>
+__device__ __forceinline__ void patch_tile(const FusedArgs &a, const DCage &c, const LevelTab &t, const LP &lp, int d,
+                                           uint32_t tile) {
     constexpr bool kStaged = TOPO && UVMODE != 0;
     __shared__ PatchLds<NT, kStaged> L;
     __shared__ LevelTab st;     // read after the tile's first barrier
@@ -224,9 +224,9 @@ constexpr int kDeferGroup = 16;
 // The vertices marked in mask words [w0, w0 + kDeferGroup) of `words`, one
 // per lane: the masks' popcounts are prefix-summed across lanes 0..15 and
 // lane l takes the l-th marked halfedge, in rounds of 64.
-template <int GATHER, int CREASED>
-__device__ __forceinline__ void deferred_group(const FusedArgs &a, const DCage &c, const LevelTab &tg,
-                                               const LevelPtrs &lpg, int d, int32_t w0, int32_t words) {
+template <int GATHER, int CREASED, class LP>
+__device__ __forceinline__ void deferred_group(const FusedArgs &a, const DCage &c, const LevelTab &tg, const LP &lpg,
+                                               int d, int32_t w0, int32_t words) {
     __shared__ LevelTab t;
     __shared__ LevelPtrs lp;
     lds_fill(t, tg);

@@ -318,8 +318,6 @@ bool ccd_Subd::refine(int32_t steps, int dmax, hipStream_t st) {
     // vertex points
     const bool topo = (steps & CCD_HALFEDGES) != 0;
     const bool points = (steps & CCD_POINTS) != 0;
-    const bool gather = (steps & CCD_GATHER) != 0;
-    const int creased = (steps & CCD_NOCREASES) ? 0 : 1;
     int uvMode = ((steps & CCD_UVS) && U0 > 0) ? 1 : 0;
     if (topo && uvMode == 0 && U0 == 0) uvMode = 2;
     if (steps & CCD_CREASES) {

@@ -352,7 +352,10 @@ int32_t ccb_SetOption(ccb_Batch *batch, int32_t key, int32_t value);
  * candidate on this batch (see ccd_Autotune) */
 int32_t ccb_Autotune(ccb_Batch *batch, int32_t steps, void *stream);
 int32_t ccb_TileOption(const ccb_Batch *batch, int32_t key, int32_t level);
-/* per-launch timing of eager (non-graph) runs */
+/* per-launch timing with HIP events: while enabled, runs are eager (no
+ * graph) and the names, levels and milliseconds read are those of the
+ * launches of the last ccb_Refine, ccb_RefinePoints or
+ * ccb_RefinePointsAdjoint */
 int32_t ccb_SetTiming(ccb_Batch *batch, int32_t enable);
 int32_t ccb_TimingCount(const ccb_Batch *batch);
 const char *ccb_TimingName(const ccb_Batch *batch, int32_t i);

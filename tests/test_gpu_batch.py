@@ -268,3 +268,21 @@ def test_batch_errors(capi):
     with pytest.raises(capi.CCError):
         b.set_option(capi.OPT_PATCH_TILE, 100)
     b.close()
+
+
+def test_batch_timings_are_the_last_run(capi):
+    cases = batch_cases()
+    b = capi.Batch([c for c, _ in cases], [d for _, d in cases])
+    b.refine(capi.ALL)
+    b.set_timing(True)
+    b.refine(capi.ALL)
+    b.synchronize()
+    first = b.timings()
+    b.refine(capi.ALL)
+    b.synchronize()
+    second = b.timings()
+    b.set_timing(False)
+    assert len(first) == len(second) > 0
+    assert [(n, lv) for n, lv, _ in first] == [(n, lv) for n, lv, _ in second]
+    assert all(ms >= 0.0 for _, _, ms in second)
+    b.close()
