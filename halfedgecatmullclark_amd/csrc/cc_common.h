@@ -125,14 +125,33 @@ __host__ __device__ __forceinline__ P3 zero3() { return {0.0f, 0.0f, 0.0f}; }
 // UV codec (CatmullClark.h:374-392).  The encode keeps the low 32 bits of a
 // 64-bit truncating conversion, which is what x86-64 compilers emit for the
 // reference's float -> uint32_t conversion, so out-of-range inputs agree too.
+//
+// The decode's single-precision k / 65535 (k < 2^16) is evaluated as the
+// product with the rounded reciprocal followed by one Newton correction with
+// fused multiply-adds (the residual k - q * 65535 is exact): three
+// instructions instead of a full division, and equal to the correctly
+// rounded quotient for every one of the 65536 codes.
+__host__ __device__ __forceinline__ float u16_unit(uint32_t k) {
+    const float x = (float)k, r = 1.0f / 65535.0f;
+    const float q = x * r;
+    return fmaf(fmaf(-q, 65535.0f, x), r, q);
+}
 __host__ __device__ __forceinline__ UV uv_decode(int32_t code) {
     uint32_t t = (uint32_t)code;
-    return {(float)(t & 0xFFFFu) / 65535.0f, (float)((t >> 16) & 0xFFFFu) / 65535.0f};
+    return {u16_unit(t & 0xFFFFu), u16_unit((t >> 16) & 0xFFFFu)};
 }
 __host__ __device__ __forceinline__ uint32_t f2u_trunc(float x) { return (uint32_t)(int64_t)x; }
 __host__ __device__ __forceinline__ int32_t uv_encode(UV uv) {
     uint32_t u = f2u_trunc(uv.u * 65535.0f);
     uint32_t v = f2u_trunc(uv.v * 65535.0f);
+    return (int32_t)((u & 0xFFFFu) | ((v & 0xFFFFu) << 16));
+}
+// uv_encode of coordinates in [0, 1] (decoded codes, their midpoints and
+// face averages: every uv of a level >= 1): the products lie in [0, 65535],
+// where a 32-bit truncating conversion equals the 64-bit one.
+__host__ __device__ __forceinline__ int32_t uv_encode_unit(UV uv) {
+    uint32_t u = (uint32_t)(int32_t)(uv.u * 65535.0f);
+    uint32_t v = (uint32_t)(int32_t)(uv.v * 65535.0f);
     return (int32_t)((u & 0xFFFFu) | ((v & 0xFFFFu) << 16));
 }
 __host__ __device__ __forceinline__ UV uv_mid(UV a, UV b) {

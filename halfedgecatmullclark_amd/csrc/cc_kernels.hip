@@ -45,9 +45,9 @@ __global__ void k_level_halfedges(const He *__restrict__ in, He *__restrict__ ou
         fuv.u /= 4.0f;
         fuv.v /= 4.0f;
         ch[0].uv = x.w;
-        ch[1].uv = uv_encode(uv_mid(uv, nuv));
-        ch[2].uv = uv_encode(fuv);
-        ch[3].uv = uv_encode(uv_mid(uv, puv));
+        ch[1].uv = uv_encode_unit(uv_mid(uv, nuv));
+        ch[2].uv = uv_encode_unit(fuv);
+        ch[3].uv = uv_encode_unit(uv_mid(uv, puv));
     } else if (uvs == 2) {
         ch[0].uv = ch[1].uv = ch[2].uv = ch[3].uv = 0;
     }

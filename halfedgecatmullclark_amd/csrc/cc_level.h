@@ -16,26 +16,26 @@ namespace cch {
 
 // face point of h's quad (lane 0 of the quad) and, for the larger halfedge
 // of each pair, the edge point
+// (g0..g3: the twin quad's points when it lies outside the block, s: the
+// sharpness of h's edge for an owner)
 template <int GATHER, int CREASED>
 __device__ __forceinline__ void level_fused_points(const FusedArgs &a, int32_t h, int j, int4 x, P3 P, P3 q0, P3 q1,
                                                    P3 q2, P3 q3, bool owner, bool twin_here, int32_t base,
-                                                   const P3 *spts, int4 tr0, int4 tr1, int4 tr2, int4 tr3) {
+                                                   const P3 *spts, P3 g0, P3 g1, P3 g2, P3 g3, float s) {
     const int32_t twin = x.x;
     const P3 F = face_rule(q0, q1, q2, q3, GATHER);
     if (j == 0) a.fp[h >> 2] = F;
     if (!owner) return;
     // this halfedge owns its edge: both contributions
     const P3 v1 = pick4(q0, q1, q2, q3, (j + 1) & 3);
-    const float s = CREASED ? lsharp(a.cr, a.C, x.y) : 0.0f;
     P3 Ev;
     if (twin >= 0) {
-        // the twin quad's points from LDS when it is in this block, else from
-        // memory; both reads are valid for every lane (tr* are the records of
-        // quad 0 for an in-block twin), and selecting afterwards keeps LDS
-        // reads and global reads in their own instructions
+        // the twin quad's points from LDS when it is in this block, else the
+        // ones loaded from memory; the LDS reads are valid for every lane, and
+        // selecting afterwards keeps LDS and global reads in their own
+        // instructions
         const int tb = twin_here ? (twin & ~3) - base : 0;
         const P3 l0 = sld(spts + tb), l1 = sld(spts + tb + 1), l2 = sld(spts + tb + 2), l3 = sld(spts + tb + 3);
-        const P3 g0 = gld(a.old + tr0.z), g1 = gld(a.old + tr1.z), g2 = gld(a.old + tr2.z), g3 = gld(a.old + tr3.z);
         const P3 t0 = twin_here ? l0 : g0, t1 = twin_here ? l1 : g1, t2 = twin_here ? l2 : g2,
                  t3 = twin_here ? l3 : g3;
         const P3 Ft = face_rule(t0, t1, t2, t3, GATHER);
@@ -80,14 +80,33 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
     const bool twin_here = owner && twin >= base && twin < base + (int32_t)blockDim.x;
     const bool twin_far = owner && twin >= 0 && !twin_here;
 
-    // issue the loads of the point work first: this halfedge's vertex point
-    // and the records of a twin quad outside the block
-    P3 P = zero3();
-    int4 tr0, tr1, tr2, tr3;
+    // the loads of the point work come first: this halfedge's vertex point,
+    // its edge's sharpness and, for a twin quad outside the block, that
+    // quad's vertex IDs and then its points, so that the latency of the
+    // second gather overlaps the topology stores below.  The vertex point
+    // goes to LDS between the two gathers: the wait for the IDs (forced
+    // inside their branch by the empty asm that reads them) covers it, and
+    // no later wait has to count the twin-quad point loads still in flight
+    // (memory loads complete in order).
+    P3 P = zero3(), g0 = zero3(), g1 = zero3(), g2 = zero3(), g3 = zero3();
+    float s = 0.0f;
     if (POINTS) {
-        if (live) P = a.old[x.z];
-        const int32_t tq = twin_far ? (twin & ~3) : (live ? (h & ~3) : 0);
-        tr0 = he4[tq]; tr1 = he4[tq + 1]; tr2 = he4[tq + 2]; tr3 = he4[tq + 3];
+        P = a.old[x.z];   // a dead lane's record is all zeros: vertex 0
+        if (CREASED) {
+            const bool has = owner && x.y < a.C;   // a level's crease array is never empty
+            const float cs = a.cr[has ? x.y : 0].sharp;
+            s = has ? cs : 0.0f;
+        }
+        int32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+        if (twin_far) {
+            const int32_t tq = twin & ~3;
+            v0 = he4[tq].z; v1 = he4[tq + 1].z; v2 = he4[tq + 2].z; v3 = he4[tq + 3].z;
+            asm volatile("" ::"v"(v0), "v"(v1), "v"(v2), "v"(v3));
+        }
+        spts[threadIdx.x] = P;
+        if (twin_far) {
+            g0 = gld(a.old + v0); g1 = gld(a.old + v1); g2 = gld(a.old + v2); g3 = gld(a.old + v3);
+        }
     }
 
     // record of the previous halfedge of the quad, by DPP
@@ -109,15 +128,14 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
             fuv.u /= 4.0f;
             fuv.v /= 4.0f;
             c[0].w = x.w;
-            c[1].w = uv_encode(uv_mid(uv, nuv));
-            c[2].w = uv_encode(fuv);
-            c[3].w = uv_encode(uv_mid(uv, puv));
+            c[1].w = uv_encode_unit(uv_mid(uv, nuv));
+            c[2].w = uv_encode_unit(fuv);
+            c[3].w = uv_encode_unit(uv_mid(uv, puv));
         }
         int4 *out4 = reinterpret_cast<int4 *>(a.out_he);
         if (kStaged) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) stage[4 * threadIdx.x + k] = c[k];
-            if (POINTS) spts[threadIdx.x] = P;
             __syncthreads();
             const int64_t obase = 4 * (int64_t)base;
             const int64_t limit = 4 * (int64_t)a.H;
@@ -143,13 +161,10 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
     }
 
     if (!POINTS) return;
-    if (!kStaged) {
-        spts[threadIdx.x] = P;
-        __syncthreads();
-    }
+    if (!kStaged) __syncthreads();
     const P3 q0 = quad_bcast3<0>(P), q1 = quad_bcast3<1>(P), q2 = quad_bcast3<2>(P), q3 = quad_bcast3<3>(P);
-    if (live) level_fused_points<GATHER, CREASED>(a, h, j, x, P, q0, q1, q2, q3, owner, twin_here, base, spts, tr0, tr1,
-                                                   tr2, tr3);
+    if (live) level_fused_points<GATHER, CREASED>(a, h, j, x, P, q0, q1, q2, q3, owner, twin_here, base, spts, g0, g1,
+                                                   g2, g3, s);
     if (a.owner_mask) {
         // one bit per halfedge: is it the representative of its vertex?  Last,
         // so that the cage lookups of a few lanes overlap the wave's stores

@@ -98,9 +98,9 @@ __device__ __forceinline__ void patch_tile(const FusedArgs &a, const DCage &c, c
             fuv.u /= 4.0f;
             fuv.v /= 4.0f;
             ch[0].w = x.w;
-            ch[1].w = uv_encode(uv_mid(uv, nuv));
-            ch[2].w = uv_encode(fuv);
-            ch[3].w = uv_encode(uv_mid(uv, puv));
+            ch[1].w = uv_encode_unit(uv_mid(uv, nuv));
+            ch[2].w = uv_encode_unit(fuv);
+            ch[3].w = uv_encode_unit(uv_mid(uv, puv));
         }
         int4 *out4 = reinterpret_cast<int4 *>(a.out_he);
         if (kStaged) {
