@@ -161,7 +161,7 @@ void batch_copy(const BatchTable &t, const float *src, float *dst, hipStream_t s
 void batch_level_patch(const BatchTable &t, char *arena, int d, int gather, int creased, int topo, int nt, int hpt,
                        int remap, hipStream_t st);
 void batch_level_deferred(const BatchTable &t, char *arena, int d, int gather, int creased, hipStream_t st);
-// fused children + face/edge points (one workgroup per 256 halfedges), then
+// fused children + face/edge points (one workgroup per kFusedBlock halfedges), then
 // vertex points (one workgroup per 256 * scan halfedges; scan 0: the owners
 // the fused kernel marked, one workgroup per 1024 mask words)
 void batch_level_fused(const BatchTable &t, char *arena, int d, int gather, int creased, int topo, int remap,

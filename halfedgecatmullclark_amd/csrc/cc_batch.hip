@@ -92,7 +92,7 @@ __global__ __launch_bounds__(NT) void k_batch_patch(const LevelJob *__restrict__
 // children with face and edge points of job k's level (one block per 256
 // halfedges)
 template <int GATHER, int CREASED, int TOPO>
-__global__ __launch_bounds__(256) void k_batch_fused(const LevelJob *__restrict__ jobs, const int32_t *__restrict__ start,
+__global__ __launch_bounds__(kFusedBlock) void k_batch_fused(const LevelJob *__restrict__ jobs, const int32_t *__restrict__ start,
                                                      int n, int d, int remap, char *arena) {
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, remap);
     const int k = find_job(start, n, lb);
@@ -100,7 +100,8 @@ __global__ __launch_bounds__(256) void k_batch_fused(const LevelJob *__restrict_
     FusedArgs a;
     DCage c;
     unpack_level(J, arena, a, c);
-    level_fused_block<GATHER, CREASED, TOPO, TOPO ? 3 : 0, 1, kStoreLds>(a, c, J.t, d, lb - (uint32_t)start[k]);
+    level_fused_block<GATHER, CREASED, TOPO, TOPO ? 3 : 0, 1, kStoreLds, kFusedBlock>(a, c, J.t, d,
+                                                                                   lb - (uint32_t)start[k]);
 }
 
 // vertex points of the owners k_batch_fused marked (kOwnerWords mask words
@@ -157,11 +158,11 @@ template <int G, int C>
 void fused_gc(const BatchTable &t, char *arena, int d, int topo, int remap, hipStream_t st) {
     const LevelJob *jobs = static_cast<const LevelJob *>(t.jobs);
     if (topo)
-        hipLaunchKernelGGL((k_batch_fused<G, C, 1>), dim3(t.blocks), dim3(256), 0, st, jobs, t.start, (int)t.n, d, remap,
-                           arena);
+        hipLaunchKernelGGL((k_batch_fused<G, C, 1>), dim3(t.blocks), dim3(kFusedBlock), 0, st, jobs, t.start, (int)t.n,
+                           d, remap, arena);
     else
-        hipLaunchKernelGGL((k_batch_fused<G, C, 0>), dim3(t.blocks), dim3(256), 0, st, jobs, t.start, (int)t.n, d, remap,
-                           arena);
+        hipLaunchKernelGGL((k_batch_fused<G, C, 0>), dim3(t.blocks), dim3(kFusedBlock), 0, st, jobs, t.start, (int)t.n,
+                           d, remap, arena);
 }
 
 // job k's deferred vertices: kDeferGroup mask words per wave

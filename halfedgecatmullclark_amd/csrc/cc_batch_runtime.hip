@@ -354,7 +354,7 @@ bool ccb_Batch::build_tables() {
             const int64_t tiles = blocks_of(g.n.H[d], cfg[d].nt);
             bp.push_back((tiles + cfg[d].hpt - 1) / cfg[d].hpt);
             bd.push_back(blocks_of(blocks_of(blocks_of(g.n.H[d], 64), kDeferGroup), 4));
-            bf.push_back(blocks_of(g.n.H[d]));
+            bf.push_back(blocks_of(g.n.H[d], kFusedBlock));
             bv.push_back(cfg[d].scan == 0 ? blocks_of(blocks_of(blocks_of(g.n.H[d], 64), kOwnerWords), 4)
                                           : blocks_of(g.n.H[d], 256 * cfg[d].scan));
         }

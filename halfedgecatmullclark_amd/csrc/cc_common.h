@@ -122,6 +122,16 @@ __host__ __device__ __forceinline__ P3 lerp(P3 a, P3 b, float u) {
 }
 __host__ __device__ __forceinline__ P3 zero3() { return {0.0f, 0.0f, 0.0f}; }
 
+// A 16-byte record store marked non-temporal: the child records of a level
+// are read again only by the next level's kernel, after gigabytes of other
+// traffic, so they need not displace the points that the vertex kernel is
+// about to gather.
+__device__ __forceinline__ void store_stream(int4 *dst, int4 v) {
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    const v4i w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<v4i *>(dst));
+}
+
 // UV codec (CatmullClark.h:374-392).  The encode keeps the low 32 bits of a
 // 64-bit truncating conversion, which is what x86-64 compilers emit for the
 // reference's float -> uint32_t conversion, so out-of-range inputs agree too.

@@ -7,6 +7,12 @@ namespace cch {
 
 enum { kStoreDirect = 0, kStoreLds = 1 };
 
+// Threads (parent halfedges) per workgroup of the fused level kernel: a
+// workgroup covers the descendants of one level d-5 face (a 16 x 16 patch of
+// quads on a regular grid), and the twin quads of all but the patch's border
+// halfedges come from its LDS rather than from memory.
+constexpr int kFusedBlock = 1024;
+
 // Level-d inputs and level-(d+1) outputs of one fused launch.
 struct FusedArgs {
     const He *he;       // level d halfedges

@@ -25,9 +25,9 @@ namespace cch {
 
 
 template <int GATHER, int CREASED, int TOPO, int UVMODE, int POINTS, int STORE>
-__global__ __launch_bounds__(256) void k_level_fused(FusedArgs a, DCage c, LevelTab t, int d) {
-    level_fused_block<GATHER, CREASED, TOPO, UVMODE, POINTS, STORE>(a, c, t, d,
-                                                                    xcd_block(blockIdx.x, gridDim.x, a.xcd_remap));
+__global__ __launch_bounds__(kFusedBlock) void k_level_fused(FusedArgs a, DCage c, LevelTab t, int d) {
+    level_fused_block<GATHER, CREASED, TOPO, UVMODE, POINTS, STORE, kFusedBlock>(
+        a, c, t, d, xcd_block(blockIdx.x, gridDim.x, a.xcd_remap));
 }
 
 // vertex points of the owners the fused kernel marked (kOwnerWords mask
@@ -242,8 +242,8 @@ static void launch_fused_impl(const FusedArgs &a, const DCage &c, const LevelTab
 
 void launch_level_fused(const FusedArgs &a, const DCage &c, const LevelTab &t, int d, int gather, int creased, int topo,
                         int uvMode, int points, int store, hipStream_t st) {
-    const dim3 block(256);
-    const dim3 grid((unsigned)((a.H + 255) / 256));
+    const dim3 block(kFusedBlock);
+    const dim3 grid((unsigned)((a.H + kFusedBlock - 1) / kFusedBlock));
     launch_fused_impl(a, c, t, d, gather ? 1 : 0, creased ? 1 : 0, topo ? 1 : 0, uvMode, points ? 1 : 0, store, grid,
                       block, st);
 }

@@ -60,12 +60,12 @@ __device__ __forceinline__ void level_fused_points(const FusedArgs &a, int32_t h
     a.ep[x.y] = Ev;
 }
 
-template <int GATHER, int CREASED, int TOPO, int UVMODE, int POINTS, int STORE>
+template <int GATHER, int CREASED, int TOPO, int UVMODE, int POINTS, int STORE, int NT>
 __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCage &cg, const LevelTab &tb, int d,
                                                   uint32_t lb) {
     constexpr bool kStaged = STORE == kStoreLds && TOPO && UVMODE != 0;
-    __shared__ int4 stage[kStaged ? 4 * 256 : 1];
-    __shared__ P3 spts[POINTS ? 256 : 1];
+    __shared__ int4 stage[kStaged ? 4 * NT : 1];
+    __shared__ P3 spts[POINTS ? NT : 1];
     __shared__ LevelTab stab;   // for the owner test (read after the block's barrier)
     if (POINTS && a.owner_mask) lds_fill(stab, tb);
     const int32_t base = (int32_t)(lb * blockDim.x);
@@ -142,7 +142,7 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const int64_t o = obase + threadIdx.x + (int64_t)k * blockDim.x;
-                if (o < limit) out4[o] = stage[threadIdx.x + k * blockDim.x];
+                if (o < limit) store_stream(out4 + o, stage[threadIdx.x + k * blockDim.x]);
             }
         } else if (live) {
             if (TOPO && UVMODE != 0) {
