@@ -295,6 +295,52 @@ __device__ __forceinline__ bool owner_fan(int32_t h, int32_t vert, int d, const 
     return true;
 }
 
+// The fan and vertex ID of a level-d halfedge h known to be the
+// representative of its vertex (marked by the fused kernel).  A face or edge
+// vertex born at level lvl >= 2 takes its ID from the child numbering
+// (child 2 of p: V + face(p), child 1: V + F + edge(p), parent-level counts),
+// so only an edge vertex reads one record, its parent's (twin and edge);
+// cage-level vertices read h's record and go through owner_fan.
+__device__ __forceinline__ int32_t marked_fan(int32_t h, int d, const DCage &c, const LevelTab &t,
+                                              const He *const *levels, Fan &f) {
+    int k = 0;
+    while (k < d - 1 && ((h >> (2 * k)) & 3) == 0) ++k;
+    const int32_t q = h >> (2 * k);
+    const int lvl = d - k;
+    const int digit = q & 3;
+    const int32_t p = q >> 2;
+    if (lvl >= 2 && digit == 2) {
+        f.kind = 1;
+        f.shift = 2 * k;
+        f.boundary = false;
+        f.ri = 0;
+        f.n = 4;
+        f.a0 = 4 * p + 2;
+        return (int32_t)t.V[lvl - 1] + (p >> 2);
+    }
+    if (lvl >= 2 && digit == 1) {
+        const int4 x = reinterpret_cast<const int4 *>(levels[lvl - 1])[p];
+        const int32_t tw = x.x, np = qnext(p);
+        f.kind = 3;
+        f.shift = 2 * k;
+        if (tw >= 0) {
+            f.boundary = false;
+            f.ri = 0;
+            f.n = 4;
+            f.a0 = 4 * p + 1; f.a1 = 4 * qnext(tw) + 3; f.a2 = 4 * tw + 1; f.a3 = 4 * np + 3;
+        } else {
+            f.boundary = true;
+            f.ri = 1;
+            f.n = 2;
+            f.a0 = 4 * np + 3; f.a1 = 4 * p + 1; f.a2 = f.a3 = 0;
+        }
+        return (int32_t)(t.V[lvl - 1] + t.F[lvl - 1]) + x.y;
+    }
+    const int32_t v = reinterpret_cast<const int4 *>(levels[d])[h].z;
+    owner_fan(h, v, d, c, t, levels, f);
+    return v;
+}
+
 
 // The fan of vertex v of level d >= 1, equal to what owner_fan gives its
 // representative (CatmullClark.h:1336): v was born at the level b with
@@ -543,44 +589,97 @@ __device__ __forceinline__ P3 fan_vertex_point(const Fan &f, const Acc &acc, P3 
     return sum;
 }
 
+// The four entries of a closed fan from its halfedges r[q] and their edges
+// re[q]: the previous edge of r[q] is the edge of r[q+1].
+__device__ __forceinline__ FanPreloaded closed_fan(const FusedArgs &a, const int32_t (&r)[4], const int32_t (&re)[4]) {
+    P3 E[4], F[4];
+    float s[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        E[q] = a.ep[re[q]];
+        F[q] = a.fp[r[q] >> 2];
+        s[q] = lsharp(a.cr, a.C, re[q]);
+    }
+    FanEntry e[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int n1 = (q + 1) & 3;
+        e[q].r = r[q];
+        e[q].re = re[q];
+        e[q].pe = re[n1];
+        e[q].E = E[q];
+        e[q].PE = E[n1];
+        e[q].F = F[q];
+        e[q].ts = s[q];
+        e[q].ps = s[n1];
+    }
+    return FanPreloaded{e[0], e[1], e[2], e[3]};
+}
+
 // vertex point of vertex v of level d with fan f (kinds 1 and 3 load their
 // at most four entries up front, cage fans are walked)
 template <int GATHER, int CREASED>
-__device__ __forceinline__ void fan_vertex_store(const FusedArgs &a, const DCage &c, const Fan &f, int32_t v) {
+__device__ __forceinline__ void fan_vertex_store(const FusedArgs &a, const DCage &c, const LevelTab &t, int d,
+                                                 const Fan &f, int32_t v) {
     const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
     const P3 old = a.old[v];
-    if (f.kind == 1 || (f.kind == 3 && !f.boundary)) {
+    const int lvl = d - (f.shift >> 1);   // the level the vertex was born at
+    if (lvl >= 2 && (f.kind == 1 || (f.kind == 3 && !f.boundary))) {
+        // A face or interior edge vertex born at level lvl >= 2: the ring at
+        // level lvl are children of level-(lvl-1) halfedges, and the ring at
+        // level d their children 0, 00, ...  Edges and twins follow from the
+        // child numbering (CatmullClark.h:2770-2793, E the parent level's
+        // edge count): child 0 of h: edge 2 edge(h) + (h > twin(h) ? 0 : 1),
+        // twin 4 next(twin(h)) + 3; child 1: edge 2E + h, twin 4 next(h) + 2;
+        // child 2: edge 2E + prev(h), twin 4 prev(h) + 1; child 3: edge
+        // 2 edge(prev h) + (prev h > twin(prev h)), twin 4 twin(prev h).  So
+        // no ring record is read.  Face ring of quad 4f': children 2 of
+        // 4f'+q.  Edge ring of e = edge(p) = edge(w), w = twin(p): 4p+1,
+        // 4 next(w)+3, 4w+1, 4 next(p)+3.
+        const int32_t Ep2 = 2 * (int32_t)t.E[lvl - 1];
+        int32_t h[4], e[4], tw[4];
+        if (f.kind == 1) {
+            const int32_t p = (f.a0 - 2) >> 2;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int32_t pp = p + ((q + 3) & 3);
+                h[q] = 4 * (p + q) + 2;
+                e[q] = Ep2 + pp;
+                tw[q] = 4 * pp + 1;
+            }
+        } else {
+            const int32_t p = (f.a0 - 1) >> 2, w = (f.a2 - 1) >> 2;
+            const int32_t e2 = 2 * (v - (int32_t)t.V[lvl - 1] - (int32_t)t.F[lvl - 1]);
+            h[0] = f.a0; h[1] = f.a1; h[2] = f.a2; h[3] = f.a3;
+            e[0] = Ep2 + p;
+            e[1] = e2 + (w > p ? 1 : 0);
+            e[2] = Ep2 + w;
+            e[3] = e2 + (p > w ? 1 : 0);
+            tw[0] = 4 * qnext(p) + 2;
+            tw[1] = 4 * p;
+            tw[2] = 4 * qnext(w) + 2;
+            tw[3] = 4 * w;
+        }
+        for (int j = lvl; j < d; ++j) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                e[q] = 2 * e[q] + (h[q] > tw[q] ? 0 : 1);
+                tw[q] = 4 * qnext(tw[q]) + 3;
+                h[q] = 4 * h[q];
+            }
+        }
+        const FanPreloaded pre = closed_fan(a, h, e);
+        a.vp[v] = fan_vertex_point<GATHER, CREASED, 4>(f, pre, old);
+    } else if (f.kind == 1 || (f.kind == 3 && !f.boundary)) {
         // a closed fan of four in forward order: r[q+1] = twin(prev(r[q])),
         // so the previous edge of r[q] is the edge of r[q+1] and only the
         // four fan records are read
-        int32_t r[4];
-        int4 rec[4];
+        int32_t r[4], re[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) r[q] = fan_at(f, c, q);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) rec[q] = he4[r[q]];
-        P3 E[4], F[4];
-        float s[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            E[q] = a.ep[rec[q].y];
-            F[q] = a.fp[r[q] >> 2];
-            s[q] = lsharp(a.cr, a.C, rec[q].y);
-        }
-        FanEntry e[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int n1 = (q + 1) & 3;
-            e[q].r = r[q];
-            e[q].re = rec[q].y;
-            e[q].pe = rec[n1].y;
-            e[q].E = E[q];
-            e[q].PE = E[n1];
-            e[q].F = F[q];
-            e[q].ts = s[q];
-            e[q].ps = s[n1];
-        }
-        FanPreloaded pre{e[0], e[1], e[2], e[3]};
+        for (int q = 0; q < 4; ++q) re[q] = he4[r[q]].y;
+        const FanPreloaded pre = closed_fan(a, r, re);
         a.vp[v] = fan_vertex_point<GATHER, CREASED, 4>(f, pre, old);
     } else if (f.kind == 3) {
         int32_t r[4];

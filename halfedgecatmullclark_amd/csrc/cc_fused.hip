@@ -221,7 +221,7 @@ __global__ __launch_bounds__(256) void k_level_vertices_direct(FusedArgs a, DCag
     if (v >= a.V) return;
     Fan f;
     vertex_fan(v, d, c, t, lp.he, f);
-    fan_vertex_store<GATHER, CREASED>(a, c, f, v);
+    fan_vertex_store<GATHER, CREASED>(a, c, t, d, f, v);
 }
 
 // ---- launcher ----------------------------------------------------------------

@@ -208,7 +208,7 @@ __device__ __forceinline__ void level_vertices_compact_block(const FusedArgs &a,
         const int32_t v = he4[h].z;
         Fan f;
         owner_fan(h, v, d, c, st, slp.he, f);
-        fan_vertex_store<GATHER, CREASED>(a, c, f, v);
+        fan_vertex_store<GATHER, CREASED>(a, c, st, d, f, v);
     }
 }
 
@@ -255,10 +255,9 @@ struct MaskedVertexFn {
     const LevelPtrs &lp;
     int d;
     __device__ void operator()(int32_t h) const {
-        const int32_t v = reinterpret_cast<const int4 *>(a.he)[h].z;
         Fan f;
-        owner_fan(h, v, d, c, t, lp.he, f);
-        fan_vertex_store<GATHER, CREASED>(a, c, f, v);
+        const int32_t v = marked_fan(h, d, c, t, lp.he, f);
+        fan_vertex_store<GATHER, CREASED>(a, c, t, d, f, v);
     }
 };
 

@@ -213,7 +213,7 @@ __device__ __forceinline__ void deferred_vertex(const FusedArgs &a, const DCage 
     const int32_t v = he4[h].z;
     Fan f;
     owner_fan(h, v, d, c, t, lp.he, f);
-    fan_vertex_store<GATHER, CREASED>(a, c, f, v);
+    fan_vertex_store<GATHER, CREASED>(a, c, t, d, f, v);
 }
 
 // Source mask words per wave of the deferred kernel: about a quarter of the
