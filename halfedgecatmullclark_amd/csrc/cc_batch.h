@@ -36,7 +36,7 @@ struct CreaseJob {
 // global rather than flat instructions); -1 marks an absent buffer.
 struct LevelJob {
     int64_t he, out_he, cr, old, vp, fp, ep, defer_mask, owner_mask;
-    int64_t c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr;
+    int64_t c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, c_vro, c_vr, c_vb, c_fro, c_fr;
     int64_t lp[kMaxDepth + 1];
     int32_t H, V, F, E, C;
     int32_t cV, cU, cH, cE, cF;
@@ -55,7 +55,7 @@ struct SpanJob {
 // adjoint arena.
 struct AdjJob {
     int64_t he, cr, gV, gF, gE, gP, W;
-    int64_t c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr;
+    int64_t c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, c_vro, c_vr, c_vb, c_fro, c_fr;
     int64_t lp[kMaxDepth + 1];
     int32_t H, C;
     int32_t cV, cU, cH, cE, cF;
@@ -67,18 +67,22 @@ __host__ __device__ __forceinline__ T *arena_ptr(char *base, int64_t off) {
     return off < 0 ? nullptr : reinterpret_cast<T *>(base + off);
 }
 
-__host__ __device__ __forceinline__ DCage arena_cage(char *base, int64_t he, int64_t v2h, int64_t e2h, int64_t f2h,
-                                                     int64_t pts, int64_t uvs, int64_t cr, int32_t V, int32_t U,
-                                                     int32_t H, int32_t E, int32_t F) {
+template <class J>
+__host__ __device__ __forceinline__ DCage arena_cage(char *base, const J &j) {
     DCage c;
-    c.he = arena_ptr<const CageHe>(base, he);
-    c.v2h = arena_ptr<const int32_t>(base, v2h);
-    c.e2h = arena_ptr<const int32_t>(base, e2h);
-    c.f2h = arena_ptr<const int32_t>(base, f2h);
-    c.pts = arena_ptr<const P3>(base, pts);
-    c.uvs = arena_ptr<const UV>(base, uvs);
-    c.cr = arena_ptr<const Crease>(base, cr);
-    c.V = V; c.U = U; c.H = H; c.E = E; c.F = F;
+    c.he = arena_ptr<const CageHe>(base, j.c_he);
+    c.v2h = arena_ptr<const int32_t>(base, j.c_v2h);
+    c.e2h = arena_ptr<const int32_t>(base, j.c_e2h);
+    c.f2h = arena_ptr<const int32_t>(base, j.c_f2h);
+    c.pts = arena_ptr<const P3>(base, j.c_pts);
+    c.uvs = arena_ptr<const UV>(base, j.c_uvs);
+    c.cr = arena_ptr<const Crease>(base, j.c_cr);
+    c.vring_off = arena_ptr<const int32_t>(base, j.c_vro);
+    c.vring = arena_ptr<const int32_t>(base, j.c_vr);
+    c.vring_bnd = arena_ptr<const int32_t>(base, j.c_vb);
+    c.fring_off = arena_ptr<const int32_t>(base, j.c_fro);
+    c.fring = arena_ptr<const int32_t>(base, j.c_fr);
+    c.V = j.cV; c.U = j.cU; c.H = j.cH; c.E = j.cE; c.F = j.cF;
     return c;
 }
 
@@ -96,7 +100,7 @@ __host__ __device__ __forceinline__ void unpack_level(const LevelJob &J, char *a
     a.H = J.H; a.V = J.V; a.F = J.F; a.E = J.E; a.C = J.C;
     a.xcd_remap = J.xcd_remap;
     a.uv_refine = J.uv_refine;
-    c = arena_cage(arena, J.c_he, J.c_v2h, J.c_e2h, J.c_f2h, J.c_pts, J.c_uvs, J.c_cr, J.cV, J.cU, J.cH, J.cE, J.cF);
+    c = arena_cage(arena, J);
 }
 
 __host__ __device__ __forceinline__ void unpack_adj(const AdjJob &J, char *arena, char *adj, AdjArgs<float> &a,
@@ -110,7 +114,7 @@ __host__ __device__ __forceinline__ void unpack_adj(const AdjJob &J, char *arena
     a.gE = arena_ptr<float>(adj, J.gE);
     a.gP = arena_ptr<float>(adj, J.gP);
     a.W = arena_ptr<float>(adj, J.W);
-    c = arena_cage(arena, J.c_he, J.c_v2h, J.c_e2h, J.c_f2h, J.c_pts, J.c_uvs, J.c_cr, J.cV, J.cU, J.cH, J.cE, J.cF);
+    c = arena_cage(arena, J);
 }
 
 // largest k in [0, n) with start[k] <= b (jobs without workgroups are skipped)

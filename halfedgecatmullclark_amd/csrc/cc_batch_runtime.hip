@@ -27,6 +27,8 @@ struct ccb_Batch {
         LevelTab tab;
         // element offsets: cage arrays, then levels 1..D, then per-cage scratch
         int64_t o_he0 = 0, o_v0 = 0, o_u0 = 0, o_e0 = 0, o_f0 = 0;
+        // element offsets of the cage's ring lists in `rings` (-1: walk the cage)
+        int64_t o_vro = -1, o_vr = -1, o_vb = -1, o_fro = -1, o_fr = -1;
         int64_t o_he = 0, o_cr = 0, o_pts = 0, o_fin = 0, o_def = 0, o_adjB = 0, o_adjW = 0;
     };
     struct Timed {
@@ -40,6 +42,8 @@ struct ccb_Batch {
     int32_t N = 0, Dmax = 0;
     std::vector<Cage> cages;
     int64_t tot_he0 = 0, tot_v0 = 0, tot_u0 = 0, tot_e0 = 0, tot_f0 = 0;
+    std::vector<int32_t> rings_host;   // every cage's ring lists, copied into `rings`
+    int32_t *rings = nullptr;
     int64_t tot_he = 0, tot_cr = 0, tot_pts = 0, tot_fin = 0, tot_def = 0, tot_adjB = 0, tot_adjW = 0;
     // device memory: every array of the forward pass is carved out of one
     // arena, the adjoint's out of a second one (job tables hold byte offsets
@@ -159,6 +163,9 @@ DCage ccb_Batch::cage_view(int i) const {
     c.pts = c_pts + g.o_v0;
     c.uvs = c_uvs + g.o_u0;
     c.cr = c_cr + g.o_e0;
+    auto ring = [&](int64_t o) -> const int32_t * { return o < 0 ? nullptr : rings + o; };
+    c.vring_off = ring(g.o_vro); c.vring = ring(g.o_vr); c.vring_bnd = ring(g.o_vb);
+    c.fring_off = ring(g.o_fro); c.fring = ring(g.o_fr);
     c.V = g.V0; c.U = g.U0; c.H = g.H0; c.E = g.E0; c.F = g.F0;
     return c;
 }
@@ -178,6 +185,20 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
         g.o_u0 = tot_u0; tot_u0 += g.U0;
         g.o_e0 = tot_e0; tot_e0 += g.E0;
         g.o_f0 = tot_f0; tot_f0 += g.F0;
+        const CageRings r = build_cage_rings(m);
+        if (r.ok) {
+            auto put = [&](const std::vector<int32_t> &v) {
+                const int64_t o = (int64_t)rings_host.size();
+                rings_host.insert(rings_host.end(), v.begin(), v.end());
+                rings_host.push_back(0);
+                return o;
+            };
+            g.o_vro = put(r.voff);
+            g.o_vb = put(r.vbnd);
+            g.o_fro = put(r.foff);
+            g.o_vr = put(r.vring);
+            g.o_fr = put(r.fring);
+        }
         g.o_he = tot_he; tot_he += g.n.cumH(g.D);
         g.o_cr = tot_cr; tot_cr += g.n.cumC(g.D);
         g.o_pts = tot_pts; tot_pts += g.n.cumV(g.D);
@@ -198,6 +219,7 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
         const size_t o_cpts = cv.take(sizeof(P3) * (size_t)tot_v0);
         const size_t o_cuvs = cv.take(sizeof(UV) * (size_t)(tot_u0 + 1));
         const size_t o_ccr = cv.take(sizeof(Crease) * (size_t)tot_e0);
+        const size_t o_rings = cv.take(sizeof(int32_t) * (rings_host.size() + 1));
         const size_t o_he = cv.take(sizeof(He) * (size_t)tot_he);
         const size_t o_cr = cv.take(sizeof(Crease) * (size_t)tot_cr);
         const size_t o_pts = cv.take(sizeof(P3) * (size_t)tot_pts);
@@ -210,6 +232,7 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
         c_pts = reinterpret_cast<P3 *>(arena + o_cpts);
         c_uvs = reinterpret_cast<UV *>(arena + o_cuvs);
         c_cr = reinterpret_cast<Crease *>(arena + o_ccr);
+        rings = reinterpret_cast<int32_t *>(arena + o_rings);
         he = reinterpret_cast<He *>(arena + o_he);
         cr = reinterpret_cast<Crease *>(arena + o_cr);
         pts = reinterpret_cast<P3 *>(arena + o_pts);
@@ -227,6 +250,9 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
             CC_TRY(hipMemcpyAsync(c_uvs + g.o_u0, m->uvs, sizeof(UV) * g.U0, hipMemcpyHostToDevice, stream), "upload");
         CC_TRY(hipMemcpyAsync(c_cr + g.o_e0, m->creases, sizeof(Crease) * g.E0, hipMemcpyHostToDevice, stream), "upload");
     }
+    if (!rings_host.empty())
+        CC_TRY(hipMemcpyAsync(rings, rings_host.data(), sizeof(int32_t) * rings_host.size(), hipMemcpyHostToDevice,
+                              stream), "upload cage rings");
     CC_TRY(hipStreamSynchronize(stream), "batch upload");
     return build_tables();
 }
@@ -275,6 +301,9 @@ bool ccb_Batch::build_tables() {
         j.c_pts = off(c_pts + g.o_v0);
         j.c_uvs = off(c_uvs + g.o_u0);
         j.c_cr = off(c_cr + g.o_e0);
+        auto roff = [&](int64_t o) -> int64_t { return o < 0 ? -1 : off(rings + o); };
+        j.c_vro = roff(g.o_vro); j.c_vr = roff(g.o_vr); j.c_vb = roff(g.o_vb);
+        j.c_fro = roff(g.o_fro); j.c_fr = roff(g.o_fr);
         j.cV = g.V0; j.cU = g.U0; j.cH = g.H0; j.cE = g.E0; j.cF = g.F0;
     };
     {   // cage step

@@ -75,6 +75,12 @@ struct DCage {
     const P3 *pts;
     const UV *uvs;
     const Crease *cr;
+    // Cage rings, listed once on the host (null: walk the cage instead).
+    // Vertex v: the halfedges of the forward walk twin(prev(g)) from v2h[v]
+    // are vring[vring_off[v] .. vring_off[v + 1]), vring_bnd[v] != 0 when the
+    // walk ends on the boundary; face f: its halfedges from f2h[f] by next
+    // are fring[fring_off[f] .. fring_off[f + 1]).
+    const int32_t *vring_off, *vring, *vring_bnd, *fring_off, *fring;
     int32_t V, U, H, E, F;
 };
 

@@ -182,24 +182,59 @@ struct Fan {
     int n;          // halfedges around the vertex
     int ri;         // index of the representative in the fan
     bool boundary;
-    int32_t a0, a1, a2, a3;
+    int32_t a0, a1, a2, a3;   // kinds 0 and 2 with cage rings: a1 = offset of the ring's list
 };
 
 __device__ __forceinline__ int32_t fan_at(const Fan &f, const DCage &c, int i) {
+    const int j = i < f.n ? (i > 0 ? i : 0) : (f.n > 1 ? f.n - 1 : 0);   // ring lists: stay in the list
     switch (f.kind) {
     case 1: return (f.a0 + 4 * i) << f.shift;
     case 3: return pick4(f.a0, f.a1, f.a2, f.a3, i) << f.shift;
     case 2: {  // the face's cage halfedges from f2h, children 2
+        if (c.fring) return (4 * c.fring[f.a1 + j] + 2) << f.shift;
         int32_t p = f.a0;
         for (int k = 0; k < i && k < kWalkCap; ++k) p = c.he[p].next;
         return (4 * p + 2) << f.shift;
     }
     default: {  // the vertex's cage halfedges in forward order, children 0
+        if (c.vring) return (4 * c.vring[f.a1 + j]) << f.shift;
         int32_t g = f.a0;
         for (int k = 0; k < i && k < kWalkCap; ++k) g = c.he[c.he[g].prev].twin;
         return (4 * g) << f.shift;
     }
     }
+}
+
+// the fan of cage vertex v with first halfedge p (kind 0): from its ring list
+// or by walking twin(prev(g)) (CatmullClark.h:1336-1345)
+__device__ __forceinline__ void cage_vertex_fan(const DCage &c, int32_t v, int32_t p, Fan &f) {
+    f.kind = 0;
+    f.a0 = p;
+    if (c.vring) {
+        f.a1 = c.vring_off[v];
+        f.n = c.vring_off[v + 1] - f.a1;
+        f.boundary = c.vring_bnd[v] != 0;
+        return;
+    }
+    int n = 1;
+    int32_t g = c.he[c.he[p].prev].twin;
+    for (; g >= 0 && g != p && n < kWalkCap; g = c.he[c.he[g].prev].twin) ++n;
+    f.n = n;
+    f.boundary = g < 0;
+}
+
+// the fan of the face point of cage face `face` with first halfedge p (kind 2)
+__device__ __forceinline__ void cage_face_fan(const DCage &c, int32_t face, int32_t p, Fan &f) {
+    f.kind = 2;
+    f.a0 = p;
+    if (c.fring) {
+        f.a1 = c.fring_off[face];
+        f.n = c.fring_off[face + 1] - f.a1;
+        return;
+    }
+    int n = 1;
+    for (int32_t it = c.he[p].next; it != p && n < kWalkCap; it = c.he[it].next) ++n;
+    f.n = n;
 }
 
 // Is h the representative of its vertex (CatmullClark.h:1336)?  The test of
@@ -232,13 +267,7 @@ __device__ __forceinline__ bool owner_fan(int32_t h, int32_t vert, int d, const 
     f.ri = 0;
     if (digit == 0) {                 // lvl == 1: a cage vertex
         if (c.v2h[vert] != p) return false;
-        f.kind = 0;
-        f.a0 = p;
-        int n = 1;
-        int32_t g = c.he[c.he[p].prev].twin;
-        for (; g >= 0 && g != p && n < kWalkCap; g = c.he[c.he[g].prev].twin) ++n;
-        f.n = n;
-        f.boundary = g < 0;
+        cage_vertex_fan(c, vert, p, f);
         return true;
     }
     if (digit == 3) return false;
@@ -252,11 +281,7 @@ __device__ __forceinline__ bool owner_fan(int32_t h, int32_t vert, int d, const 
         }
         const int32_t face = c.he[p].face;
         if (c.f2h[face] != p) return false;
-        f.kind = 2;
-        f.a0 = p;
-        int n = 1;
-        for (int32_t it = c.he[p].next; it != p && n < kWalkCap; it = c.he[it].next) ++n;
-        f.n = n;
+        cage_face_fan(c, face, p, f);
         return true;
     }
     // digit 1: an edge vertex of edge(p)
@@ -354,15 +379,8 @@ __device__ __forceinline__ void vertex_fan(int32_t v, int d, const DCage &c, con
     f.boundary = false;
     f.ri = 0;
     if (b == 0) {   // a cage vertex
-        const int32_t p = c.v2h[v];
-        f.kind = 0;
         f.shift = 2 * (d - 1);
-        f.a0 = p;
-        int n = 1;
-        int32_t g = c.he[c.he[p].prev].twin;
-        for (; g >= 0 && g != p && n < kWalkCap; g = c.he[c.he[g].prev].twin) ++n;
-        f.n = n;
-        f.boundary = g < 0;
+        cage_vertex_fan(c, v, c.v2h[v], f);
         return;
     }
     f.shift = 2 * (d - b);
@@ -376,12 +394,7 @@ __device__ __forceinline__ void vertex_fan(int32_t v, int d, const DCage &c, con
             f.n = 4;
             return;
         }
-        const int32_t p = c.f2h[face];
-        f.kind = 2;
-        f.a0 = p;
-        int n = 1;
-        for (int32_t it = c.he[p].next; it != p && n < kWalkCap; it = c.he[it].next) ++n;
-        f.n = n;
+        cage_face_fan(c, face, c.f2h[face], f);
         return;
     }
     const int32_t e = v - Vp - Fp;   // an edge vertex

@@ -1,6 +1,8 @@
 // Library-internal helpers shared by the host and runtime translation units.
 #pragma once
 
+#include <vector>
+
 #include <hip/hip_runtime.h>
 
 #include "../../include/CatmullClarkHIP.h"
@@ -31,9 +33,58 @@ struct DeviceGuard {
 // true when every cumulative count the reference evaluates in int32 fits
 bool ref_counts_fit_int32(const cc_Mesh *cage, int32_t D, const char **why);
 
+// The cage's vertex and face rings as the device walks them (kWalkCap
+// steps at most), in CSR form.  `ok` is false when the lists would exceed
+// 4 H + V entries (a cage whose walks do not close): the kernels then walk.
+struct CageRings {
+    std::vector<int32_t> voff, vring, vbnd, foff, fring;
+    bool ok = false;
+};
+
+inline CageRings build_cage_rings(const cc_Mesh *m) {
+    CageRings r;
+    const int32_t V = m->vertexCount, F = m->faceCount, H = m->halfedgeCount;
+    const CageHe *he = reinterpret_cast<const CageHe *>(m->halfedges);
+    const size_t cap = 4 * (size_t)H + (size_t)V;
+    r.voff.assign((size_t)V + 1, 0);
+    r.vbnd.assign((size_t)V, 0);
+    r.foff.assign((size_t)F + 1, 0);
+    auto valid = [&](int32_t h) { return h >= 0 && h < H; };
+    for (int32_t v = 0; v < V; ++v) {
+        r.voff[v] = (int32_t)r.vring.size();
+        const int32_t p = m->vertexToHalfedgeIDs[v];
+        if (!valid(p)) continue;
+        r.vring.push_back(p);
+        int32_t n = 1;
+        int32_t g = valid(he[p].prev) ? he[he[p].prev].twin : -1;
+        for (; g >= 0 && g != p && n < kWalkCap; ++n) {
+            if (!valid(g) || r.vring.size() >= cap) return r;
+            r.vring.push_back(g);
+            g = valid(he[g].prev) ? he[he[g].prev].twin : -1;
+        }
+        r.vbnd[v] = g < 0 ? 1 : 0;
+    }
+    r.voff[V] = (int32_t)r.vring.size();
+    for (int32_t f = 0; f < F; ++f) {
+        r.foff[f] = (int32_t)r.fring.size();
+        const int32_t p = m->faceToHalfedgeIDs[f];
+        if (!valid(p)) continue;
+        r.fring.push_back(p);
+        int32_t n = 1;
+        for (int32_t it = he[p].next; it != p && n < kWalkCap; it = he[it].next, ++n) {
+            if (!valid(it) || r.fring.size() >= cap) return r;
+            r.fring.push_back(it);
+        }
+    }
+    r.foff[F] = (int32_t)r.fring.size();
+    r.ok = true;
+    return r;
+}
+
 // host-memory view of a cage with the device-view type (for the O(depth) maps)
 inline DCage host_cage_view(const cc_Mesh *m) {
     DCage c;
+    c.vring_off = c.vring = c.vring_bnd = c.fring_off = c.fring = nullptr;
     c.he = reinterpret_cast<const CageHe *>(m->halfedges);
     c.v2h = m->vertexToHalfedgeIDs;
     c.e2h = m->edgeToHalfedgeIDs;

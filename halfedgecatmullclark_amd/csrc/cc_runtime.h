@@ -26,6 +26,12 @@ struct ccd_Subd {
     cch::P3 *c_pts = nullptr;
     cch::UV *c_uvs = nullptr;
     cch::Crease *c_cr = nullptr;
+    // the cage's vertex and face ring lists (cch::DCage) in one device
+    // buffer; rings_ok false: the kernels walk the cage
+    int32_t *rings = nullptr;
+    size_t rings_cap = 0;
+    size_t o_vro = 0, o_vr = 0, o_vb = 0, o_fro = 0, o_fr = 0;
+    bool rings_ok = false;
     // levels 1..D, cumulative, laid out as cc_Subd's arrays
     cch::He *he = nullptr;
     cch::Crease *cr = nullptr;

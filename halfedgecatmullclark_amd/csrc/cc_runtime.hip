@@ -86,7 +86,7 @@ bool ccd_Subd::alloc(const cc_Mesh *cage, int32_t depth) {
 void ccd_Subd::free_all() {
     DeviceGuard g(device);
     void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts, adj_a, adj_b, adj_w, adj_in, adj_out, adj_sharp,
-                    defer_mask};
+                    defer_mask, rings};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : timed) {
@@ -110,6 +110,33 @@ bool ccd_Subd::upload_cage(const cc_Mesh *cage, hipStream_t st) {
     CC_TRY(hipMemcpyAsync(c_pts, cage->vertexPoints, sizeof(P3) * (size_t)V0, hipMemcpyHostToDevice, st), "upload cage");
     if (U0 > 0) CC_TRY(hipMemcpyAsync(c_uvs, cage->uvs, sizeof(UV) * (size_t)U0, hipMemcpyHostToDevice, st), "upload cage");
     CC_TRY(hipMemcpyAsync(c_cr, cage->creases, sizeof(Crease) * (size_t)E0, hipMemcpyHostToDevice, st), "upload cage");
+    // ring lists: [vertex offsets | vertex boundary flags | face offsets |
+    // vertex rings | face rings], each list padded by one entry
+    const CageRings r = build_cage_rings(cage);
+    rings_ok = r.ok;
+    if (!r.ok) return true;
+    std::vector<int32_t> rings_host;
+    auto put = [&](const std::vector<int32_t> &v) {
+        const size_t o = rings_host.size();
+        rings_host.insert(rings_host.end(), v.begin(), v.end());
+        rings_host.push_back(0);
+        return o;
+    };
+    o_vro = put(r.voff);
+    o_vb = put(r.vbnd);
+    o_fro = put(r.foff);
+    o_vr = put(r.vring);
+    o_fr = put(r.fring);
+    if (rings_host.size() > rings_cap) {
+        if (rings) (void)hipFree(rings);
+        rings = nullptr;
+        rings_cap = 0;
+        CC_TRY(hipMalloc(&rings, sizeof(int32_t) * rings_host.size()), "hipMalloc(cage rings)");
+        rings_cap = rings_host.size();
+    }
+    CC_TRY(hipMemcpyAsync(rings, rings_host.data(), sizeof(int32_t) * rings_host.size(), hipMemcpyHostToDevice, st),
+           "upload cage rings");
+    CC_TRY(hipStreamSynchronize(st), "upload cage rings");   // rings_host is freed on return
     return true;
 }
 
@@ -118,6 +145,12 @@ DCage ccd_Subd::cage_view(const P3 *points) const {
     c.he = c_he; c.v2h = c_v2h; c.e2h = c_e2h; c.f2h = c_f2h;
     c.pts = points ? points : c_pts;
     c.uvs = c_uvs; c.cr = c_cr;
+    if (rings_ok) {
+        c.vring_off = rings + o_vro; c.vring = rings + o_vr; c.vring_bnd = rings + o_vb;
+        c.fring_off = rings + o_fro; c.fring = rings + o_fr;
+    } else {
+        c.vring_off = c.vring = c.vring_bnd = c.fring_off = c.fring = nullptr;
+    }
     c.V = V0; c.U = U0; c.H = H0; c.E = E0; c.F = F0;
     return c;
 }
