@@ -52,10 +52,14 @@ def level_bytes(n, d):
 
 
 def vertex_kernel_bytes(n, d):
-    """Algorithmic bytes of the vertex-point kernel d -> d+1: the level-d
-    records (16 B), points and creases (12 B) and the level-(d+1) face and
-    edge points it reads once, the level-(d+1) vertex points it writes."""
-    return 16 * n["H"][d] + 12 * n["V"][d] + 12 * n["C"][d] + 12 * (n["F"][d] + n["E"][d]) + 12 * n["V"][d]
+    """Algorithmic bytes of the masked vertex-point kernel d -> d+1: the owner
+    bits (1 bit per level-d halfedge), the level-d points and creases (12 B),
+    the level-(d+1) face and edge points it reads once, one parent record
+    (16 B) per edge vertex born at level d (the other rings follow from the
+    child numbering), and the level-(d+1) vertex points it writes."""
+    born_edges = n["E"][d - 1] if d >= 2 else 0
+    return (n["H"][d] // 8 + 12 * n["V"][d] + 12 * n["C"][d] + 12 * (n["F"][d] + n["E"][d]) + 16 * born_edges
+            + 12 * n["V"][d])
 
 
 def patch_bytes(n, d):
