@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(HERE, "libcchip.so")
 HALFEDGES, CREASES, UVS, POINTS, GATHER, NOCREASES, FP64 = 1, 2, 4, 8, 16, 32, 64
 ALL = HALFEDGES | CREASES | UVS | POINTS
 PTR_HALFEDGES, PTR_CREASES, PTR_POINTS, PTR_CAGE_POINTS = 0, 1, 2, 3
-OPT_FUSED, OPT_STORE, OPT_XCD_REMAP, OPT_FAN, OPT_PATCH_TILE, OPT_PATCH_HPT, OPT_SCAN = 0, 1, 2, 3, 4, 5, 6
+OPT_FUSED, OPT_STORE, OPT_XCD_REMAP, OPT_FAN, OPT_PATCH_TILE, OPT_PATCH_HPT, OPT_SCAN, OPT_ADJ_VERTEX = 0, 1, 2, 3, 4, 5, 6, 7
 OPT_GRAPH = 100
 (COUNT_HALFEDGES, COUNT_CREASES, COUNT_POINTS, COUNT_CAGE_POINTS, COUNT_FINEST_POINTS, COUNT_DEPTH,
  COUNT_CAGES) = range(7)

@@ -14,7 +14,7 @@ struct AdjArgs {
     const He *he;
     const Crease *cr;
     const T *sd;   // sharpness of this level in T, or null to read cr
-    int32_t H, C;
+    int32_t H, C, V;   // halfedges, creases and vertices of the level
     T *gV, *gF, *gE;
     T *gP;
     T *W;          // per-halfedge vertex-rule weights (a, b, g, dl): 4 H scalars
@@ -28,6 +28,7 @@ struct AdjPlan {
     DCage cage;
     LevelTab tab;
     LevelPtrs lp;
+    int64_t per_vertex_max;   // levels with at most this many vertices: vertex passes one thread per vertex
 };
 
 // dL/d(cage points) from dL/d(level-D points): V_D x 3 in, V_0 x 3 out.

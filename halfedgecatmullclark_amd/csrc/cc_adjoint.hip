@@ -236,46 +236,133 @@ __device__ __forceinline__ void adj_vertex_gather(const AdjArgs<T> &a, const DCa
     reinterpret_cast<V3<T> *>(a.gP)[v] = {alpha * gv.x + acc.x, alpha * gv.y + acc.y, alpha * gv.z + acc.z};
 }
 
-// the owners of level-d vertices among 256 * SCAN halfedges, collected in
-// LDS so that every thread then works on one (as the forward vertex kernel)
-template <int SCAN, class Fn>
-__device__ __forceinline__ void owners_block(const He *he, int32_t H, int d, const DCage &c, const LevelTab &t,
+// Passes 1 and 4 for the vertex v with fan f: for face and interior edge
+// vertices born at level >= 2 the ring comes from the child numbering
+// (structural_ring), so no ring record is read; the other fans take the
+// general walks above.  The sums run over the same ring in the same order as
+// the forward rule.
+template <typename T, int CREASED>
+__device__ __forceinline__ void adj_weights_fan(const AdjArgs<T> &a, const DCage &c, const LevelTab &t, int d,
+                                                const Fan &f, int32_t v) {
+    int32_t h[4], e[4];
+    if (!structural_ring(f, t, d, v, h, e)) {
+        adj_ring_weights<T, CREASED>(a, c, f);
+        return;
+    }
+    T s[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s[q] = CREASED ? sharp_at(a, e[q]) : (T)0;
+    RingT<T> rp;
+    rp.n = 4; rp.cc = 0; rp.boundary = false; rp.s1 = rp.s2 = 0; rp.e1 = rp.e2 = -1; rp.sall = 0;
+    if (CREASED) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {   // the previous edge of h[i] is e[i+1]
+            const T se = s[(i + 1) & 3];
+            rp.cc += sgnT(se);
+            if (se != (T)0) {
+                if (rp.e1 < 0) { rp.s1 = se; rp.e1 = e[(i + 1) & 3]; }
+                else if (rp.e2 < 0) { rp.s2 = se; rp.e2 = e[(i + 1) & 3]; }
+            }
+            rp.sall += se;
+        }
+    }
+    W4<T> *W = reinterpret_cast<W4<T> *>(a.W);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const VW<T> w = level_vertex_weights<T>(rp, s[i], s[(i + 1) & 3], e[i], CREASED);
+        W[h[i]] = {w.a, w.b, w.g, w.dl};
+    }
+}
+
+template <typename T, int CREASED>
+__device__ __forceinline__ void adj_gather_fan(const AdjArgs<T> &a, const DCage &c, const LevelTab &t, int d,
+                                               const Fan &f, int32_t v) {
+    int32_t h[4], e[4];
+    if (!structural_ring(f, t, d, v, h, e)) {
+        adj_vertex_gather<T, CREASED>(a, c, f, v);
+        return;
+    }
+    const W4<T> *W = reinterpret_cast<const W4<T> *>(a.W);
+    const V3<T> *gE = reinterpret_cast<const V3<T> *>(a.gE);
+    const V3<T> *gF = reinterpret_cast<const V3<T> *>(a.gF);
+    const V3<T> gv = reinterpret_cast<const V3<T> *>(a.gV)[v];
+    T alpha = 0;
+    V3<T> acc{0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {   // interior ring: no boundary edge
+        const int32_t r = h[i], re = e[i], pe = e[(i + 1) & 3];
+        alpha += W[r].a;
+        T c0, c1, cf, d0, d1, df;
+        edge_weights<T, CREASED>(a, re, false, c0, c1, cf);
+        edge_weights<T, CREASED>(a, pe, false, d0, d1, df);
+        const V3<T> ev = gE[re], ep = gE[pe], g = gF[r >> 2];
+        acc.x += c0 * ev.x; acc.y += c0 * ev.y; acc.z += c0 * ev.z;
+        acc.x += d1 * ep.x; acc.y += d1 * ep.y; acc.z += d1 * ep.z;
+        acc.x += (T)0.25 * g.x; acc.y += (T)0.25 * g.y; acc.z += (T)0.25 * g.z;
+    }
+    reinterpret_cast<V3<T> *>(a.gP)[v] = {alpha * gv.x + acc.x, alpha * gv.y + acc.y, alpha * gv.z + acc.z};
+}
+
+// The owners of level-d vertices among 256 * SCAN halfedges (structural
+// test, no scan of the records), collected in LDS so that every thread then
+// works on one, in halfedge order (neighbouring rings, shared cache lines).
+// The level tables go to LDS first; the first barrier orders the copy.
+template <int SCAN, class Fn, class LP>
+__device__ __forceinline__ void owners_block(int32_t H, int d, const DCage &c, const LevelTab &t, const LP &lp,
                                              uint32_t lb, const Fn &fn) {
     __shared__ int32_t owners[256 * SCAN];
     __shared__ int32_t nown;
+    __shared__ LevelTab st;
+    __shared__ LevelPtrs slp;
+    lds_fill(st, t);
+    lds_fill(slp, lp);
     const int32_t base = (int32_t)(lb * 256 * SCAN);
-    const int4 *he4 = reinterpret_cast<const int4 *>(he);
     if (threadIdx.x == 0) nown = 0;
     __syncthreads();
-    int32_t vs[SCAN];
 #pragma unroll
     for (int k = 0; k < SCAN; ++k) {
         const int32_t h = base + k * 256 + (int32_t)threadIdx.x;
-        vs[k] = h < H ? he4[h].z : -1;
-    }
-#pragma unroll
-    for (int k = 0; k < SCAN; ++k) {
-        const int32_t h = base + k * 256 + (int32_t)threadIdx.x;
-        if (vs[k] >= 0 && owner_test(h, vs[k], d, c, t)) owners[atomicAdd(&nown, 1)] = h;
+        if (h < H && structural_owner(h, d, c, st, slp.he)) owners[atomicAdd(&nown, 1)] = h;
     }
     __syncthreads();
     const int32_t count = nown;
-    for (int32_t i = (int32_t)threadIdx.x; i < count; i += 256) fn(owners[i]);
+    for (int32_t i = (int32_t)threadIdx.x; i < count; i += 256) {
+        Fan f;
+        const int32_t v = marked_fan(owners[i], d, c, st, slp.he, f);
+        fn(st, f, v);
+    }
 }
 
 constexpr int kAdjScan = 4;
+
+// Passes 1 and 4 with one thread per vertex and the fan from the vertex ID
+// (vertex_fan): no owner scan and one round of dependent loads fewer, but
+// consecutive vertex IDs of split edges and older vertices have rings far
+// apart.  Faster while a level has few vertices (latency-bound launches),
+// slower on large levels (AdjPlan::per_vertex_max chooses).
+
+template <typename T, int CREASED, int GATHER_PASS>
+__device__ __forceinline__ void adj_vertex_pass(const AdjArgs<T> &a, const DCage &c, const LevelTab &t,
+                                                const LevelPtrs &lp, int d, int32_t v) {
+    __shared__ LevelTab st;
+    __shared__ LevelPtrs slp;
+    lds_fill(st, t);
+    lds_fill(slp, lp);
+    __syncthreads();
+    if (v >= a.V) return;
+    Fan f;
+    vertex_fan(v, d, c, st, slp.he, f);
+    if (GATHER_PASS) adj_gather_fan<T, CREASED>(a, c, st, d, f, v);
+    else adj_weights_fan<T, CREASED>(a, c, st, d, f, v);
+}
 
 template <typename T, int CREASED>
 struct RingWeightsFn {
     const AdjArgs<T> &a;
     const DCage &c;
-    const LevelTab &t;
-    const LevelPtrs &lp;
     int d;
-    __device__ void operator()(int32_t h) const {
-        Fan f;
-        owner_fan(h, a.he[h].vert, d, c, t, lp.he, f);
-        adj_ring_weights<T, CREASED>(a, c, f);
+    __device__ void operator()(const LevelTab &t, const Fan &f, int32_t v) const {
+        adj_weights_fan<T, CREASED>(a, c, t, d, f, v);
     }
 };
 
@@ -283,38 +370,11 @@ template <typename T, int CREASED>
 struct VertexGatherFn {
     const AdjArgs<T> &a;
     const DCage &c;
-    const LevelTab &t;
-    const LevelPtrs &lp;
     int d;
-    __device__ void operator()(int32_t h) const {
-        const int32_t v = a.he[h].vert;
-        Fan f;
-        owner_fan(h, v, d, c, t, lp.he, f);
-        adj_vertex_gather<T, CREASED>(a, c, f, v);
+    __device__ void operator()(const LevelTab &t, const Fan &f, int32_t v) const {
+        adj_gather_fan<T, CREASED>(a, c, t, d, f, v);
     }
 };
-
-// (the level tables go to LDS first: owners_block's first barrier orders
-// the copy before any read)
-template <typename T, int CREASED, class LP>
-__device__ __forceinline__ void adj_weights_block(const AdjArgs<T> &a, const DCage &c, const LevelTab &t, const LP &lp,
-                                                  int d, uint32_t lb) {
-    __shared__ LevelTab st;
-    __shared__ LevelPtrs slp;
-    lds_fill(st, t);
-    lds_fill(slp, lp);
-    owners_block<kAdjScan>(a.he, a.H, d, c, st, lb, RingWeightsFn<T, CREASED>{a, c, st, slp, d});
-}
-
-template <typename T, int CREASED, class LP>
-__device__ __forceinline__ void adj_vertices_block(const AdjArgs<T> &a, const DCage &c, const LevelTab &t,
-                                                   const LP &lp, int d, uint32_t lb) {
-    __shared__ LevelTab st;
-    __shared__ LevelPtrs slp;
-    lds_fill(st, t);
-    lds_fill(slp, lp);
-    owners_block<kAdjScan>(a.he, a.H, d, c, st, lb, VertexGatherFn<T, CREASED>{a, c, st, slp, d});
-}
 
 // ---- the cage step (level 0 -> 1) -----------------------------------------------
 // vertex rule, one thread per cage halfedge with the reference's own walk
@@ -419,7 +479,7 @@ __device__ __forceinline__ void adj_cage_faces_at(const AdjArgs<T> &a, const DCa
 // ---- kernels: one thread per halfedge ------------------------------------------
 template <typename T, int CREASED>
 __global__ __launch_bounds__(256) void k_adj_weights(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
-    adj_weights_block<T, CREASED>(a, c, t, lp, d, blockIdx.x);
+    owners_block<kAdjScan>(a.H, d, c, t, lp, blockIdx.x, RingWeightsFn<T, CREASED>{a, c, d});
 }
 
 template <typename T>
@@ -434,7 +494,12 @@ __global__ __launch_bounds__(256) void k_adj_faces(AdjArgs<T> a) {
 
 template <typename T, int CREASED>
 __global__ __launch_bounds__(256) void k_adj_vertices(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
-    adj_vertices_block<T, CREASED>(a, c, t, lp, d, blockIdx.x);
+    owners_block<kAdjScan>(a.H, d, c, t, lp, blockIdx.x, VertexGatherFn<T, CREASED>{a, c, d});
+}
+
+template <typename T, int CREASED, int GATHER_PASS>
+__global__ __launch_bounds__(256) void k_adj_per_vertex(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
+    adj_vertex_pass<T, CREASED, GATHER_PASS>(a, c, t, lp, d, (int32_t)(blockIdx.x * 256 + threadIdx.x));
 }
 
 template <typename T, int CREASED>
@@ -521,23 +586,29 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
         a.cr = pl.cr + n.offC[d];
         a.H = (int32_t)n.H[d];
         a.C = (int32_t)n.C[d];
+        a.V = (int32_t)n.V[d];
         a.sd = (sharp && creased) ? sharp + pl.cage.E + n.offC[d] : nullptr;
         a.gV = G;
         a.gF = G + 3 * n.V[d];
         a.gE = G + 3 * (n.V[d] + n.F[d]);
         a.gP = P;
         a.W = W;
-        const dim3 g(grid_for(n.H[d])), gq(grid_for(n.H[d] / 4)), b(256);
+        const dim3 g(grid_for(n.H[d])), gq(grid_for(n.H[d] / 4)), gv(grid_for(n.V[d])), b(256);
+        const bool pv = n.V[d] <= pl.per_vertex_max;
         if (creased) {
-            hipLaunchKernelGGL((k_adj_weights<T, 1>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 1, 0>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            else hipLaunchKernelGGL((k_adj_weights<T, 1>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             hipLaunchKernelGGL(k_adj_edges<T>, g, b, 0, st, a);
             hipLaunchKernelGGL((k_adj_faces<T, 1>), gq, b, 0, st, a);
-            hipLaunchKernelGGL((k_adj_vertices<T, 1>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 1, 1>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            else hipLaunchKernelGGL((k_adj_vertices<T, 1>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
         } else {
-            hipLaunchKernelGGL((k_adj_weights<T, 0>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 0, 0>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            else hipLaunchKernelGGL((k_adj_weights<T, 0>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             hipLaunchKernelGGL(k_adj_edges<T>, g, b, 0, st, a);
             hipLaunchKernelGGL((k_adj_faces<T, 0>), gq, b, 0, st, a);
-            hipLaunchKernelGGL((k_adj_vertices<T, 0>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 0, 1>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            else hipLaunchKernelGGL((k_adj_vertices<T, 0>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
         }
         T *tmp = G; G = P; P = tmp;
     }
@@ -546,6 +617,7 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
     a.cr = nullptr;
     a.H = pl.cage.H;
     a.C = pl.cage.E;
+    a.V = pl.cage.V;
     a.sd = nullptr;
     a.gV = G;
     a.gF = G + 3 * (int64_t)pl.cage.V;
@@ -577,7 +649,7 @@ struct OpAdjWeights {
         DCage c;
         unpack_adj(J, arena, adj, a, c);
         const JobLevelPtrs lp{J.lp, arena};
-        adj_weights_block<float, CREASED>(a, c, J.t, lp, d, (uint32_t)(i >> 8));
+        owners_block<kAdjScan>(a.H, d, c, J.t, lp, (uint32_t)(i >> 8), RingWeightsFn<float, CREASED>{a, c, d});
     }
 };
 struct OpAdjEdges {
@@ -608,7 +680,7 @@ struct OpAdjVertices {
         DCage c;
         unpack_adj(J, arena, adj, a, c);
         const JobLevelPtrs lp{J.lp, arena};
-        adj_vertices_block<float, CREASED>(a, c, J.t, lp, d, (uint32_t)(i >> 8));
+        owners_block<kAdjScan>(a.H, d, c, J.t, lp, (uint32_t)(i >> 8), VertexGatherFn<float, CREASED>{a, c, d});
     }
 };
 template <int CREASED>

@@ -57,7 +57,7 @@ struct AdjJob {
     int64_t he, cr, gV, gF, gE, gP, W;
     int64_t c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, c_vro, c_vr, c_vb, c_fro, c_fr;
     int64_t lp[kMaxDepth + 1];
-    int32_t H, C;
+    int32_t H, C, V;
     int32_t cV, cU, cH, cE, cF;
     LevelTab t;
 };
@@ -108,7 +108,7 @@ __host__ __device__ __forceinline__ void unpack_adj(const AdjJob &J, char *arena
     a.he = arena_ptr<const He>(arena, J.he);
     a.cr = arena_ptr<const Crease>(arena, J.cr);
     a.sd = nullptr;
-    a.H = J.H; a.C = J.C;
+    a.H = J.H; a.C = J.C; a.V = J.V;
     a.gV = arena_ptr<float>(adj, J.gV);
     a.gF = arena_ptr<float>(adj, J.gF);
     a.gE = arena_ptr<float>(adj, J.gE);
@@ -174,6 +174,9 @@ void batch_level_vertices(const BatchTable &t, char *arena, int d, int gather, i
                           hipStream_t st);
 
 // adjoint launches (cc_adjoint.hip)
+// one level's adjoint: weights, faces and vertex gathers over `quads` (one
+// thread per quad; the owner passes scan 4 halfedges per thread), edges one
+// thread per halfedge
 void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, char *arena, char *adj, int d, int creased,
                      hipStream_t st);
 void batch_adj_cage(const BatchTable &t, char *arena, char *adj, int creased, hipStream_t st);

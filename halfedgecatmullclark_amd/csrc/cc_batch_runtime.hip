@@ -410,12 +410,14 @@ bool ccb_Batch::build_tables() {
                     j.cr = -1;
                     j.H = g.H0;
                     j.C = g.E0;
+                    j.V = g.V0;
                     j.gP = aoff(adjCage + 3 * g.o_v0);
                 } else {
                     j.he = off(hb + g.n.offH[d]);
                     j.cr = off(cr + g.o_cr + g.n.offC[d]);
                     j.H = (int32_t)g.n.H[d];
                     j.C = (int32_t)g.n.C[d];
+                    j.V = (int32_t)g.n.V[d];
                     j.gP = aoff(buf(d));
                 }
                 j.W = d == 0 ? -1 : aoff(adjW + 4 * g.o_adjW);

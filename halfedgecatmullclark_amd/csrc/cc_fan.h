@@ -210,6 +210,11 @@ __device__ __forceinline__ int32_t fan_at(const Fan &f, const DCage &c, int i) {
 __device__ __forceinline__ void cage_vertex_fan(const DCage &c, int32_t v, int32_t p, Fan &f) {
     f.kind = 0;
     f.a0 = p;
+    if (p < 0) {   // a vertex without halfedges: an empty fan
+        f.n = 0;
+        f.a1 = 0;
+        return;
+    }
     if (c.vring) {
         f.a1 = c.vring_off[v];
         f.n = c.vring_off[v + 1] - f.a1;
@@ -318,6 +323,30 @@ __device__ __forceinline__ bool owner_fan(int32_t h, int32_t vert, int d, const 
         f.a0 = 4 * np + 3; f.a1 = 4 * p + 1; f.a2 = f.a3 = 0;
     }
     return true;
+}
+
+// Is level-d halfedge h the representative of its vertex?  owner_test
+// without h's record: a face vertex born at level >= 2 is owned by child 2
+// of its quad's first halfedge, an edge vertex by the child 1 whose parent
+// represents the edge (one parent record read), a cage-level vertex as the
+// cage maps say.
+__device__ __forceinline__ bool structural_owner(int32_t h, int d, const DCage &c, const LevelTab &t,
+                                                 const He *const *levels) {
+    int k = 0;
+    while (k < d - 1 && ((h >> (2 * k)) & 3) == 0) ++k;
+    const int32_t q = h >> (2 * k);
+    const int lvl = d - k;
+    const int digit = q & 3;
+    const int32_t p = q >> 2;
+    if (digit == 3) return false;
+    if (lvl >= 2) {
+        if (digit == 2) return (p & 3) == 0;
+        return is_edge_rep(c, t, reinterpret_cast<const int4 *>(levels[lvl - 1])[p].y, lvl - 1, p);
+    }
+    const CageHe x = c.he[p];
+    if (digit == 0) return c.v2h[x.vert] == p;
+    if (digit == 2) return c.f2h[x.face] == p;
+    return c.e2h[x.edge] == p;
 }
 
 // The fan and vertex ID of a level-d halfedge h known to be the
@@ -629,6 +658,59 @@ __device__ __forceinline__ FanPreloaded closed_fan(const FusedArgs &a, const int
     return FanPreloaded{e[0], e[1], e[2], e[3]};
 }
 
+// The ring of level-d vertex v with fan f when it is a face or interior edge
+// vertex born at level lvl >= 2: halfedges h[q] in fan order and their edges
+// e[q] (the previous edge of h[q] is e[q+1]); false for any other fan.
+__device__ __forceinline__ bool structural_ring(const Fan &f, const LevelTab &t, int d, int32_t v, int32_t (&h)[4],
+                                                int32_t (&e)[4]) {
+    const int lvl = d - (f.shift >> 1);   // the level the vertex was born at
+    if (!(lvl >= 2 && (f.kind == 1 || (f.kind == 3 && !f.boundary)))) return false;
+    // A face or interior edge vertex born at level lvl >= 2: the ring at
+    // level lvl are children of level-(lvl-1) halfedges, and the ring at
+    // level d their children 0, 00, ...  Edges and twins follow from the
+    // child numbering (CatmullClark.h:2770-2793, E the parent level's
+    // edge count): child 0 of h: edge 2 edge(h) + (h > twin(h) ? 0 : 1),
+    // twin 4 next(twin(h)) + 3; child 1: edge 2E + h, twin 4 next(h) + 2;
+    // child 2: edge 2E + prev(h), twin 4 prev(h) + 1; child 3: edge
+    // 2 edge(prev h) + (prev h > twin(prev h)), twin 4 twin(prev h).  So
+    // no ring record is read.  Face ring of quad 4f': children 2 of
+    // 4f'+q.  Edge ring of e = edge(p) = edge(w), w = twin(p): 4p+1,
+    // 4 next(w)+3, 4w+1, 4 next(p)+3.
+    const int32_t Ep2 = 2 * (int32_t)t.E[lvl - 1];
+    int32_t tw[4];
+    if (f.kind == 1) {
+        const int32_t p = (f.a0 - 2) >> 2;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int32_t pp = p + ((q + 3) & 3);
+            h[q] = 4 * (p + q) + 2;
+            e[q] = Ep2 + pp;
+            tw[q] = 4 * pp + 1;
+        }
+    } else {
+        const int32_t p = (f.a0 - 1) >> 2, w = (f.a2 - 1) >> 2;
+        const int32_t e2 = 2 * (v - (int32_t)t.V[lvl - 1] - (int32_t)t.F[lvl - 1]);
+        h[0] = f.a0; h[1] = f.a1; h[2] = f.a2; h[3] = f.a3;
+        e[0] = Ep2 + p;
+        e[1] = e2 + (w > p ? 1 : 0);
+        e[2] = Ep2 + w;
+        e[3] = e2 + (p > w ? 1 : 0);
+        tw[0] = 4 * qnext(p) + 2;
+        tw[1] = 4 * p;
+        tw[2] = 4 * qnext(w) + 2;
+        tw[3] = 4 * w;
+    }
+    for (int j = lvl; j < d; ++j) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            e[q] = 2 * e[q] + (h[q] > tw[q] ? 0 : 1);
+            tw[q] = 4 * qnext(tw[q]) + 3;
+            h[q] = 4 * h[q];
+        }
+    }
+    return true;
+}
+
 // vertex point of vertex v of level d with fan f (kinds 1 and 3 load their
 // at most four entries up front, cage fans are walked)
 template <int GATHER, int CREASED>
@@ -636,51 +718,8 @@ __device__ __forceinline__ void fan_vertex_store(const FusedArgs &a, const DCage
                                                  const Fan &f, int32_t v) {
     const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
     const P3 old = a.old[v];
-    const int lvl = d - (f.shift >> 1);   // the level the vertex was born at
-    if (lvl >= 2 && (f.kind == 1 || (f.kind == 3 && !f.boundary))) {
-        // A face or interior edge vertex born at level lvl >= 2: the ring at
-        // level lvl are children of level-(lvl-1) halfedges, and the ring at
-        // level d their children 0, 00, ...  Edges and twins follow from the
-        // child numbering (CatmullClark.h:2770-2793, E the parent level's
-        // edge count): child 0 of h: edge 2 edge(h) + (h > twin(h) ? 0 : 1),
-        // twin 4 next(twin(h)) + 3; child 1: edge 2E + h, twin 4 next(h) + 2;
-        // child 2: edge 2E + prev(h), twin 4 prev(h) + 1; child 3: edge
-        // 2 edge(prev h) + (prev h > twin(prev h)), twin 4 twin(prev h).  So
-        // no ring record is read.  Face ring of quad 4f': children 2 of
-        // 4f'+q.  Edge ring of e = edge(p) = edge(w), w = twin(p): 4p+1,
-        // 4 next(w)+3, 4w+1, 4 next(p)+3.
-        const int32_t Ep2 = 2 * (int32_t)t.E[lvl - 1];
-        int32_t h[4], e[4], tw[4];
-        if (f.kind == 1) {
-            const int32_t p = (f.a0 - 2) >> 2;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int32_t pp = p + ((q + 3) & 3);
-                h[q] = 4 * (p + q) + 2;
-                e[q] = Ep2 + pp;
-                tw[q] = 4 * pp + 1;
-            }
-        } else {
-            const int32_t p = (f.a0 - 1) >> 2, w = (f.a2 - 1) >> 2;
-            const int32_t e2 = 2 * (v - (int32_t)t.V[lvl - 1] - (int32_t)t.F[lvl - 1]);
-            h[0] = f.a0; h[1] = f.a1; h[2] = f.a2; h[3] = f.a3;
-            e[0] = Ep2 + p;
-            e[1] = e2 + (w > p ? 1 : 0);
-            e[2] = Ep2 + w;
-            e[3] = e2 + (p > w ? 1 : 0);
-            tw[0] = 4 * qnext(p) + 2;
-            tw[1] = 4 * p;
-            tw[2] = 4 * qnext(w) + 2;
-            tw[3] = 4 * w;
-        }
-        for (int j = lvl; j < d; ++j) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                e[q] = 2 * e[q] + (h[q] > tw[q] ? 0 : 1);
-                tw[q] = 4 * qnext(tw[q]) + 3;
-                h[q] = 4 * h[q];
-            }
-        }
+    int32_t h[4], e[4];
+    if (structural_ring(f, t, d, v, h, e)) {
         const FanPreloaded pre = closed_fan(a, h, e);
         a.vp[v] = fan_vertex_point<GATHER, CREASED, 4>(f, pre, old);
     } else if (f.kind == 1 || (f.kind == 3 && !f.boundary)) {

@@ -45,6 +45,7 @@ struct ccd_Subd {
     size_t adj_scalar = 0;
     bool timing = false;
     int opt_fused = 1, opt_store = cch::kStoreLds, opt_remap = 1, opt_fan = 2;
+    int32_t opt_adj_vertex = 1 << 20;   // CCD_OPT_ADJ_VERTEX
     // per-level kernels of the fused paths (options or ccd_Autotune): kind 1
     // fused children + face/edge points, then vertex points (scan 0: the
     // representatives the fused kernel marked; 2, 4, 8: halfedges each thread

@@ -391,6 +391,7 @@ bool ccd_Subd::adjoint(const void *gradFinest, void *gradCage, int32_t steps, hi
     pl.cage = cage_view(nullptr);
     pl.tab = tab;
     pl.lp = level_ptrs();
+    pl.per_vertex_max = opt_adj_vertex;
     const int creased = (steps & CCD_NOCREASES) ? 0 : 1;
     if (scalar == sizeof(double)) {
         // the double adjoint refines sharpness in double too, as the double
@@ -766,6 +767,10 @@ int32_t ccd_SetOption(ccd_Subd *s, int32_t key, int32_t value) {
     case CCD_OPT_SCAN:
         if (value != 0 && value != 2 && value != 4 && value != 8) return 14;
         for (auto &c : s->cfg) c.scan = value;
+        return 0;
+    case CCD_OPT_ADJ_VERTEX:
+        if (value < 0) return 14;
+        s->opt_adj_vertex = value;
         return 0;
     default: return 14;
     }

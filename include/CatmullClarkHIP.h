@@ -286,10 +286,14 @@ int32_t ccd_Synchronize(ccd_Subd *subd);
  *   CCD_OPT_SCAN       vertex kernel after the fused kernel: 0 (default)
  *                      takes the vertex representatives the fused kernel
  *                      marked in a bit mask; 2, 4 or 8 scans that many
- *                      halfedges per thread for them */
+ *                      halfedges per thread for them
+ *   CCD_OPT_ADJ_VERTEX the adjoint's two vertex passes run one thread per
+ *                      vertex on levels with at most this many vertices
+ *                      (default 1048576), else over the vertex owners in
+ *                      halfedge order; both give the same bits */
 enum {
     CCD_OPT_FUSED = 0, CCD_OPT_STORE = 1, CCD_OPT_XCD_REMAP = 2, CCD_OPT_FAN = 3,
-    CCD_OPT_PATCH_TILE = 4, CCD_OPT_PATCH_HPT = 5, CCD_OPT_SCAN = 6
+    CCD_OPT_PATCH_TILE = 4, CCD_OPT_PATCH_HPT = 5, CCD_OPT_SCAN = 6, CCD_OPT_ADJ_VERTEX = 7
 };
 int32_t ccd_SetOption(ccd_Subd *subd, int32_t key, int32_t value);
 /* Picks, per level, the kernel configuration of the fused paths by timing

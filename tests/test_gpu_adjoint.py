@@ -91,3 +91,31 @@ def test_central_differences(capi):
             fd[i, k] = (loss(V0 + e) - loss(V0 - e)) / (2 * h)
     np.testing.assert_allclose(grad, fd, rtol=0, atol=1e-7 * np.abs(fd).max())
     s.close()
+
+
+@pytest.mark.parametrize("label,cage,depth", [("torus16x8_d5", C.torus(16, 8, seed=3), 5),
+                                               ("irregular3_d4", C.irregular(3, seed=2), 4),
+                                               ("cube_d4", C.cube(1), 4)])
+@pytest.mark.parametrize("steps", [0, 32], ids=["creased", "nocreases"])
+def test_vertex_pass_orders_agree(capi, label, cage, depth, steps):
+    """The adjoint's vertex passes over the owners in halfedge order and one
+    thread per vertex visit the same rings in the same order.  The cage step
+    adds with float atomics (order-dependent in the last bit), so the two
+    agree to 1e-6 of the largest gradient (a wrong ring would differ at
+    order one)."""
+    import torch
+    ds = capi.DeviceSubd(cage, depth)
+    ds.refine(capi.ALL)
+    n = ds.download()["counts"]
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    g = torch.randn((n["V"][depth], 3), generator=gen, device="cuda", dtype=torch.float32)
+    outs = []
+    for per_vertex_max in (0, 1 << 30):
+        ds.set_option(capi.OPT_ADJ_VERTEX, per_vertex_max)
+        out = torch.empty((cage.V, 3), device="cuda", dtype=torch.float32)
+        ds.refine_points_adjoint(g.data_ptr(), out.data_ptr(), steps)
+        ds.synchronize()
+        outs.append(out.cpu().numpy())
+    scale = float(np.abs(outs[1]).max())
+    assert float(np.abs(outs[0] - outs[1]).max()) <= 1e-6 * scale, label
+    ds.close()
