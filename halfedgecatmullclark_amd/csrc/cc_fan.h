@@ -658,15 +658,17 @@ __device__ __forceinline__ FanPreloaded closed_fan(const FusedArgs &a, const int
     return FanPreloaded{e[0], e[1], e[2], e[3]};
 }
 
-// The ring of level-d vertex v with fan f when it is a face or interior edge
-// vertex born at level lvl >= 2: halfedges h[q] in fan order and their edges
-// e[q] (the previous edge of h[q] is e[q+1]); false for any other fan.
+// The ring of level-d vertex v with fan f when it is a face vertex born at
+// level lvl >= 2 or an interior edge vertex (born at any level):
+// halfedges h[q] in fan order and their edges e[q] (the previous edge of
+// h[q] is e[q+1]); false for any other fan.
 __device__ __forceinline__ bool structural_ring(const Fan &f, const LevelTab &t, int d, int32_t v, int32_t (&h)[4],
                                                 int32_t (&e)[4]) {
     const int lvl = d - (f.shift >> 1);   // the level the vertex was born at
-    if (!(lvl >= 2 && (f.kind == 1 || (f.kind == 3 && !f.boundary)))) return false;
-    // A face or interior edge vertex born at level lvl >= 2: the ring at
-    // level lvl are children of level-(lvl-1) halfedges, and the ring at
+    if (!((f.kind == 1 && lvl >= 2) || (f.kind == 3 && !f.boundary))) return false;
+    // The ring at level lvl are children of level-(lvl-1) halfedges (the
+    // cage's for lvl = 1: the cage step numbers children the same way,
+    // CatmullClark.h:2693-2740), and the ring at
     // level d their children 0, 00, ...  Edges and twins follow from the
     // child numbering (CatmullClark.h:2770-2793, E the parent level's
     // edge count): child 0 of h: edge 2 edge(h) + (h > twin(h) ? 0 : 1),
@@ -675,7 +677,8 @@ __device__ __forceinline__ bool structural_ring(const Fan &f, const LevelTab &t,
     // 2 edge(prev h) + (prev h > twin(prev h)), twin 4 twin(prev h).  So
     // no ring record is read.  Face ring of quad 4f': children 2 of
     // 4f'+q.  Edge ring of e = edge(p) = edge(w), w = twin(p): 4p+1,
-    // 4 next(w)+3, 4w+1, 4 next(p)+3.
+    // 4 next(w)+3, 4w+1, 4 next(p)+3, with next from the fan (the cage's
+    // next for lvl = 1).
     const int32_t Ep2 = 2 * (int32_t)t.E[lvl - 1];
     int32_t tw[4];
     if (f.kind == 1) {
@@ -689,15 +692,16 @@ __device__ __forceinline__ bool structural_ring(const Fan &f, const LevelTab &t,
         }
     } else {
         const int32_t p = (f.a0 - 1) >> 2, w = (f.a2 - 1) >> 2;
+        const int32_t nw = (f.a1 - 3) >> 2, np = (f.a3 - 3) >> 2;
         const int32_t e2 = 2 * (v - (int32_t)t.V[lvl - 1] - (int32_t)t.F[lvl - 1]);
         h[0] = f.a0; h[1] = f.a1; h[2] = f.a2; h[3] = f.a3;
         e[0] = Ep2 + p;
         e[1] = e2 + (w > p ? 1 : 0);
         e[2] = Ep2 + w;
         e[3] = e2 + (p > w ? 1 : 0);
-        tw[0] = 4 * qnext(p) + 2;
+        tw[0] = 4 * np + 2;
         tw[1] = 4 * p;
-        tw[2] = 4 * qnext(w) + 2;
+        tw[2] = 4 * nw + 2;
         tw[3] = 4 * w;
     }
     for (int j = lvl; j < d; ++j) {
