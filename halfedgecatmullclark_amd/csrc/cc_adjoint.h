@@ -32,7 +32,7 @@ struct AdjPlan {
 };
 
 // dL/d(cage points) from dL/d(level-D points): V_D x 3 in, V_0 x 3 out.
-// bufA holds V_D x 3 and bufB V_{D-1} x 3 scalars of scratch, W 4 H_{D-1}.
+// bufA holds V_D x 3 and bufB V_{D-1} x 3 scalars of scratch, W 4 max(H_{D-1}, H_0).
 // When sharp is given (E_0 + Ccum(D-1) scalars) the sharpness of every level
 // is refined again in T and used instead of the float crease records.
 template <typename T>

@@ -476,6 +476,172 @@ __device__ __forceinline__ void adj_cage_faces_at(const AdjArgs<T> &a, const DCa
     atomic_add3(reinterpret_cast<V3<T> *>(a.gP) + x.vert, (T)1 / (T)m, gf);
 }
 
+// ---- the cage step without atomics -------------------------------------------------
+// The same transposed rules as above, in four gathers with a fixed order of
+// additions (so the adjoint is reproducible bit for bit): (1) per halfedge
+// the vertex rule's weights (to its vertex, edge, face and previous edge);
+// (2) per edge, its two halfedges' shares; (3) per face, the vertex and edge
+// rules' shares; (4) per vertex, over its outgoing halfedges in ascending
+// order, the vertex, edge and face rules' shares.  Needs the cage's lists.
+
+// the edge rule's weights of cage halfedge x (c0: its vertex, c1: the next
+// vertex, cf: its face)
+template <typename T, int CREASED>
+__device__ __forceinline__ void cage_edge_weights(const DCage &c, const CageHe &x, T &c0, T &c1, T &cf) {
+    const bool boundary = x.twin < 0;
+    if (CREASED) {
+        const T w = satT((T)c.cr[x.edge].sharp);
+        const T k = boundary ? (T)1 : (T)0.5;
+        c0 = (T)0.25 * ((T)1 - w) + (T)0.5 * k * w;
+        c1 = (T)0.5 * k * w;
+        cf = (T)0.25 * ((T)1 - w);
+    } else {
+        const T w = boundary ? (T)1 : (T)0.5;
+        c0 = c1 = (T)0.5 * w * w;
+        cf = (T)0.5 * ((T)1 - w);
+    }
+}
+
+// (1) W[h] = (to its vertex, to its edge, to its face, to its previous edge);
+// 0 marks a share the rule does not add
+template <typename T, int CREASED>
+__device__ __forceinline__ void adj_cage_weights_at(const AdjArgs<T> &a, const DCage &c, int32_t h) {
+    if (h >= c.H) return;
+    const CageHe x = c.he[h];
+    W4<T> w{0, 0, 0, 0};
+    if (!CREASED) {
+        int32_t n = 1, fw, bw;
+        for (fw = c.he[x.prev].twin; fw >= 0 && fw != h && n < kWalkCap; fw = c.he[c.he[fw].prev].twin) ++n;
+        for (bw = x.twin >= 0 ? c.he[x.twin].next : -1; fw < 0 && bw >= 0 && bw != h && n < kWalkCap;
+             bw = c.he[bw].twin >= 0 ? c.he[c.he[bw].twin].next : -1)
+            ++n;
+        const T iw = (T)1 / (T)n, s = fw < 0 ? (T)0 : (T)1;
+        w.a = iw - (T)3 * iw * iw * s;
+        if (s != (T)0) {
+            w.b = (T)4 * iw * iw * s;
+            w.g = -iw * iw * s;
+        }
+    } else {
+        const int32_t p = x.prev;
+        const T ts = (T)c.cr[x.edge].sharp, ps = (T)c.cr[c.he[p].edge].sharp;
+        T avgS = ps, cc = sgnT(ps), n = 1;
+        int32_t fw, bw;
+        for (fw = c.he[p].twin; fw >= 0 && fw != h && n < (T)kWalkCap; fw = c.he[fw].twin) {
+            const int32_t q = c.he[fw].prev;
+            const T qs = (T)c.cr[c.he[q].edge].sharp;
+            n = n + (T)1;
+            avgS = avgS + qs;
+            cc = cc + sgnT(qs);
+            fw = q;
+        }
+        for (bw = x.twin; fw < 0 && bw >= 0 && bw != h && n < (T)kWalkCap; bw = c.he[bw].twin) {
+            const int32_t q = c.he[bw].next;
+            const T qs = (T)c.cr[c.he[q].edge].sharp;
+            n = n + (T)1;
+            avgS = avgS + qs;
+            cc = cc + sgnT(qs);
+            bw = q;
+        }
+        const bool boundary = fw < 0;
+        const T nn = n;
+        const T ccAll = boundary ? cc + sgnT(ts) : cc;
+        const T nPost = boundary ? nn + (T)1 : nn;
+        if (ccAll <= (T)1) {
+            w.a = (nn - (T)3) / (nn * nn);
+            w.b = (T)4 / (nn * nn);
+            w.g = (T)-1 / (nn * nn);
+        } else if (ccAll >= (T)3 || nPost == (T)2) {
+            w.a = (T)1 / nn;
+        } else {
+            const T tt = satT(avgS * (T)0.5);
+            const T cw = sgnT(ts), pcw = boundary ? sgnT(ps) : (T)0;
+            w.a = ((T)1 - tt) / nn + tt * (T)0.25 * (cw + pcw);
+            if (cw != (T)0) w.b = tt * (T)0.25 * cw;
+            if (pcw != (T)0) w.dl = tt * (T)0.25 * pcw;
+        }
+    }
+    reinterpret_cast<W4<T> *>(a.W)[h] = w;
+}
+
+template <typename T>
+__device__ __forceinline__ void add_scaled(V3<T> &acc, T w, const V3<T> &g) {
+    if (w == (T)0) return;
+    acc.x += w * g.x; acc.y += w * g.y; acc.z += w * g.z;
+}
+
+// (2) edge e: the vertex rule's shares of its halfedges (ascending) and of
+// the halfedges whose previous edge it is
+template <typename T>
+__device__ __forceinline__ void adj_cage_edge_gather(const AdjArgs<T> &a, const DCage &c, int32_t e) {
+    if (e >= c.E) return;
+    const W4<T> *W = reinterpret_cast<const W4<T> *>(a.W);
+    const V3<T> *gV = reinterpret_cast<const V3<T> *>(a.gV);
+    V3<T> *gE = reinterpret_cast<V3<T> *>(a.gE);
+    const int32_t h0 = c.e2h[e], t0 = c.he[h0].twin;
+    int32_t hs[2] = {h0, t0};
+    int k = 1;
+    if (t0 >= 0) {
+        k = 2;
+        if (t0 < h0) { hs[0] = t0; hs[1] = h0; }
+    }
+    V3<T> acc = gE[e];
+    for (int i = 0; i < k; ++i) add_scaled(acc, W[hs[i]].b, gV[c.he[hs[i]].vert]);
+    for (int i = 0; i < k; ++i) {
+        const int32_t nx = c.he[hs[i]].next;
+        add_scaled(acc, W[nx].dl, gV[c.he[nx].vert]);
+    }
+    gE[e] = acc;
+}
+
+// (3) face f: the vertex rule's shares of its halfedges, then the edge
+// rule's face shares (in its halfedge order from f2h)
+template <typename T, int CREASED>
+__device__ __forceinline__ void adj_cage_face_gather(const AdjArgs<T> &a, const DCage &c, int32_t f) {
+    if (f >= c.F) return;
+    const W4<T> *W = reinterpret_cast<const W4<T> *>(a.W);
+    const V3<T> *gV = reinterpret_cast<const V3<T> *>(a.gV);
+    const V3<T> *gE = reinterpret_cast<const V3<T> *>(a.gE);
+    V3<T> *gF = reinterpret_cast<V3<T> *>(a.gF);
+    const int32_t o = c.fring_off[f], m = c.fring_off[f + 1] - o;
+    V3<T> acc = gF[f];
+    for (int i = 0; i < m; ++i) {
+        const int32_t h = c.fring[o + i];
+        add_scaled(acc, W[h].g, gV[c.he[h].vert]);
+    }
+    for (int i = 0; i < m; ++i) {
+        const CageHe x = c.he[c.fring[o + i]];
+        T c0, c1, cf;
+        cage_edge_weights<T, CREASED>(c, x, c0, c1, cf);
+        add_scaled(acc, cf, gE[x.edge]);
+    }
+    gF[f] = acc;
+}
+
+// (4) vertex v: over its outgoing halfedges h, ascending: the vertex rule's
+// own share, the edge rule's c0 of h and c1 of prev(h), and 1/m of h's face
+template <typename T, int CREASED>
+__device__ __forceinline__ void adj_cage_vertex_gather(const AdjArgs<T> &a, const DCage &c, int32_t v) {
+    if (v >= c.V) return;
+    const W4<T> *W = reinterpret_cast<const W4<T> *>(a.W);
+    const V3<T> *gE = reinterpret_cast<const V3<T> *>(a.gE);
+    const V3<T> *gF = reinterpret_cast<const V3<T> *>(a.gF);
+    const V3<T> gv = reinterpret_cast<const V3<T> *>(a.gV)[v];
+    V3<T> acc{0, 0, 0};
+    for (int32_t i = c.vout_off[v]; i < c.vout_off[v + 1]; ++i) {
+        const int32_t h = c.vout[i];
+        const CageHe x = c.he[h], xp = c.he[x.prev];
+        T c0, c1, cf, d0, d1, df;
+        cage_edge_weights<T, CREASED>(c, x, c0, c1, cf);
+        cage_edge_weights<T, CREASED>(c, xp, d0, d1, df);
+        add_scaled(acc, W[h].a, gv);
+        add_scaled(acc, c0, gE[x.edge]);
+        add_scaled(acc, d1, gE[xp.edge]);
+        const int32_t m = c.fring_off[x.face + 1] - c.fring_off[x.face];
+        add_scaled(acc, (T)1 / (T)m, gF[x.face]);
+    }
+    reinterpret_cast<V3<T> *>(a.gP)[v] = acc;
+}
+
 // ---- kernels: one thread per halfedge ------------------------------------------
 template <typename T, int CREASED>
 __global__ __launch_bounds__(256) void k_adj_weights(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
@@ -515,6 +681,26 @@ __global__ __launch_bounds__(256) void k_adj_cage_edges(AdjArgs<T> a, DCage c) {
 template <typename T>
 __global__ __launch_bounds__(256) void k_adj_cage_faces(AdjArgs<T> a, DCage c) {
     adj_cage_faces_at<T>(a, c, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+template <typename T, int CREASED>
+__global__ __launch_bounds__(256) void k_adj_cage_weights(AdjArgs<T> a, DCage c) {
+    adj_cage_weights_at<T, CREASED>(a, c, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_cage_edge_gather(AdjArgs<T> a, DCage c) {
+    adj_cage_edge_gather<T>(a, c, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+template <typename T, int CREASED>
+__global__ __launch_bounds__(256) void k_adj_cage_face_gather(AdjArgs<T> a, DCage c) {
+    adj_cage_face_gather<T, CREASED>(a, c, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+template <typename T, int CREASED>
+__global__ __launch_bounds__(256) void k_adj_cage_vertex_gather(AdjArgs<T> a, DCage c) {
+    adj_cage_vertex_gather<T, CREASED>(a, c, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 // crease sharpness of the next level in T (CatmullClark.h:2962-2980), for
@@ -623,10 +809,26 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
     a.gF = G + 3 * (int64_t)pl.cage.V;
     a.gE = G + 3 * ((int64_t)pl.cage.V + pl.cage.F);
     a.gP = gradCage;
+    a.W = W;
+    const dim3 g(grid_for(pl.cage.H)), b(256);
+    if (pl.cage.vout) {   // gathers in a fixed order
+        const dim3 ge(grid_for(pl.cage.E)), gf(grid_for(pl.cage.F)), gv(grid_for(pl.cage.V));
+        if (creased) {
+            hipLaunchKernelGGL((k_adj_cage_weights<T, 1>), g, b, 0, st, a, pl.cage);
+            hipLaunchKernelGGL(k_adj_cage_edge_gather<T>, ge, b, 0, st, a, pl.cage);
+            hipLaunchKernelGGL((k_adj_cage_face_gather<T, 1>), gf, b, 0, st, a, pl.cage);
+            hipLaunchKernelGGL((k_adj_cage_vertex_gather<T, 1>), gv, b, 0, st, a, pl.cage);
+        } else {
+            hipLaunchKernelGGL((k_adj_cage_weights<T, 0>), g, b, 0, st, a, pl.cage);
+            hipLaunchKernelGGL(k_adj_cage_edge_gather<T>, ge, b, 0, st, a, pl.cage);
+            hipLaunchKernelGGL((k_adj_cage_face_gather<T, 0>), gf, b, 0, st, a, pl.cage);
+            hipLaunchKernelGGL((k_adj_cage_vertex_gather<T, 0>), gv, b, 0, st, a, pl.cage);
+        }
+        return;
+    }
     a.W = nullptr;
     hipLaunchKernelGGL(k_fill0<T>, dim3(grid_stride(3 * (int64_t)pl.cage.V)), dim3(256), 0, st, gradCage,
                        3 * (int64_t)pl.cage.V);
-    const dim3 g(grid_for(pl.cage.H)), b(256);
     if (creased) {
         hipLaunchKernelGGL((k_adj_cage_vertices<T, 1>), g, b, 0, st, a, pl.cage);
         hipLaunchKernelGGL((k_adj_cage_edges<T, 1>), g, b, 0, st, a, pl.cage);
@@ -712,6 +914,21 @@ struct OpAdjCageFaces {
         adj_cage_faces_at<float>(a, c, (int32_t)i);
     }
 };
+// the cage step as gathers; `pass` 0 weights (per halfedge), 1 edges,
+// 2 faces, 3 vertices (each thread index below the element count)
+template <int CREASED, int PASS>
+struct OpAdjCageGather {
+    char *arena, *adj;
+    __device__ void operator()(const AdjJob &J, int64_t i) const {
+        AdjArgs<float> a;
+        DCage c;
+        unpack_adj(J, arena, adj, a, c);
+        if (PASS == 0) adj_cage_weights_at<float, CREASED>(a, c, (int32_t)i);
+        else if (PASS == 1) adj_cage_edge_gather<float>(a, c, (int32_t)i);
+        else if (PASS == 2) adj_cage_face_gather<float, CREASED>(a, c, (int32_t)i);
+        else adj_cage_vertex_gather<float, CREASED>(a, c, (int32_t)i);
+    }
+};
 
 }  // namespace
 
@@ -728,6 +945,21 @@ void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, char 
     } else {
         launch_batch<AdjJob>(quads, OpAdjFaces<0>{arena, adj}, st);
         launch_batch<AdjJob>(quads, OpAdjVertices<0>{d, arena, adj}, st);
+    }
+}
+
+void batch_adj_cage_gather(const BatchTable &halfedges, const BatchTable &edges, const BatchTable &faces,
+                           const BatchTable &vertices, char *arena, char *adj, int creased, hipStream_t st) {
+    if (creased) {
+        launch_batch<AdjJob>(halfedges, OpAdjCageGather<1, 0>{arena, adj}, st);
+        launch_batch<AdjJob>(edges, OpAdjCageGather<1, 1>{arena, adj}, st);
+        launch_batch<AdjJob>(faces, OpAdjCageGather<1, 2>{arena, adj}, st);
+        launch_batch<AdjJob>(vertices, OpAdjCageGather<1, 3>{arena, adj}, st);
+    } else {
+        launch_batch<AdjJob>(halfedges, OpAdjCageGather<0, 0>{arena, adj}, st);
+        launch_batch<AdjJob>(edges, OpAdjCageGather<0, 1>{arena, adj}, st);
+        launch_batch<AdjJob>(faces, OpAdjCageGather<0, 2>{arena, adj}, st);
+        launch_batch<AdjJob>(vertices, OpAdjCageGather<0, 3>{arena, adj}, st);
     }
 }
 

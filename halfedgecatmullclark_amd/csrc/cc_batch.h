@@ -36,7 +36,7 @@ struct CreaseJob {
 // global rather than flat instructions); -1 marks an absent buffer.
 struct LevelJob {
     int64_t he, out_he, cr, old, vp, fp, ep, defer_mask, owner_mask;
-    int64_t c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, c_vro, c_vr, c_vb, c_fro, c_fr;
+    int64_t c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, c_vro, c_vr, c_vb, c_fro, c_fr, c_voo, c_vo;
     int64_t lp[kMaxDepth + 1];
     int32_t H, V, F, E, C;
     int32_t cV, cU, cH, cE, cF;
@@ -55,7 +55,7 @@ struct SpanJob {
 // adjoint arena.
 struct AdjJob {
     int64_t he, cr, gV, gF, gE, gP, W;
-    int64_t c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, c_vro, c_vr, c_vb, c_fro, c_fr;
+    int64_t c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, c_vro, c_vr, c_vb, c_fro, c_fr, c_voo, c_vo;
     int64_t lp[kMaxDepth + 1];
     int32_t H, C, V;
     int32_t cV, cU, cH, cE, cF;
@@ -82,6 +82,8 @@ __host__ __device__ __forceinline__ DCage arena_cage(char *base, const J &j) {
     c.vring_bnd = arena_ptr<const int32_t>(base, j.c_vb);
     c.fring_off = arena_ptr<const int32_t>(base, j.c_fro);
     c.fring = arena_ptr<const int32_t>(base, j.c_fr);
+    c.vout_off = arena_ptr<const int32_t>(base, j.c_voo);
+    c.vout = arena_ptr<const int32_t>(base, j.c_vo);
     c.V = j.cV; c.U = j.cU; c.H = j.cH; c.E = j.cE; c.F = j.cF;
     return c;
 }
@@ -180,5 +182,9 @@ void batch_level_vertices(const BatchTable &t, char *arena, int d, int gather, i
 void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, char *arena, char *adj, int d, int creased,
                      hipStream_t st);
 void batch_adj_cage(const BatchTable &t, char *arena, char *adj, int creased, hipStream_t st);
+// the cage step as ordered gathers (every cage has its ring lists): one
+// thread per cage halfedge, edge, face and vertex in turn
+void batch_adj_cage_gather(const BatchTable &halfedges, const BatchTable &edges, const BatchTable &faces,
+                           const BatchTable &vertices, char *arena, char *adj, int creased, hipStream_t st);
 
 }  // namespace cch

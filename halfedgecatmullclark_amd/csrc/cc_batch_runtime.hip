@@ -28,7 +28,7 @@ struct ccb_Batch {
         // element offsets: cage arrays, then levels 1..D, then per-cage scratch
         int64_t o_he0 = 0, o_v0 = 0, o_u0 = 0, o_e0 = 0, o_f0 = 0;
         // element offsets of the cage's ring lists in `rings` (-1: walk the cage)
-        int64_t o_vro = -1, o_vr = -1, o_vb = -1, o_fro = -1, o_fr = -1;
+        int64_t o_vro = -1, o_vr = -1, o_vb = -1, o_fro = -1, o_fr = -1, o_voo = -1, o_vo = -1;
         int64_t o_he = 0, o_cr = 0, o_pts = 0, o_fin = 0, o_def = 0, o_adjB = 0, o_adjW = 0;
     };
     struct Timed {
@@ -65,6 +65,8 @@ struct ccb_Batch {
     BatchTable t_cage_he, t_cage_f, t_cage_e, t_cage_v, t_cage_h3, t_cage_zero, t_fin;
     BatchTable t_cr[kMaxDepth + 1], t_patch[kMaxDepth + 1], t_def[kMaxDepth + 1], t_adj[kMaxDepth + 1];
     BatchTable t_fused[kMaxDepth + 1], t_vert[kMaxDepth + 1], t_adjq[kMaxDepth + 1];
+    BatchTable t_adjc_e, t_adjc_f, t_adjc_v;   // cage-step adjoint gathers: per edge, face, vertex
+    bool all_rings = true;                     // every cage has its ring lists
     // per-level kernels: kind 1 fused + compacted vertex points (scan
     // halfedges tested per thread), kind 2 patch-local (tiles of nt, hpt
     // tiles per workgroup) + deferred vertices
@@ -166,6 +168,7 @@ DCage ccb_Batch::cage_view(int i) const {
     auto ring = [&](int64_t o) -> const int32_t * { return o < 0 ? nullptr : rings + o; };
     c.vring_off = ring(g.o_vro); c.vring = ring(g.o_vr); c.vring_bnd = ring(g.o_vb);
     c.fring_off = ring(g.o_fro); c.fring = ring(g.o_fr);
+    c.vout_off = ring(g.o_voo); c.vout = ring(g.o_vo);
     c.V = g.V0; c.U = g.U0; c.H = g.H0; c.E = g.E0; c.F = g.F0;
     return c;
 }
@@ -186,6 +189,7 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
         g.o_e0 = tot_e0; tot_e0 += g.E0;
         g.o_f0 = tot_f0; tot_f0 += g.F0;
         const CageRings r = build_cage_rings(m);
+        all_rings = all_rings && r.ok;
         if (r.ok) {
             auto put = [&](const std::vector<int32_t> &v) {
                 const int64_t o = (int64_t)rings_host.size();
@@ -198,6 +202,8 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
             g.o_fro = put(r.foff);
             g.o_vr = put(r.vring);
             g.o_fr = put(r.fring);
+            g.o_voo = put(r.ooff);
+            g.o_vo = put(r.out);
         }
         g.o_he = tot_he; tot_he += g.n.cumH(g.D);
         g.o_cr = tot_cr; tot_cr += g.n.cumC(g.D);
@@ -205,7 +211,7 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
         g.o_fin = tot_fin; tot_fin += g.n.V[g.D];
         g.o_def = tot_def; tot_def += g.D >= 2 ? (g.n.H[g.D - 1] + 63) / 64 : 1;
         g.o_adjB = tot_adjB; tot_adjB += g.n.V[g.D - 1];
-        g.o_adjW = tot_adjW; tot_adjW += g.D >= 2 ? g.n.H[g.D - 1] : 1;
+        g.o_adjW = tot_adjW; tot_adjW += std::max<int64_t>(g.D >= 2 ? g.n.H[g.D - 1] : 1, g.H0);
         Dmax = std::max(Dmax, g.D);
     }
     CC_TRY(hipGetDevice(&device), "hipGetDevice");
@@ -304,6 +310,7 @@ bool ccb_Batch::build_tables() {
         auto roff = [&](int64_t o) -> int64_t { return o < 0 ? -1 : off(rings + o); };
         j.c_vro = roff(g.o_vro); j.c_vr = roff(g.o_vr); j.c_vb = roff(g.o_vb);
         j.c_fro = roff(g.o_fro); j.c_fr = roff(g.o_fr);
+        j.c_voo = roff(g.o_voo); j.c_vo = roff(g.o_vo);
         j.cV = g.V0; j.cU = g.U0; j.cH = g.H0; j.cE = g.E0; j.cF = g.F0;
     };
     {   // cage step
@@ -395,7 +402,7 @@ bool ccb_Batch::build_tables() {
     if (adj_arena) {   // adjoint steps d+1 -> d, d = 0 the cage step
         for (int d = 0; d < Dmax; ++d) {
             std::vector<AdjJob> jobs;
-            std::vector<int64_t> b, bq;
+            std::vector<int64_t> b, bq, be, bf, bv;
             for (int i = 0; i < N; ++i) {
                 const Cage &g = cages[i];
                 if (g.D <= d) continue;
@@ -420,7 +427,7 @@ bool ccb_Batch::build_tables() {
                     j.V = (int32_t)g.n.V[d];
                     j.gP = aoff(buf(d));
                 }
-                j.W = d == 0 ? -1 : aoff(adjW + 4 * g.o_adjW);
+                j.W = aoff(adjW + 4 * g.o_adjW);
                 j.gV = aoff(G);
                 j.gF = aoff(G + 3 * g.n.V[d]);
                 j.gE = aoff(G + 3 * (g.n.V[d] + g.n.F[d]));
@@ -430,9 +437,17 @@ bool ccb_Batch::build_tables() {
                 jobs.push_back(j);
                 b.push_back(blocks_of(j.H));
                 bq.push_back(blocks_of(j.H, 1024));
+                be.push_back(blocks_of(g.E0));
+                bf.push_back(blocks_of(g.F0));
+                bv.push_back(blocks_of(g.V0));
             }
             tb.add(t_adj[d], jobs, b);
             tb.add(t_adjq[d], jobs, bq);
+            if (d == 0) {
+                tb.add(t_adjc_e, jobs, be);
+                tb.add(t_adjc_f, jobs, bf);
+                tb.add(t_adjc_v, jobs, bv);
+            }
         }
     }
     if (tb.blob.size() > tables_cap) {
@@ -511,7 +526,12 @@ void ccb_Batch::record_adjoint(int creased, hipStream_t st) {
     for (int d = Dmax - 1; d >= 1; --d)
         timed_launch("adj_level", d, st,
                      [&] { batch_adj_level(t_adjq[d], t_adj[d], arena, adj_arena, d, creased, st); });
-    timed_launch("adj_cage", 0, st, [&] { batch_adj_cage(t_adj[0], arena, adj_arena, creased, st); });
+    if (all_rings)
+        timed_launch("adj_cage", 0, st, [&] {
+            batch_adj_cage_gather(t_adj[0], t_adjc_e, t_adjc_f, t_adjc_v, arena, adj_arena, creased, st);
+        });
+    else
+        timed_launch("adj_cage", 0, st, [&] { batch_adj_cage(t_adj[0], arena, adj_arena, creased, st); });
 }
 
 bool ccb_Batch::run(int32_t key, bool adjoint, int32_t steps, hipStream_t st) {

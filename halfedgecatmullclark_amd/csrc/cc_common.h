@@ -81,6 +81,9 @@ struct DCage {
     // walk ends on the boundary; face f: its halfedges from f2h[f] by next
     // are fring[fring_off[f] .. fring_off[f + 1]).
     const int32_t *vring_off, *vring, *vring_bnd, *fring_off, *fring;
+    // every halfedge leaving vertex v, ascending: vout[vout_off[v] ..
+    // vout_off[v + 1]) (null when the rings are)
+    const int32_t *vout_off, *vout;
     int32_t V, U, H, E, F;
 };
 

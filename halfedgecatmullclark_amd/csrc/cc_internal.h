@@ -37,7 +37,7 @@ bool ref_counts_fit_int32(const cc_Mesh *cage, int32_t D, const char **why);
 // steps at most), in CSR form.  `ok` is false when the lists would exceed
 // 4 H + V entries (a cage whose walks do not close): the kernels then walk.
 struct CageRings {
-    std::vector<int32_t> voff, vring, vbnd, foff, fring;
+    std::vector<int32_t> voff, vring, vbnd, foff, fring, ooff, out;
     bool ok = false;
 };
 
@@ -77,6 +77,17 @@ inline CageRings build_cage_rings(const cc_Mesh *m) {
         }
     }
     r.foff[F] = (int32_t)r.fring.size();
+    // halfedges by vertex, ascending (a counting sort)
+    r.ooff.assign((size_t)V + 1, 0);
+    for (int32_t h = 0; h < H; ++h) {
+        const int32_t v = he[h].vert;
+        if (v < 0 || v >= V) return r;
+        ++r.ooff[(size_t)v + 1];
+    }
+    for (int32_t v = 0; v < V; ++v) r.ooff[(size_t)v + 1] += r.ooff[v];
+    r.out.assign((size_t)H, 0);
+    std::vector<int32_t> fill(r.ooff.begin(), r.ooff.end() - 1);
+    for (int32_t h = 0; h < H; ++h) r.out[(size_t)fill[he[h].vert]++] = h;
     r.ok = true;
     return r;
 }
@@ -84,7 +95,7 @@ inline CageRings build_cage_rings(const cc_Mesh *m) {
 // host-memory view of a cage with the device-view type (for the O(depth) maps)
 inline DCage host_cage_view(const cc_Mesh *m) {
     DCage c;
-    c.vring_off = c.vring = c.vring_bnd = c.fring_off = c.fring = nullptr;
+    c.vring_off = c.vring = c.vring_bnd = c.fring_off = c.fring = c.vout_off = c.vout = nullptr;
     c.he = reinterpret_cast<const CageHe *>(m->halfedges);
     c.v2h = m->vertexToHalfedgeIDs;
     c.e2h = m->edgeToHalfedgeIDs;

@@ -30,7 +30,7 @@ struct ccd_Subd {
     // buffer; rings_ok false: the kernels walk the cage
     int32_t *rings = nullptr;
     size_t rings_cap = 0;
-    size_t o_vro = 0, o_vr = 0, o_vb = 0, o_fro = 0, o_fr = 0;
+    size_t o_vro = 0, o_vr = 0, o_vb = 0, o_fro = 0, o_fr = 0, o_voo = 0, o_vo = 0;
     bool rings_ok = false;
     // levels 1..D, cumulative, laid out as cc_Subd's arrays
     cch::He *he = nullptr;

@@ -7,6 +7,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -127,6 +129,8 @@ bool ccd_Subd::upload_cage(const cc_Mesh *cage, hipStream_t st) {
     o_fro = put(r.foff);
     o_vr = put(r.vring);
     o_fr = put(r.fring);
+    o_voo = put(r.ooff);
+    o_vo = put(r.out);
     if (rings_host.size() > rings_cap) {
         if (rings) (void)hipFree(rings);
         rings = nullptr;
@@ -148,8 +152,9 @@ DCage ccd_Subd::cage_view(const P3 *points) const {
     if (rings_ok) {
         c.vring_off = rings + o_vro; c.vring = rings + o_vr; c.vring_bnd = rings + o_vb;
         c.fring_off = rings + o_fro; c.fring = rings + o_fr;
+        c.vout_off = rings + o_voo; c.vout = rings + o_vo;
     } else {
-        c.vring_off = c.vring = c.vring_bnd = c.fring_off = c.fring = nullptr;
+        c.vring_off = c.vring = c.vring_bnd = c.fring_off = c.fring = c.vout_off = c.vout = nullptr;
     }
     c.V = V0; c.U = U0; c.H = H0; c.E = E0; c.F = F0;
     return c;
@@ -380,7 +385,8 @@ bool ccd_Subd::adjoint(const void *gradFinest, void *gradCage, int32_t steps, hi
         adj_scalar = 0;
         CC_TRY(hipMalloc(&adj_a, 3 * scalar * (size_t)n.V[D]), "hipMalloc(adjoint scratch)");
         CC_TRY(hipMalloc(&adj_b, 3 * scalar * (size_t)(D >= 1 ? n.V[D - 1] : 1)), "hipMalloc(adjoint scratch)");
-        CC_TRY(hipMalloc(&adj_w, 4 * scalar * (size_t)(D >= 2 ? n.H[D - 1] : 1)), "hipMalloc(adjoint weights)");
+        CC_TRY(hipMalloc(&adj_w, 4 * scalar * (size_t)std::max<int64_t>(D >= 2 ? n.H[D - 1] : 1, H0)),
+               "hipMalloc(adjoint weights)");
         adj_scalar = scalar;
     }
     AdjPlan pl;
