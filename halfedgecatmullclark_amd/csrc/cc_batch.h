@@ -35,12 +35,12 @@ struct CreaseJob {
 // compiler knows every address derived from it is global memory and emits
 // global rather than flat instructions); -1 marks an absent buffer.
 struct LevelJob {
-    int64_t he, out_he, cr, old, vp, fp, ep, defer_mask, owner_mask;
+    int64_t he, out_he, cr, old, vp, fp, ep, defer_mask, owner_mask, rare;
     int64_t c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, c_vro, c_vr, c_vb, c_fro, c_fr, c_voo, c_vo;
     int64_t lp[kMaxDepth + 1];
     int32_t H, V, F, E, C;
     int32_t cV, cU, cH, cE, cF;
-    int32_t xcd_remap, uv_refine;
+    int32_t xcd_remap, uv_refine, rare_cap;
     LevelTab t;
 };
 
@@ -99,6 +99,8 @@ __host__ __device__ __forceinline__ void unpack_level(const LevelJob &J, char *a
     a.ep = arena_ptr<P3>(arena, J.ep);
     a.defer_mask = arena_ptr<uint64_t>(arena, J.defer_mask);
     a.owner_mask = arena_ptr<uint64_t>(arena, J.owner_mask);
+    a.rare = arena_ptr<int32_t>(arena, J.rare);
+    a.rare_cap = J.rare_cap;
     a.H = J.H; a.V = J.V; a.F = J.F; a.E = J.E; a.C = J.C;
     a.xcd_remap = J.xcd_remap;
     a.uv_refine = J.uv_refine;
@@ -174,6 +176,9 @@ void batch_level_fused(const BatchTable &t, char *arena, int d, int gather, int 
                        hipStream_t st);
 void batch_level_vertices(const BatchTable &t, char *arena, int d, int gather, int creased, int scan, int remap,
                           hipStream_t st);
+// after a scan-0 vertex launch: each job's rare list, one thread per entry
+// (`start` counts workgroups of 256 list entries)
+void batch_level_vertices_rare(const BatchTable &t, char *arena, int d, int gather, int creased, hipStream_t st);
 
 // adjoint launches (cc_adjoint.hip)
 // one level's adjoint: weights, faces and vertex gathers over `quads` (one

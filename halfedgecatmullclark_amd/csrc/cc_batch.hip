@@ -118,7 +118,21 @@ __global__ __launch_bounds__(256) void k_batch_vertices_masked(const LevelJob *_
     unpack_level(J, arena, a, c);
     const JobLevelPtrs lp{J.lp, arena};
     const int32_t wave = (int32_t)(lb - (uint32_t)start[k]) * 4 + (int32_t)(threadIdx.x >> 6);
-    level_vertices_masked<GATHER, CREASED>(a, c, J.t, lp, d, wave);
+    level_vertices_masked<GATHER, CREASED, JobLevelPtrs, 1>(a, c, J.t, lp, d, wave);
+}
+
+template <int GATHER, int CREASED>
+__global__ __launch_bounds__(256) void k_batch_vertices_rare(const LevelJob *__restrict__ jobs,
+                                                             const int32_t *__restrict__ start, int n, int d,
+                                                             char *arena) {
+    const int k = find_job(start, n, blockIdx.x);
+    const LevelJob &J = jobs[k];
+    FusedArgs a;
+    DCage c;
+    unpack_level(J, arena, a, c);
+    const JobLevelPtrs lp{J.lp, arena};
+    level_vertices_rare<GATHER, CREASED>(a, c, J.t, lp, d,
+                                         (int32_t)((blockIdx.x - (uint32_t)start[k]) * 256 + threadIdx.x));
 }
 
 // vertex points of job k's level (one block per 256 * SCAN halfedges)
@@ -254,6 +268,17 @@ void batch_level_vertices(const BatchTable &t, char *arena, int d, int gather, i
     else if (gather) vertices_gc<1, 0>(t, arena, d, scan, remap, st);
     else if (creased) vertices_gc<0, 1>(t, arena, d, scan, remap, st);
     else vertices_gc<0, 0>(t, arena, d, scan, remap, st);
+}
+
+void batch_level_vertices_rare(const BatchTable &t, char *arena, int d, int gather, int creased, hipStream_t st) {
+    if (t.blocks == 0 || t.n == 0) return;
+    const dim3 grid(t.blocks), block(256);
+    const LevelJob *jobs = static_cast<const LevelJob *>(t.jobs);
+    const int n = (int)t.n;
+    if (gather && creased) hipLaunchKernelGGL((k_batch_vertices_rare<1, 1>), grid, block, 0, st, jobs, t.start, n, d, arena);
+    else if (gather) hipLaunchKernelGGL((k_batch_vertices_rare<1, 0>), grid, block, 0, st, jobs, t.start, n, d, arena);
+    else if (creased) hipLaunchKernelGGL((k_batch_vertices_rare<0, 1>), grid, block, 0, st, jobs, t.start, n, d, arena);
+    else hipLaunchKernelGGL((k_batch_vertices_rare<0, 0>), grid, block, 0, st, jobs, t.start, n, d, arena);
 }
 
 void batch_level_deferred(const BatchTable &t, char *arena, int d, int gather, int creased, hipStream_t st) {

@@ -29,7 +29,8 @@ struct ccb_Batch {
         int64_t o_he0 = 0, o_v0 = 0, o_u0 = 0, o_e0 = 0, o_f0 = 0;
         // element offsets of the cage's ring lists in `rings` (-1: walk the cage)
         int64_t o_vro = -1, o_vr = -1, o_vb = -1, o_fro = -1, o_fr = -1, o_voo = -1, o_vo = -1;
-        int64_t o_he = 0, o_cr = 0, o_pts = 0, o_fin = 0, o_def = 0, o_adjB = 0, o_adjW = 0;
+        int64_t o_he = 0, o_cr = 0, o_pts = 0, o_fin = 0, o_def = 0, o_adjB = 0, o_adjW = 0, o_rare = 0;
+        int32_t rare_cap = 0;   // rare-vertex list entries (FusedArgs::rare), the count first
     };
     struct Timed {
         const char *name;
@@ -44,7 +45,8 @@ struct ccb_Batch {
     int64_t tot_he0 = 0, tot_v0 = 0, tot_u0 = 0, tot_e0 = 0, tot_f0 = 0;
     std::vector<int32_t> rings_host;   // every cage's ring lists, copied into `rings`
     int32_t *rings = nullptr;
-    int64_t tot_he = 0, tot_cr = 0, tot_pts = 0, tot_fin = 0, tot_def = 0, tot_adjB = 0, tot_adjW = 0;
+    int64_t tot_he = 0, tot_cr = 0, tot_pts = 0, tot_fin = 0, tot_def = 0, tot_adjB = 0, tot_adjW = 0, tot_rare = 0;
+    int32_t *rare = nullptr;
     // device memory: every array of the forward pass is carved out of one
     // arena, the adjoint's out of a second one (job tables hold byte offsets
     // into them)
@@ -64,6 +66,7 @@ struct ccb_Batch {
     // launch tables
     BatchTable t_cage_he, t_cage_f, t_cage_e, t_cage_v, t_cage_h3, t_cage_zero, t_fin;
     BatchTable t_cr[kMaxDepth + 1], t_patch[kMaxDepth + 1], t_def[kMaxDepth + 1], t_adj[kMaxDepth + 1];
+    BatchTable t_rare[kMaxDepth + 1];   // rare-vertex lists after the scan-0 vertex kernel
     BatchTable t_fused[kMaxDepth + 1], t_vert[kMaxDepth + 1], t_adjq[kMaxDepth + 1];
     BatchTable t_adjc_e, t_adjc_f, t_adjc_v;   // cage-step adjoint gathers: per edge, face, vertex
     bool all_rings = true;                     // every cage has its ring lists
@@ -210,6 +213,14 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
         g.o_pts = tot_pts; tot_pts += g.n.cumV(g.D);
         g.o_fin = tot_fin; tot_fin += g.n.V[g.D];
         g.o_def = tot_def; tot_def += g.D >= 2 ? (g.n.H[g.D - 1] + 63) / 64 : 1;
+        {
+            int64_t b0 = 0;
+            const CageHe *ch = reinterpret_cast<const CageHe *>(m->halfedges);
+            for (int32_t h = 0; h < g.H0; ++h) b0 += ch[h].twin < 0 ? 1 : 0;
+            g.rare_cap = (int32_t)std::min<int64_t>((int64_t)g.V0 + g.F0 + (b0 << (g.D - 1)), INT32_MAX - 1);
+            g.o_rare = tot_rare;
+            tot_rare += g.rare_cap + 1;
+        }
         g.o_adjB = tot_adjB; tot_adjB += g.n.V[g.D - 1];
         g.o_adjW = tot_adjW; tot_adjW += std::max<int64_t>(g.D >= 2 ? g.n.H[g.D - 1] : 1, g.H0);
         Dmax = std::max(Dmax, g.D);
@@ -230,6 +241,7 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
         const size_t o_cr = cv.take(sizeof(Crease) * (size_t)tot_cr);
         const size_t o_pts = cv.take(sizeof(P3) * (size_t)tot_pts);
         const size_t o_def = cv.take(sizeof(uint64_t) * (size_t)tot_def);
+        const size_t o_rare = cv.take(sizeof(int32_t) * (size_t)tot_rare);
         CC_TRY(hipMalloc(&arena, cv.size), "hipMalloc(batch arena)");
         c_he = reinterpret_cast<CageHe *>(arena + o_che);
         c_v2h = reinterpret_cast<int32_t *>(arena + o_cv2h);
@@ -243,6 +255,7 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
         cr = reinterpret_cast<Crease *>(arena + o_cr);
         pts = reinterpret_cast<P3 *>(arena + o_pts);
         defer_mask = reinterpret_cast<uint64_t *>(arena + o_def);
+        rare = reinterpret_cast<int32_t *>(arena + o_rare);
     }
     for (int i = 0; i < count; ++i) {
         const cc_Mesh *m = meshes[i];
@@ -362,7 +375,7 @@ bool ccb_Batch::build_tables() {
     }
     for (int d = 1; d < Dmax; ++d) {   // patch-local level steps d -> d+1
         std::vector<LevelJob> jobs;
-        std::vector<int64_t> bp, bd, bf, bv;
+        std::vector<int64_t> bp, bd, bf, bv, br;
         for (int i = 0; i < N; ++i) {
             const Cage &g = cages[i];
             if (g.D <= d) continue;
@@ -379,6 +392,8 @@ bool ccb_Batch::build_tables() {
             j.ep = off(vp + g.n.V[d] + g.n.F[d]);
             j.defer_mask = off(defer_mask + g.o_def);
             j.owner_mask = cfg[d].kind == 1 && cfg[d].scan == 0 ? j.defer_mask : -1;
+            j.rare = j.owner_mask >= 0 ? off(rare + g.o_rare) : -1;
+            j.rare_cap = j.owner_mask >= 0 ? g.rare_cap : 0;
             j.H = (int32_t)g.n.H[d]; j.V = (int32_t)g.n.V[d]; j.F = (int32_t)g.n.F[d];
             j.E = (int32_t)g.n.E[d]; j.C = (int32_t)g.n.C[d];
             j.xcd_remap = remap;
@@ -391,6 +406,7 @@ bool ccb_Batch::build_tables() {
             bp.push_back((tiles + cfg[d].hpt - 1) / cfg[d].hpt);
             bd.push_back(blocks_of(blocks_of(blocks_of(g.n.H[d], 64), kDeferGroup), 4));
             bf.push_back(blocks_of(g.n.H[d], kFusedBlock));
+            br.push_back(j.rare_cap > 0 ? blocks_of(j.rare_cap) : 0);
             bv.push_back(cfg[d].scan == 0 ? blocks_of(blocks_of(blocks_of(g.n.H[d], 64), kOwnerWords), 4)
                                           : blocks_of(g.n.H[d], 256 * cfg[d].scan));
         }
@@ -398,6 +414,7 @@ bool ccb_Batch::build_tables() {
         tb.add(t_def[d], jobs, bd);
         tb.add(t_fused[d], jobs, bf);
         tb.add(t_vert[d], jobs, bv);
+        tb.add(t_rare[d], jobs, br);
     }
     if (adj_arena) {   // adjoint steps d+1 -> d, d = 0 the cage step
         for (int d = 0; d < Dmax; ++d) {
@@ -513,6 +530,7 @@ void ccb_Batch::record_forward(int32_t steps, hipStream_t st) {
                          [&] { batch_level_fused(t_fused[d], arena, d, gather, creased, topo, remap, st); });
             timed_launch("level_vertices", d, st, [&] {
                 batch_level_vertices(t_vert[d], arena, d, gather, creased, cfg[d].scan, remap, st);
+                if (cfg[d].scan == 0) batch_level_vertices_rare(t_rare[d], arena, d, gather, creased, st);
             });
         }
     }

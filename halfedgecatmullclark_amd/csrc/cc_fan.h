@@ -349,6 +349,51 @@ __device__ __forceinline__ bool structural_owner(int32_t h, int d, const DCage &
     return c.e2h[x.edge] == p;
 }
 
+// The fan and vertex ID of a marked owner h when its vertex is a face
+// vertex born at level >= 2 or an interior edge vertex (the fans
+// structural_ring accepts), reading at most the parent's record; false for
+// any other vertex (cage vertices and faces, boundary edges).
+__device__ __forceinline__ bool marked_ring_fan(int32_t h, int d, const DCage &c, const LevelTab &t,
+                                                const He *const *levels, Fan &f, int32_t &v) {
+    int k = 0;
+    while (k < d - 1 && ((h >> (2 * k)) & 3) == 0) ++k;
+    const int32_t q = h >> (2 * k);
+    const int lvl = d - k;
+    const int digit = q & 3;
+    const int32_t p = q >> 2;
+    f.shift = 2 * k;
+    f.boundary = false;
+    f.ri = 0;
+    f.n = 4;
+    if (digit == 2) {
+        if (lvl < 2) return false;
+        f.kind = 1;
+        f.a0 = 4 * p + 2;
+        v = (int32_t)t.V[lvl - 1] + (p >> 2);
+        return true;
+    }
+    if (digit != 1) return false;
+    int32_t tw, np, nw, e;
+    if (lvl >= 2) {
+        const int4 x = reinterpret_cast<const int4 *>(levels[lvl - 1])[p];
+        tw = x.x;
+        e = x.y;
+        np = qnext(p);
+        nw = qnext(tw);
+    } else {
+        const CageHe x = c.he[p];
+        tw = x.twin;
+        e = x.edge;
+        np = x.next;
+        nw = tw >= 0 ? c.he[tw].next : -1;
+    }
+    if (tw < 0) return false;
+    f.kind = 3;
+    f.a0 = 4 * p + 1; f.a1 = 4 * nw + 3; f.a2 = 4 * tw + 1; f.a3 = 4 * np + 3;
+    v = (int32_t)(t.V[lvl - 1] + t.F[lvl - 1]) + e;
+    return true;
+}
+
 // The fan and vertex ID of a level-d halfedge h known to be the
 // representative of its vertex (marked by the fused kernel).  A face or edge
 // vertex born at level lvl >= 2 takes its ID from the child numbering

@@ -31,6 +31,13 @@ struct FusedArgs {
     // per halfedge, written by the fused kernel when set, read by the masked
     // vertex kernel
     uint64_t *owner_mask;
+    // when set: the masked vertex kernel handles the vertices whose ring
+    // follows from the child numbering and appends the others (cage-level
+    // and boundary fans) to this list, rare[0] their count (cleared by the
+    // fused kernel), rare[1 .. rare_cap] their representatives; the rare
+    // kernel then finishes them
+    int32_t *rare;
+    int32_t rare_cap;
 };
 
 // topo: write child twin/edge/vertex IDs; uvMode: 0 leave uvIDs, 1 refine

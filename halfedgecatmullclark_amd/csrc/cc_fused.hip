@@ -38,6 +38,18 @@ __global__ __launch_bounds__(256) void k_level_vertices_masked(FusedArgs a, DCag
     level_vertices_masked<GATHER, CREASED>(a, c, t, lp, d, (int32_t)(lb * 4 + (threadIdx.x >> 6)));
 }
 
+// ring vertices only; the others to the rare list, then k_level_vertices_rare
+template <int GATHER, int CREASED>
+__global__ __launch_bounds__(256) void k_level_vertices_ring(FusedArgs a, DCage c, LevelTab t, LevelPtrs lp, int d) {
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, a.xcd_remap);
+    level_vertices_masked<GATHER, CREASED, LevelPtrs, 1>(a, c, t, lp, d, (int32_t)(lb * 4 + (threadIdx.x >> 6)));
+}
+
+template <int GATHER, int CREASED>
+__global__ __launch_bounds__(256) void k_level_vertices_rare(FusedArgs a, DCage c, LevelTab t, LevelPtrs lp, int d) {
+    level_vertices_rare<GATHER, CREASED>(a, c, t, lp, d, (int32_t)(blockIdx.x * 256 + threadIdx.x));
+}
+
 template <int GATHER, int CREASED>
 __global__ __launch_bounds__(256) void k_level_vertices(FusedArgs a, DCage c, LevelTab t, int d) {
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, a.xcd_remap);
@@ -268,6 +280,25 @@ void launch_level_vertices(const FusedArgs &a, const DCage &c, const LevelTab &t
         else if (gather) hipLaunchKernelGGL((k_level_vertices_direct<1, 0>), g3, block, 0, st, a, c, t, lp, d);
         else if (creased) hipLaunchKernelGGL((k_level_vertices_direct<0, 1>), g3, block, 0, st, a, c, t, lp, d);
         else hipLaunchKernelGGL((k_level_vertices_direct<0, 0>), g3, block, 0, st, a, c, t, lp, d);
+        return;
+    }
+    if (fan == 2 && scan == 0 && a.rare) {
+        const int64_t words = ((int64_t)a.H + 63) / 64;
+        const int64_t waves = (words + kOwnerWords - 1) / kOwnerWords;
+        const dim3 g0((unsigned)((waves + 3) / 4)), gr((unsigned)((a.rare_cap + 255) / 256));
+        if (gather && creased) {
+            hipLaunchKernelGGL((k_level_vertices_ring<1, 1>), g0, block, 0, st, a, c, t, lp, d);
+            hipLaunchKernelGGL((k_level_vertices_rare<1, 1>), gr, block, 0, st, a, c, t, lp, d);
+        } else if (gather) {
+            hipLaunchKernelGGL((k_level_vertices_ring<1, 0>), g0, block, 0, st, a, c, t, lp, d);
+            hipLaunchKernelGGL((k_level_vertices_rare<1, 0>), gr, block, 0, st, a, c, t, lp, d);
+        } else if (creased) {
+            hipLaunchKernelGGL((k_level_vertices_ring<0, 1>), g0, block, 0, st, a, c, t, lp, d);
+            hipLaunchKernelGGL((k_level_vertices_rare<0, 1>), gr, block, 0, st, a, c, t, lp, d);
+        } else {
+            hipLaunchKernelGGL((k_level_vertices_ring<0, 0>), g0, block, 0, st, a, c, t, lp, d);
+            hipLaunchKernelGGL((k_level_vertices_rare<0, 0>), gr, block, 0, st, a, c, t, lp, d);
+        }
         return;
     }
     if (fan == 2 && scan == 0) {

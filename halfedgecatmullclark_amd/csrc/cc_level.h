@@ -68,6 +68,7 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
     __shared__ P3 spts[POINTS ? NT : 1];
     __shared__ LevelTab stab;   // for the owner test (read after the block's barrier)
     if (POINTS && a.owner_mask) lds_fill(stab, tb);
+    if (POINTS && a.rare && lb == 0 && threadIdx.x == 0) a.rare[0] = 0;
     const int32_t base = (int32_t)(lb * blockDim.x);
     const int32_t h = base + (int32_t)threadIdx.x;
     const bool live = h < a.H;   // H is a multiple of 4: quads are whole
@@ -261,9 +262,32 @@ struct MaskedVertexFn {
     }
 };
 
+// the same for the vertices whose ring follows from the child numbering;
+// the others go to the rare list (no general fan code here, so the kernel
+// keeps fewer registers and runs more waves)
+template <int GATHER, int CREASED>
+struct RingVertexFn {
+    const FusedArgs &a;
+    const DCage &c;
+    const LevelTab &t;
+    const LevelPtrs &lp;
+    int d;
+    __device__ void operator()(int32_t h) const {
+        Fan f;
+        int32_t v, r[4], e[4];
+        if (!marked_ring_fan(h, d, c, t, lp.he, f, v) || !structural_ring(f, t, d, v, r, e)) {
+            const int32_t slot = atomicAdd(a.rare, 1);
+            if (slot < a.rare_cap) a.rare[1 + slot] = h;
+            return;
+        }
+        const FanPreloaded pre = closed_fan(a, r, e);
+        a.vp[v] = fan_vertex_point<GATHER, CREASED, 4>(f, pre, a.old[v]);
+    }
+};
+
 // vertex points of the owners marked by the fused kernel; `wave` is the
 // index of this wave among the level's waves (kOwnerWords words each)
-template <int GATHER, int CREASED, class LP>
+template <int GATHER, int CREASED, class LP, int RING = 0>
 __device__ __forceinline__ void level_vertices_masked(const FusedArgs &a, const DCage &c, const LevelTab &t,
                                                       const LP &lp, int d, int32_t wave) {
     __shared__ LevelTab st;
@@ -272,8 +296,28 @@ __device__ __forceinline__ void level_vertices_masked(const FusedArgs &a, const 
     lds_fill(slp, lp);
     __syncthreads();
     const int32_t words = (a.H + 63) >> 6;
-    mask_group<kOwnerWords>(a.owner_mask, wave * kOwnerWords, words,
-                            MaskedVertexFn<GATHER, CREASED>{a, c, st, slp, d});
+    if (RING)
+        mask_group<kOwnerWords>(a.owner_mask, wave * kOwnerWords, words, RingVertexFn<GATHER, CREASED>{a, c, st, slp, d});
+    else
+        mask_group<kOwnerWords>(a.owner_mask, wave * kOwnerWords, words,
+                                MaskedVertexFn<GATHER, CREASED>{a, c, st, slp, d});
+}
+
+// the vertices the ring-only masked kernel listed, one thread each
+template <int GATHER, int CREASED, class LP>
+__device__ __forceinline__ void level_vertices_rare(const FusedArgs &a, const DCage &c, const LevelTab &t,
+                                                    const LP &lp, int d, int32_t i) {
+    __shared__ LevelTab st;
+    __shared__ LevelPtrs slp;
+    lds_fill(st, t);
+    lds_fill(slp, lp);
+    __syncthreads();
+    const int32_t n = a.rare[0] < a.rare_cap ? a.rare[0] : a.rare_cap;
+    if (i >= n) return;
+    const int32_t h = a.rare[1 + i];
+    Fan f;
+    const int32_t v = marked_fan(h, d, c, st, slp.he, f);
+    fan_vertex_store<GATHER, CREASED>(a, c, st, d, f, v);
 }
 
 }  // namespace cch

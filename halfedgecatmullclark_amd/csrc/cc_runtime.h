@@ -31,6 +31,11 @@ struct ccd_Subd {
     int32_t *rings = nullptr;
     size_t rings_cap = 0;
     size_t o_vro = 0, o_vr = 0, o_vb = 0, o_fro = 0, o_fr = 0, o_voo = 0, o_vo = 0;
+    // the masked vertex kernel's list of vertices without a ring from the
+    // child numbering (FusedArgs::rare): at most V0 + F0 + (boundary
+    // halfedges of the cage) * 2^(D-1) per level
+    int32_t *rare = nullptr;
+    int32_t rare_cap = 0;
     bool rings_ok = false;
     // levels 1..D, cumulative, laid out as cc_Subd's arrays
     cch::He *he = nullptr;

@@ -88,7 +88,7 @@ bool ccd_Subd::alloc(const cc_Mesh *cage, int32_t depth) {
 void ccd_Subd::free_all() {
     DeviceGuard g(device);
     void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts, adj_a, adj_b, adj_w, adj_in, adj_out, adj_sharp,
-                    defer_mask, rings};
+                    defer_mask, rings, rare};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : timed) {
@@ -112,6 +112,19 @@ bool ccd_Subd::upload_cage(const cc_Mesh *cage, hipStream_t st) {
     CC_TRY(hipMemcpyAsync(c_pts, cage->vertexPoints, sizeof(P3) * (size_t)V0, hipMemcpyHostToDevice, st), "upload cage");
     if (U0 > 0) CC_TRY(hipMemcpyAsync(c_uvs, cage->uvs, sizeof(UV) * (size_t)U0, hipMemcpyHostToDevice, st), "upload cage");
     CC_TRY(hipMemcpyAsync(c_cr, cage->creases, sizeof(Crease) * (size_t)E0, hipMemcpyHostToDevice, st), "upload cage");
+    {   // the rare-vertex list: cage vertices, cage faces and boundary vertices
+        int64_t b0 = 0;
+        const CageHe *ch = reinterpret_cast<const CageHe *>(cage->halfedges);
+        for (int32_t h = 0; h < H0; ++h) b0 += ch[h].twin < 0 ? 1 : 0;
+        const int64_t cap = std::min<int64_t>((int64_t)V0 + F0 + (b0 << (D >= 1 ? D - 1 : 0)), INT32_MAX - 1);
+        if (cap > rare_cap || !rare) {
+            if (rare) (void)hipFree(rare);
+            rare = nullptr;
+            rare_cap = 0;
+            CC_TRY(hipMalloc(&rare, sizeof(int32_t) * (size_t)(cap + 1)), "hipMalloc(rare vertices)");
+            rare_cap = (int32_t)cap;
+        }
+    }
     // ring lists: [vertex offsets | vertex boundary flags | face offsets |
     // vertex rings | face rings], each list padded by one entry
     const CageRings r = build_cage_rings(cage);
@@ -234,6 +247,8 @@ FusedArgs ccd_Subd::fused_args(int d, const P3 *points) const {
     a.uv_refine = 0;
     a.defer_mask = nullptr;
     a.owner_mask = nullptr;
+    a.rare = nullptr;
+    a.rare_cap = 0;
     return a;
 }
 
@@ -289,7 +304,11 @@ void ccd_Subd::level_fused(int d, int32_t steps, int topo, int uvMode, P3 *point
     FusedArgs a = fused_args(d, points);
     const DCage c = cage_view(nullptr);
     // the fused kernel marks the vertex representatives for the masked kernel
-    if (pts_ && opt_fan == 2 && cfg[d].scan == 0) a.owner_mask = defer_mask;
+    if (pts_ && opt_fan == 2 && cfg[d].scan == 0) {
+        a.owner_mask = defer_mask;
+        a.rare = rare;
+        a.rare_cap = rare_cap;
+    }
     if (topo || uvMode == 1 || pts_)
         launch_timed(pts_ ? (topo ? "level_fused" : "level_points") : "level_fused", n.H[d], st,
                      [&] { launch_level_fused(a, c, tab, d, gather, creased, topo, uvMode, pts_, opt_store, st); });
