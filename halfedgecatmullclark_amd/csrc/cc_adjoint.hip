@@ -103,6 +103,25 @@ __device__ __forceinline__ VW<T> level_vertex_weights(const RingT<T> &rp, T ts, 
 template <typename T>
 struct W4 { T a, b, g, dl; };
 
+// The weights live as four arrays of the level's H scalars, a | b | g | dl,
+// so that each pass reads only the one it needs.
+template <typename T>
+__device__ __forceinline__ void w_store(const AdjArgs<T> &a, int32_t h, T wa, T wb, T wg, T wdl) {
+    const int64_t H = a.H;
+    a.W[h] = wa;
+    a.W[H + h] = wb;
+    a.W[2 * H + h] = wg;
+    a.W[3 * H + h] = wdl;
+}
+template <typename T>
+__device__ __forceinline__ T w_a(const AdjArgs<T> &a, int32_t h) { return a.W[h]; }
+template <typename T>
+__device__ __forceinline__ T w_b(const AdjArgs<T> &a, int32_t h) { return a.W[(int64_t)a.H + h]; }
+template <typename T>
+__device__ __forceinline__ T w_g(const AdjArgs<T> &a, int32_t h) { return a.W[2 * (int64_t)a.H + h]; }
+template <typename T>
+__device__ __forceinline__ T w_dl(const AdjArgs<T> &a, int32_t h) { return a.W[3 * (int64_t)a.H + h]; }
+
 // weights of one halfedge h in the edge rule (creased :2221-2233, smooth
 // :2125-2131): c0 on its vertex, c1 on the next vertex, cf on its face point
 template <typename T, int CREASED>
@@ -146,37 +165,35 @@ template <typename T, int CREASED>
 __device__ __forceinline__ void adj_ring_weights(const AdjArgs<T> &a, const DCage &c, const Fan &f) {
     const He *he = a.he;
     const RingT<T> rp = ring_params<T, CREASED>(a, c, f);
-    W4<T> *W = reinterpret_cast<W4<T> *>(a.W);
     for (int i = 0; i < f.n; ++i) {
         const int32_t r = fan_at(f, c, i);
         const int32_t re = he[r].edge, pe = he[qprev(r)].edge;
         const T ts = CREASED ? sharp_at(a, re) : (T)0;
         const T ps = CREASED ? sharp_at(a, pe) : (T)0;
         const VW<T> w = level_vertex_weights<T>(rp, ts, ps, re, CREASED);
-        W[r] = {w.a, w.b, w.g, w.dl};
+        w_store(a, r, w.a, w.b, w.g, w.dl);
     }
 }
 
 // pass 2: edge(h) for its owner h (the larger halfedge, or a boundary one):
 // the b shares of the halfedges on the edge and the dl shares of the
 // halfedges whose previous edge it is
-template <typename T>
+template <typename T, int CREASED>
 __device__ __forceinline__ void adj_edge_at(const AdjArgs<T> &a, int32_t h) {
     if (h >= a.H) return;
     const He x = a.he[h];
     if (x.twin >= 0 && h < x.twin) return;
-    const W4<T> *W = reinterpret_cast<const W4<T> *>(a.W);
     const V3<T> *gV = reinterpret_cast<const V3<T> *>(a.gV);
     V3<T> *gE = reinterpret_cast<V3<T> *>(a.gE);
     V3<T> ge = gE[x.edge];
     const int32_t nh = qnext(h);
-    const T bh = W[h].b, dn = W[nh].dl;
+    const T bh = w_b(a, h), dn = CREASED ? w_dl(a, nh) : (T)0;   // smooth rules have no dl share
     const V3<T> vh = gV[x.vert], vn = gV[a.he[nh].vert];
     ge.x += bh * vh.x; ge.y += bh * vh.y; ge.z += bh * vh.z;
     ge.x += dn * vn.x; ge.y += dn * vn.y; ge.z += dn * vn.z;
     if (x.twin >= 0) {
         const int32_t t = x.twin, nt = qnext(t);
-        const T bt = W[t].b, dt = W[nt].dl;
+        const T bt = w_b(a, t), dt = CREASED ? w_dl(a, nt) : (T)0;
         const V3<T> vt = gV[a.he[t].vert], vnt = gV[a.he[nt].vert];
         ge.x += bt * vt.x; ge.y += bt * vt.y; ge.z += bt * vt.z;
         ge.x += dt * vnt.x; ge.y += dt * vnt.y; ge.z += dt * vnt.z;
@@ -189,7 +206,6 @@ __device__ __forceinline__ void adj_edge_at(const AdjArgs<T> &a, int32_t h) {
 template <typename T, int CREASED>
 __device__ __forceinline__ void adj_face_at(const AdjArgs<T> &a, int32_t f) {
     if (f >= (a.H >> 2)) return;
-    const W4<T> *W = reinterpret_cast<const W4<T> *>(a.W);
     const V3<T> *gV = reinterpret_cast<const V3<T> *>(a.gV);
     const V3<T> *gE = reinterpret_cast<const V3<T> *>(a.gE);
     V3<T> *gF = reinterpret_cast<V3<T> *>(a.gF);
@@ -200,7 +216,7 @@ __device__ __forceinline__ void adj_face_at(const AdjArgs<T> &a, int32_t f) {
         const He x = a.he[h];
         T c0, c1, cf;
         edge_weights<T, CREASED>(a, x.edge, x.twin < 0, c0, c1, cf);
-        const T g = W[h].g;
+        const T g = w_g(a, h);
         const V3<T> v = gV[x.vert], e = gE[x.edge];
         gf.x += g * v.x; gf.y += g * v.y; gf.z += g * v.z;
         gf.x += cf * e.x; gf.y += cf * e.y; gf.z += cf * e.z;
@@ -214,7 +230,6 @@ __device__ __forceinline__ void adj_face_at(const AdjArgs<T> &a, int32_t f) {
 template <typename T, int CREASED>
 __device__ __forceinline__ void adj_vertex_gather(const AdjArgs<T> &a, const DCage &c, const Fan &f, int32_t v) {
     const He *he = a.he;
-    const W4<T> *W = reinterpret_cast<const W4<T> *>(a.W);
     const V3<T> *gE = reinterpret_cast<const V3<T> *>(a.gE);
     const V3<T> *gF = reinterpret_cast<const V3<T> *>(a.gF);
     const V3<T> gv = reinterpret_cast<const V3<T> *>(a.gV)[v];
@@ -224,7 +239,7 @@ __device__ __forceinline__ void adj_vertex_gather(const AdjArgs<T> &a, const DCa
         const int32_t r = fan_at(f, c, i);
         const int32_t p = qprev(r);
         const He x = he[r], xp = he[p];
-        alpha += W[r].a;
+        alpha += w_a(a, r);
         T c0, c1, cf, d0, d1, df;
         edge_weights<T, CREASED>(a, x.edge, x.twin < 0, c0, c1, cf);     // r leaves v
         edge_weights<T, CREASED>(a, xp.edge, xp.twin < 0, d0, d1, df);  // prev(r) enters v
@@ -266,11 +281,10 @@ __device__ __forceinline__ void adj_weights_fan(const AdjArgs<T> &a, const DCage
             rp.sall += se;
         }
     }
-    W4<T> *W = reinterpret_cast<W4<T> *>(a.W);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const VW<T> w = level_vertex_weights<T>(rp, s[i], s[(i + 1) & 3], e[i], CREASED);
-        W[h[i]] = {w.a, w.b, w.g, w.dl};
+        w_store(a, h[i], w.a, w.b, w.g, w.dl);
     }
 }
 
@@ -282,7 +296,6 @@ __device__ __forceinline__ void adj_gather_fan(const AdjArgs<T> &a, const DCage 
         adj_vertex_gather<T, CREASED>(a, c, f, v);
         return;
     }
-    const W4<T> *W = reinterpret_cast<const W4<T> *>(a.W);
     const V3<T> *gE = reinterpret_cast<const V3<T> *>(a.gE);
     const V3<T> *gF = reinterpret_cast<const V3<T> *>(a.gF);
     const V3<T> gv = reinterpret_cast<const V3<T> *>(a.gV)[v];
@@ -291,7 +304,7 @@ __device__ __forceinline__ void adj_gather_fan(const AdjArgs<T> &a, const DCage 
 #pragma unroll
     for (int i = 0; i < 4; ++i) {   // interior ring: no boundary edge
         const int32_t r = h[i], re = e[i], pe = e[(i + 1) & 3];
-        alpha += W[r].a;
+        alpha += w_a(a, r);
         T c0, c1, cf, d0, d1, df;
         edge_weights<T, CREASED>(a, re, false, c0, c1, cf);
         edge_weights<T, CREASED>(a, pe, false, d0, d1, df);
@@ -560,7 +573,7 @@ __device__ __forceinline__ void adj_cage_weights_at(const AdjArgs<T> &a, const D
             if (pcw != (T)0) w.dl = tt * (T)0.25 * pcw;
         }
     }
-    reinterpret_cast<W4<T> *>(a.W)[h] = w;
+    w_store(a, h, w.a, w.b, w.g, w.dl);
 }
 
 template <typename T>
@@ -574,7 +587,6 @@ __device__ __forceinline__ void add_scaled(V3<T> &acc, T w, const V3<T> &g) {
 template <typename T>
 __device__ __forceinline__ void adj_cage_edge_gather(const AdjArgs<T> &a, const DCage &c, int32_t e) {
     if (e >= c.E) return;
-    const W4<T> *W = reinterpret_cast<const W4<T> *>(a.W);
     const V3<T> *gV = reinterpret_cast<const V3<T> *>(a.gV);
     V3<T> *gE = reinterpret_cast<V3<T> *>(a.gE);
     const int32_t h0 = c.e2h[e], t0 = c.he[h0].twin;
@@ -585,10 +597,10 @@ __device__ __forceinline__ void adj_cage_edge_gather(const AdjArgs<T> &a, const 
         if (t0 < h0) { hs[0] = t0; hs[1] = h0; }
     }
     V3<T> acc = gE[e];
-    for (int i = 0; i < k; ++i) add_scaled(acc, W[hs[i]].b, gV[c.he[hs[i]].vert]);
+    for (int i = 0; i < k; ++i) add_scaled(acc, w_b(a, hs[i]), gV[c.he[hs[i]].vert]);
     for (int i = 0; i < k; ++i) {
         const int32_t nx = c.he[hs[i]].next;
-        add_scaled(acc, W[nx].dl, gV[c.he[nx].vert]);
+        add_scaled(acc, w_dl(a, nx), gV[c.he[nx].vert]);
     }
     gE[e] = acc;
 }
@@ -598,7 +610,6 @@ __device__ __forceinline__ void adj_cage_edge_gather(const AdjArgs<T> &a, const 
 template <typename T, int CREASED>
 __device__ __forceinline__ void adj_cage_face_gather(const AdjArgs<T> &a, const DCage &c, int32_t f) {
     if (f >= c.F) return;
-    const W4<T> *W = reinterpret_cast<const W4<T> *>(a.W);
     const V3<T> *gV = reinterpret_cast<const V3<T> *>(a.gV);
     const V3<T> *gE = reinterpret_cast<const V3<T> *>(a.gE);
     V3<T> *gF = reinterpret_cast<V3<T> *>(a.gF);
@@ -606,7 +617,7 @@ __device__ __forceinline__ void adj_cage_face_gather(const AdjArgs<T> &a, const 
     V3<T> acc = gF[f];
     for (int i = 0; i < m; ++i) {
         const int32_t h = c.fring[o + i];
-        add_scaled(acc, W[h].g, gV[c.he[h].vert]);
+        add_scaled(acc, w_g(a, h), gV[c.he[h].vert]);
     }
     for (int i = 0; i < m; ++i) {
         const CageHe x = c.he[c.fring[o + i]];
@@ -622,7 +633,6 @@ __device__ __forceinline__ void adj_cage_face_gather(const AdjArgs<T> &a, const 
 template <typename T, int CREASED>
 __device__ __forceinline__ void adj_cage_vertex_gather(const AdjArgs<T> &a, const DCage &c, int32_t v) {
     if (v >= c.V) return;
-    const W4<T> *W = reinterpret_cast<const W4<T> *>(a.W);
     const V3<T> *gE = reinterpret_cast<const V3<T> *>(a.gE);
     const V3<T> *gF = reinterpret_cast<const V3<T> *>(a.gF);
     const V3<T> gv = reinterpret_cast<const V3<T> *>(a.gV)[v];
@@ -633,7 +643,7 @@ __device__ __forceinline__ void adj_cage_vertex_gather(const AdjArgs<T> &a, cons
         T c0, c1, cf, d0, d1, df;
         cage_edge_weights<T, CREASED>(c, x, c0, c1, cf);
         cage_edge_weights<T, CREASED>(c, xp, d0, d1, df);
-        add_scaled(acc, W[h].a, gv);
+        add_scaled(acc, w_a(a, h), gv);
         add_scaled(acc, c0, gE[x.edge]);
         add_scaled(acc, d1, gE[xp.edge]);
         const int32_t m = c.fring_off[x.face + 1] - c.fring_off[x.face];
@@ -648,9 +658,9 @@ __global__ __launch_bounds__(256) void k_adj_weights(AdjArgs<T> a, DCage c, Leve
     owners_block<kAdjScan>(a.H, d, c, t, lp, blockIdx.x, RingWeightsFn<T, CREASED>{a, c, d});
 }
 
-template <typename T>
+template <typename T, int CREASED>
 __global__ __launch_bounds__(256) void k_adj_edges(AdjArgs<T> a) {
-    adj_edge_at<T>(a, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
+    adj_edge_at<T, CREASED>(a, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 template <typename T, int CREASED>
@@ -784,14 +794,14 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
         if (creased) {
             if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 1, 0>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             else hipLaunchKernelGGL((k_adj_weights<T, 1>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
-            hipLaunchKernelGGL(k_adj_edges<T>, g, b, 0, st, a);
+            hipLaunchKernelGGL((k_adj_edges<T, 1>), g, b, 0, st, a);
             hipLaunchKernelGGL((k_adj_faces<T, 1>), gq, b, 0, st, a);
             if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 1, 1>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             else hipLaunchKernelGGL((k_adj_vertices<T, 1>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
         } else {
             if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 0, 0>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             else hipLaunchKernelGGL((k_adj_weights<T, 0>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
-            hipLaunchKernelGGL(k_adj_edges<T>, g, b, 0, st, a);
+            hipLaunchKernelGGL((k_adj_edges<T, 0>), g, b, 0, st, a);
             hipLaunchKernelGGL((k_adj_faces<T, 0>), gq, b, 0, st, a);
             if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 0, 1>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             else hipLaunchKernelGGL((k_adj_vertices<T, 0>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
@@ -854,13 +864,14 @@ struct OpAdjWeights {
         owners_block<kAdjScan>(a.H, d, c, J.t, lp, (uint32_t)(i >> 8), RingWeightsFn<float, CREASED>{a, c, d});
     }
 };
+template <int CREASED>
 struct OpAdjEdges {
     char *arena, *adj;
     __device__ void operator()(const AdjJob &J, int64_t i) const {
         AdjArgs<float> a;
         DCage c;
         unpack_adj(J, arena, adj, a, c);
-        adj_edge_at<float>(a, (int32_t)i);
+        adj_edge_at<float, CREASED>(a, (int32_t)i);
     }
 };
 template <int CREASED>
@@ -938,7 +949,8 @@ void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, char 
                      hipStream_t st) {
     if (creased) launch_batch<AdjJob>(quads, OpAdjWeights<1>{d, arena, adj}, st);
     else launch_batch<AdjJob>(quads, OpAdjWeights<0>{d, arena, adj}, st);
-    launch_batch<AdjJob>(halfedges, OpAdjEdges{arena, adj}, st);
+    if (creased) launch_batch<AdjJob>(halfedges, OpAdjEdges<1>{arena, adj}, st);
+    else launch_batch<AdjJob>(halfedges, OpAdjEdges<0>{arena, adj}, st);
     if (creased) {
         launch_batch<AdjJob>(quads, OpAdjFaces<1>{arena, adj}, st);
         launch_batch<AdjJob>(quads, OpAdjVertices<1>{d, arena, adj}, st);
