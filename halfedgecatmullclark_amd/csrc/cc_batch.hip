@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kFusedBlock) void k_batch_fused(const LevelJob *__r
 // vertex points of the owners k_batch_fused marked (kOwnerWords mask words
 // per wave, four waves per workgroup)
 template <int GATHER, int CREASED>
-__global__ __launch_bounds__(256) void k_batch_vertices_masked(const LevelJob *__restrict__ jobs,
+__global__ __launch_bounds__(256, 8) void k_batch_vertices_masked(const LevelJob *__restrict__ jobs,
                                                                const int32_t *__restrict__ start, int n, int d,
                                                                int remap, char *arena) {
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, remap);

@@ -39,8 +39,9 @@ __global__ __launch_bounds__(256) void k_level_vertices_masked(FusedArgs a, DCag
 }
 
 // ring vertices only; the others to the rare list, then k_level_vertices_rare
+// (8 waves per SIMD fit without spilling: 56 registers)
 template <int GATHER, int CREASED>
-__global__ __launch_bounds__(256) void k_level_vertices_ring(FusedArgs a, DCage c, LevelTab t, LevelPtrs lp, int d) {
+__global__ __launch_bounds__(256, 8) void k_level_vertices_ring(FusedArgs a, DCage c, LevelTab t, LevelPtrs lp, int d) {
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, a.xcd_remap);
     level_vertices_masked<GATHER, CREASED, LevelPtrs, 1>(a, c, t, lp, d, (int32_t)(lb * 4 + (threadIdx.x >> 6)));
 }
