@@ -118,7 +118,7 @@ __global__ __launch_bounds__(256, 8) void k_batch_vertices_masked(const LevelJob
     unpack_level(J, arena, a, c);
     const JobLevelPtrs lp{J.lp, arena};
     const int32_t wave = (int32_t)(lb - (uint32_t)start[k]) * 4 + (int32_t)(threadIdx.x >> 6);
-    level_vertices_masked<GATHER, CREASED, JobLevelPtrs, 1>(a, c, J.t, lp, d, wave);
+    level_vertices_masked<GATHER, CREASED>(a, c, J.t, lp, d, wave);
 }
 
 template <int GATHER, int CREASED>

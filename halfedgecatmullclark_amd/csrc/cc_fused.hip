@@ -30,20 +30,12 @@ __global__ __launch_bounds__(kFusedBlock) void k_level_fused(FusedArgs a, DCage 
         a, c, t, d, xcd_block(blockIdx.x, gridDim.x, a.xcd_remap));
 }
 
-// vertex points of the owners the fused kernel marked (kOwnerWords mask
-// words per wave)
-template <int GATHER, int CREASED>
-__global__ __launch_bounds__(256) void k_level_vertices_masked(FusedArgs a, DCage c, LevelTab t, LevelPtrs lp, int d) {
-    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, a.xcd_remap);
-    level_vertices_masked<GATHER, CREASED>(a, c, t, lp, d, (int32_t)(lb * 4 + (threadIdx.x >> 6)));
-}
-
 // ring vertices only; the others to the rare list, then k_level_vertices_rare
 // (8 waves per SIMD fit without spilling: 56 registers)
 template <int GATHER, int CREASED>
 __global__ __launch_bounds__(256, 8) void k_level_vertices_ring(FusedArgs a, DCage c, LevelTab t, LevelPtrs lp, int d) {
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, a.xcd_remap);
-    level_vertices_masked<GATHER, CREASED, LevelPtrs, 1>(a, c, t, lp, d, (int32_t)(lb * 4 + (threadIdx.x >> 6)));
+    level_vertices_masked<GATHER, CREASED>(a, c, t, lp, d, (int32_t)(lb * 4 + (threadIdx.x >> 6)));
 }
 
 template <int GATHER, int CREASED>
@@ -283,7 +275,7 @@ void launch_level_vertices(const FusedArgs &a, const DCage &c, const LevelTab &t
         else hipLaunchKernelGGL((k_level_vertices_direct<0, 0>), g3, block, 0, st, a, c, t, lp, d);
         return;
     }
-    if (fan == 2 && scan == 0 && a.rare) {
+    if (fan == 2 && scan == 0 && a.owner_mask && a.rare) {   // owners marked by the fused kernel
         const int64_t words = ((int64_t)a.H + 63) / 64;
         const int64_t waves = (words + kOwnerWords - 1) / kOwnerWords;
         const dim3 g0((unsigned)((waves + 3) / 4)), gr((unsigned)((a.rare_cap + 255) / 256));
@@ -300,16 +292,6 @@ void launch_level_vertices(const FusedArgs &a, const DCage &c, const LevelTab &t
             hipLaunchKernelGGL((k_level_vertices_ring<0, 0>), g0, block, 0, st, a, c, t, lp, d);
             hipLaunchKernelGGL((k_level_vertices_rare<0, 0>), gr, block, 0, st, a, c, t, lp, d);
         }
-        return;
-    }
-    if (fan == 2 && scan == 0) {
-        const int64_t words = ((int64_t)a.H + 63) / 64;
-        const int64_t waves = (words + kOwnerWords - 1) / kOwnerWords;
-        const dim3 g0((unsigned)((waves + 3) / 4));
-        if (gather && creased) hipLaunchKernelGGL((k_level_vertices_masked<1, 1>), g0, block, 0, st, a, c, t, lp, d);
-        else if (gather) hipLaunchKernelGGL((k_level_vertices_masked<1, 0>), g0, block, 0, st, a, c, t, lp, d);
-        else if (creased) hipLaunchKernelGGL((k_level_vertices_masked<0, 1>), g0, block, 0, st, a, c, t, lp, d);
-        else hipLaunchKernelGGL((k_level_vertices_masked<0, 0>), g0, block, 0, st, a, c, t, lp, d);
         return;
     }
     if (fan == 2) {

@@ -248,23 +248,9 @@ __device__ __forceinline__ void mask_group(const uint64_t *mask, int32_t w0, int
     }
 }
 
-template <int GATHER, int CREASED>
-struct MaskedVertexFn {
-    const FusedArgs &a;
-    const DCage &c;
-    const LevelTab &t;
-    const LevelPtrs &lp;
-    int d;
-    __device__ void operator()(int32_t h) const {
-        Fan f;
-        const int32_t v = marked_fan(h, d, c, t, lp.he, f);
-        fan_vertex_store<GATHER, CREASED>(a, c, t, d, f, v);
-    }
-};
-
-// the same for the vertices whose ring follows from the child numbering;
-// the others go to the rare list (no general fan code here, so the kernel
-// keeps fewer registers and runs more waves)
+// A marked owner whose ring follows from the child numbering: its vertex
+// point; any other goes to the rare list (no general fan code here, so the
+// kernel keeps fewer registers and runs more waves).
 template <int GATHER, int CREASED>
 struct RingVertexFn {
     const FusedArgs &a;
@@ -287,7 +273,7 @@ struct RingVertexFn {
 
 // vertex points of the owners marked by the fused kernel; `wave` is the
 // index of this wave among the level's waves (kOwnerWords words each)
-template <int GATHER, int CREASED, class LP, int RING = 0>
+template <int GATHER, int CREASED, class LP>
 __device__ __forceinline__ void level_vertices_masked(const FusedArgs &a, const DCage &c, const LevelTab &t,
                                                       const LP &lp, int d, int32_t wave) {
     __shared__ LevelTab st;
@@ -296,11 +282,7 @@ __device__ __forceinline__ void level_vertices_masked(const FusedArgs &a, const 
     lds_fill(slp, lp);
     __syncthreads();
     const int32_t words = (a.H + 63) >> 6;
-    if (RING)
-        mask_group<kOwnerWords>(a.owner_mask, wave * kOwnerWords, words, RingVertexFn<GATHER, CREASED>{a, c, st, slp, d});
-    else
-        mask_group<kOwnerWords>(a.owner_mask, wave * kOwnerWords, words,
-                                MaskedVertexFn<GATHER, CREASED>{a, c, st, slp, d});
+    mask_group<kOwnerWords>(a.owner_mask, wave * kOwnerWords, words, RingVertexFn<GATHER, CREASED>{a, c, st, slp, d});
 }
 
 // the vertices the ring-only masked kernel listed, one thread each
