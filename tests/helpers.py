@@ -39,3 +39,21 @@ def point_error(p: np.ndarray, ref: np.ndarray) -> tuple[float, float]:
     scale = max(float(np.abs(ref).max()), 1e-30)
     d = np.asarray(p, np.float64) - ref
     return float(np.abs(d).max()) / scale, float(np.sqrt(np.mean(d * d))) / scale
+
+
+def assert_bits_equal(a: np.ndarray, b: np.ndarray, what: str, chunk: int = 1 << 26):
+    """Bitwise equality of two large arrays, compared as 32-bit words in
+    chunks (no full-size temporaries)."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    assert a.shape == b.shape and a.dtype == b.dtype, f"{what}: {a.shape}/{a.dtype} vs {b.shape}/{b.dtype}"
+    aw = a.reshape(-1).view(np.uint32)
+    bw = b.reshape(-1).view(np.uint32)
+    per = aw.size // max(len(a), 1)
+    for lo in range(0, aw.size, chunk):
+        x, y = aw[lo:lo + chunk], bw[lo:lo + chunk]
+        if not np.array_equal(x, y):
+            bad = lo + np.nonzero(x != y)[0]
+            rec = np.unique(bad // per)
+            raise AssertionError(f"{what}: records differ from {rec[:5].tolist()} "
+                                 f"({rec.size} in words {lo}..{lo + x.size}); first {a[rec[:3]]} vs {b[rec[:3]]}")

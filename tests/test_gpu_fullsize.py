@@ -1,0 +1,75 @@
+"""GPU parity at the workloads' full sizes, bit for bit against the oracle.
+
+- torus 64x32 (8,192 halfedges, two crease rings of sharpness 1.5 and 4.0,
+  grid UVs) refined to depth 8: 716 M halfedges over levels 1..8, 537 M of
+  them at the finest level;
+- the irregular cage (icosphere level 4, seeded triangle pairs merged into
+  quads, a cap deleted to open a boundary loop, 10% of the edges creased)
+  refined to depth 7: 247 M halfedges;
+- the batch of 256 uneven cages (24..60,000 halfedges, depths 4..6, 5.26 G
+  halfedges in one arena, so the later cages sit past 2^32 halfedges):
+  every cage of at most 2 M refined halfedges (124 of them, spread over the
+  whole arena) and the last cage are checked against the oracle.
+
+The oracle runs its gather loops on all host threads (up to 16); its scatter
+sums stay sequential, so its points equal the sequential reference's.  Sizes
+like these are where 32-bit index and byte-offset mistakes show, which the
+small cages of the other parity tests cannot reach.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from halfedgecatmullclark_amd import cage as C
+from halfedgecatmullclark_amd.abi import level_counts
+from oracle import oracle as O
+from tests.helpers import assert_bits_equal
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+
+CASES = [
+    ("torus64x32_d8_scatter", lambda: C.torus(64, 32), 8, 15, O.ALL),
+    ("torus64x32_d8_gather", lambda: C.torus(64, 32), 8, 15 | 16, O.ALL | O.GATHER),
+    ("irregular4_d7_scatter", lambda: C.irregular(4), 7, 15, O.ALL),
+    ("irregular4_d7_nocreases_gather", lambda: C.irregular(4), 7, 15 | 16 | 32, O.ALL | O.GATHER | O.NOCREASES),
+]
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from halfedgecatmullclark_amd import capi as api
+    api.lib()
+    return api
+
+
+@pytest.mark.parametrize("label,make,depth,steps,flags", CASES, ids=[c[0] for c in CASES])
+def test_full_size_bitexact(capi, label, make, depth, steps, flags):
+    cage = make()
+    ds = capi.DeviceSubd(cage, depth)
+    ds.refine(steps)
+    got = ds.download()
+    ds.close()
+    want = O.refine(cage, depth, flags, threads=THREADS)
+    for k in ("halfedges", "creases", "points"):
+        assert_bits_equal(got[k], want[k], f"{label} {k}")
+        got[k] = want[k] = None
+
+
+def test_batch_workload_bitexact(capi):
+    cages, depths = C.batch_cages(256, seed=0)
+    b = capi.Batch(cages, depths)
+    b.refine(capi.ALL)
+    b.synchronize()
+    hcum = [level_counts(c.V, c.H, c.E, c.F, d)["Hcum"][d] for c, d in zip(cages, depths)]
+    pick = [i for i, h in enumerate(hcum) if h <= 2_000_000] + [len(cages) - 1]
+    assert len(pick) > 100 and sum(hcum[:pick[-1]]) > 2 ** 32
+    for i in pick:
+        cage, depth = cages[i], depths[i]
+        got = b.download(i)
+        want = O.refine(cage, depth, O.ALL, threads=THREADS)
+        for k in ("halfedges", "creases", "points"):
+            assert_bits_equal(got[k], want[k], f"cage {i} (H={cage.H}, depth {depth}) {k}")
+    b.close()
