@@ -16,12 +16,12 @@ namespace cch {
 
 // face point of h's quad (lane 0 of the quad) and, for the larger halfedge
 // of each pair, the edge point
-// (g0..g3: the twin quad's points when it lies outside the block, s: the
-// sharpness of h's edge for an owner)
+// (g2, g3: the points of the twin quad's two corners away from the edge when
+// that quad lies outside the block, s: the sharpness of h's edge for an owner)
 template <int GATHER, int CREASED>
 __device__ __forceinline__ void level_fused_points(const FusedArgs &a, int32_t h, int j, int4 x, P3 P, P3 q0, P3 q1,
                                                    P3 q2, P3 q3, bool owner, bool twin_here, int32_t base,
-                                                   const P3 *spts, P3 g0, P3 g1, P3 g2, P3 g3, float s) {
+                                                   const P3 *spts, P3 g2, P3 g3, float s) {
     const int32_t twin = x.x;
     const P3 F = face_rule(q0, q1, q2, q3, GATHER);
     if (j == 0) a.fp[h >> 2] = F;
@@ -30,22 +30,23 @@ __device__ __forceinline__ void level_fused_points(const FusedArgs &a, int32_t h
     const P3 v1 = pick4(q0, q1, q2, q3, (j + 1) & 3);
     P3 Ev;
     if (twin >= 0) {
-        // the twin quad's points from LDS when it is in this block, else the
-        // ones loaded from memory; the LDS reads are valid for every lane, and
-        // selecting afterwards keeps LDS and global reads in their own
-        // instructions
+        // The twin quad's corners, from the twin on: the twin's vertex is the
+        // head of h (v1), the twin's next vertex is h's own (P), and the other
+        // two come from LDS when the quad is in this block, else from the
+        // loads above.  The LDS reads are valid for every lane; selecting
+        // afterwards keeps LDS and global reads in their own instructions.
+        const int tj = twin & 3;
         const int tb = twin_here ? (twin & ~3) - base : 0;
-        const P3 l0 = sld(spts + tb), l1 = sld(spts + tb + 1), l2 = sld(spts + tb + 2), l3 = sld(spts + tb + 3);
-        const P3 t0 = twin_here ? l0 : g0, t1 = twin_here ? l1 : g1, t2 = twin_here ? l2 : g2,
-                 t3 = twin_here ? l3 : g3;
-        const P3 Ft = face_rule(t0, t1, t2, t3, GATHER);
+        const P3 l2 = sld(spts + tb + ((tj + 2) & 3)), l3 = sld(spts + tb + ((tj + 3) & 3));
+        const P3 o2 = twin_here ? l2 : g2, o3 = twin_here ? l3 : g3;
+        // corner k of the twin quad is corner (k - tj) & 3 counted from the twin
+        const P3 Ft = face_rule(pick4(v1, P, o2, o3, (0 - tj) & 3), pick4(v1, P, o2, o3, (1 - tj) & 3),
+                                pick4(v1, P, o2, o3, (2 - tj) & 3), pick4(v1, P, o2, o3, (3 - tj) & 3), GATHER);
         if (GATHER) {
             const float w = CREASED ? sat(s) : 1.0f;
             Ev = edge_gather(P, v1, F, Ft, w, CREASED);
         } else {
-            const int tj = twin & 3;
-            const P3 ct = edge_contrib(pick4(t0, t1, t2, t3, tj), pick4(t0, t1, t2, t3, (tj + 1) & 3), Ft, false, s,
-                                       CREASED);
+            const P3 ct = edge_contrib(v1, P, Ft, false, s, CREASED);
             const P3 ch = edge_contrib(P, v1, F, false, s, CREASED);
             Ev = add(add(zero3(), ct), ch);
         }
@@ -82,14 +83,14 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
     const bool twin_far = owner && twin >= 0 && !twin_here;
 
     // the loads of the point work come first: this halfedge's vertex point,
-    // its edge's sharpness and, for a twin quad outside the block, that
-    // quad's vertex IDs and then its points, so that the latency of the
-    // second gather overlaps the topology stores below.  The vertex point
-    // goes to LDS between the two gathers: the wait for the IDs (forced
-    // inside their branch by the empty asm that reads them) covers it, and
-    // no later wait has to count the twin-quad point loads still in flight
-    // (memory loads complete in order).
-    P3 P = zero3(), g0 = zero3(), g1 = zero3(), g2 = zero3(), g3 = zero3();
+    // its edge's sharpness and, for a twin quad outside the block, the
+    // vertex IDs of its two corners away from the shared edge and then their
+    // points, so that the latency of the second gather overlaps the topology
+    // stores below.  The vertex point goes to LDS between the two gathers:
+    // the wait for the IDs (forced inside their branch by the empty asm that
+    // reads them) covers it, and no later wait has to count the twin-quad
+    // point loads still in flight (memory loads complete in order).
+    P3 P = zero3(), g2 = zero3(), g3 = zero3();
     float s = 0.0f;
     if (POINTS) {
         P = a.old[x.z];   // a dead lane's record is all zeros: vertex 0
@@ -98,15 +99,15 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
             const float cs = a.cr[has ? x.y : 0].sharp;
             s = has ? cs : 0.0f;
         }
-        int32_t v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+        int32_t v2 = 0, v3 = 0;
         if (twin_far) {
-            const int32_t tq = twin & ~3;
-            v0 = he4[tq].z; v1 = he4[tq + 1].z; v2 = he4[tq + 2].z; v3 = he4[tq + 3].z;
-            asm volatile("" ::"v"(v0), "v"(v1), "v"(v2), "v"(v3));
+            const int32_t tq = twin & ~3, tj = twin & 3;
+            v2 = he4[tq + ((tj + 2) & 3)].z; v3 = he4[tq + ((tj + 3) & 3)].z;
+            asm volatile("" ::"v"(v2), "v"(v3));
         }
         spts[threadIdx.x] = P;
         if (twin_far) {
-            g0 = gld(a.old + v0); g1 = gld(a.old + v1); g2 = gld(a.old + v2); g3 = gld(a.old + v3);
+            g2 = gld(a.old + v2); g3 = gld(a.old + v3);
         }
     }
 
@@ -164,8 +165,7 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
     if (!POINTS) return;
     if (!kStaged) __syncthreads();
     const P3 q0 = quad_bcast3<0>(P), q1 = quad_bcast3<1>(P), q2 = quad_bcast3<2>(P), q3 = quad_bcast3<3>(P);
-    if (live) level_fused_points<GATHER, CREASED>(a, h, j, x, P, q0, q1, q2, q3, owner, twin_here, base, spts, g0, g1,
-                                                   g2, g3, s);
+    if (live) level_fused_points<GATHER, CREASED>(a, h, j, x, P, q0, q1, q2, q3, owner, twin_here, base, spts, g2, g3, s);
     if (a.owner_mask) {
         // one bit per halfedge: is it the representative of its vertex?  Last,
         // so that the cage lookups of a few lanes overlap the wave's stores
