@@ -227,7 +227,11 @@ def main():
     ds.refine(capi.ALL, sptr)
     ds.autotune(capi.ALL, sptr)
     level_cfg = ds.level_options()
-    ms_scatter, kern = timed(capi.ALL, args.steps, args.warmup, per_kernel=True)
+    # the step time with nothing between the launches; then the same steps
+    # again with a HIP event pair around every launch (read back after each
+    # step) for the per-kernel durations
+    ms_scatter, _ = timed(capi.ALL, args.steps, args.warmup)
+    ms_events, kern = timed(capi.ALL, args.steps, 1, per_kernel=True)
     ms_gather, _ = timed(capi.ALL | capi.GATHER, max(3, args.steps // 2), 1)
 
     # dominant kernel: largest mean duration among the launches of one step;
@@ -311,6 +315,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_scatter,
+        "ms_per_step_with_kernel_events": ms_events,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
