@@ -72,6 +72,14 @@ __device__ __forceinline__ T pick4(T a0, T a1, T a2, T a3, int k) {
     return k == 0 ? a0 : (k == 1 ? a1 : (k == 2 ? a2 : a3));
 }
 
+// Trailing zero base-4 digits of h >= 0 (children 0 of children 0 ...), at
+// most d - 1: h = q << 2k with the vertex of h born at level d - k.
+__device__ __forceinline__ int zero_digits(int32_t h, int d) {
+    const int cap = d > 1 ? d - 1 : 0;
+    const int z = h ? (__builtin_ctz((uint32_t)h) >> 1) : cap;
+    return z < cap ? z : cap;
+}
+
 __device__ __forceinline__ float lsharp(const Crease *cr, int32_t C, int32_t e) {
     return e < C ? cr[e].sharp : 0.0f;
 }
@@ -245,8 +253,7 @@ __device__ __forceinline__ void cage_face_fan(const DCage &c, int32_t face, int3
 // Is h the representative of its vertex (CatmullClark.h:1336)?  The test of
 // owner_fan without building the fan (no twin record is read).
 __device__ __forceinline__ bool owner_test(int32_t h, int32_t vert, int d, const DCage &c, const LevelTab &t) {
-    int k = 0;
-    while (k < d - 1 && ((h >> (2 * k)) & 3) == 0) ++k;
+    const int k = zero_digits(h, d);
     const int32_t q = h >> (2 * k);
     const int lvl = d - k;
     const int digit = q & 3;
@@ -261,8 +268,7 @@ __device__ __forceinline__ bool owner_test(int32_t h, int32_t vert, int d, const
 // Is h the representative of its vertex?  If so, describe the fan.
 __device__ __forceinline__ bool owner_fan(int32_t h, int32_t vert, int d, const DCage &c, const LevelTab &t,
                                           const He *const *levels, Fan &f) {
-    int k = 0;
-    while (k < d - 1 && ((h >> (2 * k)) & 3) == 0) ++k;
+    const int k = zero_digits(h, d);
     const int32_t q = h >> (2 * k);
     const int lvl = d - k;            // level at which the vertex appeared
     const int digit = q & 3;
@@ -332,8 +338,7 @@ __device__ __forceinline__ bool owner_fan(int32_t h, int32_t vert, int d, const 
 // cage maps say.
 __device__ __forceinline__ bool structural_owner(int32_t h, int d, const DCage &c, const LevelTab &t,
                                                  const He *const *levels) {
-    int k = 0;
-    while (k < d - 1 && ((h >> (2 * k)) & 3) == 0) ++k;
+    const int k = zero_digits(h, d);
     const int32_t q = h >> (2 * k);
     const int lvl = d - k;
     const int digit = q & 3;
@@ -355,8 +360,7 @@ __device__ __forceinline__ bool structural_owner(int32_t h, int d, const DCage &
 // any other vertex (cage vertices and faces, boundary edges).
 __device__ __forceinline__ bool marked_ring_fan(int32_t h, int d, const DCage &c, const LevelTab &t,
                                                 const He *const *levels, Fan &f, int32_t &v) {
-    int k = 0;
-    while (k < d - 1 && ((h >> (2 * k)) & 3) == 0) ++k;
+    const int k = zero_digits(h, d);
     const int32_t q = h >> (2 * k);
     const int lvl = d - k;
     const int digit = q & 3;
@@ -402,8 +406,7 @@ __device__ __forceinline__ bool marked_ring_fan(int32_t h, int d, const DCage &c
 // cage-level vertices read h's record and go through owner_fan.
 __device__ __forceinline__ int32_t marked_fan(int32_t h, int d, const DCage &c, const LevelTab &t,
                                               const He *const *levels, Fan &f) {
-    int k = 0;
-    while (k < d - 1 && ((h >> (2 * k)) & 3) == 0) ++k;
+    const int k = zero_digits(h, d);
     const int32_t q = h >> (2 * k);
     const int lvl = d - k;
     const int digit = q & 3;
@@ -749,13 +752,16 @@ __device__ __forceinline__ bool structural_ring(const Fan &f, const LevelTab &t,
         tw[2] = 4 * nw + 2;
         tw[3] = 4 * w;
     }
-    for (int j = lvl; j < d; ++j) {
+    // Down to level d through children 0: child 0 of x has edge
+    // 2 edge(x) + (x > twin(x) ? 0 : 1) and twin 4 next(twin(x)) + 3.  x and
+    // its twin lie in different quads, so 4x and 4 next(twin(x)) + 3 keep
+    // their order (and again lie in different quads): the same edge bit
+    // comes n times.
+    const int n = d - lvl;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            e[q] = 2 * e[q] + (h[q] > tw[q] ? 0 : 1);
-            tw[q] = 4 * qnext(tw[q]) + 3;
-            h[q] = 4 * h[q];
-        }
+    for (int q = 0; q < 4; ++q) {
+        e[q] = (e[q] << n) + (h[q] > tw[q] ? 0 : (1 << n) - 1);
+        h[q] <<= 2 * n;
     }
     return true;
 }

@@ -39,9 +39,11 @@ __device__ __forceinline__ void level_fused_points(const FusedArgs &a, int32_t h
         const int tb = twin_here ? (twin & ~3) - base : 0;
         const P3 l2 = sld(spts + tb + ((tj + 2) & 3)), l3 = sld(spts + tb + ((tj + 3) & 3));
         const P3 o2 = twin_here ? l2 : g2, o3 = twin_here ? l3 : g3;
-        // corner k of the twin quad is corner (k - tj) & 3 counted from the twin
-        const P3 Ft = face_rule(pick4(v1, P, o2, o3, (0 - tj) & 3), pick4(v1, P, o2, o3, (1 - tj) & 3),
-                                pick4(v1, P, o2, o3, (2 - tj) & 3), pick4(v1, P, o2, o3, (3 - tj) & 3), GATHER);
+        // corner k of the twin quad is corner (k - tj) & 3 counted from the
+        // twin: rotate by tj in two steps (by 1, then by 2)
+        const bool r1 = (tj & 1) != 0, r2 = (tj & 2) != 0;
+        const P3 y0 = r1 ? o3 : v1, y1 = r1 ? v1 : P, y2 = r1 ? P : o2, y3 = r1 ? o2 : o3;
+        const P3 Ft = face_rule(r2 ? y2 : y0, r2 ? y3 : y1, r2 ? y0 : y2, r2 ? y1 : y3, GATHER);
         if (GATHER) {
             const float w = CREASED ? sat(s) : 1.0f;
             Ev = edge_gather(P, v1, F, Ft, w, CREASED);
@@ -218,34 +220,28 @@ __device__ __forceinline__ void level_vertices_compact_block(const FusedArgs &a,
 constexpr int kOwnerWords = 4;
 
 // Calls fn(h) once per bit set in mask words [w0, w0 + GW) of `words`, one
-// halfedge per lane: popcounts prefix-summed across lanes 0..GW-1, lane l
-// takes the l-th marked halfedge, in rounds of 64.
+// halfedge per lane in ascending order, in rounds of 64.  The wave lists the
+// marked halfedges in `list` (its own 64 GW slots of LDS): lane l of word w
+// writes w * 64 + l at the count of set bits before it (mbcnt).
 template <int GW, class Fn>
-__device__ __forceinline__ void mask_group(const uint64_t *mask, int32_t w0, int32_t words, const Fn &fn) {
+__device__ __forceinline__ void mask_group(const uint64_t *mask, int32_t w0, int32_t words, int32_t *list,
+                                           const Fn &fn) {
     const int lane = (int)(threadIdx.x & 63u);
-    const int32_t w = w0 + lane;
-    const uint64_t m = (lane < GW && w < words) ? mask[w] : 0ull;
-    int32_t pre = __popcll(m);
+    int32_t total = 0;
 #pragma unroll
-    for (int off = 1; off < GW; off <<= 1) {
-        const int32_t y = __shfl_up(pre, off);
-        if (lane >= off) pre += y;
+    for (int j = 0; j < GW; ++j) {
+        const int32_t w = w0 + j;
+        const uint64_t m = w < words ? mask[w] : 0ull;   // the same word in every lane
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)m);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(m >> 32));
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+        if (((lane < 32 ? lo >> lane : hi >> (lane - 32)) & 1u) != 0u) list[total + (int32_t)rank] = w * 64 + lane;
+        total += __popc(lo) + __popc(hi);
     }
-    const int32_t total = __shfl(pre, GW - 1);
-    for (int32_t base = 0; base < total; base += 64) {
-        const int32_t e = base + lane;
-        int32_t s = 0;
-#pragma unroll
-        for (int j = 0; j < GW; ++j) s += __shfl(pre, j) <= e ? 1 : 0;
-        const int src = s < GW ? s : GW - 1;
-        const int32_t before = __shfl(pre, s > 0 ? s - 1 : 0);
-        uint64_t ms = __shfl(m, src);
-        if (e < total) {
-            const int32_t r = e - (s > 0 ? before : 0);
-            for (int32_t k = 0; k < r; ++k) ms &= ms - 1ull;
-            fn((w0 + s) * 64 + (__ffsll((unsigned long long)ms) - 1));
-        }
-    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int32_t i = lane; i < total; i += 64) fn(list[i]);
 }
 
 // A marked owner whose ring follows from the child numbering: its vertex
@@ -278,11 +274,13 @@ __device__ __forceinline__ void level_vertices_masked(const FusedArgs &a, const 
                                                       const LP &lp, int d, int32_t wave) {
     __shared__ LevelTab st;
     __shared__ LevelPtrs slp;
+    __shared__ int32_t list[4][64 * kOwnerWords];   // four waves per workgroup
     lds_fill(st, t);
     lds_fill(slp, lp);
     __syncthreads();
     const int32_t words = (a.H + 63) >> 6;
-    mask_group<kOwnerWords>(a.owner_mask, wave * kOwnerWords, words, RingVertexFn<GATHER, CREASED>{a, c, st, slp, d});
+    mask_group<kOwnerWords>(a.owner_mask, wave * kOwnerWords, words, list[threadIdx.x >> 6],
+                            RingVertexFn<GATHER, CREASED>{a, c, st, slp, d});
 }
 
 // the vertices the ring-only masked kernel listed, one thread each

@@ -133,3 +133,79 @@ def test_structural_fans(name, cage, depth):
             assert all(int(lv["vertexID"][x]) == v for x in fan)
             assert O.vertex_to_halfedge(cage, v, d) == h
         assert len(owners) == counts["V"][d], (d, len(owners), counts["V"][d])
+
+
+def birth(h, d):
+    """(k, lvl, digit, p): h = ((4 p + digit) << 2k), the vertex born at lvl = d - k
+    (k from the count of trailing zero base-4 digits, capped at d - 1)."""
+    k = min(((h & -h).bit_length() - 1) >> 1, d - 1) if h else d - 1
+    q = h >> (2 * k)
+    return k, d - k, q & 3, q >> 2
+
+
+def closed_ring(h, d, cage, counts, levels):
+    """Mirror of marked_ring_fan + structural_ring in cc_fan.h: the ring
+    (halfedges and their edges) of the vertex represented by level-d
+    halfedge h, for face vertices born at level >= 2 and interior edge
+    vertices, with the child-0 descent d - lvl levels down in closed form;
+    None for any other vertex."""
+    k, lvl, digit, p = birth(h, d)
+    Ep2 = 2 * counts["E"][lvl - 1]
+    if digit == 2 and lvl >= 2:
+        v = counts["V"][lvl - 1] + (p >> 2)
+        hs = [4 * (p + q) + 2 for q in range(4)]
+        es = [Ep2 + p + ((q + 3) & 3) for q in range(4)]
+        tws = [4 * (p + ((q + 3) & 3)) + 1 for q in range(4)]
+    elif digit == 1:
+        if lvl >= 2:
+            rec = levels[lvl - 1][p]
+            tw, e = int(rec["twinID"]), int(rec["edgeID"])
+            np_, nw = qnext(p), (qnext(tw) if tw >= 0 else -1)
+        else:
+            he = cage.halfedges
+            tw, e, np_ = int(he[p, 0]), int(he[p, 4]), int(he[p, 1])
+            nw = int(he[tw, 1]) if tw >= 0 else -1
+        if tw < 0:
+            return None
+        v = counts["V"][lvl - 1] + counts["F"][lvl - 1] + e
+        hs = [4 * p + 1, 4 * nw + 3, 4 * tw + 1, 4 * np_ + 3]
+        es = [Ep2 + p, 2 * e + (1 if tw > p else 0), Ep2 + tw, 2 * e + (1 if p > tw else 0)]
+        tws = [4 * np_ + 2, 4 * p, 4 * nw + 2, 4 * tw]
+    else:
+        return None
+    n = d - lvl
+    # h and its twin lie in different quads, so children 0 of both keep their
+    # order at every level below: the edge bit is the same n times
+    es = [(e << n) + (((1 << n) - 1) if x < t else 0) for e, x, t in zip(es, hs, tws)]
+    hs = [x << (2 * n) for x in hs]
+    return v, hs, es
+
+
+@pytest.mark.parametrize("name,cage,depth", small_cages()[:6], ids=[c[0] for c in small_cages()[:6]])
+def test_closed_form_rings(name, cage, depth):
+    """Every ring the closed form gives is the vertex's ring in the oracle's
+    topology: each halfedge carries the vertex, its edge is e[q], and the
+    edge of its previous halfedge is e[q + 1]."""
+    depth = min(depth, 4)
+    out = O.refine(cage, depth, O.HALFEDGES)
+    counts = out["counts"]
+    levels = {d: O.level(out, "halfedges", d) for d in range(1, depth + 1)}
+    for d in range(1, depth + 1):
+        lv = levels[d]
+        seen = 0
+        for h in range(counts["H"][d]):
+            v = int(lv["vertexID"][h])
+            if O.vertex_to_halfedge(cage, v, d) != h:
+                continue
+            r = closed_ring(h, d, cage, counts, levels)
+            if r is None:
+                continue
+            cv, hs, es = r
+            assert cv == v and hs[0] == h, (d, h, cv, v, hs)
+            for q in range(4):
+                x = hs[q]
+                assert int(lv["vertexID"][x]) == v, (d, h, q)
+                assert int(lv["edgeID"][x]) == es[q], (d, h, q, int(lv["edgeID"][x]), es[q])
+                assert int(lv["edgeID"][qprev(x)]) == es[(q + 1) & 3], (d, h, q)
+            seen += 1
+        assert seen > 0 or d == 1
