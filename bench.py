@@ -42,13 +42,22 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "refined halfedges/s + HBM GB/s at depth 8; fwd+adjoint and batched vs plain"
 
 
-def level_bytes(n, d):
+def level_bytes(n, d, D=None, owner_bits=False):
     """Algorithmic bytes of the fused level kernel d -> d+1: each input read
     once and each output written once.  Reads: the level-d halfedge records
     (16 B), level-d points (12 B), level-d creases (12 B); writes: the four
     16-B child records per parent halfedge and the level-(d+1) face and edge
-    points (12 B)."""
-    return 16 * n["H"][d] + 12 * n["V"][d] + 12 * n["C"][d] + 64 * n["H"][d] + 12 * (n["F"][d] + n["E"][d])
+    points (12 B).  With owner_bits (the masked vertex kernel follows) it
+    also writes one owner bit per halfedge and reads the parent level's
+    edge-representative bits; on levels d + 1 <= D - 2 it reads level d's
+    bits and writes level d+1's (one bit per halfedge each)."""
+    b = 16 * n["H"][d] + 12 * n["V"][d] + 12 * n["C"][d] + 64 * n["H"][d] + 12 * (n["F"][d] + n["E"][d])
+    if D is not None and D >= 4:
+        if owner_bits:
+            b += n["H"][d] // 8 + (n["H"][d - 1] // 8 if d >= 2 else 0)
+        if d + 1 <= D - 2:
+            b += n["H"][d] // 8 + n["H"][d + 1] // 8
+    return b
 
 
 def vertex_kernel_bytes(n, d):
@@ -246,7 +255,7 @@ def main():
     durs = kern[(idx, name)]
     mean_ms = sum(durs) / len(durs)
     if name == "level_fused":
-        nbytes = level_bytes(n, level)
+        nbytes = level_bytes(n, level, D, owner_bits=level_cfg[level - 1]["scan"] == 0)
     elif name == "level_vertices":
         nbytes = vertex_kernel_bytes(n, level)
     elif name == "level_patch":

@@ -101,6 +101,8 @@ __host__ __device__ __forceinline__ void unpack_level(const LevelJob &J, char *a
     a.owner_mask = arena_ptr<uint64_t>(arena, J.owner_mask);
     a.rare = arena_ptr<int32_t>(arena, J.rare);
     a.rare_cap = J.rare_cap;
+    a.erep = nullptr;
+    a.erep_read = a.erep_write = 0;
     a.H = J.H; a.V = J.V; a.F = J.F; a.E = J.E; a.C = J.C;
     a.xcd_remap = J.xcd_remap;
     a.uv_refine = J.uv_refine;

@@ -61,6 +61,10 @@ struct Counts {
 // edge/vertex maps.
 struct LevelTab {
     int32_t E[kMaxDepth + 1], V[kMaxDepth + 1], F[kMaxDepth + 1];
+    // word offset of level L's edge-representative bits (one bit per level-L
+    // halfedge, levels 1 ..) in a mesh's bit array: the words of the levels
+    // before it, ceil(H / 64) each
+    int32_t R[kMaxDepth + 1];
 };
 
 // Upper bound on the steps of any face or vertex walk.  Valid meshes never

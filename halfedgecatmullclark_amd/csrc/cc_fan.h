@@ -252,7 +252,8 @@ __device__ __forceinline__ void cage_face_fan(const DCage &c, int32_t face, int3
 
 // Is h the representative of its vertex (CatmullClark.h:1336)?  The test of
 // owner_fan without building the fan (no twin record is read).
-__device__ __forceinline__ bool owner_test(int32_t h, int32_t vert, int d, const DCage &c, const LevelTab &t) {
+__device__ __forceinline__ bool owner_test(int32_t h, int32_t vert, int d, const DCage &c, const LevelTab &t,
+                                           const uint64_t *erep = nullptr) {
     const int k = zero_digits(h, d);
     const int32_t q = h >> (2 * k);
     const int lvl = d - k;
@@ -262,7 +263,19 @@ __device__ __forceinline__ bool owner_test(int32_t h, int32_t vert, int d, const
     if (digit == 3) return false;
     if (digit == 2) return lvl - 1 >= 1 ? (p & 3) == 0 : c.f2h[c.he[p].face] == p;
     const int32_t e = vert - t.V[lvl - 1] - t.F[lvl - 1];
+    if (lvl - 1 >= 1 && erep) return ((erep[t.R[lvl - 1] + (p >> 6)] >> (p & 63)) & 1ull) != 0ull;
     return lvl - 1 >= 1 ? is_edge_rep(c, t, e, lvl - 1, p) : c.e2h[e] == p;
+}
+
+// OR of v over the 16 lanes of this lane's DPP row, in the row's last lane
+// (an inclusive OR scan by row_shr 1, 2, 4, 8; lanes before the row start
+// read 0)
+__device__ __forceinline__ uint32_t row_or(uint32_t v) {
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true);
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, true);
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, true);
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, true);
+    return v;
 }
 
 // Is h the representative of its vertex?  If so, describe the fan.

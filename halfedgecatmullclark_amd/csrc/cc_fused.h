@@ -38,6 +38,15 @@ struct FusedArgs {
     // kernel then finishes them
     int32_t *rare;
     int32_t rare_cap;
+    // Edge-representative bits of levels 1 .. (word offsets LevelTab::R):
+    // bit x of level L is set when halfedge x is the reference's
+    // representative of its edge (ccs_EdgeToHalfedgeID, CatmullClark.h:1206).
+    // erep_write: this launch derives level d+1's bits from level d's (and
+    // writes level 1's when d = 1); erep_read: levels 1 .. d-1 hold valid
+    // bits, so the owner test reads them instead of descending the edge's
+    // ancestry.
+    uint64_t *erep = nullptr;
+    int erep_read = 0, erep_write = 0;
 };
 
 // topo: write child twin/edge/vertex IDs; uvMode: 0 leave uvIDs, 1 refine

@@ -114,6 +114,12 @@ inline LevelTab level_tab(const Counts &n) {
         t.V[d] = (int32_t)n.V[d];
         t.F[d] = (int32_t)n.F[d];
     }
+    int64_t words = 0;   // levels past the addressable range are never used
+    t.R[0] = 0;
+    for (int d = 1; d <= kMaxDepth; ++d) {
+        t.R[d] = (int32_t)std::min<int64_t>(words, INT32_MAX);
+        words += (n.H[d] + 63) / 64;
+    }
     return t;
 }
 

@@ -78,6 +78,8 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
     const int j = h & 3;
     const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
     const int4 x = live ? he4[h] : make_int4(0, 0, 0, 0);
+    // this level's edge-representative bits of the 64 halfedges of the wave
+    const uint64_t rw = (a.erep_write && d >= 2 && live) ? a.erep[tb.R[d] + (h >> 6)] : 0ull;
     const int32_t twin = x.x;
     const bool owner = live && h > twin;   // this halfedge writes its edge point
     // twin quads inside this block come from LDS, the others from memory
@@ -164,6 +166,36 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
         }
     }
 
+    if (a.erep_write) {
+        // Edge-representative bits of the four children from this
+        // halfedge's and its quad predecessor's (CatmullClark.h:1206-1262
+        // through the child numbering): child 0 when h > twin(h) and h
+        // represents its edge, child 1 when h is the quad's last halfedge,
+        // child 2 unless it is the first, child 3 when prev(h) > twin(prev(h))
+        // and prev(h) represents its edge.  Level 1 takes its own bits from
+        // the cage maps.
+        bool rh, rp;
+        if (d == 1) {
+            rh = live && edge_rep_first(cg, x.y) == h;
+            rp = quad_rel<kQuadPrev>(rh ? 1 : 0) != 0;
+            const uint64_t m1 = __ballot(rh);
+            if ((threadIdx.x & 63u) == 0 && live) a.erep[tb.R[1] + (h >> 6)] = m1;
+        } else {
+            rh = ((rw >> (h & 63)) & 1ull) != 0ull;
+            rp = ((rw >> (qprev(h) & 63)) & 1ull) != 0ull;
+        }
+        const uint32_t nib = live ? ((h > x.x && rh ? 1u : 0u) | ((h & 3) == 3 ? 2u : 0u) | ((h & 3) != 0 ? 4u : 0u) |
+                                     (qprev(h) > p.x && rp ? 8u : 0u))
+                                  : 0u;
+        // children 4h .. 4h+3 are bits 4 (h & 15) .. + 3 of word h >> 4: a
+        // DPP row of 16 lanes fills one word
+        const int lane = (int)(threadIdx.x & 63u);
+        const uint32_t sh = 4u * (uint32_t)(lane & 7);
+        const uint32_t lo = row_or((lane & 8) ? 0u : nib << sh);
+        const uint32_t hi = row_or((lane & 8) ? nib << sh : 0u);
+        if ((lane & 15) == 15 && (h & ~15) < a.H) a.erep[tb.R[d + 1] + (h >> 4)] = ((uint64_t)hi << 32) | lo;
+    }
+
     if (!POINTS) return;
     if (!kStaged) __syncthreads();
     const P3 q0 = quad_bcast3<0>(P), q1 = quad_bcast3<1>(P), q2 = quad_bcast3<2>(P), q3 = quad_bcast3<3>(P);
@@ -171,7 +203,7 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
     if (a.owner_mask) {
         // one bit per halfedge: is it the representative of its vertex?  Last,
         // so that the cage lookups of a few lanes overlap the wave's stores
-        const uint64_t m = __ballot(live && owner_test(h, x.z, d, cg, stab));
+        const uint64_t m = __ballot(live && owner_test(h, x.z, d, cg, stab, a.erep_read ? a.erep : nullptr));
         if ((threadIdx.x & 63u) == 0 && live) a.owner_mask[h >> 6] = m;
     }
 }
@@ -202,7 +234,7 @@ __device__ __forceinline__ void level_vertices_compact_block(const FusedArgs &a,
 #pragma unroll
     for (int k = 0; k < SCAN; ++k) {
         const int32_t h = base + k * 256 + (int32_t)threadIdx.x;
-        if (vs[k] >= 0 && owner_test(h, vs[k], d, c, st)) owners[atomicAdd(&nown, 1)] = h;
+        if (vs[k] >= 0 && owner_test(h, vs[k], d, c, st, a.erep_read ? a.erep : nullptr)) owners[atomicAdd(&nown, 1)] = h;
     }
     __syncthreads();
     const int32_t count = nown;

@@ -47,6 +47,11 @@ struct ccd_Subd {
          *adj_sharp = nullptr;
     // deferred-vertex masks of the patch kernels, one word per 64 halfedges
     uint64_t *defer_mask = nullptr;
+    // edge-representative bits of levels 1..D-2 (FusedArgs::erep; the owner
+    // test of level d reads levels up to d-1), and which levels hold bits of
+    // the current cage's topology
+    uint64_t *erep = nullptr;
+    bool erep_valid[cch::kMaxDepth + 1] = {};
     size_t adj_scalar = 0;
     bool timing = false;
     int opt_fused = 1, opt_store = cch::kStoreLds, opt_remap = 1, opt_fan = 2;

@@ -82,13 +82,14 @@ bool ccd_Subd::alloc(const cc_Mesh *cage, int32_t depth) {
     CC_TRY(hipMalloc(&pts, sizeof(P3) * (size_t)n.cumV(D)), "hipMalloc(points)");
     CC_TRY(hipMalloc(&defer_mask, sizeof(uint64_t) * (size_t)(D >= 2 ? (n.H[D - 1] + 63) / 64 : 1)),
            "hipMalloc(deferred-vertex masks)");
+    if (D >= 4) CC_TRY(hipMalloc(&erep, sizeof(uint64_t) * (size_t)tab.R[D - 1]), "hipMalloc(edge representative bits)");
     return true;
 }
 
 void ccd_Subd::free_all() {
     DeviceGuard g(device);
     void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts, adj_a, adj_b, adj_w, adj_in, adj_out, adj_sharp,
-                    defer_mask, rings, rare};
+                    defer_mask, rings, rare, erep};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : timed) {
@@ -105,6 +106,7 @@ bool ccd_Subd::upload_cage(const cc_Mesh *cage, hipStream_t st) {
         set_error(10, "cage counts differ from the ones the device mirror was created for");
         return false;
     }
+    for (bool &v : erep_valid) v = false;   // bits of the previous topology
     CC_TRY(hipMemcpyAsync(c_he, cage->halfedges, sizeof(CageHe) * (size_t)H0, hipMemcpyHostToDevice, st), "upload cage");
     CC_TRY(hipMemcpyAsync(c_v2h, cage->vertexToHalfedgeIDs, sizeof(int32_t) * (size_t)V0, hipMemcpyHostToDevice, st), "upload cage");
     CC_TRY(hipMemcpyAsync(c_e2h, cage->edgeToHalfedgeIDs, sizeof(int32_t) * (size_t)E0, hipMemcpyHostToDevice, st), "upload cage");
@@ -249,6 +251,8 @@ FusedArgs ccd_Subd::fused_args(int d, const P3 *points) const {
     a.owner_mask = nullptr;
     a.rare = nullptr;
     a.rare_cap = 0;
+    a.erep = nullptr;
+    a.erep_read = a.erep_write = 0;
     return a;
 }
 
@@ -309,9 +313,23 @@ void ccd_Subd::level_fused(int d, int32_t steps, int topo, int uvMode, P3 *point
         a.rare = rare;
         a.rare_cap = rare_cap;
     }
-    if (topo || uvMode == 1 || pts_)
+    // edge-representative bits: derive level d+1's while they are still
+    // needed (levels up to D-2), read levels 1..d-1 in the owner test
+    if (erep) {
+        bool below = true;
+        for (int l = 1; l < d; ++l) below = below && erep_valid[l];
+        a.erep = erep;
+        a.erep_read = d >= 2 && below ? 1 : 0;
+        a.erep_write = d + 1 <= D - 2 && (d == 1 || erep_valid[d]) ? 1 : 0;
+    }
+    if (topo || uvMode == 1 || pts_) {
         launch_timed(pts_ ? (topo ? "level_fused" : "level_points") : "level_fused", n.H[d], st,
                      [&] { launch_level_fused(a, c, tab, d, gather, creased, topo, uvMode, pts_, opt_store, st); });
+        if (a.erep_write) {
+            erep_valid[d + 1] = true;
+            if (d == 1) erep_valid[1] = true;
+        }
+    }
     if (pts_) {
         launch_timed("level_vertices", n.H[d], st, [&] {
             launch_level_vertices(a, c, tab, level_ptrs(), d, gather, creased, opt_fan, cfg[d].scan, st);

@@ -209,3 +209,51 @@ def test_closed_form_rings(name, cage, depth):
                 assert int(lv["edgeID"][qprev(x)]) == es[(q + 1) & 3], (d, h, q)
             seen += 1
         assert seen > 0 or d == 1
+
+
+def edge_rep_bits(cage, counts, levels, depth):
+    """Mirror of the edge-representative bits the fused kernel carries from
+    level to level: R_L[x] is true when level-L halfedge x is the
+    reference's representative of its edge (ccs_EdgeToHalfedgeID,
+    CatmullClark.h:1206-1262).  Level 1 from the cage (e2h, next, prev,
+    twin); level L + 1 from level L: child 0 of p when p > twin(p) and
+    R_L[p]; child 1 when p is the quad's last halfedge; child 2 unless it is
+    the first; child 3 when prev(p) > twin(prev(p)) and R_L[prev(p)]."""
+    he = cage.halfedges
+    twin_c, next_c, prev_c, edge_c = he[:, 0], he[:, 1], he[:, 2], he[:, 4]
+    p = np.arange(cage.H)
+    r0 = cage.e2h[edge_c] == p
+    R = {1: np.zeros(4 * cage.H, bool)}
+    R[1][0::4] = (p > twin_c) & r0
+    R[1][1::4] = p > next_c
+    R[1][2::4] = p > prev_c
+    R[1][3::4] = (prev_c > twin_c[prev_c]) & r0[prev_c]
+    for L in range(1, depth):
+        H = counts["H"][L]
+        p = np.arange(H)
+        tw = levels[L]["twinID"]
+        pp = (p & ~3) | ((p - 1) & 3)
+        nxt = np.zeros(4 * H, bool)
+        nxt[0::4] = (p > tw) & R[L]
+        nxt[1::4] = (p & 3) == 3
+        nxt[2::4] = (p & 3) != 0
+        nxt[3::4] = (pp > tw[pp]) & R[L][pp]
+        R[L + 1] = nxt
+    return R
+
+
+@pytest.mark.parametrize("name,cage,depth", small_cages(), ids=[c[0] for c in small_cages()])
+def test_edge_rep_bits(name, cage, depth):
+    """The carried bits mark exactly the reference's edge representatives
+    at every level (one per edge, and it is a halfedge of that edge)."""
+    depth = min(depth, 4)
+    out = O.refine(cage, depth, O.HALFEDGES)
+    counts = out["counts"]
+    levels = {d: O.level(out, "halfedges", d) for d in range(1, depth + 1)}
+    R = edge_rep_bits(cage, counts, levels, depth)
+    for L in range(1, depth + 1):
+        marked = np.nonzero(R[L])[0]
+        edges = levels[L]["edgeID"][marked]
+        assert marked.size == counts["E"][L], (L, marked.size, counts["E"][L])
+        want = np.array([O.edge_to_halfedge(cage, int(e), L) for e in edges])
+        assert np.array_equal(marked, want), (L, marked[marked != want][:5])
