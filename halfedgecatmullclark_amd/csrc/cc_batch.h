@@ -35,12 +35,12 @@ struct CreaseJob {
 // compiler knows every address derived from it is global memory and emits
 // global rather than flat instructions); -1 marks an absent buffer.
 struct LevelJob {
-    int64_t he, out_he, cr, old, vp, fp, ep, defer_mask, owner_mask, rare;
+    int64_t he, out_he, cr, old, vp, fp, ep, defer_mask, owner_mask, rare, erep;
     int64_t c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, c_vro, c_vr, c_vb, c_fro, c_fr, c_voo, c_vo;
     int64_t lp[kMaxDepth + 1];
     int32_t H, V, F, E, C;
     int32_t cV, cU, cH, cE, cF;
-    int32_t xcd_remap, uv_refine, rare_cap;
+    int32_t xcd_remap, uv_refine, rare_cap, erep_read, erep_write;
     LevelTab t;
 };
 
@@ -101,8 +101,9 @@ __host__ __device__ __forceinline__ void unpack_level(const LevelJob &J, char *a
     a.owner_mask = arena_ptr<uint64_t>(arena, J.owner_mask);
     a.rare = arena_ptr<int32_t>(arena, J.rare);
     a.rare_cap = J.rare_cap;
-    a.erep = nullptr;
-    a.erep_read = a.erep_write = 0;
+    a.erep = arena_ptr<uint64_t>(arena, J.erep);
+    a.erep_read = J.erep_read;
+    a.erep_write = J.erep_write;
     a.H = J.H; a.V = J.V; a.F = J.F; a.E = J.E; a.C = J.C;
     a.xcd_remap = J.xcd_remap;
     a.uv_refine = J.uv_refine;

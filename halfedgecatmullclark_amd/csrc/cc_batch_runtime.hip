@@ -31,6 +31,7 @@ struct ccb_Batch {
         int64_t o_vro = -1, o_vr = -1, o_vb = -1, o_fro = -1, o_fr = -1, o_voo = -1, o_vo = -1;
         int64_t o_he = 0, o_cr = 0, o_pts = 0, o_fin = 0, o_def = 0, o_adjB = 0, o_adjW = 0, o_rare = 0;
         int32_t rare_cap = 0;   // rare-vertex list entries (FusedArgs::rare), the count first
+        int64_t o_erep = -1;    // word offset of the edge-representative bits (depth >= 4)
     };
     struct Timed {
         const char *name;
@@ -47,6 +48,8 @@ struct ccb_Batch {
     int32_t *rings = nullptr;
     int64_t tot_he = 0, tot_cr = 0, tot_pts = 0, tot_fin = 0, tot_def = 0, tot_adjB = 0, tot_adjW = 0, tot_rare = 0;
     int32_t *rare = nullptr;
+    int64_t tot_erep = 0;
+    uint64_t *erep = nullptr;   // edge-representative bits of levels 1..D-2, per cage (FusedArgs::erep)
     // device memory: every array of the forward pass is carved out of one
     // arena, the adjoint's out of a second one (job tables hold byte offsets
     // into them)
@@ -221,6 +224,10 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
             g.o_rare = tot_rare;
             tot_rare += g.rare_cap + 1;
         }
+        if (g.D >= 4) {
+            g.o_erep = tot_erep;
+            tot_erep += g.tab.R[g.D - 1];
+        }
         g.o_adjB = tot_adjB; tot_adjB += g.n.V[g.D - 1];
         g.o_adjW = tot_adjW; tot_adjW += std::max<int64_t>(g.D >= 2 ? g.n.H[g.D - 1] : 1, g.H0);
         Dmax = std::max(Dmax, g.D);
@@ -242,6 +249,7 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
         const size_t o_pts = cv.take(sizeof(P3) * (size_t)tot_pts);
         const size_t o_def = cv.take(sizeof(uint64_t) * (size_t)tot_def);
         const size_t o_rare = cv.take(sizeof(int32_t) * (size_t)tot_rare);
+        const size_t o_erep = cv.take(sizeof(uint64_t) * (size_t)(tot_erep + 1));
         CC_TRY(hipMalloc(&arena, cv.size), "hipMalloc(batch arena)");
         c_he = reinterpret_cast<CageHe *>(arena + o_che);
         c_v2h = reinterpret_cast<int32_t *>(arena + o_cv2h);
@@ -256,6 +264,7 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
         pts = reinterpret_cast<P3 *>(arena + o_pts);
         defer_mask = reinterpret_cast<uint64_t *>(arena + o_def);
         rare = reinterpret_cast<int32_t *>(arena + o_rare);
+        erep = reinterpret_cast<uint64_t *>(arena + o_erep);
     }
     for (int i = 0; i < count; ++i) {
         const cc_Mesh *m = meshes[i];
@@ -374,6 +383,9 @@ bool ccb_Batch::build_tables() {
         tb.add(t_cr[d], jobs, b);
     }
     for (int d = 1; d < Dmax; ++d) {   // patch-local level steps d -> d+1
+        bool fused_below = true;        // levels 1..d-1 run the fused kernel
+        for (int l = 1; l < d; ++l) fused_below = fused_below && cfg[l].kind == 1;
+        const bool fused_upto = fused_below && cfg[d].kind == 1;
         std::vector<LevelJob> jobs;
         std::vector<int64_t> bp, bd, bf, bv, br;
         for (int i = 0; i < N; ++i) {
@@ -394,6 +406,12 @@ bool ccb_Batch::build_tables() {
             j.owner_mask = cfg[d].kind == 1 && cfg[d].scan == 0 ? j.defer_mask : -1;
             j.rare = j.owner_mask >= 0 ? off(rare + g.o_rare) : -1;
             j.rare_cap = j.owner_mask >= 0 ? g.rare_cap : 0;
+            // edge-representative bits: the forward pass runs the fused
+            // kernel on every level up to d, so levels 1..d are written in
+            // the same pass before they are read
+            j.erep = g.o_erep >= 0 ? off(erep + g.o_erep) : -1;
+            j.erep_write = j.erep >= 0 && d + 1 <= g.D - 2 && fused_upto ? 1 : 0;
+            j.erep_read = j.erep >= 0 && d >= 2 && fused_below ? 1 : 0;
             j.H = (int32_t)g.n.H[d]; j.V = (int32_t)g.n.V[d]; j.F = (int32_t)g.n.F[d];
             j.E = (int32_t)g.n.E[d]; j.C = (int32_t)g.n.C[d];
             j.xcd_remap = remap;
