@@ -18,6 +18,8 @@ struct AdjArgs {
     T *gV, *gF, *gE;
     T *gP;
     T *W;          // per-halfedge vertex-rule weights (a, b, g, dl): 4 H scalars
+    // edge-representative bits of levels 1..d-1 (FusedArgs::erep), or null
+    const uint64_t *erep = nullptr;
 };
 
 struct AdjPlan {
@@ -29,6 +31,7 @@ struct AdjPlan {
     LevelTab tab;
     LevelPtrs lp;
     int64_t per_vertex_max;   // levels with at most this many vertices: vertex passes one thread per vertex
+    const uint64_t *erep;     // edge-representative bits of levels 1..D-2 from the forward pass, or null
 };
 
 // dL/d(cage points) from dL/d(level-D points): V_D x 3 in, V_0 x 3 out.

@@ -322,7 +322,7 @@ __device__ __forceinline__ void adj_gather_fan(const AdjArgs<T> &a, const DCage 
 // The level tables go to LDS first; the first barrier orders the copy.
 template <int SCAN, class Fn, class LP>
 __device__ __forceinline__ void owners_block(int32_t H, int d, const DCage &c, const LevelTab &t, const LP &lp,
-                                             uint32_t lb, const Fn &fn) {
+                                             uint32_t lb, const uint64_t *erep, const Fn &fn) {
     __shared__ int32_t owners[256 * SCAN];
     __shared__ int32_t nown;
     __shared__ LevelTab st;
@@ -335,7 +335,7 @@ __device__ __forceinline__ void owners_block(int32_t H, int d, const DCage &c, c
 #pragma unroll
     for (int k = 0; k < SCAN; ++k) {
         const int32_t h = base + k * 256 + (int32_t)threadIdx.x;
-        if (h < H && structural_owner(h, d, c, st, slp.he)) owners[atomicAdd(&nown, 1)] = h;
+        if (h < H && structural_owner(h, d, c, st, slp.he, erep)) owners[atomicAdd(&nown, 1)] = h;
     }
     __syncthreads();
     const int32_t count = nown;
@@ -655,7 +655,7 @@ __device__ __forceinline__ void adj_cage_vertex_gather(const AdjArgs<T> &a, cons
 // ---- kernels: one thread per halfedge ------------------------------------------
 template <typename T, int CREASED>
 __global__ __launch_bounds__(256) void k_adj_weights(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
-    owners_block<kAdjScan>(a.H, d, c, t, lp, blockIdx.x, RingWeightsFn<T, CREASED>{a, c, d});
+    owners_block<kAdjScan>(a.H, d, c, t, lp, blockIdx.x, a.erep, RingWeightsFn<T, CREASED>{a, c, d});
 }
 
 template <typename T, int CREASED>
@@ -670,7 +670,7 @@ __global__ __launch_bounds__(256) void k_adj_faces(AdjArgs<T> a) {
 
 template <typename T, int CREASED>
 __global__ __launch_bounds__(256) void k_adj_vertices(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
-    owners_block<kAdjScan>(a.H, d, c, t, lp, blockIdx.x, VertexGatherFn<T, CREASED>{a, c, d});
+    owners_block<kAdjScan>(a.H, d, c, t, lp, blockIdx.x, a.erep, VertexGatherFn<T, CREASED>{a, c, d});
 }
 
 template <typename T, int CREASED, int GATHER_PASS>
@@ -789,6 +789,7 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
         a.gE = G + 3 * (n.V[d] + n.F[d]);
         a.gP = P;
         a.W = W;
+        a.erep = pl.erep;
         const dim3 g(grid_for(n.H[d])), gq(grid_for(n.H[d] / 4)), gv(grid_for(n.V[d])), b(256);
         const bool pv = n.V[d] <= pl.per_vertex_max;
         if (creased) {
@@ -861,7 +862,7 @@ struct OpAdjWeights {
         DCage c;
         unpack_adj(J, arena, adj, a, c);
         const JobLevelPtrs lp{J.lp, arena};
-        owners_block<kAdjScan>(a.H, d, c, J.t, lp, (uint32_t)(i >> 8), RingWeightsFn<float, CREASED>{a, c, d});
+        owners_block<kAdjScan>(a.H, d, c, J.t, lp, (uint32_t)(i >> 8), a.erep, RingWeightsFn<float, CREASED>{a, c, d});
     }
 };
 template <int CREASED>
@@ -893,7 +894,7 @@ struct OpAdjVertices {
         DCage c;
         unpack_adj(J, arena, adj, a, c);
         const JobLevelPtrs lp{J.lp, arena};
-        owners_block<kAdjScan>(a.H, d, c, J.t, lp, (uint32_t)(i >> 8), VertexGatherFn<float, CREASED>{a, c, d});
+        owners_block<kAdjScan>(a.H, d, c, J.t, lp, (uint32_t)(i >> 8), a.erep, VertexGatherFn<float, CREASED>{a, c, d});
     }
 };
 template <int CREASED>

@@ -54,7 +54,7 @@ struct SpanJob {
 // cage are offsets into the arena, the gradients and weights into the
 // adjoint arena.
 struct AdjJob {
-    int64_t he, cr, gV, gF, gE, gP, W;
+    int64_t he, cr, gV, gF, gE, gP, W, erep;
     int64_t c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, c_vro, c_vr, c_vb, c_fro, c_fr, c_voo, c_vo;
     int64_t lp[kMaxDepth + 1];
     int32_t H, C, V;
@@ -121,6 +121,7 @@ __host__ __device__ __forceinline__ void unpack_adj(const AdjJob &J, char *arena
     a.gE = arena_ptr<float>(adj, J.gE);
     a.gP = arena_ptr<float>(adj, J.gP);
     a.W = arena_ptr<float>(adj, J.W);
+    a.erep = arena_ptr<const uint64_t>(arena, J.erep);
     c = arena_cage(arena, J);
 }
 

@@ -50,6 +50,10 @@ struct ccb_Batch {
     int32_t *rare = nullptr;
     int64_t tot_erep = 0;
     uint64_t *erep = nullptr;   // edge-representative bits of levels 1..D-2, per cage (FusedArgs::erep)
+    // levels 1..erep_kept hold bits written by a forward pass already queued
+    // (they depend on the topology only); the adjoint tables were built when
+    // it was erep_tables
+    int erep_kept = 0, erep_tables = 0;
     // device memory: every array of the forward pass is carved out of one
     // arena, the adjoint's out of a second one (job tables hold byte offsets
     // into them)
@@ -318,6 +322,7 @@ bool ccb_Batch::alloc_adjoint() {
 // Job tables of every launch, for the current tile options.
 bool ccb_Batch::build_tables() {
     TableBuilder tb;
+    erep_tables = erep_kept;
     auto off = [&](const void *p) -> int64_t { return static_cast<const char *>(p) - arena; };
     auto aoff = [&](const void *p) -> int64_t { return static_cast<const char *>(p) - adj_arena; };
     auto cage_offsets = [&](int i, auto &j) {
@@ -436,6 +441,9 @@ bool ccb_Batch::build_tables() {
     }
     if (adj_arena) {   // adjoint steps d+1 -> d, d = 0 the cage step
         for (int d = 0; d < Dmax; ++d) {
+            // levels 1..d-1's edge-representative bits, when a forward pass
+            // has written them
+            const bool kept = d >= 2 && d - 1 <= erep_kept;
             std::vector<AdjJob> jobs;
             std::vector<int64_t> b, bq, be, bf, bv;
             for (int i = 0; i < N; ++i) {
@@ -463,6 +471,7 @@ bool ccb_Batch::build_tables() {
                     j.gP = aoff(buf(d));
                 }
                 j.W = aoff(adjW + 4 * g.o_adjW);
+                j.erep = g.o_erep >= 0 && kept ? off(erep + g.o_erep) : -1;
                 j.gV = aoff(G);
                 j.gF = aoff(G + 3 * g.n.V[d]);
                 j.gE = aoff(G + 3 * (g.n.V[d] + g.n.F[d]));
@@ -571,6 +580,13 @@ void ccb_Batch::record_adjoint(int creased, hipStream_t st) {
 }
 
 bool ccb_Batch::run(int32_t key, bool adjoint, int32_t steps, hipStream_t st) {
+    if (adjoint && erep_kept > erep_tables && !build_tables()) return false;   // now with the bits
+    if (!adjoint) {
+        // the fused kernel on levels 1..p writes the bits of levels 1..p+1
+        int p = 0;
+        while (p + 1 < Dmax && cfg[p + 1].kind == 1) ++p;
+        if (p >= 1) erep_kept = std::max(erep_kept, p + 1);
+    }
     if (!use_graph || timing) {
         if (timing) begin_timing();   // the timings are those of the last run
         if (adjoint) record_adjoint(steps, st);

@@ -350,7 +350,7 @@ __device__ __forceinline__ bool owner_fan(int32_t h, int32_t vert, int d, const 
 // represents the edge (one parent record read), a cage-level vertex as the
 // cage maps say.
 __device__ __forceinline__ bool structural_owner(int32_t h, int d, const DCage &c, const LevelTab &t,
-                                                 const He *const *levels) {
+                                                 const He *const *levels, const uint64_t *erep = nullptr) {
     const int k = zero_digits(h, d);
     const int32_t q = h >> (2 * k);
     const int lvl = d - k;
@@ -359,6 +359,8 @@ __device__ __forceinline__ bool structural_owner(int32_t h, int d, const DCage &
     if (digit == 3) return false;
     if (lvl >= 2) {
         if (digit == 2) return (p & 3) == 0;
+        // the parent level's edge-representative bit when it is kept
+        if (erep) return ((erep[t.R[lvl - 1] + (p >> 6)] >> (p & 63)) & 1ull) != 0ull;
         return is_edge_rep(c, t, reinterpret_cast<const int4 *>(levels[lvl - 1])[p].y, lvl - 1, p);
     }
     const CageHe x = c.he[p];

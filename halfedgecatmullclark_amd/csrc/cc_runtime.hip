@@ -435,6 +435,9 @@ bool ccd_Subd::adjoint(const void *gradFinest, void *gradCage, int32_t steps, hi
     pl.tab = tab;
     pl.lp = level_ptrs();
     pl.per_vertex_max = opt_adj_vertex;
+    bool kept = erep != nullptr;
+    for (int l = 1; l <= D - 2; ++l) kept = kept && erep_valid[l];
+    pl.erep = kept ? erep : nullptr;
     const int creased = (steps & CCD_NOCREASES) ? 0 : 1;
     if (scalar == sizeof(double)) {
         // the double adjoint refines sharpness in double too, as the double
