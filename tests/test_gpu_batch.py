@@ -223,6 +223,59 @@ def test_batch_adjoint_matches_single_cage(capi, torch):
     b.close()
 
 
+def test_adjoint_with_and_without_kept_bits(capi, torch):
+    """The adjoint's owner scan reads the edge-representative bits only once
+    a forward pass has written them.  A forward pass through the patch
+    kernel writes none: the adjoint that follows, with the fused kernel
+    selected again, must fall back to the descent, and equal (bit for bit)
+    the adjoint after a fused forward pass and that of a fresh batch."""
+    cases = batch_cases()
+    cages = [c for c, _ in cases]
+    depths = [d for _, d in cases]
+    ref = capi.Batch(cages, depths)
+    ref.refine(capi.ALL)
+    nf = ref.count(capi.COUNT_FINEST_POINTS)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    g = torch.randn((nf, 3), generator=gen, device="cuda", dtype=torch.float32)
+    nc = ref.count(capi.COUNT_CAGE_POINTS)
+    want = torch.empty((nc, 3), device="cuda", dtype=torch.float32)
+    ref.refine_points_adjoint(g.data_ptr(), want.data_ptr(), 0)
+    ref.synchronize()
+    ref.close()
+    b = capi.Batch(cages, depths)
+    b.set_option(capi.OPT_FUSED, 2)
+    b.refine(capi.ALL)
+    b.set_option(capi.OPT_FUSED, 1)
+    outs = []
+    for fwd in (False, True):
+        if fwd:
+            b.refine(capi.ALL)
+        out = torch.empty((nc, 3), device="cuda", dtype=torch.float32)
+        b.refine_points_adjoint(g.data_ptr(), out.data_ptr(), 0)
+        b.synchronize()
+        outs.append(out.cpu().numpy())
+    b.close()
+    w = want.cpu().numpy()
+    assert_bytes_equal(outs[0], w, "adjoint after a patch forward")
+    assert_bytes_equal(outs[1], w, "adjoint after a fused forward")
+    # the same on one cage through the single-cage path
+    cage, depth = C.torus(16, 8, seed=1), 5
+    grads = []
+    for kind in (1, 2):
+        ds = capi.DeviceSubd(cage, depth)
+        ds.set_option(capi.OPT_FUSED, kind)
+        ds.refine(capi.ALL)
+        ds.set_option(capi.OPT_FUSED, 1)
+        gf = torch.randn((ds.n["V"][depth], 3), generator=torch.Generator(device="cuda").manual_seed(7),
+                         device="cuda", dtype=torch.float32)
+        oc = torch.empty((cage.V, 3), device="cuda", dtype=torch.float32)
+        ds.refine_points_adjoint(gf.data_ptr(), oc.data_ptr(), 0)
+        ds.synchronize()
+        grads.append(oc.cpu().numpy())
+        ds.close()
+    assert_bytes_equal(grads[1], grads[0], "single-cage adjoint after a patch forward")
+
+
 def test_batch_autotune_keeps_results(capi):
     cases = batch_cases()
     b = capi.Batch([c for c, _ in cases], [d for _, d in cases])
