@@ -781,9 +781,22 @@ __device__ __forceinline__ bool structural_ring(const Fan &f, const LevelTab &t,
     return true;
 }
 
-// vertex point of vertex v of level d with fan f (kinds 1 and 3 load their
-// at most four entries up front, cage fans are walked)
-template <int GATHER, int CREASED>
+// fans of up to eight halfedges (cage vertices and faces), every load issued
+// before any is used
+struct FanPreloaded8 {
+    FanEntry e0, e1, e2, e3, e4, e5, e6, e7;
+    __device__ __forceinline__ FanEntry get(int i) const {
+        return i < 4 ? pick4(e0, e1, e2, e3, i) : pick4(e4, e5, e6, e7, i - 4);
+    }
+    __device__ __forceinline__ int32_t key(int i) const {
+        return i < 4 ? pick4(e0.r, e1.r, e2.r, e3.r, i) : pick4(e4.r, e5.r, e6.r, e7.r, i - 4);
+    }
+};
+
+// vertex point of vertex v of level d with fan f: fans of at most four (and
+// with PRE8 of at most eight) load every entry up front, longer fans entry
+// by entry
+template <int GATHER, int CREASED, int PRE8 = 0>
 __device__ __forceinline__ void fan_vertex_store(const FusedArgs &a, const DCage &c, const LevelTab &t, int d,
                                                  const Fan &f, int32_t v) {
     const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
@@ -803,7 +816,7 @@ __device__ __forceinline__ void fan_vertex_store(const FusedArgs &a, const DCage
         for (int q = 0; q < 4; ++q) re[q] = he4[r[q]].y;
         const FanPreloaded pre = closed_fan(a, r, re);
         a.vp[v] = fan_vertex_point<GATHER, CREASED, 4>(f, pre, old);
-    } else if (f.kind == 3) {
+    } else if (f.n <= 4) {
         int32_t r[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) r[q] = fan_at(f, c, q < f.n ? q : 0);
@@ -819,6 +832,26 @@ __device__ __forceinline__ void fan_vertex_store(const FusedArgs &a, const DCage
         pre.e2 = fan_entry(a, r[2], rec[2], prv[2]);
         pre.e3 = fan_entry(a, r[3], rec[3], prv[3]);
         a.vp[v] = fan_vertex_point<GATHER, CREASED, 4>(f, pre, old);
+    } else if (PRE8 && f.n <= 8) {
+        int32_t r[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) r[q] = fan_at(f, c, q < f.n ? q : 0);
+        int4 rec[8], prv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            rec[q] = he4[r[q]];
+            prv[q] = he4[qprev(r[q])];
+        }
+        FanPreloaded8 pre;
+        pre.e0 = fan_entry(a, r[0], rec[0], prv[0]);
+        pre.e1 = fan_entry(a, r[1], rec[1], prv[1]);
+        pre.e2 = fan_entry(a, r[2], rec[2], prv[2]);
+        pre.e3 = fan_entry(a, r[3], rec[3], prv[3]);
+        pre.e4 = fan_entry(a, r[4], rec[4], prv[4]);
+        pre.e5 = fan_entry(a, r[5], rec[5], prv[5]);
+        pre.e6 = fan_entry(a, r[6], rec[6], prv[6]);
+        pre.e7 = fan_entry(a, r[7], rec[7], prv[7]);
+        a.vp[v] = fan_vertex_point<GATHER, CREASED, 8>(f, pre, old);
     } else {
         FanOnDemand od{&a, &f, &c};
         a.vp[v] = fan_vertex_point<GATHER, CREASED, 0>(f, od, old);

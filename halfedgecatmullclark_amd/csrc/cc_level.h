@@ -329,7 +329,7 @@ __device__ __forceinline__ void level_vertices_rare(const FusedArgs &a, const DC
     const int32_t h = a.rare[1 + i];
     Fan f;
     const int32_t v = marked_fan(h, d, c, st, slp.he, f);
-    fan_vertex_store<GATHER, CREASED>(a, c, st, d, f, v);
+    fan_vertex_store<GATHER, CREASED, 1>(a, c, st, d, f, v);
 }
 
 }  // namespace cch
