@@ -59,10 +59,6 @@ __device__ __forceinline__ int4 quad_rel4(int4 v) {
     return make_int4(quad_rel<CTRL>(v.x), quad_rel<CTRL>(v.y), quad_rel<CTRL>(v.z), quad_rel<CTRL>(v.w));
 }
 template <int CTRL>
-__device__ __forceinline__ P3 quad_rel3(P3 p) {
-    return {quad_relf<CTRL>(p.x), quad_relf<CTRL>(p.y), quad_relf<CTRL>(p.z)};
-}
-template <int CTRL>
 __device__ __forceinline__ UV quad_reluv(UV t) {
     return {quad_relf<CTRL>(t.u), quad_relf<CTRL>(t.v)};
 }
