@@ -184,6 +184,28 @@ def batched(torch, capi, C, steps: int, stream):
             "fwd_adj_speedup": (p_fwd + p_adj) / (b_fwd + b_adj)}
 
 
+def host_boundary(capi, C, D, runs: int = 3):
+    """ccs_Refine_Scatter through the host cc_Subd (CatmullClark.h:200): the
+    cage read from host memory and all levels written back to the subd's
+    host arrays before the call returns, so the PCIe copies are inside the
+    timed region.  Best of `runs` calls after one untimed call."""
+    s = capi.HostSubd(C.torus(64, 32, seed=0), D)
+    try:
+        s.run("ccs_Refine_Scatter")
+        best = float("inf")
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            s.run("ccs_Refine_Scatter")
+            best = min(best, time.perf_counter() - t0)
+        n = s.n
+        nbytes = 16 * n["Hcum"][D] + 12 * n["Ccum"][D] + 12 * n["Vcum"][D]
+        return {"workload": f"torus 64x32 cage, depth {D}, host arrays", "ms": best * 1e3,
+                "value": n["Hcum"][D] / best, "unit": "refined halfedges/s",
+                "host_bytes_written": nbytes, "host_gbs": nbytes / best / 1e9}
+    finally:
+        s.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,6 +213,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched comparison")
+    ap.add_argument("--host-boundary", action="store_true",
+                    help="also time ccs_Refine_Scatter on host arrays (the drop-in boundary: cage upload, "
+                         "refinement, every level copied back over PCIe)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy 1 GiB between two device buffers twice (a kernel of "
                          "known traffic for scaling hardware byte counters)")
@@ -313,6 +338,7 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    host = host_boundary(capi, C, D) if args.host_boundary else None
     batch = None if args.no_batch else batched(torch, capi, C, args.steps, stream)
     cpu = None if args.no_cpu else cpu_baseline()
 
@@ -339,6 +365,7 @@ def main():
                         "fwd_ms": res["fwd"], "fwd_adj_ms": res["fwd_adj"],
                         "ratio": res["fwd_adj"] / res["fwd"]},
         "batched": batch,
+        "host_boundary": host,
         "level_kernels": level_cfg,
         "roofline": {"bound": "hbm", "kernel": f"{name} (level {level} -> {level + 1})",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
