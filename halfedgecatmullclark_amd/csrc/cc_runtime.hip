@@ -460,6 +460,35 @@ namespace {
 std::mutex g_mu;
 std::unordered_map<const cc_Subd *, ccd_Subd *> g_mirrors;
 
+// The level arrays ccs_Create allocates are page-locked when the runtime
+// allows it, so that every refinement's copy back to them runs at the DMA
+// rate instead of through a pageable staging copy; otherwise, and for
+// arrays the caller allocated, plain malloc memory.
+std::mutex g_pin_mu;
+std::unordered_map<void *, bool> g_pinned;
+
+void *host_alloc(size_t bytes) {
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) == hipSuccess && p) {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        g_pinned[p] = true;
+        return p;
+    }
+    (void)hipGetLastError();   // no device or no pinned memory: pageable instead
+    return malloc(bytes);
+}
+
+void host_free(void *p) {
+    if (!p) return;
+    bool pinned = false;
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        pinned = g_pinned.erase(p) > 0;
+    }
+    if (pinned) (void)hipHostFree(p);
+    else free(p);
+}
+
 ccd_Subd *create_mirror(const cc_Mesh *cage, int32_t depth) {
     ccd_Subd *s = new (std::nothrow) ccd_Subd();
     if (!s) { set_error(2, "out of host memory"); return nullptr; }
@@ -568,11 +597,11 @@ cc_Subd *ccs_Create(const cc_Mesh *cage, int32_t maxDepth) {
     if (!s) { set_error(2, "ccs_Create: out of host memory"); return NULL; }
     s->maxDepth = maxDepth;
     s->cage = cage;
-    s->halfedges = (cc_Halfedge_SemiRegular *)malloc(sizeof(cc_Halfedge_SemiRegular) * (size_t)n.cumH(maxDepth));
-    s->creases = (cc_Crease *)malloc(sizeof(cc_Crease) * (size_t)n.cumC(maxDepth) + 1);
-    s->vertexPoints = (cc_VertexPoint *)malloc(sizeof(cc_VertexPoint) * (size_t)n.cumV(maxDepth));
+    s->halfedges = (cc_Halfedge_SemiRegular *)host_alloc(sizeof(cc_Halfedge_SemiRegular) * (size_t)n.cumH(maxDepth));
+    s->creases = (cc_Crease *)host_alloc(sizeof(cc_Crease) * (size_t)n.cumC(maxDepth) + 1);
+    s->vertexPoints = (cc_VertexPoint *)host_alloc(sizeof(cc_VertexPoint) * (size_t)n.cumV(maxDepth));
     if (!s->halfedges || !s->creases || !s->vertexPoints) {
-        free(s->halfedges); free(s->creases); free(s->vertexPoints); free(s);
+        host_free(s->halfedges); host_free(s->creases); host_free(s->vertexPoints); free(s);
         set_error(2, "ccs_Create: out of host memory");
         fprintf(stderr, "cc: ccs_Create: out of host memory\n");
         return NULL;
@@ -583,9 +612,9 @@ cc_Subd *ccs_Create(const cc_Mesh *cage, int32_t maxDepth) {
 void ccs_Release(cc_Subd *subd) {
     if (!subd) return;
     drop_mirror(subd);
-    free(subd->halfedges);
-    free(subd->creases);
-    free(subd->vertexPoints);
+    host_free(subd->halfedges);
+    host_free(subd->creases);
+    host_free(subd->vertexPoints);
     free(subd);
 }
 
