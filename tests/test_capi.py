@@ -66,7 +66,9 @@ def test_counts_and_maps(name, cage, depth):
 def test_create_refusals():
     L = capi.lib()
     torus = MeshView(C.torus(64, 32))
-    assert L.ccs_Create(torus.pointer, 8)  # the largest depth the int32 formulas allow
+    s = L.ccs_Create(torus.pointer, 8)  # the largest depth the int32 formulas allow
+    assert s
+    L.ccs_Release(s)
     assert not L.ccs_Create(torus.pointer, 9)
     assert "overflows int32" in capi.last_error()[1]
     assert not L.ccs_Create(torus.pointer, 0)
