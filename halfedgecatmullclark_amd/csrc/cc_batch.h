@@ -125,15 +125,12 @@ __host__ __device__ __forceinline__ void unpack_adj(const AdjJob &J, char *arena
     c = arena_cage(arena, J);
 }
 
-// largest k in [0, n) with start[k] <= b (jobs without workgroups are skipped)
+// The job of workgroup b: a job table's start array holds the first
+// workgroup of each of its n jobs and the total (n + 1 entries), then the
+// job of every workgroup, so one load replaces a binary search (jobs
+// without workgroups own none)
 __device__ __forceinline__ int find_job(const int32_t *__restrict__ start, int n, uint32_t b) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if ((uint32_t)start[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
+    return start[n + 1 + (int)b];
 }
 
 // One thread per element of each job: op(job, i) with i the job-local index.

@@ -132,10 +132,15 @@ struct TableBuilder {
     }
     template <class Job>
     void add(BatchTable &t, const std::vector<Job> &jobs, const std::vector<int64_t> &blocks) {
-        std::vector<int32_t> start(jobs.size() + 1, 0);
         int64_t total = 0;
+        for (size_t k = 0; k < jobs.size(); ++k) total += blocks[k];
+        // first workgroup of each job, the total, then each workgroup's job
+        std::vector<int32_t> start(jobs.size() + 1 + (size_t)total, 0);
+        int32_t *owner = start.data() + jobs.size() + 1;
+        total = 0;
         for (size_t k = 0; k < jobs.size(); ++k) {
             start[k] = (int32_t)total;
+            for (int64_t b = 0; b < blocks[k]; ++b) owner[total + b] = (int32_t)k;
             total += blocks[k];
         }
         start[jobs.size()] = (int32_t)total;
