@@ -160,10 +160,55 @@ __device__ __forceinline__ RingT<T> ring_params(const AdjArgs<T> &a, const DCage
     return rp;
 }
 
+// ring crease parameter update for the sharpness se of edge e (the order of
+// ring_params)
+template <typename T>
+__device__ __forceinline__ void ring_add(RingT<T> &rp, T se, int32_t e) {
+    rp.cc += sgnT(se);
+    if (se != (T)0) {
+        if (rp.e1 < 0) { rp.s1 = se; rp.e1 = e; }
+        else if (rp.e2 < 0) { rp.s2 = se; rp.e2 = e; }
+    }
+    rp.sall += se;
+}
+
 // pass 1: the vertex rule's weights of every halfedge of a vertex's ring
 template <typename T, int CREASED>
 __device__ __forceinline__ void adj_ring_weights(const AdjArgs<T> &a, const DCage &c, const Fan &f) {
     const He *he = a.he;
+    if (f.n <= 4) {
+        // a fan of at most four: every entry's loads issued before any is
+        // used, the same sums in the same order as below
+        int32_t r[4], re[4], pe[4];
+        T ts[4], ps[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[q] = fan_at(f, c, q < f.n ? q : 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            re[q] = he[r[q]].edge;
+            pe[q] = he[qprev(r[q])].edge;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            ts[q] = CREASED ? sharp_at(a, re[q]) : (T)0;
+            ps[q] = CREASED ? sharp_at(a, pe[q]) : (T)0;
+        }
+        RingT<T> rp;
+        rp.n = (T)f.n; rp.cc = 0; rp.boundary = f.boundary; rp.s1 = rp.s2 = 0; rp.e1 = rp.e2 = -1; rp.sall = 0;
+        if (CREASED) {
+            if (f.boundary) ring_add(rp, ts[0], re[0]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (i < f.n) ring_add(rp, ps[i], pe[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (i >= f.n) break;
+            const VW<T> w = level_vertex_weights<T>(rp, ts[i], ps[i], re[i], CREASED);
+            w_store(a, r[i], w.a, w.b, w.g, w.dl);
+        }
+        return;
+    }
     const RingT<T> rp = ring_params<T, CREASED>(a, c, f);
     for (int i = 0; i < f.n; ++i) {
         const int32_t r = fan_at(f, c, i);
@@ -235,6 +280,41 @@ __device__ __forceinline__ void adj_vertex_gather(const AdjArgs<T> &a, const DCa
     const V3<T> gv = reinterpret_cast<const V3<T> *>(a.gV)[v];
     T alpha = 0;
     V3<T> acc{0, 0, 0};
+    if (f.n <= 4) {
+        // a fan of at most four: every entry's loads issued before any is
+        // used, the same sums in the same order as below
+        int32_t r[4];
+        He x[4], xp[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[q] = fan_at(f, c, q < f.n ? q : 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            x[q] = he[r[q]];
+            xp[q] = he[qprev(r[q])];
+        }
+        T wa[4], c0[4], d1[4];
+        V3<T> e[4], ep[4], g[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            T c1, cf, d0, df;
+            wa[q] = w_a(a, r[q]);
+            edge_weights<T, CREASED>(a, x[q].edge, x[q].twin < 0, c0[q], c1, cf);
+            edge_weights<T, CREASED>(a, xp[q].edge, xp[q].twin < 0, d0, d1[q], df);
+            e[q] = gE[x[q].edge];
+            ep[q] = gE[xp[q].edge];
+            g[q] = gF[r[q] >> 2];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (i >= f.n) break;
+            alpha += wa[i];
+            acc.x += c0[i] * e[i].x; acc.y += c0[i] * e[i].y; acc.z += c0[i] * e[i].z;
+            acc.x += d1[i] * ep[i].x; acc.y += d1[i] * ep[i].y; acc.z += d1[i] * ep[i].z;
+            acc.x += (T)0.25 * g[i].x; acc.y += (T)0.25 * g[i].y; acc.z += (T)0.25 * g[i].z;
+        }
+        reinterpret_cast<V3<T> *>(a.gP)[v] = {alpha * gv.x + acc.x, alpha * gv.y + acc.y, alpha * gv.z + acc.z};
+        return;
+    }
     for (int i = 0; i < f.n; ++i) {
         const int32_t r = fan_at(f, c, i);
         const int32_t p = qprev(r);
