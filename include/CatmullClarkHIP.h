@@ -137,7 +137,10 @@ int32_t ccm_PrevVertexHalfedgeID(const cc_Mesh *mesh, int32_t halfedgeID);
 
 /* ---- subd lifecycle and counts (CatmullClark.h:145-160) ------------------ */
 /* ccs_Create refuses (returns NULL, ccs_LastError() != 0) a depth below 1 or
- * one whose cumulative counts overflow the reference's int32 formulas. */
+ * one whose cumulative counts overflow the reference's int32 formulas.  It
+ * allocates the three level arrays page-locked when the HIP runtime allows
+ * it (else with malloc), so refinements copy back at the DMA rate;
+ * ccs_Release frees either kind. */
 cc_Subd *ccs_Create(const cc_Mesh *cage, int32_t maxDepth);                 /* :953 */
 void ccs_Release(cc_Subd *subd);                                            /* :977 */
 int32_t ccs_MaxDepth(const cc_Subd *subd);
