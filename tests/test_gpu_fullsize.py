@@ -9,7 +9,9 @@
 - the batch of 256 uneven cages (24..60,000 halfedges, depths 4..6, 5.26 G
   halfedges in one arena, so the later cages sit past 2^32 halfedges):
   every cage of at most 2 M refined halfedges (124 of them, spread over the
-  whole arena) and the last cage are checked against the oracle.
+  whole arena) and the last cage are checked against the oracle, and the
+  batched adjoint of every eighth of them and of the last cage against the
+  single-cage adjoint.
 
 The oracle runs its gather loops on all host threads (up to 16); its scatter
 sums stay sequential, so its points equal the sequential reference's.  Sizes
@@ -73,3 +75,38 @@ def test_batch_workload_bitexact(capi):
         for k in ("halfedges", "creases", "points"):
             assert_bits_equal(got[k], want[k], f"cage {i} (H={cage.H}, depth {depth}) {k}")
     b.close()
+
+
+def test_batch_workload_adjoint_matches_single_cage(capi):
+    """The batched adjoint of the 256-cage workload, checked bit for bit
+    against the single-cage adjoint for the cages of at most 2 M refined
+    halfedges and the last cage (past 2^32 halfedges in the arena): both add
+    in the same fixed order, with the same kept edge-representative bits."""
+    import torch
+    cages, depths = C.batch_cages(256, seed=0)
+    b = capi.Batch(cages, depths)
+    b.refine(capi.ALL)
+    nf = b.count(capi.COUNT_FINEST_POINTS)
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    g = torch.randn((nf, 3), generator=gen, device="cuda", dtype=torch.float32)
+    out = torch.empty((b.count(capi.COUNT_CAGE_POINTS), 3), device="cuda", dtype=torch.float32)
+    b.refine_points_adjoint(g.data_ptr(), out.data_ptr(), 0)
+    b.synchronize()
+    hcum = [level_counts(c.V, c.H, c.E, c.F, d)["Hcum"][d] for c, d in zip(cages, depths)]
+    pick = [i for i, h in enumerate(hcum) if h <= 2_000_000][::8] + [len(cages) - 1]
+    offs = {i: (b.offset(capi.COUNT_FINEST_POINTS, i), b.count(capi.COUNT_FINEST_POINTS, i),
+                b.offset(capi.COUNT_CAGE_POINTS, i)) for i in pick}
+    got_all = out.cpu().numpy()
+    b.close()
+    del out
+    for i in pick:
+        o, n, oc = offs[i]
+        ds = capi.DeviceSubd(cages[i], depths[i])
+        ds.refine(capi.ALL)
+        gi = g[o:o + n].contiguous()
+        oi = torch.empty((cages[i].V, 3), device="cuda", dtype=torch.float32)
+        ds.refine_points_adjoint(gi.data_ptr(), oi.data_ptr(), 0)
+        ds.synchronize()
+        assert_bits_equal(np.ascontiguousarray(got_all[oc:oc + cages[i].V]), oi.cpu().numpy(),
+                          f"cage {i} (H={cages[i].H}, depth {depths[i]}) adjoint")
+        ds.close()
