@@ -57,6 +57,23 @@ def test_dot_product_identity(capi, name, cage, depth):
     s.close()
 
 
+def test_dot_product_identity_cage_fitting_workload(capi):
+    """The same identity on the cage-fitting workload itself: torus 32x16
+    (2,048 halfedges, crease rings of sharpness 1.5 and 4.0) at depth 5."""
+    cage, depth = C.torus(32, 16), 5
+    s = capi.HostSubd(cage, depth).run("ccs_RefineHalfedges", "ccs_RefineCreases")
+    rng = np.random.default_rng(1)
+    v = rng.normal(size=(cage.V, 3))
+    Sv = forward64(cage, depth, v)
+    u = rng.normal(size=Sv.shape)
+    g64 = s.adjoint(u, np.float64)
+    lhs, rhs = float(np.sum(u * Sv)), float(np.sum(g64 * v))
+    assert abs(lhs - rhs) <= 1e-11 * (abs(lhs) + np.abs(u).sum()), (lhs, rhs)
+    g32 = s.adjoint(u.astype(np.float32), np.float32)
+    assert float(np.abs(g32 - g64).max()) <= 1e-5 * float(np.abs(g64).max())
+    s.close()
+
+
 def test_against_reference_jacobian(capi):
     gold = np.load(os.path.join(HERE, "golden", "golden.npz"))
     J = gold["jacobian/cube_d3_scatter"]          # V_3 x V_0 (per coordinate)
