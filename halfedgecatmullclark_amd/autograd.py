@@ -24,7 +24,11 @@ class Subdivider:
         self.depth = depth
         self.steps = 0 if creases else capi.NOCREASES
         self.ds = capi.DeviceSubd(self.cage, depth)
-        self.ds.refine(capi.HALFEDGES | capi.CREASES)
+        # the topology pass runs on the stream that is current now; the
+        # points and adjoint kernels may later run on another torch stream,
+        # so wait for it here
+        self.ds.refine(capi.HALFEDGES | capi.CREASES, _stream())
+        self.ds.synchronize()
         n = self.ds.n
         self.V0 = n["V"][0]
         self.VD = n["V"][depth]

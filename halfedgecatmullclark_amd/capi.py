@@ -167,14 +167,21 @@ class HostSubd:
             raise CCError(f"{name}: {msg}")
         return out
 
-    def arrays(self) -> dict:
+    def arrays(self, keys=("halfedges", "creases", "points")) -> dict:
+        """Copies of the subd's host arrays named in ``keys``."""
         s = self.subd.contents
         nh, nc, nv = self.n["Hcum"][self.depth], self.n["Ccum"][self.depth], self.n["Vcum"][self.depth]
-        he = np.ctypeslib.as_array(ctypes.cast(s.halfedges, ctypes.POINTER(ctypes.c_int32)), (nh * 4,))
-        cr = np.ctypeslib.as_array(ctypes.cast(s.creases, ctypes.POINTER(ctypes.c_int32)), (nc * 3,))
-        pt = np.ctypeslib.as_array(ctypes.cast(s.vertexPoints, ctypes.POINTER(ctypes.c_float)), (nv * 3,))
-        return {"halfedges": he.view(HALFEDGE_DTYPE).copy(), "creases": cr.view(CREASE_DTYPE).copy(),
-                "points": pt.reshape(nv, 3).copy(), "counts": self.n}
+        out = {"counts": self.n}
+        if "halfedges" in keys:
+            he = np.ctypeslib.as_array(ctypes.cast(s.halfedges, ctypes.POINTER(ctypes.c_int32)), (nh * 4,))
+            out["halfedges"] = he.view(HALFEDGE_DTYPE).copy()
+        if "creases" in keys:
+            cr = np.ctypeslib.as_array(ctypes.cast(s.creases, ctypes.POINTER(ctypes.c_int32)), (nc * 3,))
+            out["creases"] = cr.view(CREASE_DTYPE).copy()
+        if "points" in keys:
+            pt = np.ctypeslib.as_array(ctypes.cast(s.vertexPoints, ctypes.POINTER(ctypes.c_float)), (nv * 3,))
+            out["points"] = pt.reshape(nv, 3).copy()
+        return out
 
     def close(self):
         if getattr(self, "subd", None):
@@ -195,6 +202,39 @@ def refine_host(cage: Cage, depth: int, *drivers: str) -> dict:
         return s.run(*(drivers or ("ccs_Refine_Scatter",))).arrays()
     finally:
         s.close()
+
+
+def refine_file(path: str, depth: int, *drivers: str) -> dict:
+    """Load a ``.ccm`` cage with the library's ``ccm_Load``, refine it with
+    host-semantics drivers (default ``ccs_Refine_Scatter``) on a subd from
+    ``ccs_Create``, and return copies of the subd's arrays."""
+    L = lib()
+    mesh = L.ccm_Load(os.fsencode(path))
+    if not mesh:
+        raise CCError(f"ccm_Load({path}) failed: {last_error()[1]}")
+    try:
+        m = mesh.contents
+        n = level_counts(m.vertexCount, m.halfedgeCount, m.edgeCount, m.faceCount, depth)
+        subd = L.ccs_Create(mesh, depth)
+        if not subd:
+            raise CCError(f"ccs_Create failed: {last_error()[1]}")
+        try:
+            for name in drivers or ("ccs_Refine_Scatter",):
+                getattr(L, name)(subd)
+                code, msg = last_error()
+                if code:
+                    raise CCError(f"{name}: {msg}")
+            s = subd.contents
+            nh, nc, nv = n["Hcum"][depth], n["Ccum"][depth], n["Vcum"][depth]
+            he = np.ctypeslib.as_array(ctypes.cast(s.halfedges, ctypes.POINTER(ctypes.c_int32)), (nh * 4,))
+            cr = np.ctypeslib.as_array(ctypes.cast(s.creases, ctypes.POINTER(ctypes.c_int32)), (nc * 3,))
+            pt = np.ctypeslib.as_array(ctypes.cast(s.vertexPoints, ctypes.POINTER(ctypes.c_float)), (nv * 3,))
+            return {"halfedges": he.view(HALFEDGE_DTYPE).copy(), "creases": cr.view(CREASE_DTYPE).copy(),
+                    "points": pt.reshape(nv, 3).copy(), "counts": n}
+        finally:
+            L.ccs_Release(subd)
+    finally:
+        L.ccm_Release(mesh)
 
 
 class DeviceSubd:

@@ -14,8 +14,9 @@
 //      gradient;
 //   4. vertices: the owner of each vertex gathers its level-d gradient from
 //      the vertex, edge and face rules.
-// The cage step (small, polygons of any size) keeps per-halfedge float
-// atomics.  Only the stored topology and creases are read; nothing of the
+// The cage step (small, polygons of any size) runs as ordered gathers over
+// the host-built ring lists (per-halfedge float atomics only for a cage
+// whose walks do not close).  Only the stored topology and creases are read; nothing of the
 // forward points is kept or recomputed.  Kernels are templated on the scalar
 // type so the same code runs in float and, for gradient checks, in double.
 #include "cc_adjoint.h"

@@ -2,9 +2,9 @@
 // and level.
 //
 // A launch covers a list of jobs (one per cage taking part) and a prefix sum
-// `start` of the workgroups each job needs (start[0] = 0, start[n] = grid).
-// A workgroup finds its job by binary search over `start`; the search reads
-// the same addresses in every lane, so it runs on the scalar unit.  All
+// `start` of the workgroups each job needs (start[0] = 0, start[n] = grid),
+// followed by the job index of every workgroup, so a workgroup finds its job
+// with one load (the same address in every lane: a scalar load).  All
 // per-cage offsets are 64-bit: a batch holds billions of halfedges, while
 // the IDs inside one cage stay 32-bit as in the reference.
 #pragma once

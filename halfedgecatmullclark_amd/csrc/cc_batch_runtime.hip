@@ -304,7 +304,11 @@ void ccb_Batch::free_all() {
     if (stream) (void)hipStreamDestroy(stream);
 }
 
+// A graph exec may still be running on any stream the caller used: wait for
+// the device before destroying it.
 void ccb_Batch::drop_graphs() {
+    if (graphs.empty()) return;
+    (void)hipDeviceSynchronize();
     for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
     graphs.clear();
 }
@@ -324,8 +328,11 @@ bool ccb_Batch::alloc_adjoint() {
     return build_tables();
 }
 
-// Job tables of every launch, for the current tile options.
+// Job tables of every launch, for the current tile options.  The tables are
+// rewritten in place, so launches already queued on any stream (the batch's
+// own or the caller's) must have finished first.
 bool ccb_Batch::build_tables() {
+    CC_TRY(hipDeviceSynchronize(), "batch tables: wait for queued launches");
     TableBuilder tb;
     erep_tables = erep_kept;
     auto off = [&](const void *p) -> int64_t { return static_cast<const char *>(p) - arena; };

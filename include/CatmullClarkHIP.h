@@ -22,12 +22,12 @@
 #ifndef CC_INCLUDE_CC_HIP_H
 #define CC_INCLUDE_CC_HIP_H
 
+#include <stdint.h>
+#include <stdbool.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
-
-#include <stdint.h>
-#include <stdbool.h>
 
 /* ---- data layouts (CatmullClark.h:17-71, :136-142) ---------------------- */
 typedef union {

@@ -336,3 +336,44 @@ def test_batch_timings_are_the_last_run(capi):
     assert [(n, lv) for n, lv, _ in first] == [(n, lv) for n, lv, _ in second]
     assert all(ms >= 0.0 for _, _, ms in second)
     b.close()
+
+
+def oracle_jacobian(cage, depth, flags=O.ALL):
+    """J (V_D x V_0) of the double-precision oracle, one basis vector per
+    cage point (the points are linear in the cage points, per coordinate)."""
+    cols = []
+    for i in range(cage.V):
+        basis = cage.copy()
+        basis.vertex_points = np.zeros((cage.V, 3), np.float64)
+        basis.vertex_points[i, 0] = 1.0
+        cols.append(O.level(O.refine(basis, depth, flags, real=np.float64), "points", depth)[:, 0])
+    return np.stack(cols, axis=1)
+
+
+def test_batch_adjoint_against_oracle_jacobian(capi, torch):
+    """The batched float adjoint of a few small cages against J^T g from the
+    double oracle's basis-vector Jacobian, to 1e-5 of the gradient's scale
+    (float sums of up to a few hundred terms)."""
+    cases = [(C.cube(0), 3), (C.torus(6, 5, seed=2), 3), (C.irregular(2, seed=8), 2), tiny_cages()[1]]
+    cages = [c for c, _ in cases]
+    depths = [d for _, d in cases]
+    b = capi.Batch(cages, depths)
+    b.refine(capi.ALL)
+    nf = b.count(capi.COUNT_FINEST_POINTS)
+    rng = np.random.default_rng(9)
+    g = rng.normal(size=(nf, 3)).astype(np.float32)
+    gd = torch.from_numpy(g).cuda()
+    out = torch.empty((b.count(capi.COUNT_CAGE_POINTS), 3), device="cuda", dtype=torch.float32)
+    b.refine_points_adjoint(gd.data_ptr(), out.data_ptr(), 0)
+    b.synchronize()
+    out = out.cpu().numpy()
+    for i, (cage, depth) in enumerate(cases):
+        J = oracle_jacobian(cage, depth)
+        o, n = b.offset(capi.COUNT_FINEST_POINTS, i), b.count(capi.COUNT_FINEST_POINTS, i)
+        assert J.shape == (n, cage.V)
+        want = J.T @ g[o:o + n].astype(np.float64)
+        oc = b.offset(capi.COUNT_CAGE_POINTS, i)
+        got = out[oc:oc + cage.V]
+        err = float(np.abs(got - want).max()) / float(np.abs(want).max())
+        assert err <= 1e-5, (i, err)
+    b.close()

@@ -8,16 +8,20 @@
   refined to depth 7: 247 M halfedges;
 - the batch of 256 uneven cages (24..60,000 halfedges, depths 4..6, 5.26 G
   halfedges in one arena, so the later cages sit past 2^32 halfedges):
-  every cage of at most 2 M refined halfedges (124 of them, spread over the
-  whole arena) and the last cage are checked against the oracle, and the
-  batched adjoint of every eighth of them and of the last cage against the
-  single-cage adjoint.
+  every cage against the oracle's SHA-256 digests (about 100 GB of output
+  in all, too much for the oracle to hold next to it, so the digests are
+  committed), and the batched adjoint of every eighth of the cages of at
+  most 2 M refined halfedges and of the last cage against the single-cage
+  adjoint.
 
 The oracle runs its gather loops on all host threads (up to 16); its scatter
 sums stay sequential, so its points equal the sequential reference's.  Sizes
 like these are where 32-bit index and byte-offset mistakes show, which the
 small cages of the other parity tests cannot reach.
 """
+import concurrent.futures
+import hashlib
+import json
 import os
 
 import numpy as np
@@ -31,6 +35,7 @@ from tests.helpers import assert_bits_equal
 pytestmark = pytest.mark.gpu
 
 THREADS = min(16, os.cpu_count() or 1)
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 CASES = [
     ("torus64x32_d8_scatter", lambda: C.torus(64, 32), 8, 15, O.ALL),
@@ -60,21 +65,43 @@ def test_full_size_bitexact(capi, label, make, depth, steps, flags):
         got[k] = want[k] = None
 
 
+def _digest(a) -> str:
+    return hashlib.sha256(memoryview(np.ascontiguousarray(a).reshape(-1).view("u1"))).hexdigest()
+
+
 def test_batch_workload_bitexact(capi):
+    """Every one of the 256 cages (5.26 G halfedges in one arena, the later
+    cages past 2^32 halfedges) against the oracle's SHA-256 digests, written
+    by tests/golden/make_batch_digests.py."""
+    gold = json.load(open(os.path.join(HERE, "golden", "batch256_digests.json")))["cages"]
     cages, depths = C.batch_cages(256, seed=0)
+    assert len(gold) == len(cages) == 256
     b = capi.Batch(cages, depths)
     b.refine(capi.ALL)
     b.synchronize()
     hcum = [level_counts(c.V, c.H, c.E, c.F, d)["Hcum"][d] for c, d in zip(cages, depths)]
-    pick = [i for i, h in enumerate(hcum) if h <= 2_000_000] + [len(cages) - 1]
-    assert len(pick) > 100 and sum(hcum[:pick[-1]]) > 2 ** 32
-    for i in pick:
-        cage, depth = cages[i], depths[i]
-        got = b.download(i)
-        want = O.refine(cage, depth, O.ALL, threads=THREADS)
+    assert sum(hcum[:-1]) > 2 ** 32
+    bad = []
+
+    def check(i, got):
+        row = gold[i]
+        assert (row["H"], row["depth"]) == (cages[i].H, depths[i])
         for k in ("halfedges", "creases", "points"):
-            assert_bits_equal(got[k], want[k], f"cage {i} (H={cage.H}, depth {depth}) {k}")
+            if _digest(got[k]) != row[k]:
+                bad.append(f"cage {i} (H={cages[i].H}, depth {depths[i]}) {k}")
+
+    # downloads in order, digests on a few threads (hashlib releases the GIL);
+    # at most 4 cages (up to 6 GB each) wait in host memory
+    with concurrent.futures.ThreadPoolExecutor(4) as pool:
+        pending = []
+        for i in range(len(cages)):
+            pending.append(pool.submit(check, i, b.download(i)))
+            if len(pending) >= 4:
+                pending.pop(0).result()
+        for f in pending:
+            f.result()
     b.close()
+    assert not bad, bad[:10]
 
 
 def test_batch_workload_adjoint_matches_single_cage(capi):

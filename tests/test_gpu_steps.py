@@ -1,0 +1,135 @@
+"""The step-wise host entry points on the GPU, through the C ABI.
+
+- The reference's own caller sequence (examples/subd_cpu.c:220-253):
+  ccs_RefineCreases, ccs_RefineHalfedges, ccs_RefineVertexPoints_Scatter,
+  ccs_RefineVertexUvs on one cc_Subd, against the oracle's ccs_Refine_Scatter.
+- Then the cage points move on the host (the topology stays) and each
+  points-only driver runs again: ccs_RefineVertexPoints_{Scatter, Gather,
+  NoCreases_Scatter, NoCreases_Gather}.  These read the subd's host
+  topology and creases, which the library uploads first; every point must
+  equal the oracle's on the moved cage bit for bit.
+- The five cage-step entry points (ccs_RefineCage*), each writing level 1
+  only, against level 1 of the oracle.
+
+Bit-exact everywhere: the scatter sums run in ascending halfedge order, as
+the reference's sequential scatter does.
+"""
+import numpy as np
+import pytest
+
+from halfedgecatmullclark_amd import cage as C
+from oracle import oracle as O
+from tests.helpers import assert_bits_equal, assert_bytes_equal, small_cages
+
+pytestmark = pytest.mark.gpu
+
+POINTS_DRIVERS = [
+    ("ccs_RefineVertexPoints_Scatter", O.ALL),
+    ("ccs_RefineVertexPoints_Gather", O.ALL | O.GATHER),
+    ("ccs_RefineVertexPoints_NoCreases_Scatter", O.ALL | O.NOCREASES),
+    ("ccs_RefineVertexPoints_NoCreases_Gather", O.ALL | O.GATHER | O.NOCREASES),
+]
+SEQUENCE = ("ccs_RefineCreases", "ccs_RefineHalfedges", "ccs_RefineVertexPoints_Scatter", "ccs_RefineVertexUvs")
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from halfedgecatmullclark_amd import capi as api
+    api.lib()
+    return api
+
+
+def moved(cage, seed):
+    rng = np.random.default_rng(seed)
+    c = cage.copy()
+    c.vertex_points = (c.vertex_points + rng.normal(0, 0.05, c.vertex_points.shape)).astype(np.float32)
+    return c
+
+
+def check_sequence_and_points_only(capi, cage, depth, compare, threads=0):
+    s = capi.HostSubd(cage, depth)
+    try:
+        s.run(*SEQUENCE)
+        got = s.arrays()
+        want = O.refine(cage, depth, O.ALL, threads=threads)
+        for k in ("halfedges", "creases", "points"):
+            compare(got[k], want[k], f"subd_cpu sequence {k}")
+        del got, want
+        # the cage points move, the topology in the subd's host arrays stays
+        c2 = moved(cage, 7)
+        s.mesh.points[:] = c2.vertex_points
+        for drv, flags in POINTS_DRIVERS:
+            s.run(drv)
+            got = s.arrays(("points",))["points"]
+            want = O.refine(c2, depth, flags, threads=threads)["points"]
+            compare(got, want, f"{drv} after moving the cage points")
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("name,cage,depth", small_cages(), ids=[c[0] for c in small_cages()])
+def test_subd_cpu_sequence_then_points_only(capi, name, cage, depth):
+    check_sequence_and_points_only(capi, cage, depth, assert_bytes_equal)
+
+
+def test_subd_cpu_sequence_torus64x32_d8(capi):
+    """The same at the bandwidth workload's full size: torus 64x32 to depth
+    8 (716 M halfedges written back into the host arrays)."""
+    import os
+    check_sequence_and_points_only(capi, C.torus(64, 32), 8, assert_bits_equal,
+                                   threads=min(16, os.cpu_count() or 1))
+
+
+CAGE_STEPS = [
+    ("ccs_RefineCageHalfedges", "halfedges", O.HALFEDGES),
+    ("ccs_RefineCageCreases", "creases", O.CREASES),
+    ("ccs_RefineCageVertexPoints_Scatter", "points", O.ALL),
+    ("ccs_RefineCageVertexPoints_Gather", "points", O.ALL | O.GATHER),
+]
+
+
+def ids(he):
+    """twin, edge and vertex IDs of halfedge records (without the uvID)"""
+    return np.stack([he["twinID"], he["edgeID"], he["vertexID"]], axis=1)
+
+
+@pytest.mark.parametrize("name,cage,depth", small_cages(), ids=[c[0] for c in small_cages()])
+def test_cage_step_entries(capi, name, cage, depth):
+    for drv, key, flags in CAGE_STEPS:
+        s = capi.HostSubd(cage, depth)
+        try:
+            s.run(drv)
+            got = O.level(s.arrays(), key, 1)
+        finally:
+            s.close()
+        want = O.level(O.refine(cage, 1, flags), key, 1)
+        if key == "halfedges" and cage.U > 0:
+            # uvIDs are left to ccs_RefineCageVertexUvs
+            got, want = ids(got), ids(want)
+        assert_bytes_equal(got, want, f"{name} {drv}")
+    # uvs after the halfedges: the full level-1 records
+    s = capi.HostSubd(cage, depth)
+    try:
+        s.run("ccs_RefineCageHalfedges", "ccs_RefineCageVertexUvs")
+        got = O.level(s.arrays(), "halfedges", 1)
+    finally:
+        s.close()
+    want = O.level(O.refine(cage, 1, O.HALFEDGES | O.UVS), "halfedges", 1)
+    if cage.U == 0:
+        got, want = ids(got), ids(want)
+    assert_bytes_equal(got, want, f"{name} cage halfedges + uvs")
+
+
+def test_cube_depth6(capi):
+    """The plain CPU workload: the cube cage (corners +-1 with N(0, 0.05)
+    jitter, 4 edges creased U[0, 3], unit-square uvs) at depth 6, 98,304
+    halfedges at the finest level, through every ccs_Refine_* driver."""
+    cube = C.cube(0)
+    for drv, flags in (("ccs_Refine_Scatter", O.ALL), ("ccs_Refine_Gather", O.ALL | O.GATHER),
+                       ("ccs_Refine_NoCreases_Scatter", O.ALL | O.NOCREASES),
+                       ("ccs_Refine_NoCreases_Gather", O.ALL | O.GATHER | O.NOCREASES)):
+        got = capi.refine_host(cube, 6, drv)
+        want = O.refine(cube, 6, flags)
+        assert got["counts"]["H"][6] == 98304
+        for k in ("halfedges", "creases", "points"):
+            assert_bytes_equal(got[k], want[k], f"cube depth 6 {drv} {k}")
