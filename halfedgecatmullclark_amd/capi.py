@@ -103,6 +103,7 @@ def lib() -> ctypes.CDLL:
         "ccm_Load": (P(cc_Mesh), [ctypes.c_char_p]),
         "ccm_Save": (ctypes.c_bool, [P(cc_Mesh), ctypes.c_char_p]),
         "ccm_Release": (None, [P(cc_Mesh)]),
+        "ccd_RefineNoAlloc": (i32, [P(cc_Mesh), P(cc_Subd), i32, vp]),
     }
     for name in HOST_DRIVERS:
         sigs[name] = (None, [P(cc_Subd)])

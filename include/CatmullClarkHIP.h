@@ -369,6 +369,29 @@ const char *ccb_TimingName(const ccb_Batch *batch, int32_t i);
 int32_t ccb_TimingLevel(const ccb_Batch *batch, int32_t i);
 float ccb_TimingMs(const ccb_Batch *batch, int32_t i);
 
+/* ---- refinement that allocates nothing -------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  The refinement of ccd_Refine (same kernels, same bits,
+ * `steps` as there), on arrays the caller already holds in device memory:
+ *   deviceCage  a host cc_Mesh whose five counts are set and whose seven
+ *               array pointers are device addresses, in cc_Mesh's layout
+ *               (uvs may be NULL when uvCount is 0);
+ *   deviceSubd  a host cc_Subd whose vertexPoints, halfedges and creases are
+ *               device arrays sized for levels 1..maxDepth (the
+ *               ccs_Cumulative*CountAtDepth counts); its cage is not read.
+ * The kernels' scratch comes from one array of CCD_NOALLOC_SCRATCH_BYTES per
+ * GPU, which the library holds until the process ends.  A cage and depth
+ * needing more are refused (error 15, nothing launched); the need is
+ *   4 (V0 + F0 + H0 2^(D-1) + 1) + H_{D-1} / 8 + (H_1 + ... + H_{D-2}) / 8
+ * bytes (each part rounded up to 256 bytes, the last one only for D >= 4),
+ * about 26 MB for a 8,192-halfedge cage at depth 8.  Every call on one GPU
+ * shares that GPU's array: calls that overlap in time on one GPU race, so a
+ * caller serializes them (one stream, or events).  Asynchronous on `stream`;
+ * returns 0 or an error code (also in ccs_LastError). */
+#define CCD_NOALLOC_SCRATCH_BYTES (64u << 20)
+int32_t ccd_RefineNoAlloc(const cc_Mesh *deviceCage, const cc_Subd *deviceSubd, int32_t steps, void *stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif
