@@ -23,7 +23,8 @@ Also reported on the same line:
   - roofline of the dominant kernel (per-kernel HIP events on the launch
     stream over the timed steps, algorithmic bytes per launch);
   - the reference's own OpenMP CPU path (CatmullClark.h built by
-    oracle/Makefile) timed on this host's cores on the same torus at depth 6.
+    oracle/Makefile) timed on this host's cores on the same workload
+    (torus 64x32 to depth 8), per stage and in total.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
 """
@@ -89,36 +90,52 @@ def refinement_bytes(n, D):
     return total
 
 
-def cpu_baseline(seconds_budget: float = 20.0):
-    """The reference's OpenMP ccs_Refine_Scatter on this host (torus 64x32,
-    depth 6), repeated up to the time budget."""
+def adjoint_bytes(n, D):
+    """Compulsory bytes of the adjoint of the points (all steps d+1 -> d,
+    d = D-1 .. 0): per step the level-d topology (16 B per halfedge; 28 B per
+    cage halfedge at d = 0) and creases (12 B) read once, the level-(d+1)
+    gradient (12 B per point) read once and the level-d gradient written
+    once."""
+    total = 0
+    for d in range(D):
+        rec = 28 if d == 0 else 16
+        total += rec * n["H"][d] + 12 * n["V"][d + 1] + 12 * n["V"][d] + 12 * n["C"][d]
+    return total
+
+
+def cpu_baseline(depth: int = 8, runs: int = 3):
+    """The reference's OpenMP ccs_Refine_Scatter on this host's cores, on the
+    same workload (torus 64x32 to depth 8), timed per stage and in total by
+    oracle/cpu_baseline.py in a child process, so that OMP_PROC_BIND=close
+    and OMP_PLACES=cores are in place before OpenMP starts.  Threads: every
+    CPU this process may run on (its affinity mask), within the host's CPU
+    share for this GPU when the environment states one in OMP_NUM_THREADS."""
+    import subprocess
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(affinity, share) if share > 0 else affinity
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--depth", str(depth), "--runs", str(runs)]
     try:
-        import numpy as np  # noqa: F401
-        from halfedgecatmullclark_amd import cage as C
-        from oracle import ref
+        r = subprocess.run(cmd, cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
+        res = json.loads(r.stdout.strip().splitlines()[-1])
     except Exception as e:  # pragma: no cover
-        return {"value": None, "unit": "refined halfedges/s", "cores": 0, "kind": "reference",
+        return {"value": None, "unit": "refined halfedges/s", "cores": affinity, "kind": "reference",
                 "sample": f"unavailable: {e}"}
-    if not ref.available():
-        return {"value": None, "unit": "refined halfedges/s", "cores": 0, "kind": "reference",
-                "sample": "reference build (oracle/_ref) not present"}
-    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 16)
-    depth = 6
-    r = ref.RefSubd(C.torus(64, 32), depth)
-    r.run("ccs_Refine_Scatter", threads=cores)  # warm up (page faults)
-    times = []
-    t_start = time.perf_counter()
-    while time.perf_counter() - t_start < seconds_budget and len(times) < 50:
-        t0 = time.perf_counter()
-        r.run("ccs_Refine_Scatter", threads=cores)
-        times.append(time.perf_counter() - t0)
-    r.close()
-    times.sort()
-    med = times[len(times) // 2]
-    hc = r.n["Hcum"][depth]
-    return {"value": hc / med, "unit": "refined halfedges/s", "cores": cores, "kind": "reference",
-            "sample": f"ccs_Refine_Scatter (OpenMP, {cores} threads), torus 64x32 at depth 6 "
-                      f"({hc} refined halfedges), median of {len(times)} runs: {med * 1e3:.1f} ms"}
+    if "total" not in res:
+        return {"value": None, "unit": "refined halfedges/s", "cores": affinity, "kind": "reference",
+                "sample": res.get("error", "no result")}
+    st = res["stages"]
+    return {"value": res["refined_halfedges_per_s"], "unit": "refined halfedges/s", "cores": res["threads"],
+            "kind": "reference", "cpu_model": res["cpu_model"], "cpus_online": res["cpus_online"],
+            "total_ms": res["total"], "stages_ms": {k: v["median_ms"] for k, v in st.items()},
+            "sample": f"CatmullClark.h ccs_Refine_Scatter (OpenMP, {res['threads']} threads, OMP_PROC_BIND=close, "
+                      f"OMP_PLACES=cores, gcc -O3 default x86-64 target) on {res['cpu_model']} "
+                      f"({res['cpus_online']} CPUs online, {res['cpus_usable']} usable): torus 64x32 to depth "
+                      f"{depth} ({res['refined_halfedges']} refined halfedges), median of {res['total']['runs']} "
+                      f"runs after one untimed run: {res['total']['median_ms']:.0f} ms; stages once each "
+                      f"(creases, halfedges, points, uvs as examples/subd_cpu.c): "
+                      + ", ".join(f"{v['median_ms']:.0f}" for v in st.values()) + " ms"}
 
 
 def batched(torch, capi, C, steps: int, stream):
@@ -268,6 +285,38 @@ def main():
     ms_events, kern = timed(capi.ALL, args.steps, 1, per_kernel=True)
     ms_gather, _ = timed(capi.ALL | capi.GATHER, max(3, args.steps // 2), 1)
 
+    # the adjoint on the same workload: points-only forward (the topology
+    # stays), the adjoint of the points, and the two in a row
+    ds.refine(capi.ALL, sptr)
+    cage_pts = torch.from_numpy(cage.vertex_points.astype("float32")).to(dev)
+    pts_out = torch.empty((n["Vcum"][D], 3), device=dev)
+    grad_fin = torch.randn((n["V"][D], 3), device=dev)
+    grad_cage = torch.empty((n["V"][0], 3), device=dev)
+
+    def ev_time(f, k):
+        for _ in range(2):
+            f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(k):
+            f()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    k1 = max(5, args.steps // 2)
+    fwd_pts_ms = ev_time(lambda: ds.refine_points(cage_pts.data_ptr(), pts_out.data_ptr(), 0, sptr), k1)
+    adj_ms = ev_time(lambda: ds.refine_points_adjoint(grad_fin.data_ptr(), grad_cage.data_ptr(), 0, sptr), k1)
+    adj_bytes = adjoint_bytes(n, D)
+    adjoint_d8 = {"workload": "torus 64x32 cage, depth 8: points-only forward (ccd_RefinePoints) and the adjoint "
+                              "of the points (ccd_RefinePointsAdjoint), topology fixed",
+                  "fwd_points_ms": fwd_pts_ms, "adj_ms": adj_ms, "fwd_adj_ms": fwd_pts_ms + adj_ms,
+                  "ratio": (fwd_pts_ms + adj_ms) / fwd_pts_ms, "ratio_vs_full_refinement": (fwd_pts_ms + adj_ms) / ms_scatter,
+                  "adj_algorithmic_bytes": adj_bytes, "adj_gbs": adj_bytes / (adj_ms * 1e-3) / 1e9,
+                  "adj_roofline_frac": adj_bytes / (adj_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    del pts_out, grad_fin, grad_cage
+
     # dominant kernel: largest mean duration among the launches of one step;
     # the per-level kernels run in level order, each level starting with its
     # fused or patch-local kernel
@@ -361,6 +410,7 @@ def main():
                    "depth": D, "refined_halfedges": hcum, "finest_halfedges": n["H"][D]},
         "hbm_gbs_algorithmic": refinement_bytes(n, D) / (ms_scatter * 1e-3) / 1e9,
         "gather": {"ms_per_step": ms_gather, "value": hcum / (ms_gather * 1e-3)},
+        "adjoint_d8": adjoint_d8,
         "fwd_adjoint": {"workload": "torus 32x16 cage, depth 5, loss 0.5*||S(V)-T||^2",
                         "fwd_ms": res["fwd"], "fwd_adj_ms": res["fwd_adj"],
                         "ratio": res["fwd_adj"] / res["fwd"]},

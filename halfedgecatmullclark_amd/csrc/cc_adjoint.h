@@ -6,9 +6,20 @@
 
 namespace cch {
 
+// The vertex rule of one level-d vertex, enough to give the weights of any
+// halfedge of its ring from that halfedge's own and previous sharpness:
+// valence, boundary and rule (0 smooth, 1 corner, 2 crease) packed in nb,
+// and the first two nonzero sharpness values of the ring (s1 on edge e1).
+// With the crease rule exactly two ring edges are sharp, so their sum is
+// s1 + s2 and a ring edge other than e1 with nonzero sharpness is the other.
+template <typename T>
+struct VParam {
+    T s1, s2;
+    int32_t nb, e1;
+};
+
 // Gradients of one level step d -> d+1: gV, gF, gE are the slices of the
-// level-(d+1) gradient (the face and edge slices receive the vertex and edge
-// rules' shares), gP receives the level-d gradient.
+// level-(d+1) gradient, gP receives the level-d gradient.
 template <typename T>
 struct AdjArgs {
     const He *he;
@@ -17,10 +28,28 @@ struct AdjArgs {
     int32_t H, C, V;   // halfedges, creases and vertices of the level
     T *gV, *gF, *gE;
     T *gP;
-    T *W;          // per-halfedge vertex-rule weights (a, b, g, dl): 4 H scalars
+    // scratch: the cage step keeps per-halfedge vertex-rule weights
+    // (a, b, g, dl) here, 4 H scalars; a level d >= 1 keeps the parameters
+    // of its vertices (VP, 4 V scalars) followed by the edge gradients
+    // after the vertex rule's shares (gEu, 3 E scalars)
+    T *W;
+    VParam<T> *VP;
+    T *gEu;
     // edge-representative bits of levels 1..d-1 (FusedArgs::erep), or null
     const uint64_t *erep = nullptr;
 };
+
+// scalars of adjoint scratch (AdjArgs::W) a cage refined to depth D needs:
+// the cage step's weights and every level's parameters and edge gradients,
+// a multiple of 4 so that consecutive cages' scratch stays 16-byte aligned
+inline int64_t adj_scratch_scalars(const Counts &n, int D) {
+    int64_t m = 4 * n.H[0];
+    for (int d = 1; d < D; ++d) {
+        const int64_t need = 4 * n.V[d] + 3 * n.E[d];
+        if (need > m) m = need;
+    }
+    return (m + 3) & ~int64_t(3);
+}
 
 struct AdjPlan {
     const Counts *n;
@@ -35,7 +64,8 @@ struct AdjPlan {
 };
 
 // dL/d(cage points) from dL/d(level-D points): V_D x 3 in, V_0 x 3 out.
-// bufA holds V_D x 3 and bufB V_{D-1} x 3 scalars of scratch, W 4 max(H_{D-1}, H_0).
+// bufA holds V_D x 3 and bufB V_{D-1} x 3 scalars of scratch, W
+// adj_scratch_scalars(n, D).
 // When sharp is given (E_0 + Ccum(D-1) scalars) the sharpness of every level
 // is refined again in T and used instead of the float crease records.
 template <typename T>

@@ -64,48 +64,14 @@ struct RingT {
     T sall;
 };
 
-// level d >= 1 scatter rule (creased :2568-2607, corner test first; smooth
-// :2339-2348)
-template <typename T>
-__device__ __forceinline__ VW<T> level_vertex_weights(const RingT<T> &rp, T ts, T ps, int32_t edge, bool creased) {
-    VW<T> w{0, 0, 0, 0};
-    const T n = rp.n;
-    if (!creased) {
-        const T iw = (T)1 / n, s = rp.boundary ? (T)0 : (T)1;
-        w.a = iw - (T)3 * iw * iw * s;
-        w.b = (T)4 * iw * iw * s;
-        w.g = -iw * iw * s;
-        return w;
-    }
-    const T nn = rp.boundary ? n + (T)1 : n;
-    if (rp.cc >= (T)3 || nn == (T)2) {
-        w.a = (T)1 / n;
-        return w;
-    }
-    if (rp.cc <= (T)1) {
-        w.a = (n - (T)3) / (n * n);
-        w.b = (T)4 / (n * n);
-        w.g = (T)-1 / (n * n);
-        return w;
-    }
-    T avg = rp.sall;
-    if (rp.boundary) avg = edge == rp.e1 ? rp.s2 : (edge == rp.e2 ? rp.s1 : rp.s1 + rp.s2);
-    const T t = satT(avg * (T)0.5);
-    const T cw = sgnT(ts), pcw = rp.boundary ? sgnT(ps) : (T)0;
-    w.a = ((T)1 - t) / n + t * (T)0.25 * (cw + pcw);
-    w.b = t * (T)0.25 * cw;
-    w.dl = t * (T)0.25 * pcw;
-    return w;
-}
-
 }  // namespace
 
 // ---- level d >= 1, gather form -------------------------------------------------
 template <typename T>
 struct W4 { T a, b, g, dl; };
 
-// The weights live as four arrays of the level's H scalars, a | b | g | dl,
-// so that each pass reads only the one it needs.
+// The cage step's weights live as four arrays of the cage's H scalars,
+// a | b | g | dl, so that each pass reads only the one it needs.
 template <typename T>
 __device__ __forceinline__ void w_store(const AdjArgs<T> &a, int32_t h, T wa, T wb, T wg, T wdl) {
     const int64_t H = a.H;
@@ -140,29 +106,9 @@ __device__ __forceinline__ void edge_weights(const AdjArgs<T> &a, int32_t edge, 
     }
 }
 
-// ring parameters of a vertex from its fan (valence, creases, boundary)
-template <typename T, int CREASED>
-__device__ __forceinline__ RingT<T> ring_params(const AdjArgs<T> &a, const DCage &c, const Fan &f) {
-    const He *he = a.he;
-    RingT<T> rp;
-    rp.n = (T)f.n; rp.cc = 0; rp.boundary = f.boundary; rp.s1 = rp.s2 = 0; rp.e1 = rp.e2 = -1; rp.sall = 0;
-    if (CREASED) {
-        for (int i = (f.boundary ? -1 : 0); i < f.n; ++i) {
-            const int32_t e = i < 0 ? he[fan_at(f, c, 0)].edge : he[qprev(fan_at(f, c, i))].edge;
-            const T se = sharp_at(a, e);
-            rp.cc += sgnT(se);
-            if (se != (T)0) {
-                if (rp.e1 < 0) { rp.s1 = se; rp.e1 = e; }
-                else if (rp.e2 < 0) { rp.s2 = se; rp.e2 = e; }
-            }
-            rp.sall += se;
-        }
-    }
-    return rp;
-}
-
-// ring crease parameter update for the sharpness se of edge e (the order of
-// ring_params)
+// ring crease parameter update for the sharpness se of edge e, in ring
+// order (the edge of the first halfedge on a boundary, then the previous
+// edge of every fan halfedge)
 template <typename T>
 __device__ __forceinline__ void ring_add(RingT<T> &rp, T se, int32_t e) {
     rp.cc += sgnT(se);
@@ -173,114 +119,283 @@ __device__ __forceinline__ void ring_add(RingT<T> &rp, T se, int32_t e) {
     rp.sall += se;
 }
 
-// pass 1: the vertex rule's weights of every halfedge of a vertex's ring
+template <typename T>
+__device__ __forceinline__ RingT<T> ring_init(int n, bool boundary) {
+    RingT<T> rp;
+    rp.n = (T)n; rp.cc = 0; rp.boundary = boundary; rp.s1 = rp.s2 = 0; rp.e1 = rp.e2 = -1; rp.sall = 0;
+    return rp;
+}
+
+// ring parameters of any fan from its records (valence, creases, boundary)
 template <typename T, int CREASED>
-__device__ __forceinline__ void adj_ring_weights(const AdjArgs<T> &a, const DCage &c, const Fan &f) {
+__device__ __forceinline__ RingT<T> ring_params(const AdjArgs<T> &a, const DCage &c, const Fan &f) {
     const He *he = a.he;
+    RingT<T> rp = ring_init<T>(f.n, f.boundary);
+    if (!CREASED) return rp;
     if (f.n <= 4) {
-        // a fan of at most four: every entry's loads issued before any is
-        // used, the same sums in the same order as below
-        int32_t r[4], re[4], pe[4];
-        T ts[4], ps[4];
+        // a fan of at most four: every entry's loads issued before any is used
+        int32_t r[4], re0 = 0, pe[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) r[q] = fan_at(f, c, q < f.n ? q : 0);
+        re0 = he[r[0]].edge;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            re[q] = he[r[q]].edge;
-            pe[q] = he[qprev(r[q])].edge;
-        }
+        for (int q = 0; q < 4; ++q) pe[q] = he[qprev(r[q])].edge;
+        T s0 = sharp_at(a, re0), ps[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            ts[q] = CREASED ? sharp_at(a, re[q]) : (T)0;
-            ps[q] = CREASED ? sharp_at(a, pe[q]) : (T)0;
-        }
-        RingT<T> rp;
-        rp.n = (T)f.n; rp.cc = 0; rp.boundary = f.boundary; rp.s1 = rp.s2 = 0; rp.e1 = rp.e2 = -1; rp.sall = 0;
-        if (CREASED) {
-            if (f.boundary) ring_add(rp, ts[0], re[0]);
+        for (int q = 0; q < 4; ++q) ps[q] = sharp_at(a, pe[q]);
+        if (f.boundary) ring_add(rp, s0, re0);
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                if (i < f.n) ring_add(rp, ps[i], pe[i]);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (i >= f.n) break;
-            const VW<T> w = level_vertex_weights<T>(rp, ts[i], ps[i], re[i], CREASED);
-            w_store(a, r[i], w.a, w.b, w.g, w.dl);
-        }
-        return;
+        for (int q = 0; q < 4; ++q)
+            if (q < f.n) ring_add(rp, ps[q], pe[q]);
+        return rp;
     }
-    const RingT<T> rp = ring_params<T, CREASED>(a, c, f);
-    for (int i = 0; i < f.n; ++i) {
-        const int32_t r = fan_at(f, c, i);
-        const int32_t re = he[r].edge, pe = he[qprev(r)].edge;
-        const T ts = CREASED ? sharp_at(a, re) : (T)0;
-        const T ps = CREASED ? sharp_at(a, pe) : (T)0;
-        const VW<T> w = level_vertex_weights<T>(rp, ts, ps, re, CREASED);
-        w_store(a, r, w.a, w.b, w.g, w.dl);
+    for (int i = (f.boundary ? -1 : 0); i < f.n; ++i) {
+        const int32_t e = i < 0 ? he[fan_at(f, c, 0)].edge : he[qprev(fan_at(f, c, i))].edge;
+        ring_add(rp, sharp_at(a, e), e);
     }
+    return rp;
 }
 
-// pass 2: edge(h) for its owner h (the larger halfedge, or a boundary one):
-// the b shares of the halfedges on the edge and the dl shares of the
-// halfedges whose previous edge it is
+// ring parameters of a structural ring (four interior halfedges h[q] with
+// edges e[q] and sharpness s[q]; the previous edge of h[q] is e[q+1])
 template <typename T, int CREASED>
-__device__ __forceinline__ void adj_edge_at(const AdjArgs<T> &a, int32_t h) {
-    if (h >= a.H) return;
-    const He x = a.he[h];
-    if (x.twin >= 0 && h < x.twin) return;
-    const V3<T> *gV = reinterpret_cast<const V3<T> *>(a.gV);
-    V3<T> *gE = reinterpret_cast<V3<T> *>(a.gE);
-    V3<T> ge = gE[x.edge];
-    const int32_t nh = qnext(h);
-    const T bh = w_b(a, h), dn = CREASED ? w_dl(a, nh) : (T)0;   // smooth rules have no dl share
-    const V3<T> vh = gV[x.vert], vn = gV[a.he[nh].vert];
-    ge.x += bh * vh.x; ge.y += bh * vh.y; ge.z += bh * vh.z;
-    ge.x += dn * vn.x; ge.y += dn * vn.y; ge.z += dn * vn.z;
-    if (x.twin >= 0) {
-        const int32_t t = x.twin, nt = qnext(t);
-        const T bt = w_b(a, t), dt = CREASED ? w_dl(a, nt) : (T)0;
-        const V3<T> vt = gV[a.he[t].vert], vnt = gV[a.he[nt].vert];
-        ge.x += bt * vt.x; ge.y += bt * vt.y; ge.z += bt * vt.z;
-        ge.x += dt * vnt.x; ge.y += dt * vnt.y; ge.z += dt * vnt.z;
+__device__ __forceinline__ RingT<T> ring_params4(const int32_t (&e)[4], const T (&s)[4]) {
+    RingT<T> rp = ring_init<T>(4, false);
+    if (CREASED) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ring_add(rp, s[(i + 1) & 3], e[(i + 1) & 3]);
     }
-    gE[x.edge] = ge;
+    return rp;
 }
 
-// pass 3: face f (halfedges 4f..4f+3): the g shares of the vertex rule and
-// the cf shares of the edge rule
+// The packed parameters of a ring and the rule they select (level d >= 1
+// scatter rule, CatmullClark.h:2568-2607, corner test first).
+template <typename T>
+__device__ __forceinline__ VParam<T> vparam_of(const RingT<T> &rp, bool creased) {
+    int rule = 0;
+    if (creased) {
+        const T nn = rp.boundary ? rp.n + (T)1 : rp.n;
+        rule = (rp.cc >= (T)3 || nn == (T)2) ? 1 : (rp.cc <= (T)1 ? 0 : 2);
+    }
+    VParam<T> p;
+    p.s1 = rp.s1;
+    p.s2 = rp.s2;
+    p.e1 = rp.e1;
+    p.nb = (int32_t)rp.n | (rule << 28) | (rp.boundary ? (1 << 30) : 0);
+    return p;
+}
+
+// weights (alpha, beta, gamma, delta) of one ring halfedge's contribution
+// alpha*V + beta*E(its edge) + gamma*F(its face) + delta*E(its previous
+// edge) to the vertex rule (creased :2568-2607; smooth :2339-2348), from
+// the vertex's parameters and the halfedge's own (ts, edge) and previous
+// (ps) sharpness
+template <typename T>
+__device__ __forceinline__ VW<T> vparam_weights(const VParam<T> &p, T ts, T ps, int32_t edge, bool creased) {
+    VW<T> w{0, 0, 0, 0};
+    const T n = (T)(p.nb & 0x0FFFFFFF);
+    const bool boundary = ((p.nb >> 30) & 1) != 0;
+    if (!creased) {
+        const T iw = (T)1 / n, s = boundary ? (T)0 : (T)1;
+        w.a = iw - (T)3 * iw * iw * s;
+        w.b = (T)4 * iw * iw * s;
+        w.g = -iw * iw * s;
+        return w;
+    }
+    const int rule = (p.nb >> 28) & 3;
+    if (rule == 1) {
+        w.a = (T)1 / n;
+        return w;
+    }
+    if (rule == 0) {
+        w.a = (n - (T)3) / (n * n);
+        w.b = (T)4 / (n * n);
+        w.g = (T)-1 / (n * n);
+        return w;
+    }
+    // two sharp ring edges: the sum of the sharpness this halfedge's walk
+    // visits, every ring edge but its own on a boundary
+    const T avg = boundary ? (edge == p.e1 ? p.s2 : (ts != (T)0 ? p.s1 : p.s1 + p.s2)) : p.s1 + p.s2;
+    const T t = satT(avg * (T)0.5);
+    const T cw = sgnT(ts), pcw = boundary ? sgnT(ps) : (T)0;
+    w.a = ((T)1 - t) / n + t * (T)0.25 * (cw + pcw);
+    w.b = t * (T)0.25 * cw;
+    w.dl = t * (T)0.25 * pcw;
+    return w;
+}
+
+template <typename T>
+__device__ __forceinline__ bool vparam_boundary_crease(const VParam<T> &p) {
+    return ((p.nb >> 30) & 1) != 0 && ((p.nb >> 28) & 3) == 2;
+}
+
+// pass 1: the parameters of the vertex rule of vertex v with fan f.  Face
+// and interior edge vertices born at level >= 2 take their ring from the
+// child numbering (structural_ring), so no ring record is read.
 template <typename T, int CREASED>
-__device__ __forceinline__ void adj_face_at(const AdjArgs<T> &a, int32_t f) {
-    if (f >= (a.H >> 2)) return;
+__device__ __forceinline__ void adj_params_fan(const AdjArgs<T> &a, const DCage &c, const LevelTab &t, int d,
+                                               const Fan &f, int32_t v) {
+    int32_t h[4], e[4];
+    RingT<T> rp;
+    if (structural_ring(f, t, d, v, h, e)) {
+        T s[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s[q] = CREASED ? sharp_at(a, e[q]) : (T)0;
+        rp = ring_params4<T, CREASED>(e, s);
+    } else {
+        rp = ring_params<T, CREASED>(a, c, f);
+    }
+    a.VP[v] = vparam_of<T>(rp, CREASED);
+}
+
+// value held by the lane of the next halfedge of the same quad, for any
+// dword-sized record (DPP quad_perm)
+template <class S>
+__device__ __forceinline__ S quad_next_rec(const S &v) {
+    static_assert(sizeof(S) % 4 == 0, "whole dwords");
+    S r;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(S) / 4); ++i)
+        reinterpret_cast<int *>(&r)[i] = quad_rel<kQuadNext>(reinterpret_cast<const int *>(&v)[i]);
+    return r;
+}
+template <int K, class S>
+__device__ __forceinline__ S quad_bcast_rec(const S &v) {
+    static_assert(sizeof(S) % 4 == 0, "whole dwords");
+    S r;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(S) / 4); ++i)
+        reinterpret_cast<int *>(&r)[i] = quad_bcast<K>(reinterpret_cast<const int *>(&v)[i]);
+    return r;
+}
+template <class S>
+__device__ __forceinline__ S quad_prev_rec(const S &v) {
+    static_assert(sizeof(S) % 4 == 0, "whole dwords");
+    S r;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(S) / 4); ++i)
+        reinterpret_cast<int *>(&r)[i] = quad_rel<kQuadPrev>(reinterpret_cast<const int *>(&v)[i]);
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void add_w(V3<T> &acc, T w, const V3<T> &g) {
+    acc.x += w * g.x; acc.y += w * g.y; acc.z += w * g.z;
+}
+
+// pass 2: one lane per halfedge h of a quad (the four lanes of a quad run
+// together).  Edge e = edge(h) joins A = vertex(h) and B = vertex(next h),
+// both corners of this quad.  The vertex rule adds E(e) to V(A) through h
+// (beta) and through next(twin h) (delta), and to V(B) through next(h)
+// (delta) and twin(h) (beta); all four weights follow from the parameters
+// of A and B and the sharpness of e (the edge of next(twin h) is read only
+// for a boundary crease vertex).  So every quad forms the updated gradient
+// of each of its edges itself, in the order of the edge's larger halfedge
+// (both quads of an edge get the same bits), and from them and its corners
+// the updated gradient of its face; the larger halfedge stores its edge's.
+template <typename T, int CREASED>
+__device__ __forceinline__ void adj_quad_at(const AdjArgs<T> &a, int32_t h) {
+    const bool live = h < a.H;   // H is a multiple of 4: quads are whole
+    const int4 x = live ? gld(reinterpret_cast<const int4 *>(a.he) + h) : make_int4(0, 0, 0, 0);
     const V3<T> *gV = reinterpret_cast<const V3<T> *>(a.gV);
-    const V3<T> *gE = reinterpret_cast<const V3<T> *>(a.gE);
+    const VParam<T> P = a.VP[x.z];   // a dead lane's record is all zeros: vertex 0
+    const V3<T> gv = gV[x.z];
+    const V3<T> ge0 = live ? reinterpret_cast<const V3<T> *>(a.gE)[x.y] : V3<T>{0, 0, 0};
+    const T ts = (CREASED && live) ? sharp_at(a, x.y) : (T)0;
+    const T ps = CREASED ? quad_prev_rec(ts) : (T)0;
+    const VW<T> w = vparam_weights<T>(P, ts, ps, x.y, CREASED);
+    T c0, c1, cf;
+    edge_weights<T, CREASED>(a, x.y, x.x < 0, c0, c1, cf);
+    const VParam<T> PB = quad_next_rec(P);
+    const V3<T> gvB = quad_next_rec(gv);
+    const T dlN = CREASED ? quad_next_rec(w.dl) : (T)0;   // next(h) leaves B, its previous edge is e
+    T bt = 0, dlt = 0;
+    if (x.x >= 0) {
+        bt = vparam_weights<T>(PB, ts, (T)0, x.y, CREASED).b;   // twin(h) leaves B along e
+        if (CREASED && ts != (T)0 && vparam_boundary_crease(P)) {
+            // next(twin h) leaves A; its previous edge is e, its own edge
+            // decides which sharpness its walk sums
+            const int32_t e2 = a.he[qnext(x.x)].edge;
+            dlt = vparam_weights<T>(P, sharp_at(a, e2), ts, e2, CREASED).dl;
+        }
+    }
+    const bool owner = x.x < 0 || h > x.x;
+    V3<T> ge = ge0;
+    if (owner) {
+        add_w(ge, w.b, gv);
+        add_w(ge, dlN, gvB);
+        if (x.x >= 0) {
+            add_w(ge, bt, gvB);
+            add_w(ge, dlt, gv);
+        }
+    } else {   // the larger halfedge is twin(h), leaving B
+        add_w(ge, bt, gvB);
+        add_w(ge, dlt, gv);
+        add_w(ge, w.b, gv);
+        add_w(ge, dlN, gvB);
+    }
+    if (live && owner) reinterpret_cast<V3<T> *>(a.gEu)[x.y] = ge;
+    // the face: its corners' gamma shares and its edges' face shares, in
+    // halfedge order
+    V3<T> pg{w.g * gv.x, w.g * gv.y, w.g * gv.z}, pe{cf * ge.x, cf * ge.y, cf * ge.z};
+    const V3<T> g0 = quad_bcast_rec<0>(pg), e0 = quad_bcast_rec<0>(pe);
+    const V3<T> g1 = quad_bcast_rec<1>(pg), e1 = quad_bcast_rec<1>(pe);
+    const V3<T> g2 = quad_bcast_rec<2>(pg), e2 = quad_bcast_rec<2>(pe);
+    const V3<T> g3 = quad_bcast_rec<3>(pg), e3 = quad_bcast_rec<3>(pe);
+    if (!live || (h & 3) != 0) return;
     V3<T> *gF = reinterpret_cast<V3<T> *>(a.gF);
-    V3<T> gf = gF[f];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int32_t h = 4 * f + j;
-        const He x = a.he[h];
-        T c0, c1, cf;
-        edge_weights<T, CREASED>(a, x.edge, x.twin < 0, c0, c1, cf);
-        const T g = w_g(a, h);
-        const V3<T> v = gV[x.vert], e = gE[x.edge];
-        gf.x += g * v.x; gf.y += g * v.y; gf.z += g * v.z;
-        gf.x += cf * e.x; gf.y += cf * e.y; gf.z += cf * e.z;
-    }
-    gF[f] = gf;
+    V3<T> gf = gF[h >> 2];
+    gf.x += g0.x; gf.y += g0.y; gf.z += g0.z;
+    gf.x += e0.x; gf.y += e0.y; gf.z += e0.z;
+    gf.x += g1.x; gf.y += g1.y; gf.z += g1.z;
+    gf.x += e1.x; gf.y += e1.y; gf.z += e1.z;
+    gf.x += g2.x; gf.y += g2.y; gf.z += g2.z;
+    gf.x += e2.x; gf.y += e2.y; gf.z += e2.z;
+    gf.x += g3.x; gf.y += g3.y; gf.z += g3.z;
+    gf.x += e3.x; gf.y += e3.y; gf.z += e3.z;
+    gF[h >> 2] = gf;
 }
 
-// pass 4: vertex v with fan f gathers a * gV[v] from the vertex rule, c0
-// and c1 shares of its incident edges from the edge rule and a quarter of
-// each incident face from the face rule
+// pass 3: vertex v with fan f gathers alpha * gV[v] from the vertex rule
+// (its own parameters, from the ring), the c0 and c1 shares of its incident
+// edges from the edge rule (edge gradients after pass 2) and a quarter of
+// each incident face (face gradients after pass 2)
 template <typename T, int CREASED>
-__device__ __forceinline__ void adj_vertex_gather(const AdjArgs<T> &a, const DCage &c, const Fan &f, int32_t v) {
+__device__ __forceinline__ void adj_gather_fan(const AdjArgs<T> &a, const DCage &c, const LevelTab &t, int d,
+                                               const Fan &f, int32_t v) {
     const He *he = a.he;
-    const V3<T> *gE = reinterpret_cast<const V3<T> *>(a.gE);
+    const V3<T> *gE = reinterpret_cast<const V3<T> *>(a.gEu);
     const V3<T> *gF = reinterpret_cast<const V3<T> *>(a.gF);
     const V3<T> gv = reinterpret_cast<const V3<T> *>(a.gV)[v];
     T alpha = 0;
     V3<T> acc{0, 0, 0};
+    int32_t h[4], e[4];
+    if (structural_ring(f, t, d, v, h, e)) {   // interior ring: no boundary edge
+        T s[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s[q] = CREASED ? sharp_at(a, e[q]) : (T)0;
+        V3<T> ge[4], gf[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            ge[q] = gE[e[q]];
+            gf[q] = gF[h[q] >> 2];
+        }
+        const VParam<T> vp = vparam_of<T>(ring_params4<T, CREASED>(e, s), CREASED);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int n1 = (i + 1) & 3;
+            alpha += vparam_weights<T>(vp, s[i], s[n1], e[i], CREASED).a;
+            T c0, c1, cf, d0, d1, df;
+            edge_weights<T, CREASED>(a, e[i], false, c0, c1, cf);
+            edge_weights<T, CREASED>(a, e[n1], false, d0, d1, df);
+            add_w(acc, c0, ge[i]);
+            add_w(acc, d1, ge[n1]);
+            add_w(acc, (T)0.25, gf[i]);
+        }
+        reinterpret_cast<V3<T> *>(a.gP)[v] = {alpha * gv.x + acc.x, alpha * gv.y + acc.y, alpha * gv.z + acc.z};
+        return;
+    }
+    const VParam<T> vp = vparam_of<T>(ring_params<T, CREASED>(a, c, f), CREASED);
     if (f.n <= 4) {
         // a fan of at most four: every entry's loads issued before any is
         // used, the same sums in the same order as below
@@ -294,14 +409,15 @@ __device__ __forceinline__ void adj_vertex_gather(const AdjArgs<T> &a, const DCa
             xp[q] = he[qprev(r[q])];
         }
         T wa[4], c0[4], d1[4];
-        V3<T> e[4], ep[4], g[4];
+        V3<T> ev[4], ep[4], g[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             T c1, cf, d0, df;
-            wa[q] = w_a(a, r[q]);
+            const T ts = CREASED ? sharp_at(a, x[q].edge) : (T)0, ps = CREASED ? sharp_at(a, xp[q].edge) : (T)0;
+            wa[q] = vparam_weights<T>(vp, ts, ps, x[q].edge, CREASED).a;
             edge_weights<T, CREASED>(a, x[q].edge, x[q].twin < 0, c0[q], c1, cf);
             edge_weights<T, CREASED>(a, xp[q].edge, xp[q].twin < 0, d0, d1[q], df);
-            e[q] = gE[x[q].edge];
+            ev[q] = gE[x[q].edge];
             ep[q] = gE[xp[q].edge];
             g[q] = gF[r[q] >> 2];
         }
@@ -309,9 +425,9 @@ __device__ __forceinline__ void adj_vertex_gather(const AdjArgs<T> &a, const DCa
         for (int i = 0; i < 4; ++i) {
             if (i >= f.n) break;
             alpha += wa[i];
-            acc.x += c0[i] * e[i].x; acc.y += c0[i] * e[i].y; acc.z += c0[i] * e[i].z;
-            acc.x += d1[i] * ep[i].x; acc.y += d1[i] * ep[i].y; acc.z += d1[i] * ep[i].z;
-            acc.x += (T)0.25 * g[i].x; acc.y += (T)0.25 * g[i].y; acc.z += (T)0.25 * g[i].z;
+            add_w(acc, c0[i], ev[i]);
+            add_w(acc, d1[i], ep[i]);
+            add_w(acc, (T)0.25, g[i]);
         }
         reinterpret_cast<V3<T> *>(a.gP)[v] = {alpha * gv.x + acc.x, alpha * gv.y + acc.y, alpha * gv.z + acc.z};
         return;
@@ -320,79 +436,14 @@ __device__ __forceinline__ void adj_vertex_gather(const AdjArgs<T> &a, const DCa
         const int32_t r = fan_at(f, c, i);
         const int32_t p = qprev(r);
         const He x = he[r], xp = he[p];
-        alpha += w_a(a, r);
+        const T ts = CREASED ? sharp_at(a, x.edge) : (T)0, ps = CREASED ? sharp_at(a, xp.edge) : (T)0;
+        alpha += vparam_weights<T>(vp, ts, ps, x.edge, CREASED).a;
         T c0, c1, cf, d0, d1, df;
         edge_weights<T, CREASED>(a, x.edge, x.twin < 0, c0, c1, cf);     // r leaves v
         edge_weights<T, CREASED>(a, xp.edge, xp.twin < 0, d0, d1, df);  // prev(r) enters v
-        const V3<T> e = gE[x.edge], ep = gE[xp.edge], g = gF[r >> 2];
-        acc.x += c0 * e.x; acc.y += c0 * e.y; acc.z += c0 * e.z;
-        acc.x += d1 * ep.x; acc.y += d1 * ep.y; acc.z += d1 * ep.z;
-        acc.x += (T)0.25 * g.x; acc.y += (T)0.25 * g.y; acc.z += (T)0.25 * g.z;
-    }
-    reinterpret_cast<V3<T> *>(a.gP)[v] = {alpha * gv.x + acc.x, alpha * gv.y + acc.y, alpha * gv.z + acc.z};
-}
-
-// Passes 1 and 4 for the vertex v with fan f: for face and interior edge
-// vertices born at level >= 2 the ring comes from the child numbering
-// (structural_ring), so no ring record is read; the other fans take the
-// general walks above.  The sums run over the same ring in the same order as
-// the forward rule.
-template <typename T, int CREASED>
-__device__ __forceinline__ void adj_weights_fan(const AdjArgs<T> &a, const DCage &c, const LevelTab &t, int d,
-                                                const Fan &f, int32_t v) {
-    int32_t h[4], e[4];
-    if (!structural_ring(f, t, d, v, h, e)) {
-        adj_ring_weights<T, CREASED>(a, c, f);
-        return;
-    }
-    T s[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) s[q] = CREASED ? sharp_at(a, e[q]) : (T)0;
-    RingT<T> rp;
-    rp.n = 4; rp.cc = 0; rp.boundary = false; rp.s1 = rp.s2 = 0; rp.e1 = rp.e2 = -1; rp.sall = 0;
-    if (CREASED) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {   // the previous edge of h[i] is e[i+1]
-            const T se = s[(i + 1) & 3];
-            rp.cc += sgnT(se);
-            if (se != (T)0) {
-                if (rp.e1 < 0) { rp.s1 = se; rp.e1 = e[(i + 1) & 3]; }
-                else if (rp.e2 < 0) { rp.s2 = se; rp.e2 = e[(i + 1) & 3]; }
-            }
-            rp.sall += se;
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const VW<T> w = level_vertex_weights<T>(rp, s[i], s[(i + 1) & 3], e[i], CREASED);
-        w_store(a, h[i], w.a, w.b, w.g, w.dl);
-    }
-}
-
-template <typename T, int CREASED>
-__device__ __forceinline__ void adj_gather_fan(const AdjArgs<T> &a, const DCage &c, const LevelTab &t, int d,
-                                               const Fan &f, int32_t v) {
-    int32_t h[4], e[4];
-    if (!structural_ring(f, t, d, v, h, e)) {
-        adj_vertex_gather<T, CREASED>(a, c, f, v);
-        return;
-    }
-    const V3<T> *gE = reinterpret_cast<const V3<T> *>(a.gE);
-    const V3<T> *gF = reinterpret_cast<const V3<T> *>(a.gF);
-    const V3<T> gv = reinterpret_cast<const V3<T> *>(a.gV)[v];
-    T alpha = 0;
-    V3<T> acc{0, 0, 0};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {   // interior ring: no boundary edge
-        const int32_t r = h[i], re = e[i], pe = e[(i + 1) & 3];
-        alpha += w_a(a, r);
-        T c0, c1, cf, d0, d1, df;
-        edge_weights<T, CREASED>(a, re, false, c0, c1, cf);
-        edge_weights<T, CREASED>(a, pe, false, d0, d1, df);
-        const V3<T> ev = gE[re], ep = gE[pe], g = gF[r >> 2];
-        acc.x += c0 * ev.x; acc.y += c0 * ev.y; acc.z += c0 * ev.z;
-        acc.x += d1 * ep.x; acc.y += d1 * ep.y; acc.z += d1 * ep.z;
-        acc.x += (T)0.25 * g.x; acc.y += (T)0.25 * g.y; acc.z += (T)0.25 * g.z;
+        add_w(acc, c0, gE[x.edge]);
+        add_w(acc, d1, gE[xp.edge]);
+        add_w(acc, (T)0.25, gF[r >> 2]);
     }
     reinterpret_cast<V3<T> *>(a.gP)[v] = {alpha * gv.x + acc.x, alpha * gv.y + acc.y, alpha * gv.z + acc.z};
 }
@@ -429,12 +480,11 @@ __device__ __forceinline__ void owners_block(int32_t H, int d, const DCage &c, c
 
 constexpr int kAdjScan = 4;
 
-// Passes 1 and 4 with one thread per vertex and the fan from the vertex ID
+// Passes 1 and 3 with one thread per vertex and the fan from the vertex ID
 // (vertex_fan): no owner scan and one round of dependent loads fewer, but
 // consecutive vertex IDs of split edges and older vertices have rings far
 // apart.  Faster while a level has few vertices (latency-bound launches),
 // slower on large levels (AdjPlan::per_vertex_max chooses).
-
 template <typename T, int CREASED, int GATHER_PASS>
 __device__ __forceinline__ void adj_vertex_pass(const AdjArgs<T> &a, const DCage &c, const LevelTab &t,
                                                 const LevelPtrs &lp, int d, int32_t v) {
@@ -447,16 +497,16 @@ __device__ __forceinline__ void adj_vertex_pass(const AdjArgs<T> &a, const DCage
     Fan f;
     vertex_fan(v, d, c, st, slp.he, f);
     if (GATHER_PASS) adj_gather_fan<T, CREASED>(a, c, st, d, f, v);
-    else adj_weights_fan<T, CREASED>(a, c, st, d, f, v);
+    else adj_params_fan<T, CREASED>(a, c, st, d, f, v);
 }
 
 template <typename T, int CREASED>
-struct RingWeightsFn {
+struct RingParamsFn {
     const AdjArgs<T> &a;
     const DCage &c;
     int d;
     __device__ void operator()(const LevelTab &t, const Fan &f, int32_t v) const {
-        adj_weights_fan<T, CREASED>(a, c, t, d, f, v);
+        adj_params_fan<T, CREASED>(a, c, t, d, f, v);
     }
 };
 
@@ -735,18 +785,13 @@ __device__ __forceinline__ void adj_cage_vertex_gather(const AdjArgs<T> &a, cons
 
 // ---- kernels: one thread per halfedge ------------------------------------------
 template <typename T, int CREASED>
-__global__ __launch_bounds__(256) void k_adj_weights(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
-    owners_block<kAdjScan>(a.H, d, c, t, lp, blockIdx.x, a.erep, RingWeightsFn<T, CREASED>{a, c, d});
+__global__ __launch_bounds__(256) void k_adj_params(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
+    owners_block<kAdjScan>(a.H, d, c, t, lp, blockIdx.x, a.erep, RingParamsFn<T, CREASED>{a, c, d});
 }
 
 template <typename T, int CREASED>
-__global__ __launch_bounds__(256) void k_adj_edges(AdjArgs<T> a) {
-    adj_edge_at<T, CREASED>(a, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
-}
-
-template <typename T, int CREASED>
-__global__ __launch_bounds__(256) void k_adj_faces(AdjArgs<T> a) {
-    adj_face_at<T, CREASED>(a, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
+__global__ __launch_bounds__(256) void k_adj_quads(AdjArgs<T> a) {
+    adj_quad_at<T, CREASED>(a, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 template <typename T, int CREASED>
@@ -870,21 +915,21 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
         a.gE = G + 3 * (n.V[d] + n.F[d]);
         a.gP = P;
         a.W = W;
+        a.VP = reinterpret_cast<VParam<T> *>(W);
+        a.gEu = W + 4 * n.V[d];
         a.erep = pl.erep;
         const dim3 g(grid_for(n.H[d])), gq(grid_for(n.H[d] / 4)), gv(grid_for(n.V[d])), b(256);
         const bool pv = n.V[d] <= pl.per_vertex_max;
         if (creased) {
             if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 1, 0>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
-            else hipLaunchKernelGGL((k_adj_weights<T, 1>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
-            hipLaunchKernelGGL((k_adj_edges<T, 1>), g, b, 0, st, a);
-            hipLaunchKernelGGL((k_adj_faces<T, 1>), gq, b, 0, st, a);
+            else hipLaunchKernelGGL((k_adj_params<T, 1>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            hipLaunchKernelGGL((k_adj_quads<T, 1>), g, b, 0, st, a);
             if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 1, 1>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             else hipLaunchKernelGGL((k_adj_vertices<T, 1>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
         } else {
             if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 0, 0>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
-            else hipLaunchKernelGGL((k_adj_weights<T, 0>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
-            hipLaunchKernelGGL((k_adj_edges<T, 0>), g, b, 0, st, a);
-            hipLaunchKernelGGL((k_adj_faces<T, 0>), gq, b, 0, st, a);
+            else hipLaunchKernelGGL((k_adj_params<T, 0>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            hipLaunchKernelGGL((k_adj_quads<T, 0>), g, b, 0, st, a);
             if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 0, 1>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             else hipLaunchKernelGGL((k_adj_vertices<T, 0>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
         }
@@ -902,6 +947,8 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
     a.gE = G + 3 * ((int64_t)pl.cage.V + pl.cage.F);
     a.gP = gradCage;
     a.W = W;
+    a.VP = nullptr;
+    a.gEu = nullptr;
     const dim3 g(grid_for(pl.cage.H)), b(256);
     if (pl.cage.vout) {   // gathers in a fixed order
         const dim3 ge(grid_for(pl.cage.E)), gf(grid_for(pl.cage.F)), gv(grid_for(pl.cage.V));
@@ -935,7 +982,7 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
 namespace {
 
 template <int CREASED>
-struct OpAdjWeights {
+struct OpAdjParams {
     int d;
     char *arena, *adj;
     __device__ void operator()(const AdjJob &J, int64_t i) const {
@@ -943,27 +990,17 @@ struct OpAdjWeights {
         DCage c;
         unpack_adj(J, arena, adj, a, c);
         const JobLevelPtrs lp{J.lp, arena};
-        owners_block<kAdjScan>(a.H, d, c, J.t, lp, (uint32_t)(i >> 8), a.erep, RingWeightsFn<float, CREASED>{a, c, d});
+        owners_block<kAdjScan>(a.H, d, c, J.t, lp, (uint32_t)(i >> 8), a.erep, RingParamsFn<float, CREASED>{a, c, d});
     }
 };
 template <int CREASED>
-struct OpAdjEdges {
+struct OpAdjQuads {
     char *arena, *adj;
     __device__ void operator()(const AdjJob &J, int64_t i) const {
         AdjArgs<float> a;
         DCage c;
         unpack_adj(J, arena, adj, a, c);
-        adj_edge_at<float, CREASED>(a, (int32_t)i);
-    }
-};
-template <int CREASED>
-struct OpAdjFaces {
-    char *arena, *adj;
-    __device__ void operator()(const AdjJob &J, int64_t i) const {
-        AdjArgs<float> a;
-        DCage c;
-        unpack_adj(J, arena, adj, a, c);
-        adj_face_at<float, CREASED>(a, (int32_t)i);
+        adj_quad_at<float, CREASED>(a, (int32_t)i);
     }
 };
 template <int CREASED>
@@ -1025,19 +1062,18 @@ struct OpAdjCageGather {
 
 }  // namespace
 
-// level d of every job: weights, edges, faces, vertices in turn; `quads`
-// has one workgroup per 1024 halfedges of each job, `halfedges` one per 256
+// level d of every job: vertex parameters, quads (edges and faces) and
+// vertex gathers in turn; `quads` has one workgroup per 1024 halfedges of
+// each job (the owner passes), `halfedges` one per 256
 void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, char *arena, char *adj, int d, int creased,
                      hipStream_t st) {
-    if (creased) launch_batch<AdjJob>(quads, OpAdjWeights<1>{d, arena, adj}, st);
-    else launch_batch<AdjJob>(quads, OpAdjWeights<0>{d, arena, adj}, st);
-    if (creased) launch_batch<AdjJob>(halfedges, OpAdjEdges<1>{arena, adj}, st);
-    else launch_batch<AdjJob>(halfedges, OpAdjEdges<0>{arena, adj}, st);
     if (creased) {
-        launch_batch<AdjJob>(quads, OpAdjFaces<1>{arena, adj}, st);
+        launch_batch<AdjJob>(quads, OpAdjParams<1>{d, arena, adj}, st);
+        launch_batch<AdjJob>(halfedges, OpAdjQuads<1>{arena, adj}, st);
         launch_batch<AdjJob>(quads, OpAdjVertices<1>{d, arena, adj}, st);
     } else {
-        launch_batch<AdjJob>(quads, OpAdjFaces<0>{arena, adj}, st);
+        launch_batch<AdjJob>(quads, OpAdjParams<0>{d, arena, adj}, st);
+        launch_batch<AdjJob>(halfedges, OpAdjQuads<0>{arena, adj}, st);
         launch_batch<AdjJob>(quads, OpAdjVertices<0>{d, arena, adj}, st);
     }
 }

@@ -121,6 +121,8 @@ __host__ __device__ __forceinline__ void unpack_adj(const AdjJob &J, char *arena
     a.gE = arena_ptr<float>(adj, J.gE);
     a.gP = arena_ptr<float>(adj, J.gP);
     a.W = arena_ptr<float>(adj, J.W);
+    a.VP = reinterpret_cast<VParam<float> *>(a.W);
+    a.gEu = a.W ? a.W + 4 * (int64_t)J.V : nullptr;
     a.erep = arena_ptr<const uint64_t>(arena, J.erep);
     c = arena_cage(arena, J);
 }
@@ -182,9 +184,9 @@ void batch_level_vertices(const BatchTable &t, char *arena, int d, int gather, i
 void batch_level_vertices_rare(const BatchTable &t, char *arena, int d, int gather, int creased, hipStream_t st);
 
 // adjoint launches (cc_adjoint.hip)
-// one level's adjoint: weights, faces and vertex gathers over `quads` (one
-// thread per quad; the owner passes scan 4 halfedges per thread), edges one
-// thread per halfedge
+// one level's adjoint: vertex parameters and vertex gathers over `quads`
+// (the owner passes scan 4 halfedges per thread), the quad pass one thread
+// per halfedge over `halfedges`
 void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, char *arena, char *adj, int d, int creased,
                      hipStream_t st);
 void batch_adj_cage(const BatchTable &t, char *arena, char *adj, int creased, hipStream_t st);

@@ -238,7 +238,7 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
             tot_erep += g.tab.R[g.D - 1];
         }
         g.o_adjB = tot_adjB; tot_adjB += g.n.V[g.D - 1];
-        g.o_adjW = tot_adjW; tot_adjW += std::max<int64_t>(g.D >= 2 ? g.n.H[g.D - 1] : 1, g.H0);
+        g.o_adjW = tot_adjW; tot_adjW += adj_scratch_scalars(g.n, g.D);
         Dmax = std::max(Dmax, g.D);
     }
     CC_TRY(hipGetDevice(&device), "hipGetDevice");
@@ -318,7 +318,7 @@ bool ccb_Batch::alloc_adjoint() {
     Carve cv;
     const size_t oA = cv.take(sizeof(float) * 3 * (size_t)tot_fin);
     const size_t oB = cv.take(sizeof(float) * 3 * (size_t)tot_adjB);
-    const size_t oW = cv.take(sizeof(float) * 4 * (size_t)tot_adjW);
+    const size_t oW = cv.take(sizeof(float) * (size_t)tot_adjW);
     const size_t oC = cv.take(sizeof(float) * 3 * (size_t)tot_v0);
     CC_TRY(hipMalloc(&adj_arena, cv.size), "hipMalloc(batch adjoint arena)");
     adjA = reinterpret_cast<float *>(adj_arena + oA);
@@ -482,7 +482,7 @@ bool ccb_Batch::build_tables() {
                     j.V = (int32_t)g.n.V[d];
                     j.gP = aoff(buf(d));
                 }
-                j.W = aoff(adjW + 4 * g.o_adjW);
+                j.W = aoff(adjW + g.o_adjW);
                 j.erep = g.o_erep >= 0 && kept ? off(erep + g.o_erep) : -1;
                 j.gV = aoff(G);
                 j.gF = aoff(G + 3 * g.n.V[d]);

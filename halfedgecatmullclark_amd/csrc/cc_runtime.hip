@@ -422,8 +422,7 @@ bool ccd_Subd::adjoint(const void *gradFinest, void *gradCage, int32_t steps, hi
         adj_scalar = 0;
         CC_TRY(hipMalloc(&adj_a, 3 * scalar * (size_t)n.V[D]), "hipMalloc(adjoint scratch)");
         CC_TRY(hipMalloc(&adj_b, 3 * scalar * (size_t)(D >= 1 ? n.V[D - 1] : 1)), "hipMalloc(adjoint scratch)");
-        CC_TRY(hipMalloc(&adj_w, 4 * scalar * (size_t)std::max<int64_t>(D >= 2 ? n.H[D - 1] : 1, H0)),
-               "hipMalloc(adjoint weights)");
+        CC_TRY(hipMalloc(&adj_w, scalar * (size_t)adj_scratch_scalars(n, D)), "hipMalloc(adjoint scratch)");
         adj_scalar = scalar;
     }
     AdjPlan pl;
