@@ -78,15 +78,17 @@ def patch_bytes(n, d):
     return level_bytes(n, d) + 12 * n["V"][d]
 
 
-def refinement_bytes(n, D):
+def refinement_bytes(n, D, rec_bytes=16):
     """Algorithmic bytes of the whole refinement (all steps, levels 0..D-1):
     per step d -> d+1 read the level-d records (28 B per cage halfedge at
-    d = 0), points and creases once, write the child records, all level-(d+1)
-    points and creases once."""
+    d = 0), points and creases once, write the child records (rec_bytes
+    each: 16 with uvIDs, 12 without), all level-(d+1) points and creases
+    once."""
     total = 0
     for d in range(D):
-        rec = 28 if d == 0 else 16
-        total += rec * n["H"][d] + 64 * n["H"][d] + 12 * n["V"][d] + 12 * n["V"][d + 1] + 12 * n["C"][d] + 24 * n["C"][d]
+        rec = 28 if d == 0 else rec_bytes
+        total += (rec * n["H"][d] + 4 * rec_bytes * n["H"][d] + 12 * n["V"][d] + 12 * n["V"][d + 1] + 12 * n["C"][d]
+                  + 24 * n["C"][d])
     return total
 
 
@@ -317,6 +319,19 @@ def main():
                   "adj_roofline_frac": adj_bytes / (adj_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     del pts_out, grad_fin, grad_cage
 
+    # the same refinement with 12-byte level records (libcchip_nouv.so, the
+    # CC_DISABLE_UV layout): no uvIDs, 48 instead of 64 bytes of children
+    # per parent halfedge
+    ds12 = capi.DeviceSubd(cage, D, variant="nouv")
+    ds12.refine(capi.ALL, sptr)
+    ds12.autotune(capi.ALL, sptr)
+    ms12 = ev_time(lambda: ds12.refine(capi.ALL, sptr), max(5, args.steps // 2))
+    ds12.close()
+    nouv = {"workload": "torus 64x32 cage to depth 8, ccs_Refine_Scatter steps without uvs, 12-byte level records "
+                        "(CC_DISABLE_UV library)",
+            "ms_per_step": ms12, "value": hcum / (ms12 * 1e-3),
+            "hbm_gbs_algorithmic": refinement_bytes(n, D, 12) / (ms12 * 1e-3) / 1e9}
+
     # dominant kernel: largest mean duration among the launches of one step;
     # the per-level kernels run in level order, each level starting with its
     # fused or patch-local kernel
@@ -411,6 +426,7 @@ def main():
         "hbm_gbs_algorithmic": refinement_bytes(n, D) / (ms_scatter * 1e-3) / 1e9,
         "gather": {"ms_per_step": ms_gather, "value": hcum / (ms_gather * 1e-3)},
         "adjoint_d8": adjoint_d8,
+        "nouv_records": nouv,
         "fwd_adjoint": {"workload": "torus 32x16 cage, depth 5, loss 0.5*||S(V)-T||^2",
                         "fwd_ms": res["fwd"], "fwd_adj_ms": res["fwd_adj"],
                         "ratio": res["fwd_adj"] / res["fwd"]},

@@ -13,6 +13,8 @@ import numpy as np
 from .cage import CREASE_DTYPE, Cage
 
 HALFEDGE_DTYPE = np.dtype([("twinID", "<i4"), ("edgeID", "<i4"), ("vertexID", "<i4"), ("uvID", "<i4")])
+# cc_Halfedge_SemiRegular of a CC_DISABLE_UV build (no uvID): 12 bytes
+HALFEDGE12_DTYPE = np.dtype([("twinID", "<i4"), ("edgeID", "<i4"), ("vertexID", "<i4")])
 
 
 class cc_Mesh(ctypes.Structure):

@@ -1,4 +1,5 @@
-"""Compile libcchip.so (HIP kernels + C ABI) for gfx950, in-tree.
+"""Compile libcchip.so (HIP kernels + C ABI) for gfx950, in-tree, and its
+12-byte-record variant libcchip_nouv.so (CC_DISABLE_UV).
 
 ``python -m halfedgecatmullclark_amd.build`` rebuilds unconditionally.
 """
@@ -32,9 +33,16 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, src + ".o")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", *FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+# the two libraries built from the same sources: 16-byte level records with
+# uvIDs, and 12-byte records without them (the reference's CC_DISABLE_UV)
+VARIANTS = {"uv": (LIB, []), "nouv": (os.path.join(HERE, "libcchip_nouv.so"), ["-DCC_DISABLE_UV"])}
+
+
+def _compile(job: tuple[str, str]) -> str:
+    variant, src = job
+    defines = VARIANTS[variant][1]
+    obj = os.path.join(BUILD, variant, src + ".o")
+    cmd = [hipcc(), f"--offload-arch={ARCH}", *FLAGS, *defines, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     noise = "is not a recognized feature for this target"
     err = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if noise not in l)
@@ -45,19 +53,24 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose: bool = False, variants=("uv", "nouv")) -> str:
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}{r.stderr}")
-    os.replace(tmp, LIB)
-    if verbose:
-        print(f"built {LIB}")
+    jobs = [(v, s) for v in variants for s in srcs]
+    for v in variants:
+        os.makedirs(os.path.join(BUILD, v), exist_ok=True)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        objs = list(ex.map(_compile, jobs))
+    for v in variants:
+        lib = VARIANTS[v][0]
+        vobjs = [o for (jv, _), o in zip(jobs, objs) if jv == v]
+        tmp = lib + ".tmp"
+        cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *vobjs, "-o", tmp]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}{r.stderr}")
+        os.replace(tmp, lib)
+        if verbose:
+            print(f"built {lib}")
     return LIB
 
 

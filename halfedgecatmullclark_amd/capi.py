@@ -9,11 +9,15 @@ import os
 
 import numpy as np
 
-from .abi import HALFEDGE_DTYPE, MeshView, cc_Mesh, cc_Subd, level_counts
+from .abi import HALFEDGE12_DTYPE, HALFEDGE_DTYPE, MeshView, cc_Mesh, cc_Subd, level_counts
 from .cage import CREASE_DTYPE, Cage
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcchip.so")
+# the library variants: 16-byte level records with uvIDs, and the 12-byte
+# records of a CC_DISABLE_UV build (libcchip_nouv.so)
+LIB_PATHS = {"uv": LIB_PATH, "nouv": os.path.join(HERE, "libcchip_nouv.so")}
+RECORD_DTYPES = {"uv": HALFEDGE_DTYPE, "nouv": HALFEDGE12_DTYPE}
 
 HALFEDGES, CREASES, UVS, POINTS, GATHER, NOCREASES, FP64 = 1, 2, 4, 8, 16, 32, 64
 ALL = HALFEDGES | CREASES | UVS | POINTS
@@ -36,15 +40,15 @@ class CCError(RuntimeError):
     pass
 
 
-_lib = None
+_libs: dict = {}
 
 
-def lib() -> ctypes.CDLL:
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise CCError(f"{LIB_PATH} is missing: build it with `python -m halfedgecatmullclark_amd.build`")
+def lib(variant: str = "uv") -> ctypes.CDLL:
+    if variant in _libs:
+        return _libs[variant]
+    path = LIB_PATHS[variant]
+    if not os.path.exists(path):
+        raise CCError(f"{path} is missing: build it with `python -m halfedgecatmullclark_amd.build`")
     # One HIP runtime per process: when torch is installed, load it first so
     # that the library binds to the libamdhip64.so.7 torch brings instead of
     # a second copy from /opt/rocm (two runtimes in one process leave torch
@@ -53,7 +57,7 @@ def lib() -> ctypes.CDLL:
         import torch  # noqa: F401
     except ImportError:
         pass
-    L = ctypes.CDLL(LIB_PATH)
+    L = ctypes.CDLL(path)
     P = ctypes.POINTER
     vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
     sigs = {
@@ -121,37 +125,39 @@ def lib() -> ctypes.CDLL:
             continue  # calling it raises AttributeError
         f.restype = res
         f.argtypes = args
-    _lib = L
+    _libs[variant] = L
     return L
 
 
-def last_error() -> tuple[int, str]:
-    L = lib()
+def last_error(variant: str = "uv") -> tuple[int, str]:
+    L = lib(variant)
     return int(L.ccs_LastError()), (L.ccs_LastErrorString() or b"").decode()
 
 
-def _check(rc: int, what: str):
+def _check(rc: int, what: str, variant: str = "uv"):
     if rc != 0:
-        code, msg = last_error()
+        code, msg = last_error(variant)
         raise CCError(f"{what} failed ({rc}): {msg}")
 
 
 class HostSubd:
     """A ``cc_Subd`` from ``ccs_Create`` (host arrays, GPU-refined)."""
 
-    def __init__(self, cage: Cage, depth: int):
-        self.L = lib()
+    def __init__(self, cage: Cage, depth: int, variant: str = "uv"):
+        self.variant = variant
+        self.L = lib(variant)
+        self.rec = RECORD_DTYPES[variant]
         self.mesh = MeshView(cage)
         self.depth = depth
         self.n = level_counts(cage.V, cage.H, cage.E, cage.F, depth)
         self.subd = self.L.ccs_Create(self.mesh.pointer, depth)
         if not self.subd:
-            raise CCError(f"ccs_Create failed: {last_error()[1]}")
+            raise CCError(f"ccs_Create failed: {last_error(variant)[1]}")
 
     def run(self, *drivers: str) -> "HostSubd":
         for name in drivers:
             getattr(self.L, name)(self.subd)
-            code, msg = last_error()
+            code, msg = last_error(self.variant)
             if code:
                 raise CCError(f"{name}: {msg}")
         return self
@@ -163,7 +169,7 @@ class HostSubd:
         out = np.zeros((self.n["V"][0], 3), dtype=real)
         name = "ccs_RefineVertexPoints_Adjoint" if real.itemsize == 4 else "ccs_RefineVertexPoints_Adjoint_f64"
         getattr(self.L, name)(self.subd, g.ctypes.data, out.ctypes.data)
-        code, msg = last_error()
+        code, msg = last_error(self.variant)
         if code:
             raise CCError(f"{name}: {msg}")
         return out
@@ -174,8 +180,9 @@ class HostSubd:
         nh, nc, nv = self.n["Hcum"][self.depth], self.n["Ccum"][self.depth], self.n["Vcum"][self.depth]
         out = {"counts": self.n}
         if "halfedges" in keys:
-            he = np.ctypeslib.as_array(ctypes.cast(s.halfedges, ctypes.POINTER(ctypes.c_int32)), (nh * 4,))
-            out["halfedges"] = he.view(HALFEDGE_DTYPE).copy()
+            words = self.rec.itemsize // 4
+            he = np.ctypeslib.as_array(ctypes.cast(s.halfedges, ctypes.POINTER(ctypes.c_int32)), (nh * words,))
+            out["halfedges"] = he.view(self.rec).copy()
         if "creases" in keys:
             cr = np.ctypeslib.as_array(ctypes.cast(s.creases, ctypes.POINTER(ctypes.c_int32)), (nc * 3,))
             out["creases"] = cr.view(CREASE_DTYPE).copy()
@@ -196,41 +203,43 @@ class HostSubd:
             pass
 
 
-def refine_host(cage: Cage, depth: int, *drivers: str) -> dict:
+def refine_host(cage: Cage, depth: int, *drivers: str, variant: str = "uv") -> dict:
     """Run host-semantics drivers (default ``ccs_Refine_Scatter``) and return copies of the arrays."""
-    s = HostSubd(cage, depth)
+    s = HostSubd(cage, depth, variant)
     try:
         return s.run(*(drivers or ("ccs_Refine_Scatter",))).arrays()
     finally:
         s.close()
 
 
-def refine_file(path: str, depth: int, *drivers: str) -> dict:
+def refine_file(path: str, depth: int, *drivers: str, variant: str = "uv") -> dict:
     """Load a ``.ccm`` cage with the library's ``ccm_Load``, refine it with
     host-semantics drivers (default ``ccs_Refine_Scatter``) on a subd from
     ``ccs_Create``, and return copies of the subd's arrays."""
-    L = lib()
+    L = lib(variant)
+    rec = RECORD_DTYPES[variant]
     mesh = L.ccm_Load(os.fsencode(path))
     if not mesh:
-        raise CCError(f"ccm_Load({path}) failed: {last_error()[1]}")
+        raise CCError(f"ccm_Load({path}) failed: {last_error(variant)[1]}")
     try:
         m = mesh.contents
         n = level_counts(m.vertexCount, m.halfedgeCount, m.edgeCount, m.faceCount, depth)
         subd = L.ccs_Create(mesh, depth)
         if not subd:
-            raise CCError(f"ccs_Create failed: {last_error()[1]}")
+            raise CCError(f"ccs_Create failed: {last_error(variant)[1]}")
         try:
             for name in drivers or ("ccs_Refine_Scatter",):
                 getattr(L, name)(subd)
-                code, msg = last_error()
+                code, msg = last_error(variant)
                 if code:
                     raise CCError(f"{name}: {msg}")
             s = subd.contents
             nh, nc, nv = n["Hcum"][depth], n["Ccum"][depth], n["Vcum"][depth]
-            he = np.ctypeslib.as_array(ctypes.cast(s.halfedges, ctypes.POINTER(ctypes.c_int32)), (nh * 4,))
+            he = np.ctypeslib.as_array(ctypes.cast(s.halfedges, ctypes.POINTER(ctypes.c_int32)),
+                                       (nh * (rec.itemsize // 4),))
             cr = np.ctypeslib.as_array(ctypes.cast(s.creases, ctypes.POINTER(ctypes.c_int32)), (nc * 3,))
             pt = np.ctypeslib.as_array(ctypes.cast(s.vertexPoints, ctypes.POINTER(ctypes.c_float)), (nv * 3,))
-            return {"halfedges": he.view(HALFEDGE_DTYPE).copy(), "creases": cr.view(CREASE_DTYPE).copy(),
+            return {"halfedges": he.view(rec).copy(), "creases": cr.view(CREASE_DTYPE).copy(),
                     "points": pt.reshape(nv, 3).copy(), "counts": n}
         finally:
             L.ccs_Release(subd)
@@ -241,53 +250,58 @@ def refine_file(path: str, depth: int, *drivers: str) -> dict:
 class DeviceSubd:
     """A ``ccd_Subd``: the cage and levels 1..depth resident in HBM."""
 
-    def __init__(self, cage: Cage, depth: int):
-        self.L = lib()
+    def __init__(self, cage: Cage, depth: int, variant: str = "uv"):
+        self.variant = variant
+        self.L = lib(variant)
+        self.rec = RECORD_DTYPES[variant]
         self.cage = cage.contiguous()
         self.mesh = MeshView(self.cage)
         self.depth = depth
         self.n = level_counts(cage.V, cage.H, cage.E, cage.F, depth)
         self.handle = self.L.ccd_Create(self.mesh.pointer, depth)
         if not self.handle:
-            raise CCError(f"ccd_Create failed: {last_error()[1]}")
+            raise CCError(f"ccd_Create failed: {last_error(variant)[1]}")
+
+    def _ok(self, rc: int, what: str):
+        _check(rc, what, self.variant)
 
     def refine(self, steps: int = ALL, stream: int | None = None):
-        _check(self.L.ccd_Refine(self.handle, steps, stream), "ccd_Refine")
+        self._ok(self.L.ccd_Refine(self.handle, steps, stream), "ccd_Refine")
 
     def refine_points(self, cage_points_ptr: int | None, out_ptr: int | None, steps: int = 0,
                       stream: int | None = None):
-        _check(self.L.ccd_RefinePoints(self.handle, cage_points_ptr, out_ptr, steps, stream), "ccd_RefinePoints")
+        self._ok(self.L.ccd_RefinePoints(self.handle, cage_points_ptr, out_ptr, steps, stream), "ccd_RefinePoints")
 
     def refine_points_adjoint(self, grad_finest_ptr: int, grad_cage_ptr: int, steps: int = 0,
                               stream: int | None = None):
-        _check(self.L.ccd_RefinePointsAdjoint(self.handle, grad_finest_ptr, grad_cage_ptr, steps, stream),
+        self._ok(self.L.ccd_RefinePointsAdjoint(self.handle, grad_finest_ptr, grad_cage_ptr, steps, stream),
                "ccd_RefinePointsAdjoint")
 
     def synchronize(self):
-        _check(self.L.ccd_Synchronize(self.handle), "ccd_Synchronize")
+        self._ok(self.L.ccd_Synchronize(self.handle), "ccd_Synchronize")
 
     def pointer(self, which: int) -> int:
         return int(self.L.ccd_DevicePointer(self.handle, which) or 0)
 
     def upload_cage_points(self, points: np.ndarray):
         p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
-        _check(self.L.ccd_UploadCagePoints(self.handle, p.ctypes.data), "ccd_UploadCagePoints")
+        self._ok(self.L.ccd_UploadCagePoints(self.handle, p.ctypes.data), "ccd_UploadCagePoints")
 
     def download(self, steps: int = ALL) -> dict:
         """Copy levels 1..depth to host numpy arrays (through a host cc_Subd)."""
         nh, nc, nv = self.n["Hcum"][self.depth], self.n["Ccum"][self.depth], self.n["Vcum"][self.depth]
-        he = np.zeros(nh, HALFEDGE_DTYPE)
+        he = np.zeros(nh, self.rec)
         cr = np.zeros(nc, CREASE_DTYPE)
         pt = np.zeros((nv, 3), np.float32)
         sub = cc_Subd(self.mesh.pointer, pt.ctypes.data, he.ctypes.data, cr.ctypes.data, self.depth)
-        _check(self.L.ccd_Download(self.handle, ctypes.byref(sub), steps), "ccd_Download")
+        self._ok(self.L.ccd_Download(self.handle, ctypes.byref(sub), steps), "ccd_Download")
         return {"halfedges": he, "creases": cr, "points": pt, "counts": self.n}
 
     def set_option(self, key: int, value: int):
-        _check(self.L.ccd_SetOption(self.handle, key, value), "ccd_SetOption")
+        self._ok(self.L.ccd_SetOption(self.handle, key, value), "ccd_SetOption")
 
     def autotune(self, steps: int = ALL, stream: int | None = None):
-        _check(self.L.ccd_Autotune(self.handle, steps, stream), "ccd_Autotune")
+        self._ok(self.L.ccd_Autotune(self.handle, steps, stream), "ccd_Autotune")
 
     def level_options(self) -> list[dict]:
         keys = {"kind": OPT_FUSED, "scan": OPT_SCAN, "tile": OPT_PATCH_TILE, "hpt": OPT_PATCH_HPT}
@@ -318,8 +332,10 @@ class Batch:
     """A ``ccb_Batch``: many cages, each with its own depth, refined together
     (one launch per kernel kind and level, replayed as a hipGraph)."""
 
-    def __init__(self, cages: list[Cage], depths: list[int]):
-        self.L = lib()
+    def __init__(self, cages: list[Cage], depths: list[int], variant: str = "uv"):
+        self.variant = variant
+        self.L = lib(variant)
+        self.rec = RECORD_DTYPES[variant]
         self.cages = [c.contiguous() for c in cages]
         self.meshes = [MeshView(c) for c in self.cages]
         self.depths = [int(d) for d in depths]
@@ -327,7 +343,10 @@ class Batch:
         dep = (ctypes.c_int32 * len(self.depths))(*self.depths)
         self.handle = self.L.ccb_Create(arr, dep, len(self.meshes))
         if not self.handle:
-            raise CCError(f"ccb_Create failed: {last_error()[1]}")
+            raise CCError(f"ccb_Create failed: {last_error(variant)[1]}")
+
+    def _ok(self, rc: int, what: str):
+        _check(rc, what, self.variant)
 
     def __len__(self) -> int:
         return len(self.cages)
@@ -342,27 +361,27 @@ class Batch:
         return int(self.L.ccb_DevicePointer(self.handle, which) or 0)
 
     def refine(self, steps: int = ALL, stream: int | None = None):
-        _check(self.L.ccb_Refine(self.handle, steps, stream), "ccb_Refine")
+        self._ok(self.L.ccb_Refine(self.handle, steps, stream), "ccb_Refine")
 
     def refine_points(self, cage_points_ptr: int | None = None, steps: int = 0, stream: int | None = None):
-        _check(self.L.ccb_RefinePoints(self.handle, cage_points_ptr, steps, stream), "ccb_RefinePoints")
+        self._ok(self.L.ccb_RefinePoints(self.handle, cage_points_ptr, steps, stream), "ccb_RefinePoints")
 
     def finest_points(self, out_ptr: int, stream: int | None = None):
-        _check(self.L.ccb_FinestPoints(self.handle, out_ptr, stream), "ccb_FinestPoints")
+        self._ok(self.L.ccb_FinestPoints(self.handle, out_ptr, stream), "ccb_FinestPoints")
 
     def refine_points_adjoint(self, grad_finest_ptr: int, grad_cage_ptr: int, steps: int = 0,
                               stream: int | None = None):
-        _check(self.L.ccb_RefinePointsAdjoint(self.handle, grad_finest_ptr, grad_cage_ptr, steps, stream),
+        self._ok(self.L.ccb_RefinePointsAdjoint(self.handle, grad_finest_ptr, grad_cage_ptr, steps, stream),
                "ccb_RefinePointsAdjoint")
 
     def synchronize(self):
-        _check(self.L.ccb_Synchronize(self.handle), "ccb_Synchronize")
+        self._ok(self.L.ccb_Synchronize(self.handle), "ccb_Synchronize")
 
     def set_option(self, key: int, value: int):
-        _check(self.L.ccb_SetOption(self.handle, key, value), "ccb_SetOption")
+        self._ok(self.L.ccb_SetOption(self.handle, key, value), "ccb_SetOption")
 
     def autotune(self, steps: int = ALL, stream: int | None = None):
-        _check(self.L.ccb_Autotune(self.handle, steps, stream), "ccb_Autotune")
+        self._ok(self.L.ccb_Autotune(self.handle, steps, stream), "ccb_Autotune")
 
     def level_options(self) -> list[dict]:
         keys = {"kind": OPT_FUSED, "scan": OPT_SCAN, "tile": OPT_PATCH_TILE, "hpt": OPT_PATCH_HPT}
@@ -381,11 +400,11 @@ class Batch:
         """Copy cage i's levels 1..D_i to host numpy arrays."""
         c, d = self.cages[i], self.depths[i]
         n = level_counts(c.V, c.H, c.E, c.F, d)
-        he = np.zeros(n["Hcum"][d], HALFEDGE_DTYPE)
+        he = np.zeros(n["Hcum"][d], self.rec)
         cr = np.zeros(n["Ccum"][d], CREASE_DTYPE)
         pt = np.zeros((n["Vcum"][d], 3), np.float32)
         sub = cc_Subd(self.meshes[i].pointer, pt.ctypes.data, he.ctypes.data, cr.ctypes.data, d)
-        _check(self.L.ccb_Download(self.handle, i, ctypes.byref(sub), steps), "ccb_Download")
+        self._ok(self.L.ccb_Download(self.handle, i, ctypes.byref(sub), steps), "ccb_Download")
         return {"halfedges": he, "creases": cr, "points": pt, "counts": n}
 
     def close(self):

@@ -296,7 +296,7 @@ __device__ __forceinline__ void add_w(V3<T> &acc, T w, const V3<T> &g) {
 template <typename T, int CREASED>
 __device__ __forceinline__ void adj_quad_at(const AdjArgs<T> &a, int32_t h) {
     const bool live = h < a.H;   // H is a multiple of 4: quads are whole
-    const int4 x = live ? gld(reinterpret_cast<const int4 *>(a.he) + h) : make_int4(0, 0, 0, 0);
+    const int4 x = live ? he_gld(a.he, h) : make_int4(0, 0, 0, 0);
     const V3<T> *gV = reinterpret_cast<const V3<T> *>(a.gV);
     const VParam<T> P = a.VP[x.z];   // a dead lane's record is all zeros: vertex 0
     const V3<T> gv = gV[x.z];

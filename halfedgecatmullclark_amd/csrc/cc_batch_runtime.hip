@@ -360,7 +360,7 @@ bool ccb_Batch::build_tables() {
             jobs[i].c = cage_view(i);
             jobs[i].he1 = he + g.o_he;
             jobs[i].pts1 = pts + g.o_pts;
-            jobs[i].uv_mode = g.U0 > 0 ? 1 : 2;
+            jobs[i].uv_mode = !kHasUv ? 0 : (g.U0 > 0 ? 1 : 2);
             bh[i] = blocks_of(g.H0);
             bf[i] = blocks_of(g.F0);
             be[i] = blocks_of(g.E0);
@@ -432,7 +432,7 @@ bool ccb_Batch::build_tables() {
             j.H = (int32_t)g.n.H[d]; j.V = (int32_t)g.n.V[d]; j.F = (int32_t)g.n.F[d];
             j.E = (int32_t)g.n.E[d]; j.C = (int32_t)g.n.C[d];
             j.xcd_remap = remap;
-            j.uv_refine = g.U0 > 0 ? 1 : 0;
+            j.uv_refine = kHasUv && g.U0 > 0 ? 1 : 0;
             cage_offsets(i, j);
             j.t = g.tab;
             for (int k = 0; k <= kMaxDepth; ++k) j.lp[k] = (k >= 1 && k <= g.D) ? off(hb + g.n.offH[k]) : -1;

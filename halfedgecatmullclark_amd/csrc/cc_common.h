@@ -17,15 +17,60 @@ constexpr int kMaxDepth = 15;
 
 // Record layouts, byte-identical to CatmullClark.h:30-55.
 struct CageHe { int32_t twin, next, prev, face, edge, vert, uv; };  // 28 B
-struct alignas(16) He { int32_t twin, edge, vert, uv; };           // 16 B
 struct Crease { int32_t next, prev; float sharp; };                 // 12 B
 struct P3 { float x, y, z; };                                       // 12 B
 struct UV { float u, v; };                                          //  8 B
 
+// The level record cc_Halfedge_SemiRegular (CatmullClark.h:47-55): 16 bytes
+// with the uv code, or 12 without it in a library built with CC_DISABLE_UV
+// (the reference's switch, which its GL demo uses).  Kernels go through
+// he_load / he_store / he_uv, which give and take the record as an int4
+// whose w is the uv code (0 without uvs).
+template <int BYTES>
+struct HeRec;
+template <>
+struct alignas(16) HeRec<16> { int32_t twin, edge, vert, uv; };
+template <>
+struct HeRec<12> { int32_t twin, edge, vert; };
+#ifdef CC_DISABLE_UV
+constexpr int kHeBytes = 12;
+#else
+constexpr int kHeBytes = 16;
+#endif
+using He = HeRec<kHeBytes>;
+constexpr bool kHasUv = kHeBytes == 16;
+
 static_assert(sizeof(CageHe) == 28, "cc_Halfedge layout");
-static_assert(sizeof(He) == 16, "cc_Halfedge_SemiRegular layout");
+static_assert(sizeof(He) == kHeBytes, "cc_Halfedge_SemiRegular layout");
 static_assert(sizeof(Crease) == 12, "cc_Crease layout");
 static_assert(sizeof(P3) == 12, "cc_VertexPoint layout");
+
+// record i of a level array as (twin, edge, vertex, uv code or 0)
+__host__ __device__ __forceinline__ int4 he_load(const HeRec<16> *p, int64_t i) {
+    return reinterpret_cast<const int4 *>(p)[i];
+}
+__host__ __device__ __forceinline__ int4 he_load(const HeRec<12> *p, int64_t i) {
+    const int32_t *q = reinterpret_cast<const int32_t *>(p) + 3 * i;
+    return make_int4(q[0], q[1], q[2], 0);
+}
+// the whole record i (the uv code is dropped without uvs)
+__host__ __device__ __forceinline__ void he_store(HeRec<16> *p, int64_t i, int4 r) {
+    reinterpret_cast<int4 *>(p)[i] = r;
+}
+__host__ __device__ __forceinline__ void he_store(HeRec<12> *p, int64_t i, int4 r) {
+    int32_t *q = reinterpret_cast<int32_t *>(p) + 3 * i;
+    q[0] = r.x; q[1] = r.y; q[2] = r.z;
+}
+// twin, edge and vertex of record i, leaving its uv code
+template <int B>
+__host__ __device__ __forceinline__ void he_store_ids(HeRec<B> *p, int64_t i, int4 r) {
+    p[i].twin = r.x; p[i].edge = r.y; p[i].vert = r.z;
+}
+// the uv code of record i alone (nothing without uvs)
+__host__ __device__ __forceinline__ void he_store_uv(HeRec<16> *p, int64_t i, int32_t uv) { p[i].uv = uv; }
+__host__ __device__ __forceinline__ void he_store_uv(HeRec<12> *, int64_t, int32_t) {}
+__host__ __device__ __forceinline__ int32_t he_uv(const HeRec<16> &x) { return x.uv; }
+__host__ __device__ __forceinline__ int32_t he_uv(const HeRec<12> &) { return 0; }
 
 // Exact per-level counts (CatmullClark.h:455-585, :811-936) from the
 // recurrences F' = H, E' = 2E + H, H' = 4H, C' = 2C, V' = V + E + F.
@@ -112,6 +157,21 @@ __device__ __forceinline__ T sld(const T *p) {
     for (int i = 0; i < (int)(sizeof(T) / 4); ++i) reinterpret_cast<uint32_t *>(&r)[i] = q[i];
     return r;
 }
+
+// record i through an explicitly global load (see gld)
+__device__ __forceinline__ int4 he_gld(const HeRec<16> *p, int64_t i) {
+    return gld(reinterpret_cast<const int4 *>(p) + i);
+}
+__device__ __forceinline__ int4 he_gld(const HeRec<12> *p, int64_t i) {
+    const int32_t *q = reinterpret_cast<const int32_t *>(p) + 3 * i;
+    return make_int4(gld(q), gld(q + 1), gld(q + 2), 0);
+}
+
+// a level array read record by record as int4s (he_load)
+struct HeView {
+    const He *p;
+    __host__ __device__ __forceinline__ int4 operator[](int64_t i) const { return he_load(p, i); }
+};
 
 // ---- scalar rules (CatmullClark.h:296-392, :667-689) ----------------------
 __host__ __device__ __forceinline__ int32_t qnext(int32_t h) {

@@ -93,7 +93,7 @@ __device__ __forceinline__ P3 face_rule(P3 p0, P3 p1, P3 p2, P3 p3, bool gather)
 }
 
 __device__ __forceinline__ P3 quad_face(const He *he, const P3 *old, int32_t quad_base, bool gather) {
-    const int4 *q4 = reinterpret_cast<const int4 *>(he) + quad_base;
+    const HeView q4{he + quad_base};
     return face_rule(old[q4[0].z], old[q4[1].z], old[q4[2].z], old[q4[3].z], gather);
 }
 
@@ -127,6 +127,41 @@ __device__ __forceinline__ int4 child_record(int k, int32_t h, int4 x, int4 p, i
     }
 }
 
+
+// int4s of LDS per parent halfedge that stage its four child records
+constexpr int kStageInt4 = kHeBytes / 4;
+
+// Child records staged through LDS: this thread's four children go to its
+// slots of `stage` (kStageInt4 int4s per thread), then, after the block's
+// barrier, the block's contiguous run of 4 * NT records (NT threads) is
+// written as whole int4s, each store instruction covering 1 KiB of
+// consecutive memory.  The run starts at record 4 * base, a multiple of 16
+// records, so it is 16-byte aligned for either record size.
+template <int NT>
+__device__ __forceinline__ void staged_children_store(int4 *stage, const int4 (&c)[4], He *out, int32_t base,
+                                                      int32_t H) {
+    if (kHasUv) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) stage[4 * threadIdx.x + k] = c[k];
+    } else {
+        int32_t *w = reinterpret_cast<int32_t *>(stage) + 12 * threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            w[3 * k] = c[k].x;
+            w[3 * k + 1] = c[k].y;
+            w[3 * k + 2] = c[k].z;
+        }
+    }
+    __syncthreads();
+    int4 *out4 = reinterpret_cast<int4 *>(out);
+    const int64_t obase = kStageInt4 * (int64_t)base;
+    const int64_t limit = kStageInt4 * (int64_t)H;
+#pragma unroll
+    for (int k = 0; k < kStageInt4; ++k) {
+        const int64_t o = obase + threadIdx.x + (int64_t)k * NT;
+        if (o < limit) store_stream(out4 + o, stage[threadIdx.x + k * NT]);
+    }
+}
 
 // ---- vertex points ----------------------------------------------------------
 // Per-halfedge scatter contribution (creased :2509-2611 with the corner test
@@ -357,7 +392,7 @@ __device__ __forceinline__ bool structural_owner(int32_t h, int d, const DCage &
         if (digit == 2) return (p & 3) == 0;
         // the parent level's edge-representative bit when it is kept
         if (erep) return ((erep[t.R[lvl - 1] + (p >> 6)] >> (p & 63)) & 1ull) != 0ull;
-        return is_edge_rep(c, t, reinterpret_cast<const int4 *>(levels[lvl - 1])[p].y, lvl - 1, p);
+        return is_edge_rep(c, t, levels[lvl - 1][p].edge, lvl - 1, p);
     }
     const CageHe x = c.he[p];
     if (digit == 0) return c.v2h[x.vert] == p;
@@ -390,7 +425,7 @@ __device__ __forceinline__ bool marked_ring_fan(int32_t h, int d, const DCage &c
     if (digit != 1) return false;
     int32_t tw, np, nw, e;
     if (lvl >= 2) {
-        const int4 x = reinterpret_cast<const int4 *>(levels[lvl - 1])[p];
+        const int4 x = he_load(levels[lvl - 1], p);
         tw = x.x;
         e = x.y;
         np = qnext(p);
@@ -432,7 +467,7 @@ __device__ __forceinline__ int32_t marked_fan(int32_t h, int d, const DCage &c, 
         return (int32_t)t.V[lvl - 1] + (p >> 2);
     }
     if (lvl >= 2 && digit == 1) {
-        const int4 x = reinterpret_cast<const int4 *>(levels[lvl - 1])[p];
+        const int4 x = he_load(levels[lvl - 1], p);
         const int32_t tw = x.x, np = qnext(p);
         f.kind = 3;
         f.shift = 2 * k;
@@ -449,7 +484,7 @@ __device__ __forceinline__ int32_t marked_fan(int32_t h, int d, const DCage &c, 
         }
         return (int32_t)(t.V[lvl - 1] + t.F[lvl - 1]) + x.y;
     }
-    const int32_t v = reinterpret_cast<const int4 *>(levels[d])[h].z;
+    const int32_t v = levels[d][h].vert;
     owner_fan(h, v, d, c, t, levels, f);
     return v;
 }
@@ -547,7 +582,7 @@ struct FanOnDemand {
     const DCage *c;
     __device__ __forceinline__ FanEntry get(int i) const {
         const int32_t r = fan_at(*f, *c, i);
-        const int4 *he4 = reinterpret_cast<const int4 *>(a->he);
+        const HeView he4{a->he};
         return fan_entry(*a, r, he4[r], he4[qprev(r)]);
     }
     __device__ __forceinline__ int32_t key(int i) const { return fan_at(*f, *c, i); }
@@ -795,7 +830,7 @@ struct FanPreloaded8 {
 template <int GATHER, int CREASED, int PRE8 = 0>
 __device__ __forceinline__ void fan_vertex_store(const FusedArgs &a, const DCage &c, const LevelTab &t, int d,
                                                  const Fan &f, int32_t v) {
-    const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
+    const HeView he4{a.he};
     const P3 old = a.old[v];
     int32_t h[4], e[4];
     if (structural_ring(f, t, d, v, h, e)) {

@@ -180,7 +180,7 @@ __global__ __launch_bounds__(256) void k_level_vertices_fan(FusedArgs a, DCage c
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, a.xcd_remap);
     const int32_t h = (int32_t)(lb * blockDim.x + threadIdx.x);
     if (h >= a.H) return;
-    const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
+    const HeView he4{a.he};
     const int32_t v = he4[h].z;
     Fan f;
     if (!owner_fan(h, v, d, c, t, lp.he, f)) return;

@@ -273,6 +273,7 @@ cc_VertexPoint ccs_VertexPoint(const cc_Subd *s, int32_t v, int32_t d) {
 cc_VertexPoint ccs_HalfedgeVertexPoint(const cc_Subd *s, int32_t h, int32_t d) {
     return ccs_VertexPoint(s, ccs_HalfedgeVertexID(s, h, d), d);
 }
+#ifndef CC_DISABLE_UV   // absent from the reference's CC_DISABLE_UV build (CatmullClark.h:170-172)
 cc_VertexUv ccs_HalfedgeVertexUv(const cc_Subd *s, int32_t h, int32_t d) {
     const UV uv = uv_decode(subd_he(s, h, d)->uvID);
     cc_VertexUv r;
@@ -280,6 +281,7 @@ cc_VertexUv ccs_HalfedgeVertexUv(const cc_Subd *s, int32_t h, int32_t d) {
     r.v = uv.v;
     return r;
 }
+#endif
 int32_t ccs_PrevVertexHalfedgeID(const cc_Subd *s, int32_t h, int32_t d) {
     return ccs_HalfedgeTwinID(s, ccs_HalfedgePrevID(s, h, d), d);
 }

@@ -20,14 +20,14 @@ __global__ void k_level_halfedges(const He *__restrict__ in, He *__restrict__ ou
                                   int32_t F, int32_t E, int topo, int uvs) {
     int32_t h = (int32_t)gid();
     if (h >= H) return;
-    const int4 *q4 = reinterpret_cast<const int4 *>(in) + (h & ~3);
+    const HeView q4{in + (h & ~3)};
     const int j = h & 3;
     int4 r[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) r[k] = q4[k];
     const int4 x = r[j];
     const int4 p = r[(j + 3) & 3];
-    He ch[4];
+    HeFull ch[4];
     if (topo) {
         child_topology(ch, h, x.x, qprev(h), qnext(h), h >> 2, x.y, x.z, p.y, p.x, qnext(x.x), V, F, E);
     }
@@ -76,7 +76,7 @@ __device__ __forceinline__ P3 lpt(const Lvl &L, int32_t h) { return L.old[L.he[h
 __global__ void k_faces(Lvl L, int gather) {
     int32_t f = (int32_t)gid();
     if (f >= L.F) return;
-    const int4 *q4 = reinterpret_cast<const int4 *>(L.he) + 4 * (int64_t)f;
+    const HeView q4{L.he + 4 * (int64_t)f};
     P3 v[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) v[k] = L.old[q4[k].z];

@@ -66,8 +66,10 @@ __device__ __forceinline__ void level_fused_points(const FusedArgs &a, int32_t h
 template <int GATHER, int CREASED, int TOPO, int UVMODE, int POINTS, int STORE, int NT>
 __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCage &cg, const LevelTab &tb, int d,
                                                   uint32_t lb) {
-    constexpr bool kStaged = STORE == kStoreLds && TOPO && UVMODE != 0;
-    __shared__ int4 stage[kStaged ? 4 * NT : 1];
+    // whole child records written (a library without uvs has nothing else)
+    constexpr bool kWhole = TOPO && (UVMODE != 0 || !kHasUv);
+    constexpr bool kStaged = STORE == kStoreLds && kWhole;
+    __shared__ int4 stage[kStaged ? kStageInt4 * NT : 1];
     __shared__ P3 spts[POINTS ? NT : 1];
     __shared__ LevelTab stab;   // for the owner test (read after the block's barrier)
     if (POINTS && a.owner_mask) lds_fill(stab, tb);
@@ -76,7 +78,7 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
     const int32_t h = base + (int32_t)threadIdx.x;
     const bool live = h < a.H;   // H is a multiple of 4: quads are whole
     const int j = h & 3;
-    const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
+    const HeView he4{a.he};
     const int4 x = live ? he4[h] : make_int4(0, 0, 0, 0);
     // this level's edge-representative bits of the 64 halfedges of the wave
     const uint64_t rw = (a.erep_write && d >= 2 && live) ? a.erep[tb.R[d] + (h >> 6)] : 0ull;
@@ -138,30 +140,18 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
             c[2].w = uv_encode_unit(fuv);
             c[3].w = uv_encode_unit(uv_mid(uv, puv));
         }
-        int4 *out4 = reinterpret_cast<int4 *>(a.out_he);
         if (kStaged) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) stage[4 * threadIdx.x + k] = c[k];
-            __syncthreads();
-            const int64_t obase = 4 * (int64_t)base;
-            const int64_t limit = 4 * (int64_t)a.H;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int64_t o = obase + threadIdx.x + (int64_t)k * blockDim.x;
-                if (o < limit) store_stream(out4 + o, stage[threadIdx.x + k * blockDim.x]);
-            }
+            staged_children_store<NT>(stage, c, a.out_he, base, a.H);
         } else if (live) {
-            if (TOPO && UVMODE != 0) {
+            if (kWhole) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) out4[4 * (int64_t)h + k] = c[k];
+                for (int k = 0; k < 4; ++k) he_store(a.out_he, 4 * (int64_t)h + k, c[k]);
             } else if (TOPO) {
-                He *o = a.out_he + 4 * (int64_t)h;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) { o[k].twin = c[k].x; o[k].edge = c[k].y; o[k].vert = c[k].z; }
+                for (int k = 0; k < 4; ++k) he_store_ids(a.out_he, 4 * (int64_t)h + k, c[k]);
             } else {
-                He *o = a.out_he + 4 * (int64_t)h;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) o[k].uv = c[k].w;
+                for (int k = 0; k < 4; ++k) he_store_uv(a.out_he, 4 * (int64_t)h + k, c[k].w);
             }
         }
     }
@@ -222,7 +212,7 @@ __device__ __forceinline__ void level_vertices_compact_block(const FusedArgs &a,
     lds_fill(st, t);
     lds_fill(slp, lp);
     const int32_t base = (int32_t)(lb * 256 * SCAN);
-    const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
+    const HeView he4{a.he};
     if (threadIdx.x == 0) nown = 0;
     __syncthreads();
     int32_t vs[SCAN];

@@ -8,22 +8,25 @@
 
 namespace cch {
 
-__device__ __forceinline__ void store_children(He *o, const He c[4], bool topo, bool uvs) {
-    if (topo && uvs) {
-        int4 *o4 = reinterpret_cast<int4 *>(o);
+// Children are built as full 16-byte records (uv code included) and stored
+// in the library's record layout.
+using HeFull = HeRec<16>;
+
+__device__ __forceinline__ void store_children(He *o, const HeFull c[4], bool topo, bool uvs) {
+    if (topo && (uvs || !kHasUv)) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) o4[k] = make_int4(c[k].twin, c[k].edge, c[k].vert, c[k].uv);
+        for (int k = 0; k < 4; ++k) he_store(o, k, make_int4(c[k].twin, c[k].edge, c[k].vert, c[k].uv));
     } else if (topo) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) { o[k].twin = c[k].twin; o[k].edge = c[k].edge; o[k].vert = c[k].vert; }
+        for (int k = 0; k < 4; ++k) he_store_ids(o, k, make_int4(c[k].twin, c[k].edge, c[k].vert, 0));
     } else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) o[k].uv = c[k].uv;
+        for (int k = 0; k < 4; ++k) he_store_uv(o, k, c[k].uv);
     }
 }
 
 // children twin/edge/vertex IDs (CatmullClark.h:2721-2737, :2777-2793)
-__device__ __forceinline__ void child_topology(He c[4], int32_t h, int32_t twin, int32_t prev, int32_t next,
+__device__ __forceinline__ void child_topology(HeFull c[4], int32_t h, int32_t twin, int32_t prev, int32_t next,
                                                int32_t face, int32_t edge, int32_t vert, int32_t prevEdge,
                                                int32_t prevTwin, int32_t twinNext, int32_t V, int32_t F,
                                                int32_t E) {
@@ -46,7 +49,7 @@ __device__ __forceinline__ void child_topology(He c[4], int32_t h, int32_t twin,
 __device__ inline void cage_halfedges_at(DCage c, He *out, int topo, int uvs, int32_t h) {
     if (h >= c.H) return;
     const CageHe x = c.he[h];
-    He ch[4];
+    HeFull ch[4];
     if (topo) {
         const CageHe p = c.he[x.prev];
         int32_t twinNext = x.twin >= 0 ? c.he[x.twin].next : -1;

@@ -48,7 +48,7 @@ __device__ __forceinline__ void defer_mark(const FusedArgs &a, bool want, int32_
 // LDS of one tile: staged children, records, points, face and edge points
 template <int NT, bool STAGED>
 struct PatchLds {
-    int4 stage[STAGED ? 4 * NT : 1];
+    int4 stage[STAGED ? kStageInt4 * NT : 1];
     int4 srec[NT];
     P3 spts[NT];
     P3 sF[NT / 4];
@@ -58,7 +58,7 @@ struct PatchLds {
 template <int GATHER, int CREASED, int TOPO, int UVMODE, int NT, class LP>
 __device__ __forceinline__ void patch_tile(const FusedArgs &a, const DCage &c, const LevelTab &t, const LP &lp, int d,
                                            uint32_t tile) {
-    constexpr bool kStaged = TOPO && UVMODE != 0;
+    constexpr bool kStaged = TOPO && (UVMODE != 0 || !kHasUv);   // whole child records
     __shared__ PatchLds<NT, kStaged> L;
     __shared__ LevelTab st;     // read after the tile's first barrier
     __shared__ LevelPtrs slp;
@@ -69,7 +69,7 @@ __device__ __forceinline__ void patch_tile(const FusedArgs &a, const DCage &c, c
     const int32_t h = base + tid;
     const bool live = h < a.H;   // H is a multiple of 4: quads are whole
     const int j = h & 3;
-    const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
+    const HeView he4{a.he};
     const int4 x = live ? he4[h] : make_int4(0, 0, 0, 0);
     const int32_t twin = x.x;
     const bool eowner = live && h > twin;
@@ -102,28 +102,17 @@ __device__ __forceinline__ void patch_tile(const FusedArgs &a, const DCage &c, c
             ch[2].w = uv_encode_unit(fuv);
             ch[3].w = uv_encode_unit(uv_mid(uv, puv));
         }
-        int4 *out4 = reinterpret_cast<int4 *>(a.out_he);
         if (kStaged) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) L.stage[4 * tid + k] = ch[k];
-            __syncthreads();
-            const int64_t obase = 4 * (int64_t)base;
-            const int64_t limit = 4 * (int64_t)a.H;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int64_t o = obase + tid + (int64_t)k * NT;
-                if (o < limit) out4[o] = L.stage[tid + k * NT];
-            }
+            staged_children_store<NT>(L.stage, ch, a.out_he, base, a.H);
         } else {
             __syncthreads();
             if (live) {
-                He *o = a.out_he + 4 * (int64_t)h;
                 if (TOPO) {
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) { o[k].twin = ch[k].x; o[k].edge = ch[k].y; o[k].vert = ch[k].z; }
+                    for (int k = 0; k < 4; ++k) he_store_ids(a.out_he, 4 * (int64_t)h + k, ch[k]);
                 } else {
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) o[k].uv = ch[k].w;
+                    for (int k = 0; k < 4; ++k) he_store_uv(a.out_he, 4 * (int64_t)h + k, ch[k].w);
                 }
             }
         }
@@ -209,7 +198,7 @@ __device__ __forceinline__ void patch_tile(const FusedArgs &a, const DCage &c, c
 template <int GATHER, int CREASED>
 __device__ __forceinline__ void deferred_vertex(const FusedArgs &a, const DCage &c, const LevelTab &t,
                                                 const LevelPtrs &lp, int d, int32_t h) {
-    const int4 *he4 = reinterpret_cast<const int4 *>(a.he);
+    const HeView he4{a.he};
     const int32_t v = he4[h].z;
     Fan f;
     owner_fan(h, v, d, c, t, lp.he, f);

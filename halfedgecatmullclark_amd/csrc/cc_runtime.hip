@@ -60,6 +60,16 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// What a refinement does with the children's uvIDs: 0 leave them, 1 refine
+// them, 2 write 0 (a cage without uvs, when the topology is written).  A
+// library built without uvs has no uvID field: always 0.
+int uv_mode(int32_t steps, int32_t U0) {
+    if (!kHasUv) return 0;
+    int m = ((steps & CCD_UVS) && U0 > 0) ? 1 : 0;
+    if ((steps & CCD_HALFEDGES) && m == 0 && U0 == 0) m = 2;
+    return m;
+}
+
 }  // namespace
 
 // ---- device mirror -----------------------------------------------------------
@@ -267,8 +277,7 @@ LevelPtrs ccd_Subd::level_ptrs() const {
 // uvIDs, 1 refine, 2 write zero.
 bool ccd_Subd::refine_topology(int32_t steps, int dmax, hipStream_t st) {
     const bool topo = (steps & CCD_HALFEDGES) != 0;
-    int uvMode = ((steps & CCD_UVS) && U0 > 0) ? 1 : 0;
-    if (topo && uvMode == 0 && U0 == 0) uvMode = 2;
+    const int uvMode = uv_mode(steps, U0);
     if (topo || uvMode == 1) {
         const DCage c = cage_view(nullptr);
         launch("cage_halfedges", k_cage_halfedges, H0, st, c, he, (int)topo, uvMode);
@@ -393,8 +402,7 @@ bool ccd_Subd::refine(int32_t steps, int dmax, hipStream_t st) {
     // vertex points
     const bool topo = (steps & CCD_HALFEDGES) != 0;
     const bool points = (steps & CCD_POINTS) != 0;
-    int uvMode = ((steps & CCD_UVS) && U0 > 0) ? 1 : 0;
-    if (topo && uvMode == 0 && U0 == 0) uvMode = 2;
+    const int uvMode = uv_mode(steps, U0);
     if (steps & CCD_CREASES) {
         launch("cage_creases", k_creases, E0, st, (const Crease *)c_cr, E0, cr);
         for (int d = 1; d < dmax; ++d)
@@ -554,9 +562,9 @@ bool host_refine(cc_Subd *subd, int32_t steps, int dmax, const char *fname) {
     const cc_Mesh *cage = subd->cage;
     if (!s->upload_cage(cage, st)) return false;
     const Counts &n = s->n;
-    const bool uvActive = (steps & CCD_UVS) && s->U0 > 0;
+    const bool uvActive = kHasUv && (steps & CCD_UVS) && s->U0 > 0;
     const bool writesHe = (steps & CCD_HALFEDGES) || uvActive;
-    const bool partialHe = (steps & CCD_HALFEDGES) ? (!uvActive && s->U0 > 0) : uvActive;
+    const bool partialHe = (steps & CCD_HALFEDGES) ? (kHasUv && !uvActive && s->U0 > 0) : uvActive;
     const bool pointsNeedTopo = (steps & CCD_POINTS) && !(steps & CCD_HALFEDGES);
     if (partialHe || pointsNeedTopo) {
         const int64_t count = partialHe ? n.cumH(dmax) : n.cumH(dmax - 1);
@@ -632,7 +640,9 @@ CC_HOST_ENTRY(ccs_RefineHalfedges, CCD_HALFEDGES, subd ? subd->maxDepth : 1)
 CC_HOST_ENTRY(ccs_RefineCreases, CCD_CREASES, subd ? subd->maxDepth : 1)
 CC_HOST_ENTRY(ccs_RefineCageHalfedges, CCD_HALFEDGES, 1)
 CC_HOST_ENTRY(ccs_RefineCageCreases, CCD_CREASES, 1)
+#ifndef CC_DISABLE_UV   // absent from the reference's CC_DISABLE_UV build
 CC_HOST_ENTRY(ccs_RefineCageVertexUvs, CCD_UVS, 1)
+#endif
 CC_HOST_ENTRY(ccs_RefineCageVertexPoints_Scatter, CCD_POINTS, 1)
 CC_HOST_ENTRY(ccs_RefineCageVertexPoints_Gather, CCD_POINTS | CCD_GATHER, 1)
 
@@ -679,11 +689,13 @@ void ccs_RefineVertexPoints_Adjoint_f64(const cc_Subd *subd, const double *dL_dF
     host_adjoint(subd, dL_dFinest, dL_dCage, sizeof(double));
 }
 
+#ifndef CC_DISABLE_UV   // absent from the reference's CC_DISABLE_UV build (CatmullClark.h:204-206)
 // no-op without cage uvs, like the reference (CatmullClark.h:2930)
 void ccs_RefineVertexUvs(cc_Subd *subd) {
     if (subd && subd->cage && subd->cage->uvCount <= 0) { clear_error(); return; }
     (void)host_refine(subd, CCD_UVS, subd ? subd->maxDepth : 1, "ccs_RefineVertexUvs");
 }
+#endif
 
 // ---- device-resident API -------------------------------------------------------
 ccd_Subd *ccd_Create(const cc_Mesh *cage, int32_t maxDepth) {

@@ -56,11 +56,17 @@ typedef struct {
     int32_t uvID;
 } cc_Halfedge;
 
+/* 16 bytes; 12 without uvID when CC_DISABLE_UV is defined, as in the
+ * reference (CatmullClark.h:47-55).  libcchip_nouv.so is the library built
+ * that way: link it, and define CC_DISABLE_UV before including this header,
+ * to keep 12-byte level records (no uv refinement). */
 typedef struct {
     int32_t twinID;
     int32_t edgeID;
     int32_t vertexID;
+#ifndef CC_DISABLE_UV
     int32_t uvID;
+#endif
 } cc_Halfedge_SemiRegular;
 
 typedef struct {
@@ -164,7 +170,9 @@ int32_t ccs_HalfedgeFaceID(const cc_Subd *subd, int32_t halfedgeID, int32_t dept
 int32_t ccs_HalfedgeEdgeID(const cc_Subd *subd, int32_t halfedgeID, int32_t depth);
 int32_t ccs_HalfedgeVertexID(const cc_Subd *subd, int32_t halfedgeID, int32_t depth);
 cc_VertexPoint ccs_HalfedgeVertexPoint(const cc_Subd *subd, int32_t halfedgeID, int32_t depth);
+#ifndef CC_DISABLE_UV
 cc_VertexUv ccs_HalfedgeVertexUv(const cc_Subd *subd, int32_t halfedgeID, int32_t depth);
+#endif
 float ccs_HalfedgeSharpness(const cc_Subd *subd, int32_t halfedgeID, int32_t depth);
 int32_t ccs_CreaseNextID_Fast(const cc_Subd *subd, int32_t edgeID, int32_t depth);
 int32_t ccs_CreaseNextID(const cc_Subd *subd, int32_t edgeID, int32_t depth);
@@ -189,7 +197,9 @@ void ccs_RefineVertexPoints_Gather(cc_Subd *subd);              /* :2659 */
 void ccs_RefineVertexPoints_Scatter(cc_Subd *subd);             /* :2631 */
 void ccs_RefineHalfedges(cc_Subd *subd);                        /* :2803 */
 void ccs_RefineCreases(cc_Subd *subd);                          /* :3033 */
+#ifndef CC_DISABLE_UV
 void ccs_RefineVertexUvs(cc_Subd *subd);                        /* :2928 */
+#endif
 void ccs_Refine_NoCreases_Gather(cc_Subd *subd);                /* :3078 */
 void ccs_Refine_NoCreases_Scatter(cc_Subd *subd);               /* :3072 */
 void ccs_RefineVertexPoints_NoCreases_Gather(cc_Subd *subd);    /* :2672 */
@@ -206,7 +216,9 @@ const char *ccs_LastErrorString(void);
  * ccs__Cage*Points_* kernels :1408-1991).  Each writes level 1 only. */
 void ccs_RefineCageHalfedges(cc_Subd *subd);
 void ccs_RefineCageCreases(cc_Subd *subd);
+#ifndef CC_DISABLE_UV
 void ccs_RefineCageVertexUvs(cc_Subd *subd);
+#endif
 void ccs_RefineCageVertexPoints_Scatter(cc_Subd *subd);
 void ccs_RefineCageVertexPoints_Gather(cc_Subd *subd);
 
