@@ -319,6 +319,18 @@ def main():
                   "adj_roofline_frac": adj_bytes / (adj_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     del pts_out, grad_fin, grad_cage
 
+    # halfedge normals of the finest level (537 M halfedges): corner
+    # normals, and vertex normals summed around each ring
+    nrm = torch.empty((n["H"][D], 3), device=dev)
+    nrm_ms = {s_: ev_time(lambda: ds.halfedge_normals(nrm.data_ptr(), s_, sptr), 5) for s_ in (False, True)}
+    del nrm
+    HD, VD = n["H"][D], n["V"][D]
+    nrm_bytes = {False: 16 * HD + 12 * VD + 12 * HD, True: 16 * HD + 12 * VD + 12 * HD + 16 * HD + 24 * HD}
+    normals = {"workload": "halfedge normals of the torus 64x32 at depth 8 (ccd_HalfedgeNormals)",
+               "corner_ms": nrm_ms[False], "vertex_ms": nrm_ms[True],
+               "corner_gbs": nrm_bytes[False] / (nrm_ms[False] * 1e-3) / 1e9,
+               "vertex_gbs": nrm_bytes[True] / (nrm_ms[True] * 1e-3) / 1e9}
+
     # the same refinement with 12-byte level records (libcchip_nouv.so, the
     # CC_DISABLE_UV layout): no uvIDs, 48 instead of 64 bytes of children
     # per parent halfedge
@@ -427,6 +439,7 @@ def main():
         "gather": {"ms_per_step": ms_gather, "value": hcum / (ms_gather * 1e-3)},
         "adjoint_d8": adjoint_d8,
         "nouv_records": nouv,
+        "normals": normals,
         "fwd_adjoint": {"workload": "torus 32x16 cage, depth 5, loss 0.5*||S(V)-T||^2",
                         "fwd_ms": res["fwd"], "fwd_adj_ms": res["fwd_adj"],
                         "ratio": res["fwd_adj"] / res["fwd"]},

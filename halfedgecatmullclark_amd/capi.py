@@ -108,6 +108,10 @@ def lib(variant: str = "uv") -> ctypes.CDLL:
         "ccm_Save": (ctypes.c_bool, [P(cc_Mesh), ctypes.c_char_p]),
         "ccm_Release": (None, [P(cc_Mesh)]),
         "ccd_RefineNoAlloc": (i32, [P(cc_Mesh), P(cc_Subd), i32, vp]),
+        "ccd_HalfedgeNormals": (i32, [vp, vp, i32, vp]),
+        "ccs_HalfedgeNormals": (None, [P(cc_Subd), vp]),
+        "ccs_HalfedgeNormals_Fast": (None, [P(cc_Subd), vp]),
+        "ccs_ExportToObj": (ctypes.c_bool, [P(cc_Subd), i32, ctypes.c_char_p]),
     }
     for name in HOST_DRIVERS:
         sigs[name] = (None, [P(cc_Subd)])
@@ -173,6 +177,22 @@ class HostSubd:
         if code:
             raise CCError(f"{name}: {msg}")
         return out
+
+    def normals(self, smooth: bool = True) -> np.ndarray:
+        """Halfedge normals of level maxDepth from the subd's host arrays
+        (ccs_HalfedgeNormals / ccs_HalfedgeNormals_Fast)."""
+        out = np.zeros((self.n["H"][self.depth], 3), np.float32)
+        name = "ccs_HalfedgeNormals" if smooth else "ccs_HalfedgeNormals_Fast"
+        getattr(self.L, name)(self.subd, out.ctypes.data)
+        code, msg = last_error(self.variant)
+        if code:
+            raise CCError(f"{name}: {msg}")
+        return out
+
+    def export_obj(self, depth: int, path: str):
+        """Level ``depth`` as an OBJ file (ccs_ExportToObj)."""
+        if not self.L.ccs_ExportToObj(self.subd, depth, os.fsencode(path)):
+            raise CCError(f"ccs_ExportToObj: {last_error(self.variant)[1]}")
 
     def arrays(self, keys=("halfedges", "creases", "points")) -> dict:
         """Copies of the subd's host arrays named in ``keys``."""
@@ -279,6 +299,9 @@ class DeviceSubd:
 
     def synchronize(self):
         self._ok(self.L.ccd_Synchronize(self.handle), "ccd_Synchronize")
+
+    def halfedge_normals(self, out_ptr: int, smooth: bool = True, stream: int | None = None):
+        self._ok(self.L.ccd_HalfedgeNormals(self.handle, out_ptr, int(smooth), stream), "ccd_HalfedgeNormals")
 
     def pointer(self, which: int) -> int:
         return int(self.L.ccd_DevicePointer(self.handle, which) or 0)

@@ -302,4 +302,82 @@ int32_t ccs_VertexPointToHalfedgeID(const cc_Subd *s, int32_t v, int32_t d) {
 }
 int32_t ccs_VertexToHalfedgeID(const cc_Subd *s, int32_t v, int32_t d) { return ccs_VertexPointToHalfedgeID(s, v, d); }
 
+// Level `depth` of a refined subd (the cage for depth 0) as a Wavefront OBJ
+// file in the text the reference's example writes (examples/subd_cpu.c:22-115,
+// ExportToObj): "v" lines, "vt" lines (the cage's uvs at depth 0, one per
+// halfedge above it), then "f" lines, 1-based, with per-halfedge uv indices
+// above depth 0; "%f" formatting throughout.  Returns false when the file
+// cannot be written.
+bool ccs_ExportToObj(const cc_Subd *subd, int32_t depth, const char *filename) {
+    clear_error();
+    if (!subd || !subd->cage || !filename || depth < 0 || depth > subd->maxDepth) {
+        set_error(11, "ccs_ExportToObj: null argument or depth out of range");
+        return false;
+    }
+    FILE *pf = fopen(filename, "w");
+    if (!pf) {
+        set_error(3, "ccs_ExportToObj: fopen failed");
+        fprintf(stderr, "cc: ccs_ExportToObj: fopen failed\n");
+        return false;
+    }
+    const cc_Mesh *cage = subd->cage;
+    const int32_t vertexCount = ccm_VertexCountAtDepth(cage, depth);
+    const int32_t faceCount = ccm_FaceCountAtDepth(cage, depth);
+    fprintf(pf, "# Vertices\n");
+    if (depth == 0) {
+        for (int32_t v = 0; v < vertexCount; ++v) {
+            const cc_VertexPoint p = ccm_VertexPoint(cage, v);
+            fprintf(pf, "v %f %f %f\n", p.x, p.y, p.z);
+        }
+        for (int32_t t = 0; t < ccm_UvCount(cage); ++t) {
+            const cc_VertexUv uv = ccm_Uv(cage, t);
+            fprintf(pf, "vt %f %f\n", uv.u, uv.v);
+        }
+    } else {
+        for (int32_t v = 0; v < vertexCount; ++v) {
+            const cc_VertexPoint p = ccs_VertexPoint(subd, v, depth);
+            fprintf(pf, "v %f %f %f\n", p.x, p.y, p.z);
+        }
+#ifndef CC_DISABLE_UV
+        const int32_t halfedgeCount = ccm_HalfedgeCountAtDepth(cage, depth);
+        for (int32_t h = 0; h < halfedgeCount; ++h) {
+            const cc_VertexUv uv = ccs_HalfedgeVertexUv(subd, h, depth);
+            fprintf(pf, "vt %f %f\n", uv.u, uv.v);
+        }
+#endif
+    }
+    fprintf(pf, "\n");
+    fprintf(pf, "# Topology\n");
+    if (depth == 0) {
+        for (int32_t f = 0; f < faceCount; ++f) {
+            const int32_t h0 = ccm_FaceToHalfedgeID(cage, f);
+            fprintf(pf, "f %i/%i", ccm_HalfedgeVertexID(cage, h0) + 1, ccm_HalfedgeUvID(cage, h0) + 1);
+            int32_t n = 1;
+            for (int32_t h = ccm_HalfedgeNextID(cage, h0); h != h0 && n < kWalkCap; h = ccm_HalfedgeNextID(cage, h), ++n)
+                fprintf(pf, " %i/%i", ccm_HalfedgeVertexID(cage, h) + 1, ccm_HalfedgeUvID(cage, h) + 1);
+            fprintf(pf, "\n");
+        }
+    } else {
+        for (int32_t f = 0; f < faceCount; ++f) {
+            const int32_t h = 4 * f;
+#ifndef CC_DISABLE_UV
+            fprintf(pf, "f %i/%i %i/%i %i/%i %i/%i\n", ccs_HalfedgeVertexID(subd, h, depth) + 1, h + 1,
+                    ccs_HalfedgeVertexID(subd, h + 1, depth) + 1, h + 2, ccs_HalfedgeVertexID(subd, h + 2, depth) + 1,
+                    h + 3, ccs_HalfedgeVertexID(subd, h + 3, depth) + 1, h + 4);
+#else
+            fprintf(pf, "f %i %i %i %i\n", ccs_HalfedgeVertexID(subd, h, depth) + 1,
+                    ccs_HalfedgeVertexID(subd, h + 1, depth) + 1, ccs_HalfedgeVertexID(subd, h + 2, depth) + 1,
+                    ccs_HalfedgeVertexID(subd, h + 3, depth) + 1);
+#endif
+        }
+        fprintf(pf, "\n");
+    }
+    const bool ok = !ferror(pf);
+    if (fclose(pf) != 0 || !ok) {
+        set_error(6, "ccs_ExportToObj: write failed");
+        return false;
+    }
+    return true;
+}
+
 }  // extern "C"

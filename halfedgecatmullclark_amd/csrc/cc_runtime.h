@@ -53,6 +53,9 @@ struct ccd_Subd {
     uint64_t *erep = nullptr;
     bool erep_valid[cch::kMaxDepth + 1] = {};
     size_t adj_scalar = 0;
+    // halfedge normals (allocated on first use, H_D points each): the corner
+    // normals the vertex normals sum, and the host entry points' output
+    cch::P3 *nrm_tmp = nullptr, *nrm_out = nullptr;
     bool timing = false;
     int opt_fused = 1, opt_store = cch::kStoreLds, opt_remap = 1, opt_fan = 2;
     int32_t opt_adj_vertex = 1 << 20;   // CCD_OPT_ADJ_VERTEX
@@ -88,4 +91,11 @@ struct ccd_Subd {
     void launch(const char *name, K kernel, int64_t threads, hipStream_t st, Args... args);
     template <typename Fn>
     void launch_timed(const char *name, int64_t threads, hipStream_t st, Fn &&fn);
+    bool normals(cch::P3 *out, bool smooth, hipStream_t st);
 };
+
+namespace cch {
+// the device mirror of a host cc_Subd (created on first use, freed by
+// ccs_Release), or null with the error channel set
+ccd_Subd *host_mirror(const cc_Subd *subd);
+}  // namespace cch

@@ -99,7 +99,7 @@ bool ccd_Subd::alloc(const cc_Mesh *cage, int32_t depth) {
 void ccd_Subd::free_all() {
     DeviceGuard g(device);
     void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts, adj_a, adj_b, adj_w, adj_in, adj_out, adj_sharp,
-                    defer_mask, rings, rare, erep};
+                    defer_mask, rings, rare, erep, nrm_tmp, nrm_out};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : timed) {
@@ -585,6 +585,21 @@ bool host_refine(cc_Subd *subd, int32_t steps, int dmax, const char *fname) {
 }
 
 }  // namespace
+
+namespace cch {
+ccd_Subd *host_mirror(const cc_Subd *subd) {
+    if (!subd || !subd->cage) {
+        set_error(11, "null subd or cage");
+        return nullptr;
+    }
+    const char *why = nullptr;
+    if (!ref_counts_fit_int32(subd->cage, subd->maxDepth, &why)) {
+        set_error(12, why);
+        return nullptr;
+    }
+    return mirror_for(subd);
+}
+}  // namespace cch
 
 extern "C" {
 

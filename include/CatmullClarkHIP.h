@@ -236,6 +236,21 @@ void ccs_RefineVertexPoints_Adjoint_f64(const cc_Subd *subd,
                                         const double *dL_dFinest,
                                         double *dL_dCage);
 
+/* ---- extension: normals and OBJ export ----------------------------------
+ * The step after refinement.  Normals of every halfedge of level maxDepth
+ * (ccm_HalfedgeCountAtDepth(cage, maxDepth) records), as the reference's
+ * shaders define them (glsl/CatmullClark_Scatter.glsl:931-960,
+ * ccs_HalfedgeNormal_Fast / ccs_HalfedgeNormal): the corner normal
+ * normalize(cross(P(next) - P(h), P(prev) - P(h))), or the normalized sum
+ * of the corner normals around h's vertex (the corner normal at a boundary).
+ * Computed on the GPU from the subd's host arrays of level maxDepth. */
+void ccs_HalfedgeNormals(const cc_Subd *subd, cc_VertexPoint *normals);
+void ccs_HalfedgeNormals_Fast(const cc_Subd *subd, cc_VertexPoint *normals);
+/* Level `depth` (0: the cage) as an OBJ file, in the text the reference's
+ * example writes (examples/subd_cpu.c:22-115, ExportToObj); false when the
+ * file cannot be written. */
+bool ccs_ExportToObj(const cc_Subd *subd, int32_t depth, const char *filename);
+
 /* ---- extension: device-resident refinement ------------------------------
  * A ccd_Subd owns device copies of a cage and of levels 1..maxDepth laid out
  * exactly as cc_Subd's arrays.  Calls are asynchronous on `stream` (a
@@ -280,6 +295,10 @@ int32_t ccd_RefinePointsAdjoint(ccd_Subd *subd, const float *devGradFinest, floa
                                 int32_t steps, void *stream);
 int32_t ccd_Download(const ccd_Subd *subd, cc_Subd *dst, int32_t steps);
 int32_t ccd_Synchronize(ccd_Subd *subd);
+/* halfedge normals of level maxDepth into devNormals (H_D x 3 floats) from
+ * the device topology and points (refined already): vertex normals when
+ * smooth != 0, else corner normals (see ccs_HalfedgeNormals) */
+int32_t ccd_HalfedgeNormals(ccd_Subd *subd, float *devNormals, int32_t smooth, void *stream);
 /* kernel-selection options (all have working defaults):
  *   CCD_OPT_FUSED      1: per level one fused kernel for children, face and
  *                      edge points plus one for vertex points (default);

@@ -29,6 +29,7 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <math.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -928,4 +929,60 @@ int32_t ORC_API(map)(int32_t V, int32_t H, int32_t E, int32_t F, const void *cag
     c.v2h = v2h; c.e2h = e2h; c.f2h = f2h;
     init_counts(&c);
     return which == 0 ? edge_rep(&c, id, depth) : vertex_rep(&c, id, depth);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Per-halfedge normals of one level of quads                               */
+/* (glsl/CatmullClark_Scatter.glsl:931-960, the only place the reference   */
+/* defines them).  The corner normal of h is                                */
+/* normalize(cross(P(next h) - P(h), P(prev h) - P(h))); the vertex normal  */
+/* is normalize(corner(h) + sum of corner(it)) over the halfedges it met    */
+/* walking it = twin(prev(it)) from h, or corner(h) when the walk reaches   */
+/* a boundary.  normalize(v) = v * (1 / sqrt(dot(v, v))) with a correctly   */
+/* rounded square root.                                                     */
+/* ------------------------------------------------------------------------ */
+static Pt psub(Pt a, Pt b) { Pt r; for (int i = 0; i < 3; ++i) r.p[i] = a.p[i] - b.p[i]; return r; }
+
+static Pt pcross(Pt a, Pt b)
+{
+    Pt r;
+    r.p[0] = a.p[1] * b.p[2] - b.p[1] * a.p[2];
+    r.p[1] = a.p[2] * b.p[0] - b.p[2] * a.p[0];
+    r.p[2] = a.p[0] * b.p[1] - b.p[0] * a.p[1];
+    return r;
+}
+
+static Pt pnormalize(Pt v)
+{
+    const R dd = v.p[0] * v.p[0] + v.p[1] * v.p[1] + v.p[2] * v.p[2];
+    const R inv = (R)1 / (R)sqrt((double)dd);
+    return pmul(v, inv);
+}
+
+static Pt corner_normal(const He *he, const Pt *pts, int32_t h)
+{
+    const Pt v0 = pts[he[h].vert], v1 = pts[he[qprev(h)].vert], v2 = pts[he[qnext(h)].vert];
+    return pnormalize(pcross(psub(v2, v0), psub(v1, v0)));
+}
+
+int32_t ORC_API(halfedge_normals)(int64_t H, const void *levelHalfedges, const void *levelPoints, int32_t smooth,
+                                  void *out)
+{
+    const He *he = (const He *)levelHalfedges;
+    const Pt *pts = (const Pt *)levelPoints;
+    Pt *o = (Pt *)out;
+#pragma omp parallel for
+    for (int64_t k = 0; k < H; ++k) {
+        const int32_t h = (int32_t)k;
+        const Pt n = corner_normal(he, pts, h);
+        if (!smooth) { o[k] = n; continue; }
+        Pt avg = pzero();
+        int32_t it = he[qprev(h)].twin;
+        for (int64_t g = 0; it >= 0 && it != h && g < H; ++g) {
+            avg = padd(avg, corner_normal(he, pts, it));
+            it = he[qprev(it)].twin;
+        }
+        o[k] = it < 0 ? n : pnormalize(padd(n, avg));
+    }
+    return 0;
 }

@@ -38,6 +38,9 @@ def lib():
             m = getattr(L, p + "map")
             m.restype = ctypes.c_int32
             m.argtypes = [ctypes.c_int32] * 4 + [ctypes.c_void_p] * 4 + [ctypes.c_int32] * 3
+            nm = getattr(L, p + "halfedge_normals")
+            nm.restype = ctypes.c_int32
+            nm.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -90,3 +93,17 @@ def level(arrays: dict, name: str, depth: int) -> np.ndarray:
     key = {"halfedges": ("Hcum", "H"), "creases": ("Ccum", "C"), "points": ("Vcum", "V")}[name]
     lo = n[key[0]][depth - 1]
     return arrays[name][lo:lo + n[key[1]][depth]]
+
+
+def halfedge_normals(arrays: dict, depth: int, smooth: bool = True) -> np.ndarray:
+    """Per-halfedge normals of level ``depth`` (>= 1) of refined ``arrays``
+    (glsl/CatmullClark_Scatter.glsl:931-960): corner normals, or with
+    ``smooth`` the normalized sum of the corner normals around each
+    halfedge's vertex (the corner normal on a boundary)."""
+    he = np.ascontiguousarray(level(arrays, "halfedges", depth))
+    pts = np.ascontiguousarray(level(arrays, "points", depth))
+    real = pts.dtype
+    out = np.zeros((len(he), 3), dtype=real)
+    f = getattr(lib(), ("orc32_" if real.itemsize == 4 else "orc64_") + "halfedge_normals")
+    f(len(he), _p(he), _p(pts), int(smooth), _p(out))
+    return out

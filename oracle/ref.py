@@ -123,3 +123,23 @@ def refine(cage: Cage, depth: int, *drivers: str, real=np.float32, threads: int 
         return r.run(*(drivers or ("ccs_Refine_Scatter",)), threads=threads).arrays()
     finally:
         r.close()
+
+
+def export_available(variant: str = "uv") -> bool:
+    return os.path.exists(_export_path(variant))
+
+
+def _export_path(variant: str) -> str:
+    return os.path.join(REF_DIR, "libccref_export_nouv.so" if variant == "nouv" else "libccref_export.so")
+
+
+def export_obj(subd_ptr, depth: int, path: str, variant: str = "uv"):
+    """The reference example's ExportToObj (examples/subd_cpu.c:22-115) on a
+    cc_Subd (any library's: the struct and its arrays are the ABI's)."""
+    key = ("export", variant)
+    if key not in _libs:
+        L = ctypes.CDLL(_export_path(variant), mode=ctypes.RTLD_LOCAL)
+        L.ref_export_obj.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_char_p]
+        L.ref_export_obj.restype = None
+        _libs[key] = L
+    _libs[key].ref_export_obj(ctypes.cast(subd_ptr, ctypes.c_void_p), depth, os.fsencode(path))
