@@ -19,7 +19,8 @@ LIB = os.path.join(HERE, "libcchip.so")
 ARCH = os.environ.get("CCHIP_ARCH", "gfx950")
 
 SOURCES = ["cc_kernels.hip", "cc_fused.hip", "cc_patch.hip", "cc_batch.hip", "cc_adjoint.hip", "cc_runtime.hip",
-           "cc_batch_runtime.hip", "cc_noalloc.hip", "cc_normals.hip", "cc_host.cpp"]
+           "cc_batch_runtime.hip", "cc_noalloc.hip", "cc_normals.hip", "cc_builder.hip",
+           "cc_host.cpp"]
 # -ffp-contract=off keeps the reference's rounding (no fused multiply-add);
 # packed FP32 instructions are disabled on the device.
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",

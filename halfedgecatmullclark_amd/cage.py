@@ -262,6 +262,29 @@ def to_obj(points, faces, uvs=None, face_uvs=None, creases=()) -> str:
     return "\n".join(lines) + "\n"
 
 
+def cage_polygons(cage: Cage):
+    """The polygons of a cage (from f2h by next), their per-corner uv IDs and
+    one (v0, v1, sharpness) tag per sharp edge: the arguments of from_faces
+    and to_obj that describe the cage again."""
+    he = cage.halfedges
+    faces, fuv = [], []
+    for f in range(cage.F):
+        h0 = int(cage.f2h[f])
+        vs, ts, h = [], [], h0
+        while True:
+            vs.append(int(he[h, 5]))
+            ts.append(int(he[h, 6]))
+            h = int(he[h, 1])
+            if h == h0:
+                break
+        faces.append(vs)
+        fuv.append(ts)
+    sharp = np.nonzero(cage.creases["sharpness"] > 0)[0]
+    creases = [(int(he[cage.e2h[e], 5]), int(he[he[cage.e2h[e], 1], 5]), float(cage.creases["sharpness"][e]))
+               for e in sharp.tolist()]
+    return faces, fuv, creases
+
+
 # --------------------------------------------------------------------------
 # seeded generators for the benchmark and test workloads
 # --------------------------------------------------------------------------

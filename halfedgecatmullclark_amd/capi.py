@@ -109,6 +109,8 @@ def lib(variant: str = "uv") -> ctypes.CDLL:
         "ccm_Release": (None, [P(cc_Mesh)]),
         "ccd_RefineNoAlloc": (i32, [P(cc_Mesh), P(cc_Subd), i32, vp]),
         "ccd_HalfedgeNormals": (i32, [vp, vp, i32, vp]),
+        "ccm_CreateFromFaces": (P(cc_Mesh), [i32, vp, i32, vp, vp, i32, vp, vp, i32, vp, vp, i32]),
+        "ccm_LoadObj": (P(cc_Mesh), [ctypes.c_char_p]),
         "ccs_HalfedgeNormals": (None, [P(cc_Subd), vp]),
         "ccs_HalfedgeNormals_Fast": (None, [P(cc_Subd), vp]),
         "ccs_ExportToObj": (ctypes.c_bool, [P(cc_Subd), i32, ctypes.c_char_p]),
@@ -265,6 +267,64 @@ def refine_file(path: str, depth: int, *drivers: str, variant: str = "uv") -> di
             L.ccs_Release(subd)
     finally:
         L.ccm_Release(mesh)
+
+
+def cage_from_mesh(mesh_ptr) -> Cage:
+    """Copy a library-allocated cc_Mesh into a Cage (numpy arrays)."""
+    m = mesh_ptr.contents
+    V, U, H, E, F = m.vertexCount, m.uvCount, m.halfedgeCount, m.edgeCount, m.faceCount
+
+    def arr(ptr, ctype, n, dtype):
+        if n == 0:
+            return np.zeros(0, dtype)
+        return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), (n,)).copy().view(dtype)
+
+    return Cage(arr(m.vertexPoints, ctypes.c_float, 3 * V, np.float32).reshape(V, 3),
+                arr(m.uvs, ctypes.c_float, 2 * U, np.float32).reshape(U, 2),
+                arr(m.halfedges, ctypes.c_int32, 7 * H, np.int32).reshape(H, 7),
+                arr(m.creases, ctypes.c_int32, 3 * E, np.int32).view(CREASE_DTYPE),
+                arr(m.vertexToHalfedgeIDs, ctypes.c_int32, V, np.int32),
+                arr(m.edgeToHalfedgeIDs, ctypes.c_int32, E, np.int32),
+                arr(m.faceToHalfedgeIDs, ctypes.c_int32, F, np.int32))
+
+
+def build_cage(points, faces, uvs=None, face_uvs=None, creases=(), sharpen_boundaries=True) -> Cage:
+    """Polygons to a cage with the GPU cage builder (ccm_CreateFromFaces);
+    the arguments are those of cage.from_faces."""
+    L = lib()
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    sizes = np.array([len(f) for f in faces], np.int64)
+    off = np.zeros(len(faces) + 1, np.int32)
+    np.cumsum(sizes, out=off[1:])
+    fv = np.ascontiguousarray(np.concatenate([np.asarray(f, np.int32) for f in faces]))
+    uv = np.ascontiguousarray(uvs if uvs is not None else np.zeros((0, 2)), np.float32).reshape(-1, 2)
+    fuv = (np.ascontiguousarray(np.concatenate([np.asarray(f, np.int32) for f in face_uvs]))
+           if face_uvs is not None else None)
+    tags = list(creases)
+    tv = np.ascontiguousarray(np.array([[a, b] for a, b, _ in tags], np.int32).reshape(-1))
+    ts = np.ascontiguousarray(np.array([s for _, _, s in tags], np.float32))
+    p = lambda a: a.ctypes.data if a is not None and a.size else None
+    m = L.ccm_CreateFromFaces(len(pts), p(pts), len(faces), p(off), p(fv), len(uv), p(uv), p(fuv), len(tags),
+                              p(tv), p(ts), int(sharpen_boundaries))
+    if not m:
+        raise CCError(f"ccm_CreateFromFaces failed: {last_error()[1]}")
+    try:
+        return cage_from_mesh(m)
+    finally:
+        L.ccm_Release(m)
+
+
+def load_obj(path: str) -> Cage:
+    """An OBJ file through ccm_LoadObj (parsed as the reference's converter
+    parses it, built on the GPU)."""
+    L = lib()
+    m = L.ccm_LoadObj(os.fsencode(path))
+    if not m:
+        raise CCError(f"ccm_LoadObj({path}) failed: {last_error()[1]}")
+    try:
+        return cage_from_mesh(m)
+    finally:
+        L.ccm_Release(m)
 
 
 class DeviceSubd:

@@ -236,6 +236,28 @@ void ccs_RefineVertexPoints_Adjoint_f64(const cc_Subd *subd,
                                         const double *dL_dFinest,
                                         double *dL_dCage);
 
+/* ---- extension: the cage builder on the GPU ------------------------------
+ * The reference builds cages in its converter (examples/obj_to_ccm.c:88-450,
+ * LoadObj): twins by a bitonic sort and binary search, edge IDs by a prefix
+ * sum, the vertex -> halfedge choice, crease tags, boundary sharpness 16 and
+ * crease neighbours.  The same steps run here as GPU kernels (the twin sort
+ * with the converter's own compare-exchange network), so the result is the
+ * converter's byte for byte.  Inputs are host arrays: faceOffsets holds
+ * faceCount + 1 offsets (from 0) into faceVertices (0-based vertex
+ * indices); faceUvs (per corner, or NULL for uvID 0) indexes uvs; crease
+ * tags give two vertices (creaseTagVertices[2k], [2k+1], 0-based as the
+ * converter reads "t crease 2/1/0 v0 v1 s") and a sharpness each; boundary
+ * edges get sharpness 16 when sharpenBoundaries is nonzero.  The cage comes
+ * back in host memory (ccm_Release), NULL on failure (ccs_LastError). */
+cc_Mesh *ccm_CreateFromFaces(int32_t vertexCount, const cc_VertexPoint *vertexPoints, int32_t faceCount,
+                             const int32_t *faceOffsets, const int32_t *faceVertices, int32_t uvCount,
+                             const cc_VertexUv *uvs, const int32_t *faceUvs, int32_t creaseTagCount,
+                             const int32_t *creaseTagVertices, const float *creaseTagSharpness,
+                             int32_t sharpenBoundaries);
+/* An OBJ file parsed as the converter parses it (v, vt, f in its four vertex
+ * formats, "t crease 2/1/0" tags) and built by ccm_CreateFromFaces. */
+cc_Mesh *ccm_LoadObj(const char *filename);
+
 /* ---- extension: normals and OBJ export ----------------------------------
  * The step after refinement.  Normals of every halfedge of level maxDepth
  * (ccm_HalfedgeCountAtDepth(cage, maxDepth) records), as the reference's
