@@ -23,6 +23,7 @@ HALFEDGES, CREASES, UVS, POINTS, GATHER, NOCREASES, FP64 = 1, 2, 4, 8, 16, 32, 6
 ALL = HALFEDGES | CREASES | UVS | POINTS
 PTR_HALFEDGES, PTR_CREASES, PTR_POINTS, PTR_CAGE_POINTS = 0, 1, 2, 3
 OPT_FUSED, OPT_STORE, OPT_XCD_REMAP, OPT_FAN, OPT_PATCH_TILE, OPT_PATCH_HPT, OPT_SCAN, OPT_ADJ_VERTEX = 0, 1, 2, 3, 4, 5, 6, 7
+OPT_FUSED_BLOCK, OPT_FUSED_TILES, OPT_VERTEX_WAVES = 8, 9, 10
 OPT_GRAPH = 100
 (COUNT_HALFEDGES, COUNT_CREASES, COUNT_POINTS, COUNT_CAGE_POINTS, COUNT_FINEST_POINTS, COUNT_DEPTH,
  COUNT_CAGES) = range(7)
@@ -387,7 +388,8 @@ class DeviceSubd:
         self._ok(self.L.ccd_Autotune(self.handle, steps, stream), "ccd_Autotune")
 
     def level_options(self) -> list[dict]:
-        keys = {"kind": OPT_FUSED, "scan": OPT_SCAN, "tile": OPT_PATCH_TILE, "hpt": OPT_PATCH_HPT}
+        keys = {"kind": OPT_FUSED, "scan": OPT_SCAN, "tile": OPT_PATCH_TILE, "hpt": OPT_PATCH_HPT,
+                "block": OPT_FUSED_BLOCK, "tiles": OPT_FUSED_TILES, "waves": OPT_VERTEX_WAVES}
         return [{k: int(self.L.ccd_TileOption(self.handle, v, d)) for k, v in keys.items()}
                 for d in range(1, self.depth)]
 
@@ -467,7 +469,8 @@ class Batch:
         self._ok(self.L.ccb_Autotune(self.handle, steps, stream), "ccb_Autotune")
 
     def level_options(self) -> list[dict]:
-        keys = {"kind": OPT_FUSED, "scan": OPT_SCAN, "tile": OPT_PATCH_TILE, "hpt": OPT_PATCH_HPT}
+        keys = {"kind": OPT_FUSED, "scan": OPT_SCAN, "tile": OPT_PATCH_TILE, "hpt": OPT_PATCH_HPT,
+                "block": OPT_FUSED_BLOCK, "tiles": OPT_FUSED_TILES, "waves": OPT_VERTEX_WAVES}
         dmax = self.count(COUNT_DEPTH)
         return [{k: int(self.L.ccb_TileOption(self.handle, v, d)) for k, v in keys.items()} for d in range(1, dmax)]
 

@@ -172,13 +172,14 @@ void batch_copy(const BatchTable &t, const float *src, float *dst, hipStream_t s
 void batch_level_patch(const BatchTable &t, char *arena, int d, int gather, int creased, int topo, int nt, int hpt,
                        int remap, hipStream_t st);
 void batch_level_deferred(const BatchTable &t, char *arena, int d, int gather, int creased, hipStream_t st);
-// fused children + face/edge points (one workgroup per kFusedBlock halfedges), then
-// vertex points (one workgroup per 256 * scan halfedges; scan 0: the owners
-// the fused kernel marked, one workgroup per 1024 mask words)
-void batch_level_fused(const BatchTable &t, char *arena, int d, int gather, int creased, int topo, int remap,
-                       hipStream_t st);
+// fused children + face/edge points (workgroups of nt threads, one per
+// nt * tiles halfedges), then vertex points (one workgroup per 256 * scan
+// halfedges; scan 0: the owners the fused kernel marked, one workgroup per
+// 16 mask words, kernel compiled for `waves` waves per SIMD)
+void batch_level_fused(const BatchTable &t, char *arena, int d, int gather, int creased, int topo, int remap, int nt,
+                       int tiles, hipStream_t st);
 void batch_level_vertices(const BatchTable &t, char *arena, int d, int gather, int creased, int scan, int remap,
-                          hipStream_t st);
+                          int waves, hipStream_t st);
 // after a scan-0 vertex launch: each job's rare list, one thread per entry
 // (`start` counts workgroups of 256 list entries)
 void batch_level_vertices_rare(const BatchTable &t, char *arena, int d, int gather, int creased, hipStream_t st);

@@ -89,25 +89,34 @@ __global__ __launch_bounds__(NT) void k_batch_patch(const LevelJob *__restrict__
     }
 }
 
-// children with face and edge points of job k's level (one block per 256
-// halfedges)
-template <int GATHER, int CREASED, int TOPO>
-__global__ __launch_bounds__(kFusedBlock) void k_batch_fused(const LevelJob *__restrict__ jobs, const int32_t *__restrict__ start,
-                                                     int n, int d, int remap, char *arena) {
+// children with face and edge points of job k's level (workgroups of NT
+// threads; MULTI: each processing `tiles` tiles of NT halfedges in turn, a
+// kernel of its own since the loop costs registers)
+template <int GATHER, int CREASED, int TOPO, int NT, int MULTI>
+__global__ __launch_bounds__(NT) void k_batch_fused(const LevelJob *__restrict__ jobs, const int32_t *__restrict__ start,
+                                                    int n, int d, int remap, int tiles, char *arena) {
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, remap);
     const int k = find_job(start, n, lb);
     const LevelJob &J = jobs[k];
     FusedArgs a;
     DCage c;
     unpack_level(J, arena, a, c);
-    level_fused_block<GATHER, CREASED, TOPO, TOPO ? 3 : 0, 1, kStoreLds, kFusedBlock>(a, c, J.t, d,
-                                                                                   lb - (uint32_t)start[k]);
+    constexpr int UV = TOPO ? 3 : 0;
+    if constexpr (!MULTI) {
+        level_fused_block<GATHER, CREASED, TOPO, UV, 1, kStoreLds, NT>(a, c, J.t, d, lb - (uint32_t)start[k]);
+    } else {
+        const uint32_t tile0 = (lb - (uint32_t)start[k]) * (uint32_t)tiles;
+        for (int q = 0; q < tiles; ++q) {
+            if (q) __syncthreads();
+            level_fused_block<GATHER, CREASED, TOPO, UV, 1, kStoreLds, NT>(a, c, J.t, d, tile0 + (uint32_t)q);
+        }
+    }
 }
 
 // vertex points of the owners k_batch_fused marked (kOwnerWords mask words
-// per wave, four waves per workgroup)
-template <int GATHER, int CREASED>
-__global__ __launch_bounds__(256, 8) void k_batch_vertices_masked(const LevelJob *__restrict__ jobs,
+// per wave, four waves per workgroup; W waves per SIMD)
+template <int GATHER, int CREASED, int W>
+__global__ __launch_bounds__(256, W) void k_batch_vertices_masked(const LevelJob *__restrict__ jobs,
                                                                const int32_t *__restrict__ start, int n, int d,
                                                                int remap, char *arena) {
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, remap);
@@ -156,11 +165,18 @@ void vertices_scan(const BatchTable &t, char *arena, int d, int remap, hipStream
                        static_cast<const LevelJob *>(t.jobs), t.start, (int)t.n, d, remap, arena);
 }
 
+template <int G, int C, int W>
+void vertices_masked(const BatchTable &t, char *arena, int d, int remap, hipStream_t st) {
+    hipLaunchKernelGGL((k_batch_vertices_masked<G, C, W>), dim3(t.blocks), dim3(256), 0, st,
+                       static_cast<const LevelJob *>(t.jobs), t.start, (int)t.n, d, remap, arena);
+}
+
 template <int G, int C>
-void vertices_gc(const BatchTable &t, char *arena, int d, int scan, int remap, hipStream_t st) {
+void vertices_gc(const BatchTable &t, char *arena, int d, int scan, int remap, int waves, hipStream_t st) {
     if (scan == 0) {
-        hipLaunchKernelGGL((k_batch_vertices_masked<G, C>), dim3(t.blocks), dim3(256), 0, st,
-                           static_cast<const LevelJob *>(t.jobs), t.start, (int)t.n, d, remap, arena);
+        if (waves == 4) vertices_masked<G, C, 4>(t, arena, d, remap, st);
+        else if (waves == 6) vertices_masked<G, C, 6>(t, arena, d, remap, st);
+        else vertices_masked<G, C, kVertexWaves>(t, arena, d, remap, st);
         return;
     }
     if (scan == 2) vertices_scan<G, C, 2>(t, arena, d, remap, st);
@@ -168,15 +184,29 @@ void vertices_gc(const BatchTable &t, char *arena, int d, int scan, int remap, h
     else vertices_scan<G, C, 4>(t, arena, d, remap, st);
 }
 
-template <int G, int C>
-void fused_gc(const BatchTable &t, char *arena, int d, int topo, int remap, hipStream_t st) {
+template <int G, int C, int T, int NT>
+void fused_tiles(const BatchTable &t, char *arena, int d, int remap, int tiles, hipStream_t st) {
     const LevelJob *jobs = static_cast<const LevelJob *>(t.jobs);
-    if (topo)
-        hipLaunchKernelGGL((k_batch_fused<G, C, 1>), dim3(t.blocks), dim3(kFusedBlock), 0, st, jobs, t.start, (int)t.n,
-                           d, remap, arena);
+    if (tiles > 1)
+        hipLaunchKernelGGL((k_batch_fused<G, C, T, NT, 1>), dim3(t.blocks), dim3(NT), 0, st, jobs, t.start, (int)t.n, d,
+                           remap, tiles, arena);
     else
-        hipLaunchKernelGGL((k_batch_fused<G, C, 0>), dim3(t.blocks), dim3(kFusedBlock), 0, st, jobs, t.start, (int)t.n,
-                           d, remap, arena);
+        hipLaunchKernelGGL((k_batch_fused<G, C, T, NT, 0>), dim3(t.blocks), dim3(NT), 0, st, jobs, t.start, (int)t.n, d,
+                           remap, 1, arena);
+}
+
+// (the table's workgroup counts are per nt * tiles halfedges)
+template <int G, int C, int T>
+void fused_nt(const BatchTable &t, char *arena, int d, int remap, int nt, int tiles, hipStream_t st) {
+    if (nt == 256) fused_tiles<G, C, T, 256>(t, arena, d, remap, tiles, st);
+    else if (nt == 512) fused_tiles<G, C, T, 512>(t, arena, d, remap, tiles, st);
+    else fused_tiles<G, C, T, kFusedBlock>(t, arena, d, remap, tiles, st);
+}
+
+template <int G, int C>
+void fused_gc(const BatchTable &t, char *arena, int d, int topo, int remap, int nt, int tiles, hipStream_t st) {
+    if (topo) fused_nt<G, C, 1>(t, arena, d, remap, nt, tiles, st);
+    else fused_nt<G, C, 0>(t, arena, d, remap, nt, tiles, st);
 }
 
 // job k's deferred vertices: kDeferGroup mask words per wave
@@ -252,22 +282,22 @@ void batch_level_patch(const BatchTable &t, char *arena, int d, int gather, int 
     else patch_gc<0, 0>(t, arena, d, topo, nt, hpt, remap, st);
 }
 
-void batch_level_fused(const BatchTable &t, char *arena, int d, int gather, int creased, int topo, int remap,
-                       hipStream_t st) {
+void batch_level_fused(const BatchTable &t, char *arena, int d, int gather, int creased, int topo, int remap, int nt,
+                       int tiles, hipStream_t st) {
     if (t.blocks == 0 || t.n == 0) return;
-    if (gather && creased) fused_gc<1, 1>(t, arena, d, topo, remap, st);
-    else if (gather) fused_gc<1, 0>(t, arena, d, topo, remap, st);
-    else if (creased) fused_gc<0, 1>(t, arena, d, topo, remap, st);
-    else fused_gc<0, 0>(t, arena, d, topo, remap, st);
+    if (gather && creased) fused_gc<1, 1>(t, arena, d, topo, remap, nt, tiles, st);
+    else if (gather) fused_gc<1, 0>(t, arena, d, topo, remap, nt, tiles, st);
+    else if (creased) fused_gc<0, 1>(t, arena, d, topo, remap, nt, tiles, st);
+    else fused_gc<0, 0>(t, arena, d, topo, remap, nt, tiles, st);
 }
 
 void batch_level_vertices(const BatchTable &t, char *arena, int d, int gather, int creased, int scan, int remap,
-                          hipStream_t st) {
+                          int waves, hipStream_t st) {
     if (t.blocks == 0 || t.n == 0) return;
-    if (gather && creased) vertices_gc<1, 1>(t, arena, d, scan, remap, st);
-    else if (gather) vertices_gc<1, 0>(t, arena, d, scan, remap, st);
-    else if (creased) vertices_gc<0, 1>(t, arena, d, scan, remap, st);
-    else vertices_gc<0, 0>(t, arena, d, scan, remap, st);
+    if (gather && creased) vertices_gc<1, 1>(t, arena, d, scan, remap, waves, st);
+    else if (gather) vertices_gc<1, 0>(t, arena, d, scan, remap, waves, st);
+    else if (creased) vertices_gc<0, 1>(t, arena, d, scan, remap, waves, st);
+    else vertices_gc<0, 0>(t, arena, d, scan, remap, waves, st);
 }
 
 void batch_level_vertices_rare(const BatchTable &t, char *arena, int d, int gather, int creased, hipStream_t st) {

@@ -77,11 +77,14 @@ struct ccb_Batch {
     BatchTable t_fused[kMaxDepth + 1], t_vert[kMaxDepth + 1], t_adjq[kMaxDepth + 1];
     BatchTable t_adjc_e, t_adjc_f, t_adjc_v;   // cage-step adjoint gathers: per edge, face, vertex
     bool all_rings = true;                     // every cage has its ring lists
-    // per-level kernels: kind 1 fused + compacted vertex points (scan
-    // halfedges tested per thread), kind 2 patch-local (tiles of nt, hpt
-    // tiles per workgroup) + deferred vertices
+    // per-level kernels: kind 1 fused (fb threads per workgroup, ft tiles
+    // of fb halfedges per workgroup) + vertex points (scan 0: the masked
+    // kernel compiled for vw waves per SIMD; else the compacted kernel with
+    // scan halfedges tested per thread), kind 2 patch-local (tiles of nt,
+    // hpt tiles per workgroup) + deferred vertices
     struct LevelCfg {
         int kind = 1, scan = 0, nt = 256, hpt = 1;
+        int fb = kFusedBlock, ft = 1, vw = kVertexWaves;
     };
     LevelCfg cfg[kMaxDepth + 1];
     int remap = 1, use_graph = 1;
@@ -440,7 +443,7 @@ bool ccb_Batch::build_tables() {
             const int64_t tiles = blocks_of(g.n.H[d], cfg[d].nt);
             bp.push_back((tiles + cfg[d].hpt - 1) / cfg[d].hpt);
             bd.push_back(blocks_of(blocks_of(blocks_of(g.n.H[d], 64), kDeferGroup), 4));
-            bf.push_back(blocks_of(g.n.H[d], kFusedBlock));
+            bf.push_back(blocks_of(g.n.H[d], (int64_t)cfg[d].fb * cfg[d].ft));
             br.push_back(j.rare_cap > 0 ? blocks_of(j.rare_cap) : 0);
             bv.push_back(cfg[d].scan == 0 ? blocks_of(blocks_of(blocks_of(g.n.H[d], 64), kOwnerWords), 4)
                                           : blocks_of(g.n.H[d], 256 * cfg[d].scan));
@@ -566,9 +569,12 @@ void ccb_Batch::record_forward(int32_t steps, hipStream_t st) {
                          [&] { batch_level_deferred(t_def[d], arena, d, gather, creased, st); });
         } else {
             timed_launch("level_fused", d, st,
-                         [&] { batch_level_fused(t_fused[d], arena, d, gather, creased, topo, remap, st); });
+                         [&] {
+                             batch_level_fused(t_fused[d], arena, d, gather, creased, topo, remap, cfg[d].fb,
+                                               cfg[d].ft, st);
+                         });
             timed_launch("level_vertices", d, st, [&] {
-                batch_level_vertices(t_vert[d], arena, d, gather, creased, cfg[d].scan, remap, st);
+                batch_level_vertices(t_vert[d], arena, d, gather, creased, cfg[d].scan, remap, cfg[d].vw, st);
                 if (cfg[d].scan == 0) batch_level_vertices_rare(t_rare[d], arena, d, gather, creased, st);
             });
         }
@@ -623,26 +629,30 @@ bool ccb_Batch::run(int32_t key, bool adjoint, int32_t steps, hipStream_t st) {
     return true;
 }
 
-// Per level, the fastest of the fused kernels with 2, 4 or 8 halfedges
-// tested per thread of the vertex kernel and the patch-local kernel with
-// tiles of 128, 256 or 512 halfedges, timed with HIP events on this batch.
+// Times candidate kernel configurations on this batch with HIP events and
+// keeps, per level, the fastest, in three rounds:
+//   1. the fused kernel's geometry: 256, 512 or 1024 threads per workgroup
+//      times 1, 2 or 4 tiles per workgroup (masked vertex kernel);
+//   2. from each level's pick: the masked vertex kernel compiled for 4 or 6
+//      waves per SIMD, or the compacted vertex kernel testing 2, 4 or 8
+//      halfedges per thread;
+//   3. the patch-local kernel with tiles of 128, 256 or 512 halfedges.
+// Each candidate runs the given steps three times (the first discarded); a
+// level's time is that of its fused (or patch) and vertex kernels.
 int32_t ccb_Batch::autotune(int32_t steps, hipStream_t st) {
-    std::vector<LevelCfg> cands;
-    for (int scan : {0, 2, 4, 8}) cands.push_back({1, scan, 256, 1});
-    for (int t : {128, 256, 512}) cands.push_back({2, 4, t, 1});
     float best[kMaxDepth + 1];
     LevelCfg pick[kMaxDepth + 1];
     for (int d = 0; d <= kMaxDepth; ++d) { best[d] = 1e30f; pick[d] = cfg[d]; }
     const bool was = timing;
     timing = true;
-    for (const LevelCfg &cand : cands) {
-        for (auto &c : cfg) c = cand;
-        if (!build_tables()) { timing = was; return last_error(); }
-        float ms[kMaxDepth + 1] = {0};
+    // level times of the candidate in cfg, false on error
+    auto measure = [&](float (&ms)[kMaxDepth + 1]) -> bool {
+        for (auto &x : ms) x = 0.0f;
+        if (!build_tables()) return false;
         for (int rep = 0; rep < 3; ++rep) {
             begin_timing();
             record_forward(steps, st);
-            if (hip_failed(hipStreamSynchronize(st), "batch autotune")) { timing = was; return last_error(); }
+            if (hip_failed(hipStreamSynchronize(st), "batch autotune")) return false;
             if (rep == 0) continue;
             for (auto &x : timed) {
                 if (strncmp(x.name, "level_", 6) != 0) continue;
@@ -651,12 +661,38 @@ int32_t ccb_Batch::autotune(int32_t steps, hipStream_t st) {
                 ms[x.level] += v;
             }
         }
+        return true;
+    };
+    auto keep = [&](const float (&ms)[kMaxDepth + 1]) {
         for (int d = 1; d < Dmax; ++d)
-            if (ms[d] < best[d]) { best[d] = ms[d]; pick[d] = cand; }
+            if (ms[d] < best[d]) { best[d] = ms[d]; pick[d] = cfg[d]; }
+    };
+    float ms[kMaxDepth + 1];
+    bool ok = true;
+    for (int fb : {256, 512, 1024}) {
+        for (int ft : {1, 2, 4}) {
+            if (!ok) break;
+            for (auto &c : cfg) { c = LevelCfg{}; c.fb = fb; c.ft = ft; }
+            if ((ok = measure(ms))) keep(ms);
+        }
+    }
+    LevelCfg geometry[kMaxDepth + 1];
+    for (int d = 0; d <= kMaxDepth; ++d) geometry[d] = pick[d];
+    const int variants[][2] = {{0, 4}, {0, 6}, {2, kVertexWaves}, {4, kVertexWaves}, {8, kVertexWaves}};
+    for (const auto &v : variants) {
+        if (!ok) break;
+        for (int d = 0; d <= kMaxDepth; ++d) { cfg[d] = geometry[d]; cfg[d].scan = v[0]; cfg[d].vw = v[1]; }
+        if ((ok = measure(ms))) keep(ms);
+    }
+    for (int nt : {128, 256, 512}) {
+        if (!ok) break;
+        for (auto &c : cfg) { c = LevelCfg{}; c.kind = 2; c.scan = 4; c.nt = nt; }
+        if ((ok = measure(ms))) keep(ms);
     }
     begin_timing();
     timing = was;
     for (int d = 0; d <= kMaxDepth; ++d) cfg[d] = pick[d];
+    if (!ok) return last_error() ? last_error() : 1;
     return build_tables() ? 0 : last_error();
 }
 
@@ -867,6 +903,19 @@ int32_t ccb_SetOption(ccb_Batch *b, int32_t key, int32_t value) {
         if (value != 0 && value != 2 && value != 4 && value != 8) return 14;
         for (auto &c : b->cfg) c.scan = value;
         return b->build_tables() ? 0 : last_error();
+    case CCD_OPT_FUSED_BLOCK:
+        if (value != 256 && value != 512 && value != 1024) return 14;
+        for (auto &c : b->cfg) c.fb = value;
+        return b->build_tables() ? 0 : last_error();
+    case CCD_OPT_FUSED_TILES:
+        if (value != 1 && value != 2 && value != 4) return 14;
+        for (auto &c : b->cfg) c.ft = value;
+        return b->build_tables() ? 0 : last_error();
+    case CCD_OPT_VERTEX_WAVES:
+        if (value != 4 && value != 6 && value != 8) return 14;
+        for (auto &c : b->cfg) c.vw = value;
+        b->drop_graphs();
+        return 0;
     default: return 14;
     }
 }
@@ -892,6 +941,9 @@ int32_t ccb_TileOption(const ccb_Batch *b, int32_t key, int32_t level) {
     case CCD_OPT_PATCH_TILE: return b->cfg[level].nt;
     case CCD_OPT_PATCH_HPT: return b->cfg[level].hpt;
     case CCD_OPT_SCAN: return b->cfg[level].scan;
+    case CCD_OPT_FUSED_BLOCK: return b->cfg[level].fb;
+    case CCD_OPT_FUSED_TILES: return b->cfg[level].ft;
+    case CCD_OPT_VERTEX_WAVES: return b->cfg[level].vw;
     default: return -1;
     }
 }

@@ -7,11 +7,16 @@ namespace cch {
 
 enum { kStoreDirect = 0, kStoreLds = 1 };
 
-// Threads (parent halfedges) per workgroup of the fused level kernel: a
-// workgroup covers the descendants of one level d-5 face (a 16 x 16 patch of
-// quads on a regular grid), and the twin quads of all but the patch's border
-// halfedges come from its LDS rather than from memory.
+// Threads (parent halfedges) per workgroup of the fused level kernel by
+// default: a workgroup covers the descendants of one level d-5 face (a 16 x
+// 16 patch of quads on a regular grid), and the twin quads of all but the
+// patch's border halfedges come from its LDS rather than from memory.  The
+// autotuner may pick 256 or 512 per level, and 2 or 4 tiles (of one
+// workgroup's size) processed in turn by each workgroup.
 constexpr int kFusedBlock = 1024;
+// waves per SIMD the masked vertex kernel is compiled for by default (it
+// needs 56 registers: 8 fit without spilling); the autotuner may pick 4 or 6
+constexpr int kVertexWaves = 8;
 
 // Level-d inputs and level-(d+1) outputs of one fused launch.
 struct FusedArgs {
@@ -52,8 +57,11 @@ struct FusedArgs {
 // topo: write child twin/edge/vertex IDs; uvMode: 0 leave uvIDs, 1 refine
 // them, 2 write 0; points: face and edge points of level d+1.
 // (c, t, d: the cage and counts for the owner bits when a.owner_mask is set)
+// nt: threads (halfedges) per workgroup, 256, 512 or 1024 with LDS-staged
+// stores (direct stores always use 1024); tiles: tiles of nt halfedges each
+// workgroup processes in turn, 1, 2 or 4
 void launch_level_fused(const FusedArgs &a, const DCage &c, const LevelTab &t, int d, int gather, int creased, int topo,
-                        int uvMode, int points, int store, hipStream_t st);
+                        int uvMode, int points, int store, int nt, int tiles, hipStream_t st);
 // base pointers of the halfedges of every level (index 1..D)
 struct LevelPtrs {
     const He *he[kMaxDepth + 1];
@@ -87,8 +95,9 @@ void launch_level_deferred(const FusedArgs &a, const DCage &c, const LevelTab &t
 // vertex points of level d+1 (after launch_level_fused with points); fan
 // selects the structurally enumerated rings over the ring walk
 // (scan: halfedges each thread of the compacted kernel tests, 2, 4 or 8;
-// 0: the owners marked in a.owner_mask by the fused kernel)
+// 0: the owners marked in a.owner_mask by the fused kernel, with the masked
+// kernel compiled for `waves` waves per SIMD, 4, 6 or 8)
 void launch_level_vertices(const FusedArgs &a, const DCage &c, const LevelTab &t, const LevelPtrs &lp, int d,
-                           int gather, int creased, int fan, int scan, hipStream_t st);
+                           int gather, int creased, int fan, int scan, int waves, hipStream_t st);
 
 }  // namespace cch

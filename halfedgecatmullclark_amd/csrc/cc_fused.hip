@@ -23,17 +23,30 @@
 
 namespace cch {
 
-
-template <int GATHER, int CREASED, int TOPO, int UVMODE, int POINTS, int STORE>
-__global__ __launch_bounds__(kFusedBlock) void k_level_fused(FusedArgs a, DCage c, LevelTab t, int d) {
-    level_fused_block<GATHER, CREASED, TOPO, UVMODE, POINTS, STORE, kFusedBlock>(
-        a, c, t, d, xcd_block(blockIdx.x, gridDim.x, a.xcd_remap));
+// NT threads per workgroup, one tile of NT halfedges each; MULTI: each
+// workgroup processes `tiles` consecutive tiles in turn (the barrier between
+// tiles keeps one tile's LDS reads ahead of the next tile's writes).  The
+// loop is a kernel of its own: keeping the arguments live across iterations
+// takes about 45 more scalar and 17 more vector registers, which at 1024
+// threads leaves room for one workgroup per CU instead of two.
+template <int GATHER, int CREASED, int TOPO, int UVMODE, int POINTS, int STORE, int NT, int MULTI>
+__global__ __launch_bounds__(NT) void k_level_fused(FusedArgs a, DCage c, LevelTab t, int d, int tiles) {
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, a.xcd_remap);
+    if constexpr (!MULTI) {
+        level_fused_block<GATHER, CREASED, TOPO, UVMODE, POINTS, STORE, NT>(a, c, t, d, lb);
+    } else {
+        for (int k = 0; k < tiles; ++k) {
+            if (k) __syncthreads();
+            level_fused_block<GATHER, CREASED, TOPO, UVMODE, POINTS, STORE, NT>(a, c, t, d,
+                                                                              lb * (uint32_t)tiles + (uint32_t)k);
+        }
+    }
 }
 
 // ring vertices only; the others to the rare list, then k_level_vertices_rare
-// (8 waves per SIMD fit without spilling: 56 registers)
-template <int GATHER, int CREASED>
-__global__ __launch_bounds__(256, 8) void k_level_vertices_ring(FusedArgs a, DCage c, LevelTab t, LevelPtrs lp, int d) {
+// (W waves per SIMD: 8 fit without spilling, 56 registers)
+template <int GATHER, int CREASED, int W>
+__global__ __launch_bounds__(256, W) void k_level_vertices_ring(FusedArgs a, DCage c, LevelTab t, LevelPtrs lp, int d) {
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, a.xcd_remap);
     level_vertices_masked<GATHER, CREASED>(a, c, t, lp, d, (int32_t)(lb * 4 + (threadIdx.x >> 6)));
 }
@@ -230,14 +243,42 @@ __global__ __launch_bounds__(256) void k_level_vertices_direct(FusedArgs a, DCag
 }
 
 // ---- launcher ----------------------------------------------------------------
+template <int G, int C, int T, int U, int P, int S, int NT>
+static void launch_fused_tiles(const FusedArgs &a, const DCage &c, const LevelTab &t, int d, int tiles,
+                               hipStream_t st) {
+    constexpr bool kMulti = S == kStoreLds && P == 1;
+    if (!kMulti) tiles = 1;
+    const int64_t per = (int64_t)NT * tiles;
+    const dim3 grid((unsigned)((a.H + per - 1) / per));
+    if constexpr (kMulti) {
+        if (tiles > 1) {
+            hipLaunchKernelGGL((k_level_fused<G, C, T, U, P, S, NT, 1>), grid, dim3(NT), 0, st, a, c, t, d, tiles);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_level_fused<G, C, T, U, P, S, NT, 0>), grid, dim3(NT), 0, st, a, c, t, d, 1);
+}
+
+// 256 and 512 threads, and several tiles per workgroup, with LDS-staged
+// stores and points only; 1024 threads and one tile otherwise
+template <int G, int C, int T, int U, int P, int S>
+static void launch_fused_nt(const FusedArgs &a, const DCage &c, const LevelTab &t, int d, int nt, int tiles,
+                            hipStream_t st) {
+    if constexpr (S == kStoreLds && P == 1) {
+        if (nt == 256) return launch_fused_tiles<G, C, T, U, P, S, 256>(a, c, t, d, tiles, st);
+        if (nt == 512) return launch_fused_tiles<G, C, T, U, P, S, 512>(a, c, t, d, tiles, st);
+    }
+    launch_fused_tiles<G, C, T, U, P, S, kFusedBlock>(a, c, t, d, tiles, st);
+}
+
 #define CC_FUSED_CASE(G, C, T, U, P, S)                                                      \
     if (gather == G && creased == C && topo == T && uvMode == U && points == P && store == S) { \
-        hipLaunchKernelGGL((k_level_fused<G, C, T, U, P, S>), grid, block, 0, st, a, c, t, d);  \
+        launch_fused_nt<G, C, T, U, P, S>(a, c, t, d, nt, tiles, st);                         \
         return;                                                                               \
     }
 
 static void launch_fused_impl(const FusedArgs &a, const DCage &c, const LevelTab &t, int d, int gather, int creased,
-                              int topo, int uvMode, int points, int store, dim3 grid, dim3 block, hipStream_t st) {
+                              int topo, int uvMode, int points, int store, int nt, int tiles, hipStream_t st) {
 #define CC_FUSED_STORES(G, C, T, U, P) CC_FUSED_CASE(G, C, T, U, P, kStoreDirect) CC_FUSED_CASE(G, C, T, U, P, kStoreLds)
 #define CC_FUSED_POINTS(G, C, T, U) CC_FUSED_STORES(G, C, T, U, 0) CC_FUSED_STORES(G, C, T, U, 1)
 #define CC_FUSED_UV(G, C, T) CC_FUSED_POINTS(G, C, T, 0) CC_FUSED_POINTS(G, C, T, 1) CC_FUSED_POINTS(G, C, T, 2)
@@ -246,11 +287,11 @@ static void launch_fused_impl(const FusedArgs &a, const DCage &c, const LevelTab
 }
 
 void launch_level_fused(const FusedArgs &a, const DCage &c, const LevelTab &t, int d, int gather, int creased, int topo,
-                        int uvMode, int points, int store, hipStream_t st) {
-    const dim3 block(kFusedBlock);
-    const dim3 grid((unsigned)((a.H + kFusedBlock - 1) / kFusedBlock));
-    launch_fused_impl(a, c, t, d, gather ? 1 : 0, creased ? 1 : 0, topo ? 1 : 0, uvMode, points ? 1 : 0, store, grid,
-                      block, st);
+                        int uvMode, int points, int store, int nt, int tiles, hipStream_t st) {
+    if (nt != 256 && nt != 512) nt = kFusedBlock;
+    if (tiles < 1) tiles = 1;
+    launch_fused_impl(a, c, t, d, gather ? 1 : 0, creased ? 1 : 0, topo ? 1 : 0, uvMode, points ? 1 : 0, store, nt,
+                      tiles, st);
 }
 
 template <int SCAN>
@@ -263,8 +304,16 @@ static void launch_compact(const FusedArgs &a, const DCage &c, const LevelTab &t
     else hipLaunchKernelGGL((k_level_vertices_compact<0, 0, SCAN>), grid, block, 0, st, a, c, t, lp, d);
 }
 
+template <int G, int C>
+static void launch_ring(const FusedArgs &a, const DCage &c, const LevelTab &t, const LevelPtrs &lp, int d, int waves,
+                        dim3 grid, hipStream_t st) {
+    if (waves == 4) hipLaunchKernelGGL((k_level_vertices_ring<G, C, 4>), grid, dim3(256), 0, st, a, c, t, lp, d);
+    else if (waves == 6) hipLaunchKernelGGL((k_level_vertices_ring<G, C, 6>), grid, dim3(256), 0, st, a, c, t, lp, d);
+    else hipLaunchKernelGGL((k_level_vertices_ring<G, C, kVertexWaves>), grid, dim3(256), 0, st, a, c, t, lp, d);
+}
+
 void launch_level_vertices(const FusedArgs &a, const DCage &c, const LevelTab &t, const LevelPtrs &lp, int d,
-                           int gather, int creased, int fan, int scan, hipStream_t st) {
+                           int gather, int creased, int fan, int scan, int waves, hipStream_t st) {
     const dim3 block(256);
     const dim3 grid((unsigned)((a.H + 255) / 256));
     if (fan == 3) {
@@ -280,16 +329,16 @@ void launch_level_vertices(const FusedArgs &a, const DCage &c, const LevelTab &t
         const int64_t waves = (words + kOwnerWords - 1) / kOwnerWords;
         const dim3 g0((unsigned)((waves + 3) / 4)), gr((unsigned)((a.rare_cap + 255) / 256));
         if (gather && creased) {
-            hipLaunchKernelGGL((k_level_vertices_ring<1, 1>), g0, block, 0, st, a, c, t, lp, d);
+            launch_ring<1, 1>(a, c, t, lp, d, waves, g0, st);
             hipLaunchKernelGGL((k_level_vertices_rare<1, 1>), gr, block, 0, st, a, c, t, lp, d);
         } else if (gather) {
-            hipLaunchKernelGGL((k_level_vertices_ring<1, 0>), g0, block, 0, st, a, c, t, lp, d);
+            launch_ring<1, 0>(a, c, t, lp, d, waves, g0, st);
             hipLaunchKernelGGL((k_level_vertices_rare<1, 0>), gr, block, 0, st, a, c, t, lp, d);
         } else if (creased) {
-            hipLaunchKernelGGL((k_level_vertices_ring<0, 1>), g0, block, 0, st, a, c, t, lp, d);
+            launch_ring<0, 1>(a, c, t, lp, d, waves, g0, st);
             hipLaunchKernelGGL((k_level_vertices_rare<0, 1>), gr, block, 0, st, a, c, t, lp, d);
         } else {
-            hipLaunchKernelGGL((k_level_vertices_ring<0, 0>), g0, block, 0, st, a, c, t, lp, d);
+            launch_ring<0, 0>(a, c, t, lp, d, waves, g0, st);
             hipLaunchKernelGGL((k_level_vertices_rare<0, 0>), gr, block, 0, st, a, c, t, lp, d);
         }
         return;

@@ -74,7 +74,7 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
     __shared__ LevelTab stab;   // for the owner test (read after the block's barrier)
     if (POINTS && a.owner_mask) lds_fill(stab, tb);
     if (POINTS && a.rare && lb == 0 && threadIdx.x == 0) a.rare[0] = 0;
-    const int32_t base = (int32_t)(lb * blockDim.x);
+    const int32_t base = (int32_t)(lb * NT);   // NT threads per block, one tile of NT halfedges
     const int32_t h = base + (int32_t)threadIdx.x;
     const bool live = h < a.H;   // H is a multiple of 4: quads are whole
     const int j = h & 3;
@@ -85,7 +85,7 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
     const int32_t twin = x.x;
     const bool owner = live && h > twin;   // this halfedge writes its edge point
     // twin quads inside this block come from LDS, the others from memory
-    const bool twin_here = owner && twin >= base && twin < base + (int32_t)blockDim.x;
+    const bool twin_here = owner && twin >= base && twin < base + NT;
     const bool twin_far = owner && twin >= 0 && !twin_here;
 
     // the loads of the point work come first: this halfedge's vertex point,

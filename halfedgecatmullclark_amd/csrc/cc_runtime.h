@@ -60,12 +60,15 @@ struct ccd_Subd {
     int opt_fused = 1, opt_store = cch::kStoreLds, opt_remap = 1, opt_fan = 2;
     int32_t opt_adj_vertex = 1 << 20;   // CCD_OPT_ADJ_VERTEX
     // per-level kernels of the fused paths (options or ccd_Autotune): kind 1
-    // fused children + face/edge points, then vertex points (scan 0: the
-    // representatives the fused kernel marked; 2, 4, 8: halfedges each thread
-    // of the compacted kernel tests); kind 2 patch-local (nt halfedges per
-    // tile, hpt tiles per workgroup) plus deferred vertices
+    // fused children + face/edge points (fb threads per workgroup, ft tiles
+    // of fb halfedges per workgroup), then vertex points (scan 0: the
+    // representatives the fused kernel marked, kernel compiled for vw waves
+    // per SIMD; 2, 4, 8: halfedges each thread of the compacted kernel
+    // tests); kind 2 patch-local (nt halfedges per tile, hpt tiles per
+    // workgroup) plus deferred vertices
     struct LevelCfg {
         int kind = 1, scan = 0, nt = 256, hpt = 1;
+        int fb = cch::kFusedBlock, ft = 1, vw = cch::kVertexWaves;
     };
     LevelCfg cfg[cch::kMaxDepth + 1];
     std::vector<Timed> timed;

@@ -333,7 +333,10 @@ void ccd_Subd::level_fused(int d, int32_t steps, int topo, int uvMode, P3 *point
     }
     if (topo || uvMode == 1 || pts_) {
         launch_timed(pts_ ? (topo ? "level_fused" : "level_points") : "level_fused", n.H[d], st,
-                     [&] { launch_level_fused(a, c, tab, d, gather, creased, topo, uvMode, pts_, opt_store, st); });
+                     [&] {
+                         launch_level_fused(a, c, tab, d, gather, creased, topo, uvMode, pts_, opt_store, cfg[d].fb,
+                                            cfg[d].ft, st);
+                     });
         if (a.erep_write) {
             erep_valid[d + 1] = true;
             if (d == 1) erep_valid[1] = true;
@@ -341,7 +344,7 @@ void ccd_Subd::level_fused(int d, int32_t steps, int topo, int uvMode, P3 *point
     }
     if (pts_) {
         launch_timed("level_vertices", n.H[d], st, [&] {
-            launch_level_vertices(a, c, tab, level_ptrs(), d, gather, creased, opt_fan, cfg[d].scan, st);
+            launch_level_vertices(a, c, tab, level_ptrs(), d, gather, creased, opt_fan, cfg[d].scan, cfg[d].vw, st);
         });
     }
 }
@@ -873,31 +876,46 @@ int32_t ccd_SetOption(ccd_Subd *s, int32_t key, int32_t value) {
         if (value < 0) return 14;
         s->opt_adj_vertex = value;
         return 0;
+    case CCD_OPT_FUSED_BLOCK:
+        if (value != 256 && value != 512 && value != 1024) return 14;
+        for (auto &c : s->cfg) c.fb = value;
+        return 0;
+    case CCD_OPT_FUSED_TILES:
+        if (value != 1 && value != 2 && value != 4) return 14;
+        for (auto &c : s->cfg) c.ft = value;
+        return 0;
+    case CCD_OPT_VERTEX_WAVES:
+        if (value != 4 && value != 6 && value != 8) return 14;
+        for (auto &c : s->cfg) c.vw = value;
+        return 0;
     default: return 14;
     }
 }
 
-// Times every candidate kernel configuration of the fused paths on this
-// subd with HIP events and keeps, per level, the fastest: the fused kernels
-// with 2, 4 or 8 halfedges tested per thread of the vertex kernel, and the
-// patch-local kernel with tiles of 128, 256 or 512 halfedges.
+// Times candidate kernel configurations on this subd with HIP events and
+// keeps, per level, the fastest, in three rounds:
+//   1. the fused kernel's geometry: 256, 512 or 1024 threads per workgroup
+//      times 1, 2 or 4 tiles per workgroup (masked vertex kernel);
+//   2. from each level's pick: the masked vertex kernel compiled for 4 or 6
+//      waves per SIMD, or the compacted vertex kernel testing 2, 4 or 8
+//      halfedges per thread;
+//   3. the patch-local kernel with tiles of 128, 256 or 512 halfedges.
+// Each candidate runs the given steps three times (the first discarded); a
+// level's time is that of its fused (or patch) and vertex kernels.
 int32_t ccd_Subd::autotune(int32_t steps, hipStream_t st) {
     if (!(steps & CCD_POINTS) || opt_fused == 0 || D < 2) return 0;
-    std::vector<LevelCfg> cands;
-    for (int scan : {0, 2, 4, 8}) cands.push_back({1, scan, 256, 1});
-    for (int nt : {128, 256, 512}) cands.push_back({2, 4, nt, 1});
     float best[kMaxDepth + 1];
     LevelCfg pick[kMaxDepth + 1];
     for (int d = 0; d <= kMaxDepth; ++d) { best[d] = 1e30f; pick[d] = cfg[d]; }
     const bool was_timing = timing;
     timing = true;
-    for (const LevelCfg &cand : cands) {
-        for (auto &c : cfg) c = cand;
-        float ms[kMaxDepth + 1] = {0};
+    // level times of one candidate (cfg set per level), false on error
+    auto measure = [&](float (&ms)[kMaxDepth + 1]) -> bool {
+        for (auto &x : ms) x = 0.0f;
         for (int rep = 0; rep < 3; ++rep) {
             begin_timing();
-            if (!refine(steps, D, st)) { timing = was_timing; return t_err ? t_err : 1; }
-            if (hip_failed(hipStreamSynchronize(st), "autotune")) { timing = was_timing; return t_err; }
+            if (!refine(steps, D, st)) return false;
+            if (hip_failed(hipStreamSynchronize(st), "autotune")) return false;
             if (rep == 0) continue;  // warm-up
             // level d's kernels are the timed launches between its first
             // kernel and the next level's; they come in level order
@@ -913,13 +931,39 @@ int32_t ccd_Subd::autotune(int32_t steps, hipStream_t st) {
                 if (d >= 1 && d <= kMaxDepth) ms[d] += x;
             }
         }
+        return true;
+    };
+    auto keep = [&](const float (&ms)[kMaxDepth + 1]) {
         for (int d = 1; d < D; ++d)
-            if (ms[d] < best[d]) { best[d] = ms[d]; pick[d] = cand; }
+            if (ms[d] < best[d]) { best[d] = ms[d]; pick[d] = cfg[d]; }
+    };
+    float ms[kMaxDepth + 1];
+    bool ok = true;
+    for (int fb : {256, 512, 1024}) {
+        for (int ft : {1, 2, 4}) {
+            for (auto &c : cfg) { c = LevelCfg{}; c.fb = fb; c.ft = ft; }
+            if (!(ok = measure(ms))) break;
+            keep(ms);
+        }
+        if (!ok) break;
+    }
+    LevelCfg geometry[kMaxDepth + 1];
+    for (int d = 0; d <= kMaxDepth; ++d) geometry[d] = pick[d];
+    const int variants[][2] = {{0, 4}, {0, 6}, {2, kVertexWaves}, {4, kVertexWaves}, {8, kVertexWaves}};
+    for (const auto &v : variants) {
+        if (!ok) break;
+        for (int d = 0; d <= kMaxDepth; ++d) { cfg[d] = geometry[d]; cfg[d].scan = v[0]; cfg[d].vw = v[1]; }
+        if ((ok = measure(ms))) keep(ms);
+    }
+    for (int nt : {128, 256, 512}) {
+        if (!ok) break;
+        for (auto &c : cfg) { c = LevelCfg{}; c.kind = 2; c.scan = 4; c.nt = nt; }
+        if ((ok = measure(ms))) keep(ms);
     }
     begin_timing();
     timing = was_timing;
     for (int d = 0; d <= kMaxDepth; ++d) cfg[d] = pick[d];
-    return 0;
+    return ok ? 0 : (last_error() ? last_error() : 1);
 }
 
 int32_t ccd_TileOption(const ccd_Subd *s, int32_t key, int32_t level) {
@@ -929,6 +973,9 @@ int32_t ccd_TileOption(const ccd_Subd *s, int32_t key, int32_t level) {
     case CCD_OPT_PATCH_TILE: return s->cfg[level].nt;
     case CCD_OPT_PATCH_HPT: return s->cfg[level].hpt;
     case CCD_OPT_SCAN: return s->cfg[level].scan;
+    case CCD_OPT_FUSED_BLOCK: return s->cfg[level].fb;
+    case CCD_OPT_FUSED_TILES: return s->cfg[level].ft;
+    case CCD_OPT_VERTEX_WAVES: return s->cfg[level].vw;
     default: return -1;
     }
 }

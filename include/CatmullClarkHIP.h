@@ -346,21 +346,31 @@ int32_t ccd_HalfedgeNormals(ccd_Subd *subd, float *devNormals, int32_t smooth, v
  *   CCD_OPT_ADJ_VERTEX the adjoint's two vertex passes run one thread per
  *                      vertex on levels with at most this many vertices
  *                      (default 1048576), else over the vertex owners in
- *                      halfedge order; both give the same bits */
+ *                      halfedge order; both give the same bits
+ *   CCD_OPT_FUSED_BLOCK threads (parent halfedges) per workgroup of the fused
+ *                      kernel: 256, 512 or 1024 (default), every level
+ *   CCD_OPT_FUSED_TILES tiles of that many halfedges each fused workgroup
+ *                      processes in turn: 1 (default), 2 or 4
+ *   CCD_OPT_VERTEX_WAVES waves per SIMD the masked vertex kernel is compiled
+ *                      for: 4, 6 or 8 (default) */
 enum {
     CCD_OPT_FUSED = 0, CCD_OPT_STORE = 1, CCD_OPT_XCD_REMAP = 2, CCD_OPT_FAN = 3,
-    CCD_OPT_PATCH_TILE = 4, CCD_OPT_PATCH_HPT = 5, CCD_OPT_SCAN = 6, CCD_OPT_ADJ_VERTEX = 7
+    CCD_OPT_PATCH_TILE = 4, CCD_OPT_PATCH_HPT = 5, CCD_OPT_SCAN = 6, CCD_OPT_ADJ_VERTEX = 7,
+    CCD_OPT_FUSED_BLOCK = 8, CCD_OPT_FUSED_TILES = 9, CCD_OPT_VERTEX_WAVES = 10
 };
 int32_t ccd_SetOption(ccd_Subd *subd, int32_t key, int32_t value);
 /* Picks, per level, the kernel configuration of the fused paths by timing
- * every candidate on this subd: the fused kernels with each CCD_OPT_SCAN
- * vertex kernel, and the patch-local kernel with tiles of 128, 256 or 512
- * halfedges.  Runs the given steps a few times per
- * candidate on the stream, synchronizing it; the choice is kept until the
- * next ccd_SetOption of these keys.  Returns 0 or an error code. */
+ * candidates on this subd: first the fused kernel's workgroup size (256,
+ * 512, 1024) and tiles per workgroup (1, 2, 4); then, from each level's
+ * pick, the masked vertex kernel at 4 or 6 waves per SIMD and the compacted
+ * vertex kernel at 2, 4 or 8 halfedges per thread; and the patch-local
+ * kernel with tiles of 128, 256 or 512 halfedges.  Every candidate gives
+ * the same bits.  Runs the given steps a few times per candidate on the
+ * stream, synchronizing it; the choice is kept until the next ccd_SetOption
+ * of these keys.  Returns 0 or an error code. */
 int32_t ccd_Autotune(ccd_Subd *subd, int32_t steps, void *stream);
-/* the value of CCD_OPT_FUSED (1 or 2), _PATCH_TILE, _PATCH_HPT or _SCAN used
- * at one level, or -1 */
+/* the value of CCD_OPT_FUSED (1 or 2), _PATCH_TILE, _PATCH_HPT, _SCAN,
+ * _FUSED_BLOCK, _FUSED_TILES or _VERTEX_WAVES used at one level, or -1 */
 int32_t ccd_TileOption(const ccd_Subd *subd, int32_t key, int32_t level);
 /* per-kernel timing with HIP events on the launch stream: enable, run, then
  * read the kernel names and mean milliseconds of the last ccd_Refine. */
@@ -406,7 +416,9 @@ int32_t ccb_RefinePointsAdjoint(ccb_Batch *batch, const float *devGradFinest, fl
                                 int32_t steps, void *stream);
 int32_t ccb_Download(const ccb_Batch *batch, int32_t cage, cc_Subd *dst, int32_t steps);
 int32_t ccb_Synchronize(ccb_Batch *batch);
-/* CCB_OPT_GRAPH, CCD_OPT_XCD_REMAP, CCD_OPT_PATCH_TILE, CCD_OPT_PATCH_HPT */
+/* CCB_OPT_GRAPH, CCD_OPT_XCD_REMAP, CCD_OPT_FUSED, CCD_OPT_PATCH_TILE,
+ * CCD_OPT_PATCH_HPT, CCD_OPT_SCAN, CCD_OPT_FUSED_BLOCK, CCD_OPT_FUSED_TILES,
+ * CCD_OPT_VERTEX_WAVES */
 int32_t ccb_SetOption(ccb_Batch *batch, int32_t key, int32_t value);
 /* per-level choice of tile size and tiles per workgroup by timing every
  * candidate on this batch (see ccd_Autotune) */
