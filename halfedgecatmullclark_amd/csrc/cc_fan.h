@@ -131,6 +131,15 @@ __device__ __forceinline__ int4 child_record(int k, int32_t h, int4 x, int4 p, i
 // int4s of LDS per parent halfedge that stage its four child records
 constexpr int kStageInt4 = kHeBytes / 4;
 
+// Slot of child k of thread t among the thread's four 16-byte LDS slots.
+// Written in place (slot k), the four store passes put lanes t, t + 4, ...
+// on the same banks (a 64-byte stride); rotating by s(t) = t/2 + t/8 makes
+// every group of 8 lanes (128 bytes a cycle) and of 16 lanes (256 bytes)
+// cover distinct banks in each pass.
+__device__ __forceinline__ int stage_slot(uint32_t t, int k) {
+    return (int)(4 * t) + ((k + (int)(t >> 1) + (int)(t >> 3)) & 3);
+}
+
 // Child records staged through LDS: this thread's four children go to its
 // slots of `stage` (kStageInt4 int4s per thread), then, after the block's
 // barrier, the block's contiguous run of 4 * NT records (NT threads) is
@@ -142,7 +151,7 @@ __device__ __forceinline__ void staged_children_store(int4 *stage, const int4 (&
                                                       int32_t H) {
     if (kHasUv) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) stage[4 * threadIdx.x + k] = c[k];
+        for (int k = 0; k < 4; ++k) stage[stage_slot(threadIdx.x, k)] = c[k];
     } else {
         int32_t *w = reinterpret_cast<int32_t *>(stage) + 12 * threadIdx.x;
 #pragma unroll
@@ -158,8 +167,10 @@ __device__ __forceinline__ void staged_children_store(int4 *stage, const int4 (&
     const int64_t limit = kStageInt4 * (int64_t)H;
 #pragma unroll
     for (int k = 0; k < kStageInt4; ++k) {
-        const int64_t o = obase + threadIdx.x + (int64_t)k * NT;
-        if (o < limit) store_stream(out4 + o, stage[threadIdx.x + k * NT]);
+        const uint32_t q = threadIdx.x + (uint32_t)k * NT;   // record q & 3 of thread q / 4
+        const int64_t o = obase + q;
+        const int4 v = kHasUv ? stage[stage_slot(q >> 2, (int)(q & 3))] : stage[q];
+        if (o < limit) store_stream(out4 + o, v);
     }
 }
 

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <vector>
 
 #include "cc_internal.h"
@@ -72,6 +73,12 @@ struct ccd_Subd {
     };
     LevelCfg cfg[cch::kMaxDepth + 1];
     std::vector<Timed> timed;
+    // ccd_Refine's launch sequences captured as graphs (CCB_OPT_GRAPH), keyed
+    // by the steps and by the levels whose edge-representative bits were
+    // valid at capture; dropped when an option, a level's kernels or the
+    // cage change
+    std::map<int64_t, hipGraphExec_t> graphs;
+    int opt_graph = 1;
 
     bool alloc(const cc_Mesh *cage, int32_t depth);
     int32_t autotune(int32_t steps, hipStream_t st);
@@ -83,6 +90,8 @@ struct ccd_Subd {
     cch::LevelPtrs level_ptrs() const;
     void cage_points(int32_t steps, const cch::P3 *cagePts, cch::P3 *out, hipStream_t st);
     bool refine(int32_t steps, int dmax, hipStream_t st);
+    bool refine_graph(int32_t steps, hipStream_t st);
+    void drop_graphs();
     bool refine_topology(int32_t steps, int dmax, hipStream_t st);
     bool refine_points(int32_t steps, int dmax, const cch::P3 *cagePts, cch::P3 *out, hipStream_t st);
     void level_patch(int d, int32_t steps, int topo, int uvMode, cch::P3 *points, hipStream_t st);

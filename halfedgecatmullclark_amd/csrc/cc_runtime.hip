@@ -107,7 +107,13 @@ void ccd_Subd::free_all() {
         (void)hipEventDestroy(t.stop);
     }
     timed.clear();
+    drop_graphs();
     if (stream) (void)hipStreamDestroy(stream);
+}
+
+void ccd_Subd::drop_graphs() {
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
 }
 
 bool ccd_Subd::upload_cage(const cc_Mesh *cage, hipStream_t st) {
@@ -117,6 +123,7 @@ bool ccd_Subd::upload_cage(const cc_Mesh *cage, hipStream_t st) {
         return false;
     }
     for (bool &v : erep_valid) v = false;   // bits of the previous topology
+    drop_graphs();
     CC_TRY(hipMemcpyAsync(c_he, cage->halfedges, sizeof(CageHe) * (size_t)H0, hipMemcpyHostToDevice, st), "upload cage");
     CC_TRY(hipMemcpyAsync(c_v2h, cage->vertexToHalfedgeIDs, sizeof(int32_t) * (size_t)V0, hipMemcpyHostToDevice, st), "upload cage");
     CC_TRY(hipMemcpyAsync(c_e2h, cage->edgeToHalfedgeIDs, sizeof(int32_t) * (size_t)E0, hipMemcpyHostToDevice, st), "upload cage");
@@ -419,6 +426,35 @@ bool ccd_Subd::refine(int32_t steps, int dmax, hipStream_t st) {
         else level_fused(d, steps, topo, uvMode, pts, st);
     }
     CC_TRY(hipGetLastError(), "kernel launch");
+    return true;
+}
+
+// refine(steps, D) as a graph captured on the subd's own stream on first
+// use and launched on st.  The host-side choices a capture bakes in (the
+// level kernels, options, which levels' edge-representative bits to read or
+// write) are part of the key or drop the graphs when they change.
+bool ccd_Subd::refine_graph(int32_t steps, hipStream_t st) {
+    int64_t key = (uint32_t)steps;
+    for (int l = 1; l <= kMaxDepth; ++l)
+        if (erep_valid[l]) key |= (int64_t)1 << (32 + l);
+    auto it = graphs.find(key);
+    if (it == graphs.end()) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t exec = nullptr;
+        CC_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        const bool ok = refine(steps, D, stream);
+        const hipError_t e = hipStreamEndCapture(stream, &g);
+        if (!ok) {
+            if (g) (void)hipGraphDestroy(g);
+            return false;
+        }
+        CC_TRY(e, "hipStreamEndCapture");
+        const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        CC_TRY(ei, "hipGraphInstantiate");
+        it = graphs.emplace(key, exec).first;
+    }
+    CC_TRY(hipGraphLaunch(it->second, st), "hipGraphLaunch");
     return true;
 }
 
@@ -794,8 +830,8 @@ int32_t ccd_Refine(ccd_Subd *s, int32_t steps, void *stream) {
     if (!s) { set_error(11, "ccd_Refine: null subd"); return t_err; }
     DeviceGuard g(s->device);
     hipStream_t st = (hipStream_t)stream;
-    if (!s->refine(steps, s->D, st)) return t_err ? t_err : 1;
-    return 0;
+    const bool ok = (s->opt_graph && !s->timing) ? s->refine_graph(steps, st) : s->refine(steps, s->D, st);
+    return ok ? 0 : (t_err ? t_err : 1);
 }
 
 int32_t ccd_RefinePoints(ccd_Subd *s, const float *devCagePoints, float *devPoints, int32_t steps, void *stream) {
@@ -851,7 +887,10 @@ int32_t ccd_Synchronize(ccd_Subd *s) {
 
 int32_t ccd_SetOption(ccd_Subd *s, int32_t key, int32_t value) {
     if (!s) return 11;
+    DeviceGuard g(s->device);
+    s->drop_graphs();   // captured with the previous choices
     switch (key) {
+    case CCB_OPT_GRAPH: s->opt_graph = value != 0; return 0;
     case CCD_OPT_FUSED:
         s->opt_fused = value < 0 ? 0 : (value > 2 ? 2 : value);
         if (s->opt_fused)
@@ -904,6 +943,7 @@ int32_t ccd_SetOption(ccd_Subd *s, int32_t key, int32_t value) {
 // level's time is that of its fused (or patch) and vertex kernels.
 int32_t ccd_Subd::autotune(int32_t steps, hipStream_t st) {
     if (!(steps & CCD_POINTS) || opt_fused == 0 || D < 2) return 0;
+    drop_graphs();
     float best[kMaxDepth + 1];
     LevelCfg pick[kMaxDepth + 1];
     for (int d = 0; d <= kMaxDepth; ++d) { best[d] = 1e30f; pick[d] = cfg[d]; }

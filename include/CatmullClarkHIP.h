@@ -303,6 +303,9 @@ int32_t ccd_MaxDepth(const ccd_Subd *subd);
 int64_t ccd_Count(const ccd_Subd *subd, int32_t what, int32_t depth);
 void *ccd_DevicePointer(const ccd_Subd *subd, int32_t which);
 int32_t ccd_UploadCagePoints(ccd_Subd *subd, const cc_VertexPoint *hostPoints);
+/* refines on the device; the launch sequence of each steps value is captured
+ * as a hipGraph on first use and replayed on `stream` (CCB_OPT_GRAPH 0:
+ * launched one by one; so are runs with timing on) */
 int32_t ccd_Refine(ccd_Subd *subd, int32_t steps, void *stream);
 /* points only, from cage points already on the device (or the uploaded ones
  * when devCagePoints is NULL) into devPoints (or the ccd_Subd's own buffer
@@ -321,7 +324,9 @@ int32_t ccd_Synchronize(ccd_Subd *subd);
  * the device topology and points (refined already): vertex normals when
  * smooth != 0, else corner normals (see ccs_HalfedgeNormals) */
 int32_t ccd_HalfedgeNormals(ccd_Subd *subd, float *devNormals, int32_t smooth, void *stream);
-/* kernel-selection options (all have working defaults):
+/* kernel-selection options (all have working defaults; setting any drops
+ * ccd_Refine's captured graphs):
+ *   CCB_OPT_GRAPH      1: ccd_Refine replays captured graphs (default)
  *   CCD_OPT_FUSED      1: per level one fused kernel for children, face and
  *                      edge points plus one for vertex points (default);
  *                      2: one patch-local kernel per level for children,

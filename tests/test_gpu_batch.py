@@ -161,6 +161,32 @@ def test_graph_replays_match_eager(capi, torch):
     b.close()
 
 
+@pytest.mark.parametrize("label,steps,flags", STEPS[:2], ids=[s[0] for s in STEPS[:2]])
+def test_single_cage_graph_replays_match_oracle(capi, torch, label, steps, flags):
+    """ccd_Refine replays a captured graph: every replay, with unrelated
+    torch work between them, new cage points uploaded before some, and the
+    graphs dropped by an option change halfway, equals the oracle."""
+    cage, depth = C.torus(6, 5, seed=4, ring_sharpness=((0, 1.3), (2, 0.4))), 5
+    rng = np.random.default_rng(3)
+    ds = capi.DeviceSubd(cage, depth)
+    stream = torch.cuda.current_stream().cuda_stream
+    for rep in range(6):
+        x = torch.randn(1 << 16, device="cuda")
+        _ = (x * 2).sum().item()
+        if rep in (2, 5):
+            cage.vertex_points = (cage.vertex_points + rng.normal(0, 0.05, cage.vertex_points.shape)).astype(np.float32)
+            ds.upload_cage_points(cage.vertex_points)
+        if rep == 3:
+            ds.set_option(capi.OPT_SCAN, 4)
+        ds.refine(steps, stream)
+        torch.cuda.synchronize()
+        got = ds.download()
+        want = O.refine(cage, depth, flags)
+        for k in ("halfedges", "creases", "points"):
+            assert_bytes_equal(got[k], want[k], f"replay {rep} {k}")
+    ds.close()
+
+
 def test_batch_refine_points_and_finest(capi, torch):
     cases = batch_cases()
     cages = [c for c, _ in cases]
