@@ -353,6 +353,9 @@ def main():
             d += 1
         labelled.append(((i, nm), d))
     ((idx, name), level) = max(labelled, key=lambda kv: sum(kern[kv[0]]) / len(kern[kv[0]]))
+    # every launch of one step: (name, level, mean ms), the level a kernel
+    # reads (0: cage step and creases)
+    step_kernels = [(nm, lv, round(sum(kern[(i, nm)]) / len(kern[(i, nm)]), 4)) for (i, nm), lv in labelled]
     durs = kern[(idx, name)]
     mean_ms = sum(durs) / len(durs)
     if name == "level_fused":
@@ -446,6 +449,7 @@ def main():
         "batched": batch,
         "host_boundary": host,
         "level_kernels": level_cfg,
+        "step_kernels_ms": step_kernels,
         "roofline": {"bound": "hbm", "kernel": f"{name} (level {level} -> {level + 1})",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,

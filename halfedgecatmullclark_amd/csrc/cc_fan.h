@@ -43,6 +43,21 @@ __device__ __forceinline__ P3 quad_bcast3(P3 p) {
     return {quad_bcastf<K>(p.x), quad_bcastf<K>(p.y), quad_bcastf<K>(p.z)};
 }
 
+// A row of 16 lanes rotated right by N: lane i receives lane (i - N) mod 16
+// of its row (DPP row_ror).
+template <int N>
+__device__ __forceinline__ int row_rori(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x120 + N, 0xF, 0xF, false);
+}
+template <int N>
+__device__ __forceinline__ float row_ror(float v) {
+    return __int_as_float(row_rori<N>(__float_as_int(v)));
+}
+template <int N>
+__device__ __forceinline__ P3 row_ror3(P3 v) {
+    return P3{row_ror<N>(v.x), row_ror<N>(v.y), row_ror<N>(v.z)};
+}
+
 // value held by the lane of the next (+1), opposite (+2) or previous (+3)
 // halfedge of the same quad (DPP quad_perm [1,2,3,0], [2,3,0,1], [3,0,1,2])
 constexpr int kQuadNext = 0x39, kQuadOpp = 0x4E, kQuadPrev = 0x93;

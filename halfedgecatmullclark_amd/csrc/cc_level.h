@@ -17,18 +17,18 @@ namespace cch {
 // face point of h's quad (lane 0 of the quad) and, for the larger halfedge
 // of each pair, the edge point
 // (g2, g3: the points of the twin quad's two corners away from the edge when
-// that quad lies outside the block, s: the sharpness of h's edge for an owner)
+// that quad lies outside the block, s: the sharpness of h's edge for an owner;
+// F and Ev receive the face point and, for an owner, the edge point)
 template <int GATHER, int CREASED>
 __device__ __forceinline__ void level_fused_points(const FusedArgs &a, int32_t h, int j, int4 x, P3 P, P3 q0, P3 q1,
                                                    P3 q2, P3 q3, bool owner, bool twin_here, int32_t base,
-                                                   const P3 *spts, P3 g2, P3 g3, float s) {
+                                                   const P3 *spts, P3 g2, P3 g3, float s, P3 &F, P3 &Ev) {
     const int32_t twin = x.x;
-    const P3 F = face_rule(q0, q1, q2, q3, GATHER);
+    F = face_rule(q0, q1, q2, q3, GATHER);
     if (j == 0) a.fp[h >> 2] = F;
     if (!owner) return;
     // this halfedge owns its edge: both contributions
     const P3 v1 = pick4(q0, q1, q2, q3, (j + 1) & 3);
-    P3 Ev;
     if (twin >= 0) {
         // The twin quad's corners, from the twin on: the twin's vertex is the
         // head of h (v1), the twin's next vertex is h's own (P), and the other
@@ -61,6 +61,86 @@ __device__ __forceinline__ void level_fused_points(const FusedArgs &a, int32_t h
         }
     }
     a.ep[x.y] = Ev;
+}
+
+// Vertex points of the face vertices born at level d >= 2, inside the fused
+// kernel.  The face vertex of level-(d-1) quad Q (halfedges p = 4Q .. 4Q+3)
+// has the ring of children 2 of p + q, level-d halfedges 16Q + 4q + 2: one
+// row of 16 lanes.  Their quads' face points are in the ring lanes; the edge
+// of ring halfedge q, 2E' + prev(p + q) (E' the level-(d-1) edge count),
+// is owned by the ring lane itself for q = 1, 2, 3 and by lane 16Q + 13
+// (child 1 of p + 3, its twin) for q = 0.  Lane 16Q + 2, the vertex's
+// representative, gathers the four face points and four edge points by row
+// rotations and evaluates the masked vertex kernel's own function on the
+// entries that kernel would load (sharpness 0: edges from 2E' on have no
+// crease record, 2E' >= C), so the point is the same bit for bit.  Every
+// lane of the wave runs the rotations; a row is live or dead as a whole
+// (H is a multiple of 16 from level 2 on).  The scatter rules add one
+// contribution per ring halfedge from zero in ascending order: each ring
+// lane computes its own (the ring's parameters are those of a smooth
+// valence-4 vertex), and lane 16Q + 2 adds the four.
+template <int GATHER, int CREASED>
+__device__ __forceinline__ void face_vertex_in_row(const FusedArgs &a, const LevelTab &t, int d, int32_t h,
+                                                   bool live, P3 old, P3 F, P3 E) {
+    if (!GATHER) {
+        // ring lane q (offset 4q + 2): its edge point (offset 2: from 13)
+        // and the previous edge's, the next ring lane's (offset 14: from 13)
+        const int off = (int)(h & 15);
+        const P3 E5 = row_ror3<5>(E), E12 = row_ror3<12>(E), E1 = row_ror3<1>(E);
+        const P3 Eown = off == 2 ? E5 : E, PE = off == 14 ? E1 : E12;
+        RingParams rp;
+        rp.n = 4.0f; rp.cc = 0.0f; rp.boundary = false; rp.s1 = rp.s2 = 0.0f; rp.e1 = rp.e2 = -1; rp.sall = 0.0f;
+        const int32_t p = (h >> 4) << 2;
+        const int32_t re = 2 * (int32_t)t.E[d - 1] + p + ((((h >> 2) & 3) + 3) & 3);
+        const P3 c = vertex_scatter_contrib(rp, old, Eown, PE, F, 0.0f, 0.0f, re, CREASED);
+        const P3 c1 = row_ror3<12>(c), c2 = row_ror3<8>(c), c3 = row_ror3<4>(c);
+        if (!live || off != 2) return;
+        P3 sum = add(zero3(), c);
+        sum = add(sum, c1);
+        sum = add(sum, c2);
+        sum = add(sum, c3);
+        a.vp[(int32_t)t.V[d - 1] + (p >> 2)] = sum;
+        return;
+    }
+    const P3 F1 = row_ror3<12>(F), F2 = row_ror3<8>(F), F3 = row_ror3<4>(F);
+    const P3 E0 = row_ror3<5>(E), E1 = row_ror3<12>(E), E2 = row_ror3<8>(E), E3 = row_ror3<4>(E);
+    if (!live || (h & 15) != 2) return;
+    const int32_t p = (h - 2) >> 2;
+    const int32_t Ep2 = 2 * (int32_t)t.E[d - 1];
+    const P3 Fq[4] = {F, F1, F2, F3}, Eq[4] = {E0, E1, E2, E3};
+    FanEntry en[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int n1 = (q + 1) & 3;
+        en[q].r = h + 4 * q;
+        en[q].re = Ep2 + p + ((q + 3) & 3);
+        en[q].pe = Ep2 + p + ((n1 + 3) & 3);
+        en[q].E = Eq[q];
+        en[q].PE = Eq[n1];
+        en[q].F = Fq[q];
+        en[q].ts = 0.0f;
+        en[q].ps = 0.0f;
+    }
+    Fan f;
+    f.kind = 1; f.shift = 0; f.n = 4; f.ri = 0; f.boundary = false;
+    f.a0 = h; f.a1 = f.a2 = f.a3 = 0;
+    a.vp[(int32_t)t.V[d - 1] + (p >> 2)] =
+        fan_vertex_point<GATHER, CREASED, 4>(f, FanPreloaded{en[0], en[1], en[2], en[3]}, old);
+}
+
+// uvIDs of children 1, 2, 3 of a halfedge from its own, the next, the
+// opposite and the previous halfedge's uvs (:2878-2921; the face average sums
+// them in that order, :2904-2913); child 0 keeps the halfedge's own code
+__device__ __forceinline__ void child_uv_codes(UV uv, UV nuv, UV ouv, UV puv, int32_t &c1, int32_t &c2, int32_t &c3) {
+    UV fuv = uv;
+    fuv.u += nuv.u; fuv.v += nuv.v;
+    fuv.u += ouv.u; fuv.v += ouv.v;
+    fuv.u += puv.u; fuv.v += puv.v;
+    fuv.u /= 4.0f;
+    fuv.v /= 4.0f;
+    c1 = uv_encode_unit(uv_mid(uv, nuv));
+    c2 = uv_encode_unit(fuv);
+    c3 = uv_encode_unit(uv_mid(uv, puv));
 }
 
 template <int GATHER, int CREASED, int TOPO, int UVMODE, int POINTS, int STORE, int NT>
@@ -108,7 +188,8 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
         int32_t v2 = 0, v3 = 0;
         if (twin_far) {
             const int32_t tq = twin & ~3, tj = twin & 3;
-            v2 = he4[tq + ((tj + 2) & 3)].z; v3 = he4[tq + ((tj + 3) & 3)].z;
+            v2 = he4[tq + ((tj + 2) & 3)].z;
+            v3 = he4[tq + ((tj + 3) & 3)].z;
             asm volatile("" ::"v"(v2), "v"(v3));
         }
         spts[threadIdx.x] = P;
@@ -129,16 +210,8 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
             // the face average sums them in that order (:2904-2913)
             const UV uv = uv_decode(x.w);
             const UV nuv = quad_reluv<kQuadNext>(uv), ouv = quad_reluv<kQuadOpp>(uv), puv = quad_reluv<kQuadPrev>(uv);
-            UV fuv = uv;
-            fuv.u += nuv.u; fuv.v += nuv.v;
-            fuv.u += ouv.u; fuv.v += ouv.v;
-            fuv.u += puv.u; fuv.v += puv.v;
-            fuv.u /= 4.0f;
-            fuv.v /= 4.0f;
             c[0].w = x.w;
-            c[1].w = uv_encode_unit(uv_mid(uv, nuv));
-            c[2].w = uv_encode_unit(fuv);
-            c[3].w = uv_encode_unit(uv_mid(uv, puv));
+            child_uv_codes(uv, nuv, ouv, puv, c[1].w, c[2].w, c[3].w);
         }
         if (kStaged) {
             staged_children_store<NT>(stage, c, a.out_he, base, a.H);
@@ -189,11 +262,21 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
     if (!POINTS) return;
     if (!kStaged) __syncthreads();
     const P3 q0 = quad_bcast3<0>(P), q1 = quad_bcast3<1>(P), q2 = quad_bcast3<2>(P), q3 = quad_bcast3<3>(P);
-    if (live) level_fused_points<GATHER, CREASED>(a, h, j, x, P, q0, q1, q2, q3, owner, twin_here, base, spts, g2, g3, s);
+    P3 Fq = zero3(), Ev = zero3();
+    if (live)
+        level_fused_points<GATHER, CREASED>(a, h, j, x, P, q0, q1, q2, q3, owner, twin_here, base, spts, g2, g3, s, Fq,
+                                            Ev);
     if (a.owner_mask) {
+        // the face vertices born at this level are finished here: their
+        // representatives (children 2 of a quad's first halfedge, h % 16 = 2)
+        // leave no bit for the vertex kernel
+        const bool face_here = d >= 2;
+        if (face_here) face_vertex_in_row<GATHER, CREASED>(a, stab, d, h, live, P, Fq, Ev);
         // one bit per halfedge: is it the representative of its vertex?  Last,
         // so that the cage lookups of a few lanes overlap the wave's stores
-        const uint64_t m = __ballot(live && owner_test(h, x.z, d, cg, stab, a.erep_read ? a.erep : nullptr));
+        const bool skip = face_here && (h & 15) == 2;
+        const uint64_t m =
+            __ballot(live && !skip && owner_test(h, x.z, d, cg, stab, a.erep_read ? a.erep : nullptr));
         if ((threadIdx.x & 63u) == 0 && live) a.owner_mask[h >> 6] = m;
     }
 }
