@@ -49,10 +49,14 @@ def level_bytes(n, d, D=None, owner_bits=False):
     (16 B), level-d points (12 B), level-d creases (12 B); writes: the four
     16-B child records per parent halfedge and the level-(d+1) face and edge
     points (12 B).  With owner_bits (the masked vertex kernel follows) it
-    also writes one owner bit per halfedge and reads the parent level's
-    edge-representative bits; on levels d + 1 <= D - 2 it reads level d's
-    bits and writes level d+1's (one bit per halfedge each)."""
+    also writes one owner bit per halfedge, reads the parent level's
+    edge-representative bits and, from level 2 on, writes the vertex points
+    of the face vertices born at level d (12 B each, one per parent-level
+    face); on levels d + 1 <= D - 2 it reads level d's bits and writes level
+    d+1's (one bit per halfedge each)."""
     b = 16 * n["H"][d] + 12 * n["V"][d] + 12 * n["C"][d] + 64 * n["H"][d] + 12 * (n["F"][d] + n["E"][d])
+    if owner_bits and d >= 2:
+        b += 12 * n["F"][d - 1]
     if D is not None and D >= 4:
         if owner_bits:
             b += n["H"][d] // 8 + (n["H"][d - 1] // 8 if d >= 2 else 0)
@@ -63,13 +67,16 @@ def level_bytes(n, d, D=None, owner_bits=False):
 
 def vertex_kernel_bytes(n, d):
     """Algorithmic bytes of the masked vertex-point kernel d -> d+1: the owner
-    bits (1 bit per level-d halfedge), the level-d points and creases (12 B),
-    the level-(d+1) face and edge points it reads once, one parent record
-    (16 B) per edge vertex born at level d (the other rings follow from the
-    child numbering), and the level-(d+1) vertex points it writes."""
+    bits (1 bit per level-d halfedge), the level-d points of its vertices and
+    the creases (12 B), the level-(d+1) face and edge points it reads once,
+    one parent record (16 B) per edge vertex born at level d (the other rings
+    follow from the child numbering), and the level-(d+1) vertex points it
+    writes.  From level 2 on the fused kernel finishes the face vertices born
+    at level d (one per parent-level face), so they are left out."""
     born_edges = n["E"][d - 1] if d >= 2 else 0
-    return (n["H"][d] // 8 + 12 * n["V"][d] + 12 * n["C"][d] + 12 * (n["F"][d] + n["E"][d]) + 16 * born_edges
-            + 12 * n["V"][d])
+    verts = n["V"][d] - (n["F"][d - 1] if d >= 2 else 0)
+    return (n["H"][d] // 8 + 12 * verts + 12 * n["C"][d] + 12 * (n["F"][d] + n["E"][d]) + 16 * born_edges
+            + 12 * verts)
 
 
 def patch_bytes(n, d):
