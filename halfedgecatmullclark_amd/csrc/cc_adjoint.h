@@ -8,10 +8,13 @@ namespace cch {
 
 // The vertex rule of one level-d vertex, enough to give the weights of any
 // halfedge of its ring from that halfedge's own and previous sharpness:
-// valence, boundary and rule (0 smooth, 1 corner, 2 crease) packed in nb,
-// and the first two nonzero sharpness values of the ring (s1 on edge e1).
-// With the crease rule exactly two ring edges are sharp, so their sum is
-// s1 + s2 and a ring edge other than e1 with nonzero sharpness is the other.
+// valence, boundary and rule (0 smooth, 1 corner, 2 crease) packed in nb.
+// The smooth and corner rules' weights are the same for every ring
+// halfedge, so s1 and s2 hold them already divided out (s1 the weight a of
+// the vertex, s2 the weight g of the face; the edge's b is -4 g); the crease
+// rule keeps the first two nonzero sharpness values of the ring (s1 on edge
+// e1): exactly two ring edges are sharp, so their sum is s1 + s2 and a ring
+// edge other than e1 with nonzero sharpness is the other.
 template <typename T>
 struct VParam {
     T s1, s2;
@@ -61,6 +64,12 @@ struct AdjPlan {
     LevelPtrs lp;
     int64_t per_vertex_max;   // levels with at most this many vertices: vertex passes one thread per vertex
     const uint64_t *erep;     // edge-representative bits of levels 1..D-2 from the forward pass, or null
+    // float adjoint: the vertex rules' parameters of levels 1..D-1 (VParam,
+    // level d at n.offV[d]), kept across calls for the same topology and
+    // creases; vp_valid: they hold this topology's, so the parameter pass is
+    // skipped (null: the parameters go to the scratch W, every call)
+    void *vp_cache = nullptr;
+    bool vp_valid = false;
 };
 
 // dL/d(cage points) from dL/d(level-D points): V_D x 3 in, V_0 x 3 out.

@@ -5,20 +5,23 @@
 // CatmullClark.h:1437-1991 and :2032-2615, depend only on valence, crease
 // count, boundary and sharpness).  The adjoint applies the transposes level
 // by level, finest first.  For a level step d -> d+1 (d >= 1) it runs in
-// gather form, without atomics, in four passes:
-//   1. weights: the owner of each level-d vertex evaluates the vertex rule's
-//      weights (a, b, g, dl) of every halfedge of its ring;
-//   2. edges: the owner of each edge adds the vertex rule's shares to the
-//      edge's gradient;
-//   3. faces: each quad adds the vertex and edge rules' shares to its face's
-//      gradient;
-//   4. vertices: the owner of each vertex gathers its level-d gradient from
-//      the vertex, edge and face rules.
-// The cage step (small, polygons of any size) runs as ordered gathers over
-// the host-built ring lists (per-halfedge float atomics only for a cage
-// whose walks do not close).  Only the stored topology and creases are read; nothing of the
-// forward points is kept or recomputed.  Kernels are templated on the scalar
-// type so the same code runs in float and, for gradient checks, in double.
+// gather form, without atomics, in three passes:
+//   P. parameters: the owner of each level-d vertex packs its vertex rule's
+//      parameters (valence, boundary, rule, the first two sharp ring edges:
+//      16 bytes, VParam) from its ring;
+//   Q. quads: the four lanes of each quad form, from their corners'
+//      parameters, the vertex rule's shares of each of the quad's edges and
+//      the updated edge gradients (the larger halfedge of a pair stores its
+//      edge's), then the updated face gradient;
+//   R. vertices: the owner of each vertex gathers its level-d gradient from
+//      the vertex rule (its own weight), its edges' and its faces' updated
+//      gradients.
+// The cage step (small, polygons of any size) runs as four ordered gathers
+// over the host-built ring lists with per-halfedge weights (per-halfedge
+// float atomics only for a cage whose walks do not close).  Only the stored
+// topology and creases are read; nothing of the forward points is kept or
+// recomputed.  Kernels are templated on the scalar type so the same code
+// runs in float and, for gradient checks, in double.
 #include "cc_adjoint.h"
 #include "cc_batch.h"
 #include "cc_fan.h"
@@ -178,10 +181,23 @@ __device__ __forceinline__ VParam<T> vparam_of(const RingT<T> &rp, bool creased)
         rule = (rp.cc >= (T)3 || nn == (T)2) ? 1 : (rp.cc <= (T)1 ? 0 : 2);
     }
     VParam<T> p;
-    p.s1 = rp.s1;
-    p.s2 = rp.s2;
-    p.e1 = rp.e1;
     p.nb = (int32_t)rp.n | (rule << 28) | (rp.boundary ? (1 << 30) : 0);
+    p.e1 = rp.e1;
+    const T n = rp.n;
+    if (!creased) {   // the same operations as the rule's weights in vparam_weights
+        const T iw = (T)1 / n, s = rp.boundary ? (T)0 : (T)1;
+        p.s1 = iw - (T)3 * iw * iw * s;
+        p.s2 = -iw * iw * s;
+    } else if (rule == 0) {
+        p.s1 = (n - (T)3) / (n * n);
+        p.s2 = (T)-1 / (n * n);
+    } else if (rule == 1) {
+        p.s1 = (T)1 / n;
+        p.s2 = (T)0;
+    } else {
+        p.s1 = rp.s1;
+        p.s2 = rp.s2;
+    }
     return p;
 }
 
@@ -190,29 +206,29 @@ __device__ __forceinline__ VParam<T> vparam_of(const RingT<T> &rp, bool creased)
 // edge) to the vertex rule (creased :2568-2607; smooth :2339-2348), from
 // the vertex's parameters and the halfedge's own (ts, edge) and previous
 // (ps) sharpness
+//
+// The smooth and corner weights come stored in p (vparam_of): the smooth
+// rule's are 1/n - 3/n^2, 4/n^2, -1/n^2 uncreased (times 0 but the first on
+// a boundary) and (n - 3)/n^2, 4/n^2, -1/n^2 creased, the corner rule's 1/n.
+// b = -4 g holds bit for bit: both are correctly rounded quotients (or
+// products) and the factor 4 is exact, signed zeros included (a boundary's
+// g is -0, its b +0).
 template <typename T>
 __device__ __forceinline__ VW<T> vparam_weights(const VParam<T> &p, T ts, T ps, int32_t edge, bool creased) {
     VW<T> w{0, 0, 0, 0};
+    const int rule = creased ? (p.nb >> 28) & 3 : 0;
+    if (rule == 0) {
+        w.a = p.s1;
+        w.b = (T)-4 * p.s2;
+        w.g = p.s2;
+        return w;
+    }
+    if (rule == 1) {
+        w.a = p.s1;
+        return w;
+    }
     const T n = (T)(p.nb & 0x0FFFFFFF);
     const bool boundary = ((p.nb >> 30) & 1) != 0;
-    if (!creased) {
-        const T iw = (T)1 / n, s = boundary ? (T)0 : (T)1;
-        w.a = iw - (T)3 * iw * iw * s;
-        w.b = (T)4 * iw * iw * s;
-        w.g = -iw * iw * s;
-        return w;
-    }
-    const int rule = (p.nb >> 28) & 3;
-    if (rule == 1) {
-        w.a = (T)1 / n;
-        return w;
-    }
-    if (rule == 0) {
-        w.a = (n - (T)3) / (n * n);
-        w.b = (T)4 / (n * n);
-        w.g = (T)-1 / (n * n);
-        return w;
-    }
     // two sharp ring edges: the sum of the sharpness this halfedge's walk
     // visits, every ring edge but its own on a boundary
     const T avg = boundary ? (edge == p.e1 ? p.s2 : (ts != (T)0 ? p.s1 : p.s1 + p.s2)) : p.s1 + p.s2;
@@ -901,6 +917,8 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
     }
     // G holds the gradient of level d+1 (modified in place), P receives level d
     T *G = bufA, *P = bufB;
+    VParam<T> *cache = sizeof(T) == sizeof(float) ? static_cast<VParam<T> *>(pl.vp_cache) : nullptr;
+    const bool params = !(cache && pl.vp_valid);   // run the parameter pass
     hipLaunchKernelGGL(k_copy<T>, dim3(grid_stride(3 * n.V[D])), dim3(256), 0, st, gradFinest, G, 3 * n.V[D]);
     for (int d = D - 1; d >= 1; --d) {
         AdjArgs<T> a;
@@ -915,20 +933,28 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
         a.gE = G + 3 * (n.V[d] + n.F[d]);
         a.gP = P;
         a.W = W;
-        a.VP = reinterpret_cast<VParam<T> *>(W);
+        a.VP = cache ? cache + n.offV[d] : reinterpret_cast<VParam<T> *>(W);
         a.gEu = W + 4 * n.V[d];
         a.erep = pl.erep;
         const dim3 g(grid_for(n.H[d])), gq(grid_for(n.H[d] / 4)), gv(grid_for(n.V[d])), b(256);
         const bool pv = n.V[d] <= pl.per_vertex_max;
         if (creased) {
-            if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 1, 0>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
-            else hipLaunchKernelGGL((k_adj_params<T, 1>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            if (!params) {
+            } else if (pv) {
+                hipLaunchKernelGGL((k_adj_per_vertex<T, 1, 0>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            } else {
+                hipLaunchKernelGGL((k_adj_params<T, 1>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            }
             hipLaunchKernelGGL((k_adj_quads<T, 1>), g, b, 0, st, a);
             if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 1, 1>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             else hipLaunchKernelGGL((k_adj_vertices<T, 1>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
         } else {
-            if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 0, 0>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
-            else hipLaunchKernelGGL((k_adj_params<T, 0>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            if (!params) {
+            } else if (pv) {
+                hipLaunchKernelGGL((k_adj_per_vertex<T, 0, 0>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            } else {
+                hipLaunchKernelGGL((k_adj_params<T, 0>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            }
             hipLaunchKernelGGL((k_adj_quads<T, 0>), g, b, 0, st, a);
             if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 0, 1>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             else hipLaunchKernelGGL((k_adj_vertices<T, 0>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);

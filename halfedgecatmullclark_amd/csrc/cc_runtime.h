@@ -46,6 +46,12 @@ struct ccd_Subd {
     // adjoint scratch (allocated on first use, per scalar size)
     void *adj_a = nullptr, *adj_b = nullptr, *adj_w = nullptr, *adj_in = nullptr, *adj_out = nullptr,
          *adj_sharp = nullptr;
+    // the float adjoint's vertex-rule parameters of levels 1..D-1 (16 bytes
+    // per vertex), valid for the creased (1) or uncreased (0) rules of the
+    // current topology and creases, -1 when they must be computed again
+    // (a topology or crease refinement, a cage upload)
+    void *adj_vp = nullptr;
+    int adj_vp_key = -1;
     // deferred-vertex masks of the patch kernels, one word per 64 halfedges
     uint64_t *defer_mask = nullptr;
     // edge-representative bits of levels 1..D-2 (FusedArgs::erep; the owner

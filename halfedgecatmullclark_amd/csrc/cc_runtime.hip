@@ -99,7 +99,7 @@ bool ccd_Subd::alloc(const cc_Mesh *cage, int32_t depth) {
 void ccd_Subd::free_all() {
     DeviceGuard g(device);
     void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts, adj_a, adj_b, adj_w, adj_in, adj_out, adj_sharp,
-                    defer_mask, rings, rare, erep, nrm_tmp, nrm_out};
+                    adj_vp, defer_mask, rings, rare, erep, nrm_tmp, nrm_out};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : timed) {
@@ -123,6 +123,7 @@ bool ccd_Subd::upload_cage(const cc_Mesh *cage, hipStream_t st) {
         return false;
     }
     for (bool &v : erep_valid) v = false;   // bits of the previous topology
+    adj_vp_key = -1;
     drop_graphs();
     CC_TRY(hipMemcpyAsync(c_he, cage->halfedges, sizeof(CageHe) * (size_t)H0, hipMemcpyHostToDevice, st), "upload cage");
     CC_TRY(hipMemcpyAsync(c_v2h, cage->vertexToHalfedgeIDs, sizeof(int32_t) * (size_t)V0, hipMemcpyHostToDevice, st), "upload cage");
@@ -401,6 +402,7 @@ bool ccd_Subd::refine_points(int32_t steps, int dmax, const P3 *cagePts, P3 *out
 
 bool ccd_Subd::refine(int32_t steps, int dmax, hipStream_t st) {
     if (timing) begin_timing();
+    if (steps & (CCD_HALFEDGES | CCD_CREASES)) adj_vp_key = -1;   // the adjoint's parameters follow these
     if (!opt_fused) {
         if (!refine_topology(steps, dmax, st)) return false;
         if ((steps & CCD_POINTS) && !refine_points(steps, dmax, nullptr, pts, st)) return false;
@@ -493,8 +495,13 @@ bool ccd_Subd::adjoint(const void *gradFinest, void *gradCage, int32_t steps, hi
         adjoint_levels<double>(pl, (const double *)gradFinest, (double *)gradCage, (double *)adj_a, (double *)adj_b,
                                (double *)adj_w, (double *)adj_sharp, creased, st);
     } else {
+        if (!adj_vp && D >= 2)
+            CC_TRY(hipMalloc(&adj_vp, sizeof(VParam<float>) * (size_t)n.cumV(D - 1)), "hipMalloc(adjoint parameters)");
+        pl.vp_cache = adj_vp;
+        pl.vp_valid = adj_vp_key == creased;
         adjoint_levels<float>(pl, (const float *)gradFinest, (float *)gradCage, (float *)adj_a, (float *)adj_b,
                               (float *)adj_w, nullptr, creased, st);
+        adj_vp_key = creased;
     }
     CC_TRY(hipGetLastError(), "adjoint launch");
     return true;

@@ -135,3 +135,42 @@ def test_vertex_pass_orders_agree(capi, label, cage, depth, steps):
     assert outs[0].tobytes() == outs[1].tobytes(), label
     assert outs[0].tobytes() == outs[2].tobytes(), label
     ds.close()
+
+
+def test_parameter_cache_follows_rules_and_topology(capi):
+    """The float adjoint keeps the vertex rules' parameters of the current
+    topology and creases between calls: a cached run, a run after the rules
+    change (creased / uncreased) and a run after the topology and creases
+    are refined again give the bits of a fresh subd's first run."""
+    import torch
+    cage, depth = C.torus(6, 5, seed=4, ring_sharpness=((0, 1.3), (2, 0.4))), 5
+
+    def first_run(steps):
+        ds = capi.DeviceSubd(cage, depth)
+        ds.refine(capi.ALL)
+        out = run(ds, steps)
+        ds.close()
+        return out
+
+    n = capi.DeviceSubd(cage, depth)
+    n.refine(capi.ALL)
+    V = n.download()["counts"]["V"]
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    g = torch.randn((V[depth], 3), generator=gen, device="cuda", dtype=torch.float32)
+
+    def run(ds, steps):
+        out = torch.empty((cage.V, 3), device="cuda", dtype=torch.float32)
+        ds.refine_points_adjoint(g.data_ptr(), out.data_ptr(), steps)
+        ds.synchronize()
+        return out.cpu().numpy().tobytes()
+
+    want = {0: first_run(0), 32: first_run(32)}
+    assert want[0] != want[32]
+    got = [run(n, 0), run(n, 0), run(n, 32), run(n, 32), run(n, 0)]
+    n.refine(capi.ALL)
+    got += [run(n, 0), run(n, 0)]
+    n.refine(capi.POINTS)      # points only: the cached parameters stay
+    got.append(run(n, 0))
+    for i, (steps, b) in enumerate(zip([0, 0, 32, 32, 0, 0, 0, 0], got)):
+        assert b == want[steps], (i, steps)
+    n.close()
