@@ -8,11 +8,18 @@ refinement of all 8 levels from the cage resident in HBM; value is refined
 halfedges (levels 1..8, 715,816,960) per second.
 
 Before timing, ccd_Autotune picks each level's kernel configuration (the
-fused kernels with 2, 4 or 8 halfedges tested per vertex-kernel thread, or
-the patch-local kernel with 128/256/512-halfedge tiles) by timing them.
+fused kernel's workgroup size and tiles per workgroup, the vertex kernel's
+form and waves per SIMD, or the patch-local kernel) by timing them; the
+timed steps replay ccd_Refine's captured graph.
 
 Also reported on the same line:
+  - every kernel of one step (per-kernel HIP events, a second pass);
   - the gather point rules on the same workload;
+  - the points-only forward and the adjoint of the points at depth 8;
+  - the same refinement with 12-byte level records (CC_DISABLE_UV library);
+  - halfedge normals of level 8;
+  - the cage builder on a 131,072-face OBJ (ccm_LoadObj, ccm_CreateFromFaces),
+    with the reference's converter timed on the same file in the CPU leg;
   - batched refinement: 256 cages of uneven size (halfedges log-uniform in
     [24, 60000], depths 4-6, 5% of edges creased U[0,4]) refined in one
     launch per kernel kind and level from a captured hipGraph, forward and
@@ -145,6 +152,29 @@ def cpu_baseline(depth: int = 8, runs: int = 3):
                       f"runs after one untimed run: {res['total']['median_ms']:.0f} ms; stages once each "
                       f"(creases, halfedges, points, uvs as examples/subd_cpu.c): "
                       + ", ".join(f"{v['median_ms']:.0f}" for v in st.values()) + " ms"}
+
+
+def builder_cpu_baseline(obj_path: str, runs: int = 3):
+    """The reference's converter (examples/obj_to_ccm.c built by
+    oracle/Makefile, OpenMP on every core) on the builder workload's OBJ
+    file: parse, build, write the .ccm; wall-clock median of `runs` after one
+    untimed run."""
+    import subprocess
+    from oracle import ref
+    exe = ref.obj_to_ccm_path()
+    if exe is None:
+        return {"ms": None, "sample": "the reference converter is not built"}
+    cwd = os.path.dirname(obj_path)
+    ts = []
+    for i in range(runs + 1):
+        t0 = time.perf_counter()
+        r = subprocess.run([exe, obj_path], cwd=cwd, capture_output=True)
+        if r.returncode != 0:
+            return {"ms": None, "sample": f"obj_to_ccm failed ({r.returncode})"}
+        if i:
+            ts.append((time.perf_counter() - t0) * 1e3)
+    return {"ms": sorted(ts)[runs // 2], "sample": f"examples/obj_to_ccm.c (gcc -O2 -fopenmp) on the same OBJ, "
+                                                   f"median of {runs} runs"}
 
 
 def batched(torch, capi, C, steps: int, stream):
@@ -338,6 +368,36 @@ def main():
                "corner_gbs": nrm_bytes[False] / (nrm_ms[False] * 1e-3) / 1e9,
                "vertex_gbs": nrm_bytes[True] / (nrm_ms[True] * 1e-3) / 1e9}
 
+    # the cage builder on a 512 x 256 quad torus with uvs and two crease
+    # rings (131,072 faces, 524,288 halfedges): its OBJ text through
+    # ccm_LoadObj (parsed on the host as the reference's converter parses,
+    # built on the GPU, copied back into a cc_Mesh), and ccm_CreateFromFaces
+    # on the flat face arrays (upload, GPU build, the cc_Mesh copied back);
+    # wall-clock medians of 3 after one warm-up
+    import tempfile
+    mesh = C.torus_mesh(512, 256, seed=0)
+    obj_dir = tempfile.mkdtemp()
+    obj_path = os.path.join(obj_dir, "torus512x256.obj")
+    with open(obj_path, "w") as f:
+        f.write(C.to_obj(*mesh))
+
+    def wall_ms(f, k=3):
+        f()
+        ts = []
+        for _ in range(k):
+            t0 = time.perf_counter()
+            f()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return sorted(ts)[k // 2]
+
+    built = capi.build_cage(*mesh)
+    arrs = capi.face_arrays(*mesh)
+    builder = {"workload": "OBJ of a 512x256 quad torus with uvs and two crease rings (131072 faces, "
+                           f"{built.H} halfedges) to a cc_Mesh",
+               "load_obj_ms": wall_ms(lambda: capi.release_mesh(capi.load_obj_mesh(obj_path))),
+               "from_faces_ms": wall_ms(lambda: capi.release_mesh(capi.create_mesh(arrs))), "halfedges": built.H}
+    del built, arrs
+
     # the same refinement with 12-byte level records (libcchip_nouv.so, the
     # CC_DISABLE_UV layout): no uvIDs, 48 instead of 64 bytes of children
     # per parent halfedge
@@ -427,6 +487,10 @@ def main():
     host = host_boundary(capi, C, D) if args.host_boundary else None
     batch = None if args.no_batch else batched(torch, capi, C, args.steps, stream)
     cpu = None if args.no_cpu else cpu_baseline()
+    if cpu is not None:
+        cpu["builder"] = builder_cpu_baseline(obj_path)
+    import shutil
+    shutil.rmtree(obj_dir, ignore_errors=True)
 
     line = {
         "metric": METRIC,
@@ -450,6 +514,7 @@ def main():
         "adjoint_d8": adjoint_d8,
         "nouv_records": nouv,
         "normals": normals,
+        "builder": builder,
         "fwd_adjoint": {"workload": "torus 32x16 cage, depth 5, loss 0.5*||S(V)-T||^2",
                         "fwd_ms": res["fwd"], "fwd_adj_ms": res["fwd_adj"],
                         "ratio": res["fwd_adj"] / res["fwd"]},

@@ -289,10 +289,9 @@ def cage_from_mesh(mesh_ptr) -> Cage:
                 arr(m.faceToHalfedgeIDs, ctypes.c_int32, F, np.int32))
 
 
-def build_cage(points, faces, uvs=None, face_uvs=None, creases=(), sharpen_boundaries=True) -> Cage:
-    """Polygons to a cage with the GPU cage builder (ccm_CreateFromFaces);
-    the arguments are those of cage.from_faces."""
-    L = lib()
+def face_arrays(points, faces, uvs=None, face_uvs=None, creases=()) -> dict:
+    """The flat arrays ccm_CreateFromFaces takes, from the arguments of
+    cage.from_faces."""
     pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
     sizes = np.array([len(f) for f in faces], np.int64)
     off = np.zeros(len(faces) + 1, np.int32)
@@ -304,15 +303,42 @@ def build_cage(points, faces, uvs=None, face_uvs=None, creases=(), sharpen_bound
     tags = list(creases)
     tv = np.ascontiguousarray(np.array([[a, b] for a, b, _ in tags], np.int32).reshape(-1))
     ts = np.ascontiguousarray(np.array([s for _, _, s in tags], np.float32))
+    return {"pts": pts, "off": off, "fv": fv, "uv": uv, "fuv": fuv, "tv": tv, "ts": ts}
+
+
+def create_mesh(arrs: dict, sharpen_boundaries=True):
+    """ccm_CreateFromFaces on face_arrays' arrays: the library-allocated
+    cc_Mesh (free it with release_mesh)."""
+    L = lib()
     p = lambda a: a.ctypes.data if a is not None and a.size else None
-    m = L.ccm_CreateFromFaces(len(pts), p(pts), len(faces), p(off), p(fv), len(uv), p(uv), p(fuv), len(tags),
-                              p(tv), p(ts), int(sharpen_boundaries))
+    m = L.ccm_CreateFromFaces(len(arrs["pts"]), p(arrs["pts"]), len(arrs["off"]) - 1, p(arrs["off"]), p(arrs["fv"]),
+                              len(arrs["uv"]), p(arrs["uv"]), p(arrs["fuv"]), len(arrs["ts"]), p(arrs["tv"]),
+                              p(arrs["ts"]), int(sharpen_boundaries))
     if not m:
         raise CCError(f"ccm_CreateFromFaces failed: {last_error()[1]}")
+    return m
+
+
+def release_mesh(m):
+    lib().ccm_Release(m)
+
+
+def build_cage(points, faces, uvs=None, face_uvs=None, creases=(), sharpen_boundaries=True) -> Cage:
+    """Polygons to a cage with the GPU cage builder (ccm_CreateFromFaces);
+    the arguments are those of cage.from_faces."""
+    m = create_mesh(face_arrays(points, faces, uvs, face_uvs, creases), sharpen_boundaries)
     try:
         return cage_from_mesh(m)
     finally:
-        L.ccm_Release(m)
+        release_mesh(m)
+
+
+def load_obj_mesh(path: str):
+    """ccm_LoadObj: the library-allocated cc_Mesh (free it with release_mesh)."""
+    m = lib().ccm_LoadObj(os.fsencode(path))
+    if not m:
+        raise CCError(f"ccm_LoadObj({path}) failed: {last_error()[1]}")
+    return m
 
 
 def load_obj(path: str) -> Cage:

@@ -1092,13 +1092,13 @@ struct OpAdjCageGather {
 // vertex gathers in turn; `quads` has one workgroup per 1024 halfedges of
 // each job (the owner passes), `halfedges` one per 256
 void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, char *arena, char *adj, int d, int creased,
-                     hipStream_t st) {
+                     int params, hipStream_t st) {
     if (creased) {
-        launch_batch<AdjJob>(quads, OpAdjParams<1>{d, arena, adj}, st);
+        if (params) launch_batch<AdjJob>(quads, OpAdjParams<1>{d, arena, adj}, st);
         launch_batch<AdjJob>(halfedges, OpAdjQuads<1>{arena, adj}, st);
         launch_batch<AdjJob>(quads, OpAdjVertices<1>{d, arena, adj}, st);
     } else {
-        launch_batch<AdjJob>(quads, OpAdjParams<0>{d, arena, adj}, st);
+        if (params) launch_batch<AdjJob>(quads, OpAdjParams<0>{d, arena, adj}, st);
         launch_batch<AdjJob>(halfedges, OpAdjQuads<0>{arena, adj}, st);
         launch_batch<AdjJob>(quads, OpAdjVertices<0>{d, arena, adj}, st);
     }

@@ -55,6 +55,7 @@ struct SpanJob {
 // adjoint arena.
 struct AdjJob {
     int64_t he, cr, gV, gF, gE, gP, W, erep;
+    int64_t vp;   // the level's vertex-rule parameters (kept across calls), or -1: in W
     int64_t c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, c_vro, c_vr, c_vb, c_fro, c_fr, c_voo, c_vo;
     int64_t lp[kMaxDepth + 1];
     int32_t H, C, V;
@@ -121,7 +122,7 @@ __host__ __device__ __forceinline__ void unpack_adj(const AdjJob &J, char *arena
     a.gE = arena_ptr<float>(adj, J.gE);
     a.gP = arena_ptr<float>(adj, J.gP);
     a.W = arena_ptr<float>(adj, J.W);
-    a.VP = reinterpret_cast<VParam<float> *>(a.W);
+    a.VP = J.vp >= 0 ? arena_ptr<VParam<float>>(adj, J.vp) : reinterpret_cast<VParam<float> *>(a.W);
     a.gEu = a.W ? a.W + 4 * (int64_t)J.V : nullptr;
     a.erep = arena_ptr<const uint64_t>(arena, J.erep);
     c = arena_cage(arena, J);
@@ -185,11 +186,12 @@ void batch_level_vertices(const BatchTable &t, char *arena, int d, int gather, i
 void batch_level_vertices_rare(const BatchTable &t, char *arena, int d, int gather, int creased, hipStream_t st);
 
 // adjoint launches (cc_adjoint.hip)
-// one level's adjoint: vertex parameters and vertex gathers over `quads`
-// (the owner passes scan 4 halfedges per thread), the quad pass one thread
-// per halfedge over `halfedges`
+// one level's adjoint: vertex parameters (when `params`; else the ones a
+// previous call left) and vertex gathers over `quads` (the owner passes scan
+// 4 halfedges per thread), the quad pass one thread per halfedge over
+// `halfedges`
 void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, char *arena, char *adj, int d, int creased,
-                     hipStream_t st);
+                     int params, hipStream_t st);
 void batch_adj_cage(const BatchTable &t, char *arena, char *adj, int creased, hipStream_t st);
 // the cage step as ordered gathers (every cage has its ring lists): one
 // thread per cage halfedge, edge, face and vertex in turn

@@ -30,6 +30,7 @@ struct ccb_Batch {
         // element offsets of the cage's ring lists in `rings` (-1: walk the cage)
         int64_t o_vro = -1, o_vr = -1, o_vb = -1, o_fro = -1, o_fr = -1, o_voo = -1, o_vo = -1;
         int64_t o_he = 0, o_cr = 0, o_pts = 0, o_fin = 0, o_def = 0, o_adjB = 0, o_adjW = 0, o_rare = 0;
+        int64_t o_adjVP = 0;   // VParam offset of levels 1..D-1's vertex-rule parameters
         int32_t rare_cap = 0;   // rare-vertex list entries (FusedArgs::rare), the count first
         int64_t o_erep = -1;    // word offset of the edge-representative bits (depth >= 4)
     };
@@ -47,6 +48,7 @@ struct ccb_Batch {
     std::vector<int32_t> rings_host;   // every cage's ring lists, copied into `rings`
     int32_t *rings = nullptr;
     int64_t tot_he = 0, tot_cr = 0, tot_pts = 0, tot_fin = 0, tot_def = 0, tot_adjB = 0, tot_adjW = 0, tot_rare = 0;
+    int64_t tot_adjVP = 0;
     int32_t *rare = nullptr;
     int64_t tot_erep = 0;
     uint64_t *erep = nullptr;   // edge-representative bits of levels 1..D-2, per cage (FusedArgs::erep)
@@ -68,6 +70,11 @@ struct ccb_Batch {
     P3 *pts = nullptr;
     uint64_t *defer_mask = nullptr;   // deferred-vertex masks, per cage one word per 64 halfedges
     float *adjA = nullptr, *adjB = nullptr, *adjW = nullptr, *adjCage = nullptr;
+    // every cage's vertex-rule parameters of levels 1..D-1, kept across
+    // adjoint calls (a batch's topology and creases never change); valid for
+    // the creased (1) or uncreased (0) rules, -1 before the first call
+    VParam<float> *adjVP = nullptr;
+    int adj_vp_key = -1;
     void *tables = nullptr;
     size_t tables_cap = 0;
     // launch tables
@@ -90,7 +97,7 @@ struct ccb_Batch {
     int remap = 1, use_graph = 1;
     bool timing = false;
     std::vector<Timed> timed;
-    std::map<int32_t, hipGraphExec_t> graphs;   // forward: steps; adjoint: -1 - creased
+    std::map<int32_t, hipGraphExec_t> graphs;   // forward: steps; adjoint: -1 - creased - 2 params
     bool topo_done = false, creases_done = false;
 
     bool create(const cc_Mesh *const *meshes, const int32_t *depths, int32_t count);
@@ -99,7 +106,8 @@ struct ccb_Batch {
     void drop_graphs();
     bool alloc_adjoint();
     void record_forward(int32_t steps, hipStream_t st);
-    void record_adjoint(int creased, hipStream_t st);
+    // flags: 1 creased rules, 2 run the parameter pass
+    void record_adjoint(int flags, hipStream_t st);
     bool run(int32_t key, bool adjoint, int32_t steps, hipStream_t st);
     int32_t autotune(int32_t steps, hipStream_t st);
     void begin_timing();
@@ -242,6 +250,7 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
         }
         g.o_adjB = tot_adjB; tot_adjB += g.n.V[g.D - 1];
         g.o_adjW = tot_adjW; tot_adjW += adj_scratch_scalars(g.n, g.D);
+        g.o_adjVP = tot_adjVP; tot_adjVP += g.n.cumV(g.D - 1);
         Dmax = std::max(Dmax, g.D);
     }
     CC_TRY(hipGetDevice(&device), "hipGetDevice");
@@ -323,7 +332,9 @@ bool ccb_Batch::alloc_adjoint() {
     const size_t oB = cv.take(sizeof(float) * 3 * (size_t)tot_adjB);
     const size_t oW = cv.take(sizeof(float) * (size_t)tot_adjW);
     const size_t oC = cv.take(sizeof(float) * 3 * (size_t)tot_v0);
+    const size_t oP = cv.take(sizeof(VParam<float>) * (size_t)tot_adjVP);
     CC_TRY(hipMalloc(&adj_arena, cv.size), "hipMalloc(batch adjoint arena)");
+    adjVP = reinterpret_cast<VParam<float> *>(adj_arena + oP);
     adjA = reinterpret_cast<float *>(adj_arena + oA);
     adjB = reinterpret_cast<float *>(adj_arena + oB);
     adjW = reinterpret_cast<float *>(adj_arena + oW);
@@ -486,6 +497,7 @@ bool ccb_Batch::build_tables() {
                     j.gP = aoff(buf(d));
                 }
                 j.W = aoff(adjW + g.o_adjW);
+                j.vp = d >= 1 ? aoff(adjVP + g.o_adjVP + g.n.offV[d]) : -1;
                 j.erep = g.o_erep >= 0 && kept ? off(erep + g.o_erep) : -1;
                 j.gV = aoff(G);
                 j.gF = aoff(G + 3 * g.n.V[d]);
@@ -583,12 +595,13 @@ void ccb_Batch::record_forward(int32_t steps, hipStream_t st) {
 
 // adjoint of record_forward's points, finest level first (gradients of the
 // finest points in adjA, cage gradients into adjCage)
-void ccb_Batch::record_adjoint(int creased, hipStream_t st) {
+void ccb_Batch::record_adjoint(int flags, hipStream_t st) {
+    const int creased = flags & 1, params = (flags >> 1) & 1;
     const int64_t nc = 3 * tot_v0;
     hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<int64_t>((nc + 255) / 256, 65536)), dim3(256), 0, st, adjCage, nc);
     for (int d = Dmax - 1; d >= 1; --d)
         timed_launch("adj_level", d, st,
-                     [&] { batch_adj_level(t_adjq[d], t_adj[d], arena, adj_arena, d, creased, st); });
+                     [&] { batch_adj_level(t_adjq[d], t_adj[d], arena, adj_arena, d, creased, params, st); });
     if (all_rings)
         timed_launch("adj_cage", 0, st, [&] {
             batch_adj_cage_gather(t_adj[0], t_adjc_e, t_adjc_f, t_adjc_v, arena, adj_arena, creased, st);
@@ -843,7 +856,10 @@ int32_t ccb_RefinePointsAdjoint(ccb_Batch *b, const float *devGradFinest, float 
     if (hip_failed(hipMemcpyAsync(b->adjA, devGradFinest, sizeof(float) * 3 * (size_t)b->tot_fin,
                                   hipMemcpyDeviceToDevice, st), "ccb_RefinePointsAdjoint"))
         return last_error();
-    if (!b->run(-1 - creased, true, creased, st)) return last_error();
+    // the parameter pass runs on the first call for these rules only
+    const int params = b->adj_vp_key != creased ? 1 : 0;
+    if (!b->run(-1 - creased - 2 * params, true, creased | (params << 1), st)) return last_error();
+    b->adj_vp_key = creased;
     if (hip_failed(hipMemcpyAsync(devGradCage, b->adjCage, sizeof(float) * 3 * (size_t)b->tot_v0,
                                   hipMemcpyDeviceToDevice, st), "ccb_RefinePointsAdjoint"))
         return last_error();

@@ -376,6 +376,39 @@ def oracle_jacobian(cage, depth, flags=O.ALL):
     return np.stack(cols, axis=1)
 
 
+def test_batch_adjoint_parameter_cache(capi, torch):
+    """The batched adjoint keeps its vertex-rule parameters after the first
+    call for a rule set: later calls (graph replays without the parameter
+    pass), and switching between the creased and uncreased rules, give the
+    bits of a fresh batch's first call."""
+    cases = batch_cases()
+    cages = [c for c, _ in cases]
+    depths = [d for _, d in cases]
+
+    def fresh(steps):
+        b = capi.Batch(cages, depths)
+        b.refine(capi.ALL)
+        r = adj(b, steps)
+        b.close()
+        return r
+
+    b = capi.Batch(cages, depths)
+    b.refine(capi.ALL)
+    gen = torch.Generator(device="cuda").manual_seed(13)
+    g = torch.randn((b.count(capi.COUNT_FINEST_POINTS), 3), generator=gen, device="cuda", dtype=torch.float32)
+
+    def adj(bb, steps):
+        out = torch.empty((bb.count(capi.COUNT_CAGE_POINTS), 3), device="cuda", dtype=torch.float32)
+        bb.refine_points_adjoint(g.data_ptr(), out.data_ptr(), steps)
+        bb.synchronize()
+        return out.cpu().numpy().tobytes()
+
+    want = {0: fresh(0), 32: fresh(32)}
+    for i, steps in enumerate([0, 0, 0, 32, 32, 0]):
+        assert adj(b, steps) == want[steps], (i, steps)
+    b.close()
+
+
 def test_batch_adjoint_against_oracle_jacobian(capi, torch):
     """The batched float adjoint of a few small cages against J^T g from the
     double oracle's basis-vector Jacobian, to 1e-5 of the gradient's scale
