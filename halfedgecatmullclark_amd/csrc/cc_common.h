@@ -49,6 +49,16 @@ static_assert(sizeof(P3) == 12, "cc_VertexPoint layout");
 __host__ __device__ __forceinline__ int4 he_load(const HeRec<16> *p, int64_t i) {
     return reinterpret_cast<const int4 *>(p)[i];
 }
+// element i < 2^28 of p: a 32-bit unsigned byte offset, so that a uniform p
+// stays in scalar registers (global loads with a scalar base)
+template <class T>
+__host__ __device__ __forceinline__ const T *at32(const T *p, uint32_t i) {
+    return reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + i * (uint32_t)sizeof(T));
+}
+template <class T>
+__host__ __device__ __forceinline__ T *at32(T *p, uint32_t i) {
+    return reinterpret_cast<T *>(reinterpret_cast<char *>(p) + i * (uint32_t)sizeof(T));
+}
 __host__ __device__ __forceinline__ int4 he_load(const HeRec<12> *p, int64_t i) {
     const int32_t *q = reinterpret_cast<const int32_t *>(p) + 3 * i;
     return make_int4(q[0], q[1], q[2], 0);

@@ -177,15 +177,15 @@ __device__ __forceinline__ void staged_children_store(int4 *stage, const int4 (&
         }
     }
     __syncthreads();
-    int4 *out4 = reinterpret_cast<int4 *>(out);
-    const int64_t obase = kStageInt4 * (int64_t)base;
-    const int64_t limit = kStageInt4 * (int64_t)H;
+    // the block's run from a block-uniform base (scalar registers), lanes at
+    // 32-bit offsets within it
+    int4 *const run = reinterpret_cast<int4 *>(out) + kStageInt4 * (int64_t)base;
+    const int64_t left = kStageInt4 * ((int64_t)H - base);   // int4s of the run inside the array
 #pragma unroll
     for (int k = 0; k < kStageInt4; ++k) {
         const uint32_t q = threadIdx.x + (uint32_t)k * NT;   // record q & 3 of thread q / 4
-        const int64_t o = obase + q;
         const int4 v = kHasUv ? stage[stage_slot(q >> 2, (int)(q & 3))] : stage[q];
-        if (o < limit) store_stream(out4 + o, v);
+        if ((int64_t)q < left) store_stream(at32(run, q), v);
     }
 }
 

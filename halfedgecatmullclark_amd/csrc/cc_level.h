@@ -25,7 +25,7 @@ __device__ __forceinline__ void level_fused_points(const FusedArgs &a, int32_t h
                                                    const P3 *spts, P3 g2, P3 g3, float s, P3 &F, P3 &Ev) {
     const int32_t twin = x.x;
     F = face_rule(q0, q1, q2, q3, GATHER);
-    if (j == 0) a.fp[h >> 2] = F;
+    if (j == 0) *at32(a.fp + (base >> 2), threadIdx.x >> 2) = F;   // quad h / 4 (base: a multiple of 256)
     if (!owner) return;
     // this halfedge owns its edge: both contributions
     const P3 v1 = pick4(q0, q1, q2, q3, (j + 1) & 3);
@@ -81,7 +81,9 @@ __device__ __forceinline__ void level_fused_points(const FusedArgs &a, int32_t h
 // valence-4 vertex), and lane 16Q + 2 adds the four.
 template <int GATHER, int CREASED>
 __device__ __forceinline__ void face_vertex_in_row(const FusedArgs &a, const LevelTab &t, int d, int32_t h,
-                                                   bool live, P3 old, P3 F, P3 E) {
+                                                   int32_t base, bool live, P3 old, P3 F, P3 E) {
+    // the vertex point of parent-level quad h / 16, from a block-uniform base
+    P3 *const vps = a.vp + ((int32_t)t.V[d - 1] + (base >> 4));
     if (!GATHER) {
         // ring lane q (offset 4q + 2): its edge point (offset 2: from 13)
         // and the previous edge's, the next ring lane's (offset 14: from 13)
@@ -99,7 +101,7 @@ __device__ __forceinline__ void face_vertex_in_row(const FusedArgs &a, const Lev
         sum = add(sum, c1);
         sum = add(sum, c2);
         sum = add(sum, c3);
-        a.vp[(int32_t)t.V[d - 1] + (p >> 2)] = sum;
+        *at32(vps, threadIdx.x >> 4) = sum;
         return;
     }
     const P3 F1 = row_ror3<12>(F), F2 = row_ror3<8>(F), F3 = row_ror3<4>(F);
@@ -124,8 +126,7 @@ __device__ __forceinline__ void face_vertex_in_row(const FusedArgs &a, const Lev
     Fan f;
     f.kind = 1; f.shift = 0; f.n = 4; f.ri = 0; f.boundary = false;
     f.a0 = h; f.a1 = f.a2 = f.a3 = 0;
-    a.vp[(int32_t)t.V[d - 1] + (p >> 2)] =
-        fan_vertex_point<GATHER, CREASED, 4>(f, FanPreloaded{en[0], en[1], en[2], en[3]}, old);
+    *at32(vps, threadIdx.x >> 4) = fan_vertex_point<GATHER, CREASED, 4>(f, FanPreloaded{en[0], en[1], en[2], en[3]}, old);
 }
 
 // uvIDs of children 1, 2, 3 of a halfedge from its own, the next, the
@@ -159,9 +160,10 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
     const bool live = h < a.H;   // H is a multiple of 4: quads are whole
     const int j = h & 3;
     const HeView he4{a.he};
-    const int4 x = live ? he4[h] : make_int4(0, 0, 0, 0);
+    // the block's records from a block-uniform base, lanes at small offsets
+    const int4 x = live ? he_load(at32(a.he + base, threadIdx.x), 0) : make_int4(0, 0, 0, 0);
     // this level's edge-representative bits of the 64 halfedges of the wave
-    const uint64_t rw = (a.erep_write && d >= 2 && live) ? a.erep[tb.R[d] + (h >> 6)] : 0ull;
+    const uint64_t rw = (a.erep_write && d >= 2 && live) ? *at32(a.erep + (tb.R[d] + (base >> 6)), threadIdx.x >> 6) : 0ull;
     const int32_t twin = x.x;
     const bool owner = live && h > twin;   // this halfedge writes its edge point
     // twin quads inside this block come from LDS, the others from memory
@@ -242,7 +244,7 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
             rh = live && edge_rep_first(cg, x.y) == h;
             rp = quad_rel<kQuadPrev>(rh ? 1 : 0) != 0;
             const uint64_t m1 = __ballot(rh);
-            if ((threadIdx.x & 63u) == 0 && live) a.erep[tb.R[1] + (h >> 6)] = m1;
+            if ((threadIdx.x & 63u) == 0 && live) *at32(a.erep + (tb.R[1] + (base >> 6)), threadIdx.x >> 6) = m1;
         } else {
             rh = ((rw >> (h & 63)) & 1ull) != 0ull;
             rp = ((rw >> (qprev(h) & 63)) & 1ull) != 0ull;
@@ -256,7 +258,8 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
         const uint32_t sh = 4u * (uint32_t)(lane & 7);
         const uint32_t lo = row_or((lane & 8) ? 0u : nib << sh);
         const uint32_t hi = row_or((lane & 8) ? nib << sh : 0u);
-        if ((lane & 15) == 15 && (h & ~15) < a.H) a.erep[tb.R[d + 1] + (h >> 4)] = ((uint64_t)hi << 32) | lo;
+        if ((lane & 15) == 15 && (h & ~15) < a.H)
+            *at32(a.erep + (tb.R[d + 1] + (base >> 4)), threadIdx.x >> 4) = ((uint64_t)hi << 32) | lo;
     }
 
     if (!POINTS) return;
@@ -271,13 +274,13 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
         // representatives (children 2 of a quad's first halfedge, h % 16 = 2)
         // leave no bit for the vertex kernel
         const bool face_here = d >= 2;
-        if (face_here) face_vertex_in_row<GATHER, CREASED>(a, stab, d, h, live, P, Fq, Ev);
+        if (face_here) face_vertex_in_row<GATHER, CREASED>(a, tb, d, h, base, live, P, Fq, Ev);
         // one bit per halfedge: is it the representative of its vertex?  Last,
         // so that the cage lookups of a few lanes overlap the wave's stores
         const bool skip = face_here && (h & 15) == 2;
         const uint64_t m =
             __ballot(live && !skip && owner_test(h, x.z, d, cg, stab, a.erep_read ? a.erep : nullptr));
-        if ((threadIdx.x & 63u) == 0 && live) a.owner_mask[h >> 6] = m;
+        if ((threadIdx.x & 63u) == 0 && live) *at32(a.owner_mask + (base >> 6), threadIdx.x >> 6) = m;
     }
 }
 
