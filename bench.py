@@ -212,8 +212,12 @@ def batched(torch, capi, C, steps: int, stream):
     offs = [b.offset(capi.COUNT_FINEST_POINTS, i) for i in range(len(cages))]
     b.close()
     torch.cuda.empty_cache()
+    # the plain side: each cage refined by its own launch sequence
+    # (ccd_Refine kernel by kernel); the same cages replaying a captured
+    # graph each are reported beside it
     subs = [capi.DeviceSubd(c, d) for c, d in zip(cages, depths)]
     for s_ in subs:
+        s_.set_option(capi.OPT_GRAPH, 0)
         s_.refine(capi.ALL, sptr)
     gslices = [g[o:] for o in offs]
     gcs = [torch.empty((c.V, 3), device="cuda", dtype=torch.float32) for c in cages]
@@ -229,11 +233,15 @@ def batched(torch, capi, C, steps: int, stream):
     p_fwd = run(plain_fwd, k)
     p_adj = run(plain_adj, k)
     for s_ in subs:
+        s_.set_option(capi.OPT_GRAPH, 1)
+    p_fwd_graphs = run(plain_fwd, k)
+    for s_ in subs:
         s_.close()
     return {"workload": "256 cages (cube/torus/irregular generators, halfedges log-uniform in [24, 60000], "
                         "depths uniform in {4,5,6}, 5% of edges creased U[0,4], seed 0)",
             "refined_halfedges": total, "cage_generation_s": gen_s, "autotune_s": tune_s,
             "batched_fwd_ms": b_fwd, "plain_fwd_ms": p_fwd, "fwd_speedup": p_fwd / b_fwd,
+            "plain_fwd_graph_per_cage_ms": p_fwd_graphs,
             "batched_fwd_value": total / (b_fwd * 1e-3),
             "batched_adj_ms": b_adj, "plain_adj_ms": p_adj, "adj_speedup": p_adj / b_adj,
             "batched_fwd_adj_ms": b_fwd + b_adj, "plain_fwd_adj_ms": p_fwd + p_adj,

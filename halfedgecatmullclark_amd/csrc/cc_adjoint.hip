@@ -464,36 +464,6 @@ __device__ __forceinline__ void adj_gather_fan(const AdjArgs<T> &a, const DCage 
     reinterpret_cast<V3<T> *>(a.gP)[v] = {alpha * gv.x + acc.x, alpha * gv.y + acc.y, alpha * gv.z + acc.z};
 }
 
-// The owners of level-d vertices among 256 * SCAN halfedges (structural
-// test, no scan of the records), collected in LDS so that every thread then
-// works on one, in halfedge order (neighbouring rings, shared cache lines).
-// The level tables go to LDS first; the first barrier orders the copy.
-template <int SCAN, class Fn, class LP>
-__device__ __forceinline__ void owners_block(int32_t H, int d, const DCage &c, const LevelTab &t, const LP &lp,
-                                             uint32_t lb, const uint64_t *erep, const Fn &fn) {
-    __shared__ int32_t owners[256 * SCAN];
-    __shared__ int32_t nown;
-    __shared__ LevelTab st;
-    __shared__ LevelPtrs slp;
-    lds_fill(st, t);
-    lds_fill(slp, lp);
-    const int32_t base = (int32_t)(lb * 256 * SCAN);
-    if (threadIdx.x == 0) nown = 0;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < SCAN; ++k) {
-        const int32_t h = base + k * 256 + (int32_t)threadIdx.x;
-        if (h < H && structural_owner(h, d, c, st, slp.he, erep)) owners[atomicAdd(&nown, 1)] = h;
-    }
-    __syncthreads();
-    const int32_t count = nown;
-    for (int32_t i = (int32_t)threadIdx.x; i < count; i += 256) {
-        Fan f;
-        const int32_t v = marked_fan(owners[i], d, c, st, slp.he, f);
-        fn(st, f, v);
-    }
-}
-
 constexpr int kAdjScan = 4;
 
 // Passes 1 and 3 with one thread per vertex and the fan from the vertex ID

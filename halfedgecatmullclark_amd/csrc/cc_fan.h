@@ -515,6 +515,36 @@ __device__ __forceinline__ int32_t marked_fan(int32_t h, int d, const DCage &c, 
     return v;
 }
 
+// The owners of level-d vertices among 256 * SCAN halfedges (structural
+// test, no scan of the records), collected in LDS so that every thread then
+// works on one, in halfedge order (neighbouring rings, shared cache lines).
+// The level tables go to LDS first; the first barrier orders the copy.
+template <int SCAN, class Fn, class LP>
+__device__ __forceinline__ void owners_block(int32_t H, int d, const DCage &c, const LevelTab &t, const LP &lp,
+                                             uint32_t lb, const uint64_t *erep, const Fn &fn) {
+    __shared__ int32_t owners[256 * SCAN];
+    __shared__ int32_t nown;
+    __shared__ LevelTab st;
+    __shared__ LevelPtrs slp;
+    lds_fill(st, t);
+    lds_fill(slp, lp);
+    const int32_t base = (int32_t)(lb * 256 * SCAN);
+    if (threadIdx.x == 0) nown = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SCAN; ++k) {
+        const int32_t h = base + k * 256 + (int32_t)threadIdx.x;
+        if (h < H && structural_owner(h, d, c, st, slp.he, erep)) owners[atomicAdd(&nown, 1)] = h;
+    }
+    __syncthreads();
+    const int32_t count = nown;
+    for (int32_t i = (int32_t)threadIdx.x; i < count; i += 256) {
+        Fan f;
+        const int32_t v = marked_fan(owners[i], d, c, st, slp.he, f);
+        fn(st, f, v);
+    }
+}
+
 
 // The fan of vertex v of level d >= 1, equal to what owner_fan gives its
 // representative (CatmullClark.h:1336): v was born at the level b with

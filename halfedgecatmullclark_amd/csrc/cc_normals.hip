@@ -44,7 +44,104 @@ __global__ __launch_bounds__(256) void k_corner_normals(const He *__restrict__ h
     if (live) out[h] = normalize3(cross3(sub3(v2, v0), sub3(v1, v0)));
 }
 
-// vertex normals from the corner normals
+// One owner per level-d vertex, decided from the child numbering as the
+// structural owner test does, except that an edge vertex born at level >= 2
+// is owned by child 1 of the larger parent halfedge of its edge (or the
+// parent on a boundary): one parent twin read instead of the edge's
+// representative descent.  Any owner gives the same fan as a cyclic list.
+__device__ __forceinline__ bool normal_owner(int32_t h, int d, const DCage &c, const He *const *levels) {
+    const int k = zero_digits(h, d);
+    const int32_t q = h >> (2 * k);
+    const int lvl = d - k;
+    const int digit = q & 3;
+    const int32_t p = q >> 2;
+    if (digit == 3) return false;
+    if (lvl >= 2) {
+        if (digit == 2) return (p & 3) == 0;
+        const int32_t tw = levels[lvl - 1][p].twin;
+        return tw < 0 || p > tw;
+    }
+    const CageHe x = c.he[p];
+    if (digit == 0) return c.v2h[x.vert] == p;
+    if (digit == 2) return c.f2h[x.face] == p;
+    return c.e2h[x.edge] == p;
+}
+
+// Vertex normals of every halfedge of a vertex's fan, from its owner.  The
+// fan lists the ring in the walk order twin(prev(x)), so the walk from
+// member i visits members i+1, ..., i-1: its normal is
+// normalize(c_i + ((0 + c_{i+1}) + ... + c_{i-1})), the per-halfedge walk's
+// own sums.  On a boundary every member's walk ends at the boundary and its
+// normal is its corner normal.
+struct VertexNormalsFn {
+    const P3 *corner;
+    P3 *out;
+    const DCage &c;
+    __device__ void operator()(const Fan &f) const {
+        const int n = f.n;
+        if (f.boundary) {
+            for (int i = 0; i < n; ++i) {
+                const int32_t r = fan_at(f, c, i);
+                out[r] = corner[r];
+            }
+            return;
+        }
+        if (n <= 4) {
+            int32_t r[4];
+            P3 cn[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) r[q] = fan_at(f, c, q < n ? q : 0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cn[q] = corner[r[q]];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (i >= n) break;
+                P3 avg = zero3();
+#pragma unroll
+                for (int k = 1; k < 4; ++k)
+                    if (k < n) avg = add(avg, pick4(cn[0], cn[1], cn[2], cn[3], (i + k) % n));
+                out[r[i]] = normalize3(add(cn[i], avg));
+            }
+            return;
+        }
+        for (int i = 0; i < n; ++i) {
+            const int32_t ri = fan_at(f, c, i);
+            P3 avg = zero3();
+            for (int k = 1; k < n; ++k) avg = add(avg, corner[fan_at(f, c, (i + k) % n)]);
+            out[ri] = normalize3(add(corner[ri], avg));
+        }
+    }
+};
+
+// vertex normals by owner: a block lists the owners among its 1024
+// halfedges in LDS (halfedge order), then each thread finishes one vertex
+__global__ __launch_bounds__(256) void k_vertex_normals_owners(DCage c, LevelTab t, LevelPtrs lp, int d, int32_t H,
+                                                               const P3 *__restrict__ corner, P3 *__restrict__ out) {
+    __shared__ int32_t owners[1024];
+    __shared__ int32_t nown;
+    __shared__ LevelTab st;
+    __shared__ LevelPtrs slp;
+    lds_fill(st, t);
+    lds_fill(slp, lp);
+    const int32_t base = (int32_t)(xcd_block(blockIdx.x, gridDim.x, 1) * 1024u);
+    if (threadIdx.x == 0) nown = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int32_t h = base + k * 256 + (int32_t)threadIdx.x;
+        if (h < H && normal_owner(h, d, c, slp.he)) owners[atomicAdd(&nown, 1)] = h;
+    }
+    __syncthreads();
+    const int32_t count = nown;
+    const VertexNormalsFn fn{corner, out, c};
+    for (int32_t i = (int32_t)threadIdx.x; i < count; i += 256) {
+        Fan f;
+        (void)marked_fan(owners[i], d, c, st, slp.he, f);
+        fn(f);
+    }
+}
+
+// vertex normals from the corner normals, one walk per halfedge
 __global__ __launch_bounds__(256) void k_vertex_normals(const He *__restrict__ he, const P3 *__restrict__ corner,
                                                         int32_t H, P3 *__restrict__ out) {
     const int32_t h = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -75,9 +172,14 @@ bool ccd_Subd::normals(P3 *out, bool smooth, hipStream_t st) {
         corner = nrm_tmp;
     }
     hipLaunchKernelGGL(k_corner_normals, dim3(blocks_for(H)), dim3(256), 0, st, lhe, lpts, (int32_t)H, corner);
-    if (smooth)
+    if (smooth && D >= 1 && rings_ok) {
+        // by owner: each ring's corner normals read once
+        hipLaunchKernelGGL(k_vertex_normals_owners, dim3((unsigned)((H + 1023) / 1024)), dim3(256), 0, st,
+                           cage_view(nullptr), tab, level_ptrs(), D, (int32_t)H, (const P3 *)corner, out);
+    } else if (smooth) {
         hipLaunchKernelGGL(k_vertex_normals, dim3(blocks_for(H)), dim3(256), 0, st, lhe, (const P3 *)corner,
                            (int32_t)H, out);
+    }
     return !hip_failed(hipGetLastError(), "halfedge normals");
 }
 
