@@ -618,7 +618,10 @@ bool ccb_Batch::run(int32_t key, bool adjoint, int32_t steps, hipStream_t st) {
         while (p + 1 < Dmax && cfg[p + 1].kind == 1) ++p;
         if (p >= 1) erep_kept = std::max(erep_kept, p + 1);
     }
-    if (!use_graph || timing) {
+    // a caller capturing its own stream gets the launches in its capture
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (st) CC_TRY(hipStreamIsCapturing(st, &cap), "hipStreamIsCapturing");
+    if (!use_graph || timing || cap != hipStreamCaptureStatusNone) {
         if (timing) begin_timing();   // the timings are those of the last run
         if (adjoint) record_adjoint(steps, st);
         else record_forward(steps, st);

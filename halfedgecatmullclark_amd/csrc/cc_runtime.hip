@@ -837,7 +837,11 @@ int32_t ccd_Refine(ccd_Subd *s, int32_t steps, void *stream) {
     if (!s) { set_error(11, "ccd_Refine: null subd"); return t_err; }
     DeviceGuard g(s->device);
     hipStream_t st = (hipStream_t)stream;
-    const bool ok = (s->opt_graph && !s->timing) ? s->refine_graph(steps, st) : s->refine(steps, s->D, st);
+    // a caller capturing its own stream gets the launches in its capture
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (st && hip_failed(hipStreamIsCapturing(st, &cap), "ccd_Refine")) return t_err;
+    const bool replay = s->opt_graph && !s->timing && cap == hipStreamCaptureStatusNone;
+    const bool ok = replay ? s->refine_graph(steps, st) : s->refine(steps, s->D, st);
     return ok ? 0 : (t_err ? t_err : 1);
 }
 

@@ -187,6 +187,44 @@ def test_single_cage_graph_replays_match_oracle(capi, torch, label, steps, flags
     ds.close()
 
 
+def test_refine_inside_a_caller_graph_capture(capi, torch):
+    """ccd_Refine and ccb_Refine on a stream the caller is capturing put their
+    launches into the caller's graph (no capture of their own): the torch
+    graph, replayed with eager work and new cage points between replays,
+    equals the oracle each time."""
+    cage, depth = C.torus(6, 5, seed=4, ring_sharpness=((0, 1.3), (2, 0.4))), 4
+    rng = np.random.default_rng(21)
+    ds = capi.DeviceSubd(cage, depth)
+    b = capi.Batch([cage], [depth])
+    ds.refine(capi.ALL)
+    b.refine(capi.ALL)               # warm-up outside the capture (tables, first launches)
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        ds.refine(capi.ALL, s.cuda_stream)
+        b.refine(capi.ALL, s.cuda_stream)
+    for rep in range(3):
+        x = torch.randn(1 << 16, device="cuda")
+        _ = (x * 2).sum().item()
+        if rep:
+            cage.vertex_points = (cage.vertex_points + rng.normal(0, 0.05, cage.vertex_points.shape)).astype(np.float32)
+            ds.upload_cage_points(cage.vertex_points)
+        g.replay()
+        torch.cuda.synchronize()
+        want = O.refine(cage, depth, O.ALL)
+        got = ds.download()
+        for k in ("halfedges", "creases", "points"):
+            assert_bytes_equal(got[k], want[k], f"ccd replay {rep} {k}")
+        if rep == 0:   # the batch's cage points were not moved
+            gb = b.download(0)
+            for k in ("halfedges", "creases", "points"):
+                assert_bytes_equal(gb[k], want[k], f"ccb replay {rep} {k}")
+    del g
+    ds.close()
+    b.close()
+
+
 def test_batch_refine_points_and_finest(capi, torch):
     cases = batch_cases()
     cages = [c for c, _ in cases]
