@@ -28,7 +28,9 @@ Also reported on the same line:
   - forward + adjoint against forward alone on the 32x16 torus at depth 5
     (the cage-fitting workload);
   - roofline of the dominant kernel (per-kernel HIP events on the launch
-    stream over the timed steps, algorithmic bytes per launch);
+    stream over the timed steps, algorithmic bytes per launch), also as a
+    fraction of what torch's device copy reaches on the same GPU;
+  - streaming ceilings: torch's device copy and fill of 2 GiB buffers;
   - the reference's own OpenMP CPU path (CatmullClark.h built by
     oracle/Makefile) timed on this host's cores on the same workload
     (torus 64x32 to depth 8), per stage and in total.
@@ -485,6 +487,17 @@ def main():
     sub.close()
 
     ds.close()
+    # what plain streaming kernels reach on this GPU, for comparison with the
+    # level kernels: torch's device copy (2 GiB read + 2 GiB written) and
+    # fill (2 GiB written)
+    ca = torch.ones(512 * 1024 * 1024, dtype=torch.float32, device=dev)
+    cb = torch.empty_like(ca)
+    copy_ms = ev_time(lambda: cb.copy_(ca), 10)
+    fill_ms = ev_time(lambda: cb.fill_(1.0), 10)
+    nb = ca.numel() * 4
+    ceilings = {"copy_gbs": 2 * nb / (copy_ms * 1e-3) / 1e9, "fill_gbs": nb / (fill_ms * 1e-3) / 1e9,
+                "bytes_per_buffer": nb}
+    del ca, cb
     if args.copy_calibration:
         a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
@@ -533,7 +546,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": f"{name} (level {level} -> {level + 1})",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
-                     "kernel_ms": mean_ms, "algorithmic_bytes": nbytes},
+                     "kernel_ms": mean_ms, "algorithmic_bytes": nbytes,
+                     "frac_of_copy": (achieved / ceilings["copy_gbs"]) if achieved else None},
+        "streaming_ceilings": ceilings,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
