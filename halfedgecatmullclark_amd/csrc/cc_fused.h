@@ -52,6 +52,9 @@ struct FusedArgs {
     // ancestry.
     uint64_t *erep = nullptr;
     int erep_read = 0, erep_write = 0;
+    // 0: owner_mask already holds this level's bits (a points-only pass over
+    // an unchanged topology), the fused kernel leaves them
+    int owner_write = 1;
 };
 
 // topo: write child twin/edge/vertex IDs; uvMode: 0 leave uvIDs, 1 refine

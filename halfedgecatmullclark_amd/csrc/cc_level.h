@@ -277,10 +277,12 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
         if (face_here) face_vertex_in_row<GATHER, CREASED>(a, tb, d, h, base, live, P, Fq, Ev);
         // one bit per halfedge: is it the representative of its vertex?  Last,
         // so that the cage lookups of a few lanes overlap the wave's stores
-        const bool skip = face_here && (h & 15) == 2;
-        const uint64_t m =
-            __ballot(live && !skip && owner_test(h, x.z, d, cg, stab, a.erep_read ? a.erep : nullptr));
-        if ((threadIdx.x & 63u) == 0 && live) *at32(a.owner_mask + (base >> 6), threadIdx.x >> 6) = m;
+        if (a.owner_write) {
+            const bool skip = face_here && (h & 15) == 2;
+            const uint64_t m =
+                __ballot(live && !skip && owner_test(h, x.z, d, cg, stab, a.erep_read ? a.erep : nullptr));
+            if ((threadIdx.x & 63u) == 0 && live) *at32(a.owner_mask + (base >> 6), threadIdx.x >> 6) = m;
+        }
     }
 }
 

@@ -54,6 +54,12 @@ struct ccd_Subd {
     int adj_vp_key = -1;
     // deferred-vertex masks of the patch kernels, one word per 64 halfedges
     uint64_t *defer_mask = nullptr;
+    // owner bits of levels 1..D-1 (FusedArgs::owner_mask; level d from word
+    // own_off[d]), and which levels hold the bits of the current cage's
+    // topology: points-only passes reuse them
+    uint64_t *own_masks = nullptr;
+    int64_t own_off[cch::kMaxDepth + 1] = {};
+    bool own_valid[cch::kMaxDepth + 1] = {};
     // edge-representative bits of levels 1..D-2 (FusedArgs::erep; the owner
     // test of level d reads levels up to d-1), and which levels hold bits of
     // the current cage's topology

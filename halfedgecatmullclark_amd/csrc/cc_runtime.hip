@@ -93,13 +93,19 @@ bool ccd_Subd::alloc(const cc_Mesh *cage, int32_t depth) {
     CC_TRY(hipMalloc(&defer_mask, sizeof(uint64_t) * (size_t)(D >= 2 ? (n.H[D - 1] + 63) / 64 : 1)),
            "hipMalloc(deferred-vertex masks)");
     if (D >= 4) CC_TRY(hipMalloc(&erep, sizeof(uint64_t) * (size_t)tab.R[D - 1]), "hipMalloc(edge representative bits)");
+    int64_t words = 0;
+    for (int d = 1; d < D; ++d) {
+        own_off[d] = words;
+        words += (n.H[d] + 63) / 64;
+    }
+    CC_TRY(hipMalloc(&own_masks, sizeof(uint64_t) * (size_t)(words > 0 ? words : 1)), "hipMalloc(owner bits)");
     return true;
 }
 
 void ccd_Subd::free_all() {
     DeviceGuard g(device);
     void *ptrs[] = {c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, he, cr, pts, adj_a, adj_b, adj_w, adj_in, adj_out, adj_sharp,
-                    adj_vp, defer_mask, rings, rare, erep, nrm_tmp, nrm_out};
+                    adj_vp, defer_mask, own_masks, rings, rare, erep, nrm_tmp, nrm_out};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &t : timed) {
@@ -123,6 +129,7 @@ bool ccd_Subd::upload_cage(const cc_Mesh *cage, hipStream_t st) {
         return false;
     }
     for (bool &v : erep_valid) v = false;   // bits of the previous topology
+    for (bool &v : own_valid) v = false;
     adj_vp_key = -1;
     drop_graphs();
     CC_TRY(hipMemcpyAsync(c_he, cage->halfedges, sizeof(CageHe) * (size_t)H0, hipMemcpyHostToDevice, st), "upload cage");
@@ -324,9 +331,12 @@ void ccd_Subd::level_fused(int d, int32_t steps, int topo, int uvMode, P3 *point
     const bool pts_ = (steps & CCD_POINTS) != 0;
     FusedArgs a = fused_args(d, points);
     const DCage c = cage_view(nullptr);
-    // the fused kernel marks the vertex representatives for the masked kernel
+    // the fused kernel marks the vertex representatives for the masked
+    // kernel (the bits depend on the topology only: a points-only pass keeps
+    // the bits of an earlier pass over the same cage)
     if (pts_ && opt_fan == 2 && cfg[d].scan == 0) {
-        a.owner_mask = defer_mask;
+        a.owner_mask = own_masks + own_off[d];
+        a.owner_write = own_valid[d] && !topo ? 0 : 1;
         a.rare = rare;
         a.rare_cap = rare_cap;
     }
@@ -349,6 +359,7 @@ void ccd_Subd::level_fused(int d, int32_t steps, int topo, int uvMode, P3 *point
             erep_valid[d + 1] = true;
             if (d == 1) erep_valid[1] = true;
         }
+        if (a.owner_mask) own_valid[d] = true;
     }
     if (pts_) {
         launch_timed("level_vertices", n.H[d], st, [&] {
@@ -439,6 +450,8 @@ bool ccd_Subd::refine_graph(int32_t steps, hipStream_t st) {
     int64_t key = (uint32_t)steps;
     for (int l = 1; l <= kMaxDepth; ++l)
         if (erep_valid[l]) key |= (int64_t)1 << (32 + l);
+    for (int l = 1; l <= kMaxDepth; ++l)
+        if (own_valid[l]) key |= (int64_t)1 << (48 + l);
     auto it = graphs.find(key);
     if (it == graphs.end()) {
         hipGraph_t g = nullptr;

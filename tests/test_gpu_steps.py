@@ -133,3 +133,29 @@ def test_cube_depth6(capi):
         assert got["counts"]["H"][6] == 98304
         for k in ("halfedges", "creases", "points"):
             assert_bytes_equal(got[k], want[k], f"cube depth 6 {drv} {k}")
+
+
+@pytest.mark.parametrize("name,cage,depth", small_cages(), ids=[c[0] for c in small_cages()])
+def test_device_points_only_passes(capi, name, cage, depth):
+    """ccd_RefinePoints after one full ccd_Refine: points-only passes over
+    the same topology (later passes keep the owner bits of the vertex
+    kernel from the first) with moved cage points, the scatter, gather and
+    uncreased rules interleaved; every pass equals the oracle on the moved
+    cage bit for bit."""
+    import torch
+    st = torch.cuda.current_stream().cuda_stream
+    ds = capi.DeviceSubd(cage, depth)
+    try:
+        ds.refine(capi.ALL, st)
+        out = torch.empty((ds.n["Vcum"][depth], 3), device="cuda")
+        passes = [(0, O.ALL), (capi.GATHER, O.ALL | O.GATHER), (0, O.ALL), (capi.NOCREASES, O.ALL | O.NOCREASES),
+                  (0, O.ALL)]
+        for i, (steps, flags) in enumerate(passes):
+            c2 = moved(cage, 11 + i)
+            cp = torch.from_numpy(np.ascontiguousarray(c2.vertex_points, np.float32)).cuda()
+            ds.refine_points(cp.data_ptr(), out.data_ptr(), steps, st)
+            got = out.cpu().numpy()
+            want = O.refine(c2, depth, flags)["points"]
+            assert_bytes_equal(got, want, f"{name} points-only pass {i}")
+    finally:
+        ds.close()
