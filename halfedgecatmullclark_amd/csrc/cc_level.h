@@ -14,43 +14,63 @@
 
 namespace cch {
 
-// face point of h's quad (lane 0 of the quad) and, for the larger halfedge
-// of each pair, the edge point
-// (g2, g3: the points of the twin quad's two corners away from the edge when
-// that quad lies outside the block, s: the sharpness of h's edge for an owner;
-// F and Ev receive the face point and, for an owner, the edge point)
+// This halfedge's side of its edge, which the lanes of the twin's quad read
+// from LDS when the twin owns the edge: the face point of h's quad (gather
+// rules) or h's own scatter contribution to the edge point, with v1 the
+// point of the next halfedge's vertex and s the edge's sharpness.
 template <int GATHER, int CREASED>
-__device__ __forceinline__ void level_fused_points(const FusedArgs &a, int32_t h, int j, int4 x, P3 P, P3 q0, P3 q1,
-                                                   P3 q2, P3 q3, bool owner, bool twin_here, int32_t base,
-                                                   const P3 *spts, P3 g2, P3 g3, float s, P3 &F, P3 &Ev) {
+__device__ __forceinline__ P3 edge_side(P3 P, P3 v1, P3 F, float s) {
+    return GATHER ? F : edge_contrib(P, v1, F, false, s, CREASED);
+}
+
+// face point F of h's quad (stored by lane 0 of the quad) and, for the
+// larger halfedge of each pair, the edge point Ev; mine: this lane's
+// edge_side.  The twin's side: SIDES, spts holds every lane's edge_side and
+// a twin inside the block gives its own; else spts holds every lane's vertex
+// point and a twin quad inside the block gives its corners, from which its
+// face point follows as for a twin quad outside the block (g2, g3: the
+// points of its two corners away from the edge, loaded by the caller).
+template <int GATHER, int CREASED, bool SIDES>
+__device__ __forceinline__ void level_fused_points(const FusedArgs &a, int32_t h, int4 x, P3 P, P3 v1, P3 F, P3 mine,
+                                                   bool owner, bool twin_here, int32_t base, const P3 *spts, P3 g2,
+                                                   P3 g3, float s, P3 &Ev) {
     const int32_t twin = x.x;
-    F = face_rule(q0, q1, q2, q3, GATHER);
-    if (j == 0) *at32(a.fp + (base >> 2), threadIdx.x >> 2) = F;   // quad h / 4 (base: a multiple of 256)
+    if ((h & 3) == 0) *at32(a.fp + (base >> 2), threadIdx.x >> 2) = F;   // quad h / 4 (base: a multiple of 256)
     if (!owner) return;
-    // this halfedge owns its edge: both contributions
-    const P3 v1 = pick4(q0, q1, q2, q3, (j + 1) & 3);
+    // this halfedge owns its edge: both sides
     if (twin >= 0) {
-        // The twin quad's corners, from the twin on: the twin's vertex is the
-        // head of h (v1), the twin's next vertex is h's own (P), and the other
-        // two come from LDS when the quad is in this block, else from the
-        // loads above.  The LDS reads are valid for every lane; selecting
-        // afterwards keeps LDS and global reads in their own instructions.
-        const int tj = twin & 3;
-        const int tb = twin_here ? (twin & ~3) - base : 0;
-        const P3 l2 = sld(spts + tb + ((tj + 2) & 3)), l3 = sld(spts + tb + ((tj + 3) & 3));
-        const P3 o2 = twin_here ? l2 : g2, o3 = twin_here ? l3 : g3;
-        // corner k of the twin quad is corner (k - tj) & 3 counted from the
-        // twin: rotate by tj in two steps (by 1, then by 2)
-        const bool r1 = (tj & 1) != 0, r2 = (tj & 2) != 0;
-        const P3 y0 = r1 ? o3 : v1, y1 = r1 ? v1 : P, y2 = r1 ? P : o2, y3 = r1 ? o2 : o3;
-        const P3 Ft = face_rule(r2 ? y2 : y0, r2 ? y3 : y1, r2 ? y0 : y2, r2 ? y1 : y3, GATHER);
+        P3 Ft = zero3(), ct = zero3();   // the twin quad's face point (gather), the twin's contribution (scatter)
+        if (SIDES && twin_here) {
+            const P3 o = sld(spts + (twin - base));
+            if (GATHER) Ft = o;
+            else ct = o;
+        } else {
+            // The twin quad's corners, from the twin on: the twin's vertex
+            // is the head of h (v1), the twin's next vertex is h's own (P),
+            // the other two come from LDS when the quad is in this block
+            // (the reads are valid for every lane; selecting afterwards
+            // keeps LDS and global reads in their own instructions), else
+            // from the loads above.  Corner k of the twin quad is corner
+            // (k - tj) & 3 counted from the twin: rotate by tj in two steps
+            // (by 1, then by 2).
+            const int tj = twin & 3;
+            P3 o2 = g2, o3 = g3;
+            if (!SIDES) {
+                const int tb = twin_here ? (twin & ~3) - base : 0;
+                const P3 l2 = sld(spts + tb + ((tj + 2) & 3)), l3 = sld(spts + tb + ((tj + 3) & 3));
+                o2 = twin_here ? l2 : g2;
+                o3 = twin_here ? l3 : g3;
+            }
+            const bool r1 = (tj & 1) != 0, r2 = (tj & 2) != 0;
+            const P3 y0 = r1 ? o3 : v1, y1 = r1 ? v1 : P, y2 = r1 ? P : o2, y3 = r1 ? o2 : o3;
+            Ft = face_rule(r2 ? y2 : y0, r2 ? y3 : y1, r2 ? y0 : y2, r2 ? y1 : y3, GATHER);
+            if (!GATHER) ct = edge_contrib(v1, P, Ft, false, s, CREASED);
+        }
         if (GATHER) {
             const float w = CREASED ? sat(s) : 1.0f;
             Ev = edge_gather(P, v1, F, Ft, w, CREASED);
         } else {
-            const P3 ct = edge_contrib(v1, P, Ft, false, s, CREASED);
-            const P3 ch = edge_contrib(P, v1, F, false, s, CREASED);
-            Ev = add(add(zero3(), ct), ch);
+            Ev = add(add(zero3(), ct), mine);
         }
     } else {
         if (GATHER) {
@@ -158,7 +178,6 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
     const int32_t base = (int32_t)(lb * NT);   // NT threads per block, one tile of NT halfedges
     const int32_t h = base + (int32_t)threadIdx.x;
     const bool live = h < a.H;   // H is a multiple of 4: quads are whole
-    const int j = h & 3;
     const HeView he4{a.he};
     // the block's records from a block-uniform base, lanes at small offsets
     const int4 x = live ? he_load(at32(a.he + base, threadIdx.x), 0) : make_int4(0, 0, 0, 0);
@@ -166,24 +185,31 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
     const uint64_t rw = (a.erep_write && d >= 2 && live) ? *at32(a.erep + (tb.R[d] + (base >> 6)), threadIdx.x >> 6) : 0ull;
     const int32_t twin = x.x;
     const bool owner = live && h > twin;   // this halfedge writes its edge point
-    // twin quads inside this block come from LDS, the others from memory
+    // twins inside this block give their side of the edge (or their quad's
+    // corners) through LDS, the others' quads come from memory
     const bool twin_here = owner && twin >= base && twin < base + NT;
+    // Without topology work the sides go to LDS in the prologue, and the
+    // barrier before the point work follows at once.  With it, computing
+    // them before the child stores lengthens the path to those stores (the
+    // level-7 fused kernel 2.43 -> 2.62 ms), so LDS holds the corner points.
+    constexpr bool kSides = POINTS && !TOPO && !UVMODE;
     const bool twin_far = owner && twin >= 0 && !twin_here;
 
     // the loads of the point work come first: this halfedge's vertex point,
     // its edge's sharpness and, for a twin quad outside the block, the
     // vertex IDs of its two corners away from the shared edge and then their
     // points, so that the latency of the second gather overlaps the topology
-    // stores below.  The vertex point goes to LDS between the two gathers:
-    // the wait for the IDs (forced inside their branch by the empty asm that
-    // reads them) covers it, and no later wait has to count the twin-quad
-    // point loads still in flight (memory loads complete in order).
-    P3 P = zero3(), g2 = zero3(), g3 = zero3();
+    // stores below (the empty asm that reads the IDs keeps their wait inside
+    // the branch).  With kSides this halfedge's side of its edge then goes
+    // to LDS; else its vertex point, between the two gathers: the wait for
+    // the IDs covers it, and no later wait has to count the twin-quad point
+    // loads still in flight (memory loads complete in order).
+    P3 P = zero3(), g2 = zero3(), g3 = zero3(), v1 = zero3(), F = zero3(), mine = zero3();
     float s = 0.0f;
     if (POINTS) {
         P = a.old[x.z];   // a dead lane's record is all zeros: vertex 0
-        if (CREASED) {
-            const bool has = owner && x.y < a.C;   // a level's crease array is never empty
+        if (CREASED) {   // both halves of an edge with kSides, else its owner
+            const bool has = (kSides ? live : owner) && x.y < a.C;   // a level's crease array is never empty
             const float cs = a.cr[has ? x.y : 0].sharp;
             s = has ? cs : 0.0f;
         }
@@ -194,9 +220,19 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
             v3 = he4[tq + ((tj + 3) & 3)].z;
             asm volatile("" ::"v"(v2), "v"(v3));
         }
-        spts[threadIdx.x] = P;
-        if (twin_far) {
-            g2 = gld(a.old + v2); g3 = gld(a.old + v3);
+        if (kSides) {
+            if (twin_far) {
+                g2 = gld(a.old + v2); g3 = gld(a.old + v3);
+            }
+            v1 = P3{quad_relf<kQuadNext>(P.x), quad_relf<kQuadNext>(P.y), quad_relf<kQuadNext>(P.z)};
+            F = face_rule(quad_bcast3<0>(P), quad_bcast3<1>(P), quad_bcast3<2>(P), quad_bcast3<3>(P), GATHER);
+            mine = edge_side<GATHER, CREASED>(P, v1, F, s);
+            spts[threadIdx.x] = mine;
+        } else {
+            spts[threadIdx.x] = P;
+            if (twin_far) {
+                g2 = gld(a.old + v2); g3 = gld(a.old + v3);
+            }
         }
     }
 
@@ -264,17 +300,21 @@ __device__ __forceinline__ void level_fused_block(const FusedArgs &a, const DCag
 
     if (!POINTS) return;
     if (!kStaged) __syncthreads();
-    const P3 q0 = quad_bcast3<0>(P), q1 = quad_bcast3<1>(P), q2 = quad_bcast3<2>(P), q3 = quad_bcast3<3>(P);
-    P3 Fq = zero3(), Ev = zero3();
+    if (!kSides) {
+        v1 = P3{quad_relf<kQuadNext>(P.x), quad_relf<kQuadNext>(P.y), quad_relf<kQuadNext>(P.z)};
+        F = face_rule(quad_bcast3<0>(P), quad_bcast3<1>(P), quad_bcast3<2>(P), quad_bcast3<3>(P), GATHER);
+        mine = edge_side<GATHER, CREASED>(P, v1, F, s);
+    }
+    P3 Ev = zero3();
     if (live)
-        level_fused_points<GATHER, CREASED>(a, h, j, x, P, q0, q1, q2, q3, owner, twin_here, base, spts, g2, g3, s, Fq,
-                                            Ev);
+        level_fused_points<GATHER, CREASED, kSides>(a, h, x, P, v1, F, mine, owner, twin_here, base, spts, g2, g3, s,
+                                                    Ev);
     if (a.owner_mask) {
         // the face vertices born at this level are finished here: their
         // representatives (children 2 of a quad's first halfedge, h % 16 = 2)
         // leave no bit for the vertex kernel
         const bool face_here = d >= 2;
-        if (face_here) face_vertex_in_row<GATHER, CREASED>(a, tb, d, h, base, live, P, Fq, Ev);
+        if (face_here) face_vertex_in_row<GATHER, CREASED>(a, tb, d, h, base, live, P, F, Ev);
         // one bit per halfedge: is it the representative of its vertex?  Last,
         // so that the cage lookups of a few lanes overlap the wave's stores
         if (a.owner_write) {
