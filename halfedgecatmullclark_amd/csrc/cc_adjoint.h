@@ -38,17 +38,21 @@ struct AdjArgs {
     T *W;
     VParam<T> *VP;
     T *gEu;
+    // the face gradients after the quad pass (3 F scalars), or null: the
+    // quad pass then updates gF in place
+    T *gFu = nullptr;
     // edge-representative bits of levels 1..d-1 (FusedArgs::erep), or null
     const uint64_t *erep = nullptr;
 };
 
 // scalars of adjoint scratch (AdjArgs::W) a cage refined to depth D needs:
-// the cage step's weights and every level's parameters and edge gradients,
-// a multiple of 4 so that consecutive cages' scratch stays 16-byte aligned
-inline int64_t adj_scratch_scalars(const Counts &n, int D) {
+// the cage step's weights and every level's parameters and edge gradients
+// (face_out: and its face gradients, AdjArgs::gFu), a multiple of 4 so that
+// consecutive cages' scratch stays 16-byte aligned
+inline int64_t adj_scratch_scalars(const Counts &n, int D, bool face_out = false) {
     int64_t m = 4 * n.H[0];
     for (int d = 1; d < D; ++d) {
-        const int64_t need = 4 * n.V[d] + 3 * n.E[d];
+        const int64_t need = 4 * n.V[d] + 3 * n.E[d] + (face_out ? 3 * n.F[d] : 0);
         if (need > m) m = need;
     }
     return (m + 3) & ~int64_t(3);
@@ -72,9 +76,9 @@ struct AdjPlan {
     bool vp_valid = false;
 };
 
-// dL/d(cage points) from dL/d(level-D points): V_D x 3 in, V_0 x 3 out.
-// bufA holds V_D x 3 and bufB V_{D-1} x 3 scalars of scratch, W
-// adj_scratch_scalars(n, D).
+// dL/d(cage points) from dL/d(level-D points): V_D x 3 in (only read),
+// V_0 x 3 out.  bufA holds V_{D-2} x 3 scalars of scratch (V_D x 3 when
+// D = 1), bufB V_{D-1} x 3, W adj_scratch_scalars(n, D, true).
 // When sharp is given (E_0 + Ccum(D-1) scalars) the sharpness of every level
 // is refined again in T and used instead of the float crease records.
 template <typename T>

@@ -359,7 +359,8 @@ __device__ __forceinline__ void adj_quad_at(const AdjArgs<T> &a, int32_t h) {
     const V3<T> g2 = quad_bcast_rec<2>(pg), e2 = quad_bcast_rec<2>(pe);
     const V3<T> g3 = quad_bcast_rec<3>(pg), e3 = quad_bcast_rec<3>(pe);
     if (!live || (h & 3) != 0) return;
-    V3<T> *gF = reinterpret_cast<V3<T> *>(a.gF);
+    const V3<T> *gF = reinterpret_cast<const V3<T> *>(a.gF);
+    V3<T> *gFo = reinterpret_cast<V3<T> *>(a.gFu ? a.gFu : a.gF);
     V3<T> gf = gF[h >> 2];
     gf.x += g0.x; gf.y += g0.y; gf.z += g0.z;
     gf.x += e0.x; gf.y += e0.y; gf.z += e0.z;
@@ -369,7 +370,7 @@ __device__ __forceinline__ void adj_quad_at(const AdjArgs<T> &a, int32_t h) {
     gf.x += e2.x; gf.y += e2.y; gf.z += e2.z;
     gf.x += g3.x; gf.y += g3.y; gf.z += g3.z;
     gf.x += e3.x; gf.y += e3.y; gf.z += e3.z;
-    gF[h >> 2] = gf;
+    gFo[h >> 2] = gf;
 }
 
 // pass 3: vertex v with fan f gathers alpha * gV[v] from the vertex rule
@@ -381,7 +382,7 @@ __device__ __forceinline__ void adj_gather_fan(const AdjArgs<T> &a, const DCage 
                                                const Fan &f, int32_t v) {
     const He *he = a.he;
     const V3<T> *gE = reinterpret_cast<const V3<T> *>(a.gEu);
-    const V3<T> *gF = reinterpret_cast<const V3<T> *>(a.gF);
+    const V3<T> *gF = reinterpret_cast<const V3<T> *>(a.gFu ? a.gFu : a.gF);
     const V3<T> gv = reinterpret_cast<const V3<T> *>(a.gV)[v];
     T alpha = 0;
     V3<T> acc{0, 0, 0};
@@ -885,11 +886,17 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
                                sharp + pl.cage.E + n.offC[d + 1]);
         }
     }
-    // G holds the gradient of level d+1 (modified in place), P receives level d
-    T *G = bufA, *P = bufB;
+    // G holds the gradient of level d+1, P receives level d.  The level
+    // passes only read G (the quad pass writes the updated face gradients to
+    // W), so the finest level reads the caller's gradient itself; the cage
+    // step updates G in place, so a cage refined once works on a copy.
+    T *G = const_cast<T *>(gradFinest), *P = bufB;
     VParam<T> *cache = sizeof(T) == sizeof(float) ? static_cast<VParam<T> *>(pl.vp_cache) : nullptr;
     const bool params = !(cache && pl.vp_valid);   // run the parameter pass
-    hipLaunchKernelGGL(k_copy<T>, dim3(grid_stride(3 * n.V[D])), dim3(256), 0, st, gradFinest, G, 3 * n.V[D]);
+    if (D == 1) {
+        G = bufA;
+        hipLaunchKernelGGL(k_copy<T>, dim3(grid_stride(3 * n.V[D])), dim3(256), 0, st, gradFinest, G, 3 * n.V[D]);
+    }
     for (int d = D - 1; d >= 1; --d) {
         AdjArgs<T> a;
         a.he = pl.he + n.offH[d];
@@ -905,6 +912,7 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
         a.W = W;
         a.VP = cache ? cache + n.offV[d] : reinterpret_cast<VParam<T> *>(W);
         a.gEu = W + 4 * n.V[d];
+        a.gFu = W + 4 * n.V[d] + 3 * n.E[d];
         a.erep = pl.erep;
         const dim3 g(grid_for(n.H[d])), gq(grid_for(n.H[d] / 4)), gv(grid_for(n.V[d])), b(256);
         const bool pv = n.V[d] <= pl.per_vertex_max;
@@ -929,7 +937,11 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
             if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 0, 1>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             else hipLaunchKernelGGL((k_adj_vertices<T, 0>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
         }
-        T *tmp = G; G = P; P = tmp;
+        // the next level reads P; the one after writes bufA (never the
+        // caller's gradient), then the two alternate
+        T *const next = d == D - 1 ? bufA : G;
+        G = P;
+        P = next;
     }
     AdjArgs<T> a;
     a.he = nullptr;

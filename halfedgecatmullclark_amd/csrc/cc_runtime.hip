@@ -482,9 +482,11 @@ bool ccd_Subd::adjoint(const void *gradFinest, void *gradCage, int32_t steps, hi
         if (adj_w) (void)hipFree(adj_w);
         adj_a = adj_b = adj_w = nullptr;
         adj_scalar = 0;
-        CC_TRY(hipMalloc(&adj_a, 3 * scalar * (size_t)n.V[D]), "hipMalloc(adjoint scratch)");
+        // level gradients: D-1 in adj_b, then D-2, D-3, ... alternating with
+        // adj_a (a cage refined once copies the finest gradient to adj_a)
+        CC_TRY(hipMalloc(&adj_a, 3 * scalar * (size_t)(D >= 2 ? n.V[D - 2] : n.V[D])), "hipMalloc(adjoint scratch)");
         CC_TRY(hipMalloc(&adj_b, 3 * scalar * (size_t)(D >= 1 ? n.V[D - 1] : 1)), "hipMalloc(adjoint scratch)");
-        CC_TRY(hipMalloc(&adj_w, scalar * (size_t)adj_scratch_scalars(n, D)), "hipMalloc(adjoint scratch)");
+        CC_TRY(hipMalloc(&adj_w, scalar * (size_t)adj_scratch_scalars(n, D, true)), "hipMalloc(adjoint scratch)");
         adj_scalar = scalar;
     }
     AdjPlan pl;
