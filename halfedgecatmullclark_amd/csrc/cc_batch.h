@@ -52,10 +52,14 @@ struct SpanJob {
 
 // One cage's adjoint step d+1 -> d (d = 0 is the cage step): he, cr and the
 // cage are offsets into the arena, the gradients and weights into the
-// adjoint arena.
+// adjoint arena; with gin set gV, gF and gE are offsets into the caller's
+// gradient of the finest points, whose address is the adjoint arena's first
+// 8 bytes (written before each adjoint run).
 struct AdjJob {
     int64_t he, cr, gV, gF, gE, gP, W, erep;
     int64_t vp;   // the level's vertex-rule parameters (kept across calls), or -1: in W
+    int64_t gFu;  // the face gradients after the quad pass, or -1: in place (gF)
+    int32_t gin;
     int64_t c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, c_vro, c_vr, c_vb, c_fro, c_fr, c_voo, c_vo;
     int64_t lp[kMaxDepth + 1];
     int32_t H, C, V;
@@ -117,9 +121,11 @@ __host__ __device__ __forceinline__ void unpack_adj(const AdjJob &J, char *arena
     a.cr = arena_ptr<const Crease>(arena, J.cr);
     a.sd = nullptr;
     a.H = J.H; a.C = J.C; a.V = J.V;
-    a.gV = arena_ptr<float>(adj, J.gV);
-    a.gF = arena_ptr<float>(adj, J.gF);
-    a.gE = arena_ptr<float>(adj, J.gE);
+    char *const g = J.gin ? *reinterpret_cast<char *const *>(adj) : adj;
+    a.gV = arena_ptr<float>(g, J.gV);
+    a.gF = arena_ptr<float>(g, J.gF);
+    a.gE = arena_ptr<float>(g, J.gE);
+    a.gFu = arena_ptr<float>(adj, J.gFu);
     a.gP = arena_ptr<float>(adj, J.gP);
     a.W = arena_ptr<float>(adj, J.W);
     a.VP = J.vp >= 0 ? arena_ptr<VParam<float>>(adj, J.vp) : reinterpret_cast<VParam<float> *>(a.W);

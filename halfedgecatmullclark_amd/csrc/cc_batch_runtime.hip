@@ -84,6 +84,11 @@ struct ccb_Batch {
     BatchTable t_fused[kMaxDepth + 1], t_vert[kMaxDepth + 1], t_adjq[kMaxDepth + 1];
     BatchTable t_adjc_e, t_adjc_f, t_adjc_v;   // cage-step adjoint gathers: per edge, face, vertex
     bool all_rings = true;                     // every cage has its ring lists
+    // the adjoint's finest steps read the caller's gradient (its address in
+    // the adjoint arena's first bytes); false when a cage is refined once:
+    // its cage step updates the finest gradient in place, so every call
+    // copies the gradient to adjA
+    bool gin_ok = true;
     // per-level kernels: kind 1 fused (fb threads per workgroup, ft tiles
     // of fb halfedges per workgroup) + vertex points (scan 0: the masked
     // kernel compiled for vw waves per SIMD; else the compacted kernel with
@@ -249,7 +254,8 @@ bool ccb_Batch::create(const cc_Mesh *const *meshes, const int32_t *depths, int3
             tot_erep += g.tab.R[g.D - 1];
         }
         g.o_adjB = tot_adjB; tot_adjB += g.n.V[g.D - 1];
-        g.o_adjW = tot_adjW; tot_adjW += adj_scratch_scalars(g.n, g.D);
+        g.o_adjW = tot_adjW; tot_adjW += adj_scratch_scalars(g.n, g.D, true);
+        if (g.D == 1) gin_ok = false;
         g.o_adjVP = tot_adjVP; tot_adjVP += g.n.cumV(g.D - 1);
         Dmax = std::max(Dmax, g.D);
     }
@@ -328,6 +334,7 @@ void ccb_Batch::drop_graphs() {
 bool ccb_Batch::alloc_adjoint() {
     if (adj_arena) return true;
     Carve cv;
+    (void)cv.take(sizeof(void *));   // offset 0: the address of the caller's finest gradient
     const size_t oA = cv.take(sizeof(float) * 3 * (size_t)tot_fin);
     const size_t oB = cv.take(sizeof(float) * 3 * (size_t)tot_adjB);
     const size_t oW = cv.take(sizeof(float) * (size_t)tot_adjW);
@@ -499,9 +506,18 @@ bool ccb_Batch::build_tables() {
                 j.W = aoff(adjW + g.o_adjW);
                 j.vp = d >= 1 ? aoff(adjVP + g.o_adjVP + g.n.offV[d]) : -1;
                 j.erep = g.o_erep >= 0 && kept ? off(erep + g.o_erep) : -1;
-                j.gV = aoff(G);
-                j.gF = aoff(G + 3 * g.n.V[d]);
-                j.gE = aoff(G + 3 * (g.n.V[d] + g.n.F[d]));
+                j.gin = gin_ok && d == g.D - 1 ? 1 : 0;
+                if (j.gin) {   // byte offsets into the caller's gradient
+                    const int64_t o = (int64_t)sizeof(float) * 3 * g.o_fin;
+                    j.gV = o;
+                    j.gF = o + (int64_t)sizeof(float) * 3 * g.n.V[d];
+                    j.gE = o + (int64_t)sizeof(float) * 3 * (g.n.V[d] + g.n.F[d]);
+                } else {
+                    j.gV = aoff(G);
+                    j.gF = aoff(G + 3 * g.n.V[d]);
+                    j.gE = aoff(G + 3 * (g.n.V[d] + g.n.F[d]));
+                }
+                j.gFu = d >= 1 ? aoff(adjW + g.o_adjW + 4 * g.n.V[d] + 3 * g.n.E[d]) : -1;
                 cage_offsets(i, j);
                 j.t = g.tab;
                 for (int k = 0; k <= kMaxDepth; ++k) j.lp[k] = (k >= 1 && k <= g.D) ? off(hb + g.n.offH[k]) : -1;
@@ -856,9 +872,14 @@ int32_t ccb_RefinePointsAdjoint(ccb_Batch *b, const float *devGradFinest, float 
     DeviceGuard g(b->device);
     if (!b->alloc_adjoint()) return last_error();
     hipStream_t st = (hipStream_t)stream;
-    if (hip_failed(hipMemcpyAsync(b->adjA, devGradFinest, sizeof(float) * 3 * (size_t)b->tot_fin,
-                                  hipMemcpyDeviceToDevice, st), "ccb_RefinePointsAdjoint"))
+    if (b->gin_ok) {   // the finest steps read the caller's gradient
+        hipLaunchKernelGGL(k_set_address, dim3(1), dim3(64), 0, st, reinterpret_cast<const void **>(b->adj_arena),
+                           devGradFinest);
+        if (hip_failed(hipGetLastError(), "ccb_RefinePointsAdjoint")) return last_error();
+    } else if (hip_failed(hipMemcpyAsync(b->adjA, devGradFinest, sizeof(float) * 3 * (size_t)b->tot_fin,
+                                         hipMemcpyDeviceToDevice, st), "ccb_RefinePointsAdjoint")) {
         return last_error();
+    }
     // the parameter pass runs on the first call for these rules only
     const int params = b->adj_vp_key != creased ? 1 : 0;
     if (!b->run(-1 - creased - 2 * params, true, creased | (params << 1), st)) return last_error();

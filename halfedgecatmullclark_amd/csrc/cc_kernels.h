@@ -20,6 +20,8 @@ __global__ void k_level_halfedges(const He *__restrict__ in, He *__restrict__ ou
                                   int32_t F, int32_t E, int topo, int uvs);
 __global__ void k_creases(const Crease *__restrict__ in, int32_t C, Crease *__restrict__ out);
 __global__ void k_zero(float *p, int64_t n);
+// *slot = p (one lane of one workgroup)
+__global__ void k_set_address(const void **slot, const void *p);
 
 __global__ void k_cage_faces(DCage c, P3 *fp, int gather);
 __global__ void k_cage_edges(DCage c, const P3 *fp, P3 *ep, int gather, int creased);

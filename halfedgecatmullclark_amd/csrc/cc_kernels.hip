@@ -56,6 +56,10 @@ __global__ void k_level_halfedges(const He *__restrict__ in, He *__restrict__ ou
 
 __global__ void k_creases(const Crease *__restrict__ in, int32_t C, Crease *__restrict__ out) { creases_at(in, C, out, (int32_t)gid()); }
 
+__global__ void k_set_address(const void **slot, const void *p) {
+    if (threadIdx.x == 0) *slot = p;
+}
+
 __global__ void k_zero(float *p, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;

@@ -287,6 +287,50 @@ def test_batch_adjoint_matches_single_cage(capi, torch):
     b.close()
 
 
+def test_batch_adjoint_reads_callers_gradient(capi, torch):
+    """With every cage refined at least twice, the batched adjoint's finest
+    steps read the caller's gradient where it lies (its address is written
+    before each run; the captured graph replays): two calls with gradients
+    in different buffers each equal the single-cage adjoints bit for bit,
+    and leave the callers' gradients unchanged."""
+    cases = [(c, d) for c, d in batch_cases() if d >= 2]
+    cages = [c for c, _ in cases]
+    depths = [d for _, d in cases]
+    b = capi.Batch(cages, depths)
+    b.refine(capi.ALL)
+    nf = b.count(capi.COUNT_FINEST_POINTS)
+    singles = []
+    for cage, depth in cases:
+        ds = capi.DeviceSubd(cage, depth)
+        ds.refine(capi.ALL)
+        singles.append(ds)
+    for seed in (5, 6):
+        gen = torch.Generator(device="cuda").manual_seed(seed)
+        g = torch.randn((nf, 3), generator=gen, device="cuda", dtype=torch.float32)
+        g0 = g.clone()
+        out = torch.empty((b.count(capi.COUNT_CAGE_POINTS), 3), device="cuda", dtype=torch.float32)
+        b.refine_points_adjoint(g.data_ptr(), out.data_ptr(), 0)
+        b.synchronize()
+        assert torch.equal(g, g0), "the batched adjoint wrote into its input"
+        got_all = out.cpu().numpy()
+        g_host = g.cpu().numpy()
+        for i, (cage, depth) in enumerate(cases):
+            o = b.offset(capi.COUNT_FINEST_POINTS, i)
+            n = b.count(capi.COUNT_FINEST_POINTS, i)
+            gi = torch.from_numpy(np.ascontiguousarray(g_host[o:o + n])).cuda()
+            gi0 = gi.clone()
+            oi = torch.empty((cage.V, 3), device="cuda", dtype=torch.float32)
+            singles[i].refine_points_adjoint(gi.data_ptr(), oi.data_ptr(), 0)
+            singles[i].synchronize()
+            assert torch.equal(gi, gi0), f"cage {i}: the adjoint wrote into its input"
+            oc = b.offset(capi.COUNT_CAGE_POINTS, i)
+            assert_bytes_equal(np.ascontiguousarray(got_all[oc:oc + cage.V]), oi.cpu().numpy(),
+                               f"seed {seed} cage {i} adjoint")
+    for ds in singles:
+        ds.close()
+    b.close()
+
+
 def test_adjoint_with_and_without_kept_bits(capi, torch):
     """The adjoint's owner scan reads the edge-representative bits only once
     a forward pass has written them.  A forward pass through the patch
