@@ -773,17 +773,19 @@ __device__ __forceinline__ void adj_cage_vertex_gather(const AdjArgs<T> &a, cons
 // ---- kernels: one thread per halfedge ------------------------------------------
 template <typename T, int CREASED>
 __global__ __launch_bounds__(256) void k_adj_params(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
-    owners_block<kAdjScan>(a.H, d, c, t, lp, blockIdx.x, a.erep, RingParamsFn<T, CREASED>{a, c, d});
+    owners_block<kAdjScan>(a.H, d, c, t, lp, xcd_block(blockIdx.x, gridDim.x, 1), a.erep,
+                           RingParamsFn<T, CREASED>{a, c, d});
 }
 
 template <typename T, int CREASED>
 __global__ __launch_bounds__(256) void k_adj_quads(AdjArgs<T> a) {
-    adj_quad_at<T, CREASED>(a, (int32_t)(blockIdx.x * blockDim.x + threadIdx.x));
+    adj_quad_at<T, CREASED>(a, (int32_t)(xcd_block(blockIdx.x, gridDim.x, 1) * blockDim.x + threadIdx.x));
 }
 
 template <typename T, int CREASED>
 __global__ __launch_bounds__(256) void k_adj_vertices(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
-    owners_block<kAdjScan>(a.H, d, c, t, lp, blockIdx.x, a.erep, VertexGatherFn<T, CREASED>{a, c, d});
+    owners_block<kAdjScan>(a.H, d, c, t, lp, xcd_block(blockIdx.x, gridDim.x, 1), a.erep,
+                           VertexGatherFn<T, CREASED>{a, c, d});
 }
 
 template <typename T, int CREASED, int GATHER_PASS>

@@ -142,12 +142,14 @@ __device__ __forceinline__ int find_job(const int32_t *__restrict__ start, int n
     return start[n + 1 + (int)b];
 }
 
-// One thread per element of each job: op(job, i) with i the job-local index.
+// One thread per element of each job: op(job, i) with i the job-local index
+// (workgroups in contiguous ranges per XCD).
 template <class Job, class Op>
 __global__ __launch_bounds__(256) void k_batch(const Job *__restrict__ jobs, const int32_t *__restrict__ start, int n,
                                                Op op) {
-    const int k = find_job(start, n, blockIdx.x);
-    const int64_t i = (int64_t)(blockIdx.x - (uint32_t)start[k]) * 256 + threadIdx.x;
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x, 1);
+    const int k = find_job(start, n, lb);
+    const int64_t i = (int64_t)(lb - (uint32_t)start[k]) * 256 + threadIdx.x;
     op(jobs[k], i);
 }
 

@@ -183,6 +183,15 @@ struct HeView {
     __host__ __device__ __forceinline__ int4 operator[](int64_t i) const { return he_load(p, i); }
 };
 
+// Contiguous block ranges per XCD: hardware hands consecutive workgroups to
+// the 8 XCDs in turn; this returns the logical block so that XCD x sees
+// blocks [x*q + min(x, r) ...) of a launch with nb = 8q + r blocks.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb, int remap) {
+    if (!remap) return b;
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, i = b >> 3;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 // ---- scalar rules (CatmullClark.h:296-392, :667-689) ----------------------
 __host__ __device__ __forceinline__ int32_t qnext(int32_t h) {
     return (int32_t)(((uint32_t)h & ~3u) | (((uint32_t)h + 1u) & 3u));

@@ -8,15 +8,6 @@
 
 namespace cch {
 
-// Contiguous block ranges per XCD: hardware hands consecutive workgroups to
-// the 8 XCDs in turn; this returns the logical block so that XCD x sees
-// blocks [x*q + min(x, r) ...) of a launch with nb = 8q + r blocks.
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb, int remap) {
-    if (!remap) return b;
-    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, i = b >> 3;
-    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
-}
-
 // Copies a small table (kernel argument or job field) into LDS, one dword
 // per thread.  Reads of LevelTab / LevelPtrs at a per-lane index become
 // vector memory loads when the table is a kernel argument, with one memory
