@@ -32,10 +32,12 @@ __device__ __forceinline__ P3 normalize3(P3 v) {
 }
 
 // corner normals, one thread per halfedge of the level (H a multiple of 4,
-// blocks of 256: the four lanes of a quad are live together)
+// blocks of 256: the four lanes of a quad are live together; workgroups in
+// contiguous ranges per XCD, so the points a vertex's quads share meet in
+// one L2)
 __global__ __launch_bounds__(256) void k_corner_normals(const He *__restrict__ he, const P3 *__restrict__ pts,
                                                         int32_t H, P3 *__restrict__ out) {
-    const int32_t h = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int32_t h = (int32_t)(xcd_block(blockIdx.x, gridDim.x, 1) * blockDim.x + threadIdx.x);
     const bool live = h < H;
     const int4 x = live ? he_gld(he, h) : make_int4(0, 0, 0, 0);
     const P3 v0 = gld(pts + x.z);
