@@ -43,6 +43,12 @@ struct AdjArgs {
     T *gFu = nullptr;
     // edge-representative bits of levels 1..d-1 (FusedArgs::erep), or null
     const uint64_t *erep = nullptr;
+    // when set: the vertex pass leaves the vertices whose ring does not
+    // follow from the child numbering (cage vertices and faces, boundary
+    // edge vertices) to the rare kernel through this list, rare[0] their
+    // count (cleared by the quad pass), rare[1 .. rare_cap] their IDs
+    int32_t *rare = nullptr;
+    int32_t rare_cap = 0;
 };
 
 // scalars of adjoint scratch (AdjArgs::W) a cage refined to depth D needs:
@@ -74,6 +80,9 @@ struct AdjPlan {
     // skipped (null: the parameters go to the scratch W, every call)
     void *vp_cache = nullptr;
     bool vp_valid = false;
+    // the vertex pass's rare-vertex list (AdjArgs::rare), or null
+    int32_t *rare = nullptr;
+    int32_t rare_cap = 0;
 };
 
 // dL/d(cage points) from dL/d(level-D points): V_D x 3 in (only read),

@@ -311,6 +311,7 @@ __device__ __forceinline__ void add_w(V3<T> &acc, T w, const V3<T> &g) {
 // the updated gradient of its face; the larger halfedge stores its edge's.
 template <typename T, int CREASED>
 __device__ __forceinline__ void adj_quad_at(const AdjArgs<T> &a, int32_t h) {
+    if (a.rare && h == 0) a.rare[0] = 0;   // the vertex pass that follows appends
     const bool live = h < a.H;   // H is a multiple of 4: quads are whole
     const int4 x = live ? he_gld(a.he, h) : make_int4(0, 0, 0, 0);
     const V3<T> *gV = reinterpret_cast<const V3<T> *>(a.gV);
@@ -503,9 +504,37 @@ struct VertexGatherFn {
     const DCage &c;
     int d;
     __device__ void operator()(const LevelTab &t, const Fan &f, int32_t v) const {
+        if (a.rare) {
+            // a ring that does not follow from the child numbering walks
+            // records: one such lane would hold up its whole wave
+            int32_t h[4], e[4];
+            if (!structural_ring(f, t, d, v, h, e)) {
+                const int32_t slot = atomicAdd(a.rare, 1);
+                if (slot < a.rare_cap) a.rare[1 + slot] = v;
+                return;
+            }
+        }
         adj_gather_fan<T, CREASED>(a, c, t, d, f, v);
     }
 };
+
+// the vertices the vertex pass listed (a.rare), one thread each, with the
+// fan from the vertex ID; the caller passes this thread's list index
+template <typename T, int CREASED, class LP>
+__device__ __forceinline__ void adj_vertices_rare(const AdjArgs<T> &a, const DCage &c, const LevelTab &t, const LP &lp,
+                                                  int d, int32_t i) {
+    __shared__ LevelTab st;
+    __shared__ LevelPtrs slp;
+    lds_fill(st, t);
+    lds_fill(slp, lp);
+    __syncthreads();
+    const int32_t n = a.rare[0] < a.rare_cap ? a.rare[0] : a.rare_cap;
+    if (i >= n) return;
+    const int32_t v = a.rare[1 + i];
+    Fan f;
+    vertex_fan(v, d, c, st, slp.he, f);
+    adj_gather_fan<T, CREASED>(a, c, st, d, f, v);
+}
 
 // ---- the cage step (level 0 -> 1) -----------------------------------------------
 // vertex rule, one thread per cage halfedge with the reference's own walk
@@ -788,6 +817,11 @@ __global__ __launch_bounds__(256) void k_adj_vertices(AdjArgs<T> a, DCage c, Lev
                            VertexGatherFn<T, CREASED>{a, c, d});
 }
 
+template <typename T, int CREASED>
+__global__ __launch_bounds__(256) void k_adj_vertices_rare(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
+    adj_vertices_rare<T, CREASED>(a, c, t, lp, d, (int32_t)(blockIdx.x * 256 + threadIdx.x));
+}
+
 template <typename T, int CREASED, int GATHER_PASS>
 __global__ __launch_bounds__(256) void k_adj_per_vertex(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
     adj_vertex_pass<T, CREASED, GATHER_PASS>(a, c, t, lp, d, (int32_t)(blockIdx.x * 256 + threadIdx.x));
@@ -918,6 +952,11 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
         a.erep = pl.erep;
         const dim3 g(grid_for(n.H[d])), gq(grid_for(n.H[d] / 4)), gv(grid_for(n.V[d])), b(256);
         const bool pv = n.V[d] <= pl.per_vertex_max;
+        if (!pv && pl.rare) {
+            a.rare = pl.rare;
+            a.rare_cap = pl.rare_cap;
+        }
+        const dim3 gr(grid_for(pl.rare_cap > 0 ? pl.rare_cap : 1));
         if (creased) {
             if (!params) {
             } else if (pv) {
@@ -928,6 +967,7 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
             hipLaunchKernelGGL((k_adj_quads<T, 1>), g, b, 0, st, a);
             if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 1, 1>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             else hipLaunchKernelGGL((k_adj_vertices<T, 1>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            if (a.rare) hipLaunchKernelGGL((k_adj_vertices_rare<T, 1>), gr, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
         } else {
             if (!params) {
             } else if (pv) {
@@ -938,6 +978,7 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
             hipLaunchKernelGGL((k_adj_quads<T, 0>), g, b, 0, st, a);
             if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 0, 1>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             else hipLaunchKernelGGL((k_adj_vertices<T, 0>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            if (a.rare) hipLaunchKernelGGL((k_adj_vertices_rare<T, 0>), gr, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
         }
         // the next level reads P; the one after writes bufA (never the
         // caller's gradient), then the two alternate
@@ -1054,6 +1095,20 @@ struct OpAdjCageFaces {
         adj_cage_faces_at<float>(a, c, (int32_t)i);
     }
 };
+// the vertices job k's vertex pass listed, one thread each
+template <int CREASED>
+__global__ __launch_bounds__(256) void k_batch_adj_vertices_rare(const AdjJob *__restrict__ jobs,
+                                                                 const int32_t *__restrict__ start, int n, int d,
+                                                                 char *arena, char *adj) {
+    const int k = find_job(start, n, blockIdx.x);
+    const AdjJob &J = jobs[k];
+    AdjArgs<float> a;
+    DCage c;
+    unpack_adj(J, arena, adj, a, c);
+    const JobLevelPtrs lp{J.lp, arena};
+    adj_vertices_rare<float, CREASED>(a, c, J.t, lp, d, (int32_t)((blockIdx.x - (uint32_t)start[k]) * 256 + threadIdx.x));
+}
+
 // the cage step as gathers; `pass` 0 weights (per halfedge), 1 edges,
 // 2 faces, 3 vertices (each thread index below the element count)
 template <int CREASED, int PASS>
@@ -1075,16 +1130,23 @@ struct OpAdjCageGather {
 // level d of every job: vertex parameters, quads (edges and faces) and
 // vertex gathers in turn; `quads` has one workgroup per 1024 halfedges of
 // each job (the owner passes), `halfedges` one per 256
-void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, char *arena, char *adj, int d, int creased,
-                     int params, hipStream_t st) {
+void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, const BatchTable &rare, char *arena, char *adj,
+                     int d, int creased, int params, hipStream_t st) {
+    const auto rare_pass = [&](auto kernel) {
+        if (rare.blocks == 0 || rare.n == 0) return;
+        hipLaunchKernelGGL(kernel, dim3(rare.blocks), dim3(256), 0, st, static_cast<const AdjJob *>(rare.jobs),
+                           rare.start, (int)rare.n, d, arena, adj);
+    };
     if (creased) {
         if (params) launch_batch<AdjJob>(quads, OpAdjParams<1>{d, arena, adj}, st);
         launch_batch<AdjJob>(halfedges, OpAdjQuads<1>{arena, adj}, st);
         launch_batch<AdjJob>(quads, OpAdjVertices<1>{d, arena, adj}, st);
+        rare_pass(k_batch_adj_vertices_rare<1>);
     } else {
         if (params) launch_batch<AdjJob>(quads, OpAdjParams<0>{d, arena, adj}, st);
         launch_batch<AdjJob>(halfedges, OpAdjQuads<0>{arena, adj}, st);
         launch_batch<AdjJob>(quads, OpAdjVertices<0>{d, arena, adj}, st);
+        rare_pass(k_batch_adj_vertices_rare<0>);
     }
 }
 

@@ -59,6 +59,8 @@ struct AdjJob {
     int64_t he, cr, gV, gF, gE, gP, W, erep;
     int64_t vp;   // the level's vertex-rule parameters (kept across calls), or -1: in W
     int64_t gFu;  // the face gradients after the quad pass, or -1: in place (gF)
+    int64_t rare; // the vertex pass's rare-vertex list (arena offset), or -1
+    int32_t rare_cap;
     int32_t gin;
     int64_t c_he, c_v2h, c_e2h, c_f2h, c_pts, c_uvs, c_cr, c_vro, c_vr, c_vb, c_fro, c_fr, c_voo, c_vo;
     int64_t lp[kMaxDepth + 1];
@@ -131,6 +133,8 @@ __host__ __device__ __forceinline__ void unpack_adj(const AdjJob &J, char *arena
     a.VP = J.vp >= 0 ? arena_ptr<VParam<float>>(adj, J.vp) : reinterpret_cast<VParam<float> *>(a.W);
     a.gEu = a.W ? a.W + 4 * (int64_t)J.V : nullptr;
     a.erep = arena_ptr<const uint64_t>(arena, J.erep);
+    a.rare = arena_ptr<int32_t>(arena, J.rare);
+    a.rare_cap = J.rare_cap;
     c = arena_cage(arena, J);
 }
 
@@ -198,7 +202,8 @@ void batch_level_vertices_rare(const BatchTable &t, char *arena, int d, int gath
 // previous call left) and vertex gathers over `quads` (the owner passes scan
 // 4 halfedges per thread), the quad pass one thread per halfedge over
 // `halfedges`
-void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, char *arena, char *adj, int d, int creased,
+void batch_adj_level(const BatchTable &quads, const BatchTable &halfedges, const BatchTable &rare, char *arena, char *adj,
+                     int d, int creased,
                      int params, hipStream_t st);
 void batch_adj_cage(const BatchTable &t, char *arena, char *adj, int creased, hipStream_t st);
 // the cage step as ordered gathers (every cage has its ring lists): one

@@ -82,6 +82,7 @@ struct ccb_Batch {
     BatchTable t_cr[kMaxDepth + 1], t_patch[kMaxDepth + 1], t_def[kMaxDepth + 1], t_adj[kMaxDepth + 1];
     BatchTable t_rare[kMaxDepth + 1];   // rare-vertex lists after the scan-0 vertex kernel
     BatchTable t_fused[kMaxDepth + 1], t_vert[kMaxDepth + 1], t_adjq[kMaxDepth + 1];
+    BatchTable t_adjr[kMaxDepth + 1];   // the adjoint vertex pass's rare-vertex lists
     BatchTable t_adjc_e, t_adjc_f, t_adjc_v;   // cage-step adjoint gathers: per edge, face, vertex
     bool all_rings = true;                     // every cage has its ring lists
     // the adjoint's finest steps read the caller's gradient (its address in
@@ -478,7 +479,7 @@ bool ccb_Batch::build_tables() {
             // has written them
             const bool kept = d >= 2 && d - 1 <= erep_kept;
             std::vector<AdjJob> jobs;
-            std::vector<int64_t> b, bq, be, bf, bv;
+            std::vector<int64_t> b, bq, br, be, bf, bv;
             for (int i = 0; i < N; ++i) {
                 const Cage &g = cages[i];
                 if (g.D <= d) continue;
@@ -518,18 +519,23 @@ bool ccb_Batch::build_tables() {
                     j.gE = aoff(G + 3 * (g.n.V[d] + g.n.F[d]));
                 }
                 j.gFu = d >= 1 ? aoff(adjW + g.o_adjW + 4 * g.n.V[d] + 3 * g.n.E[d]) : -1;
+                // the forward's rare-vertex list, free between passes
+                j.rare = d >= 1 ? off(rare + g.o_rare) : -1;
+                j.rare_cap = d >= 1 ? g.rare_cap : 0;
                 cage_offsets(i, j);
                 j.t = g.tab;
                 for (int k = 0; k <= kMaxDepth; ++k) j.lp[k] = (k >= 1 && k <= g.D) ? off(hb + g.n.offH[k]) : -1;
                 jobs.push_back(j);
                 b.push_back(blocks_of(j.H));
                 bq.push_back(blocks_of(j.H, 1024));
+                br.push_back(j.rare_cap > 0 ? blocks_of(j.rare_cap) : 0);
                 be.push_back(blocks_of(g.E0));
                 bf.push_back(blocks_of(g.F0));
                 bv.push_back(blocks_of(g.V0));
             }
             tb.add(t_adj[d], jobs, b);
             tb.add(t_adjq[d], jobs, bq);
+            tb.add(t_adjr[d], jobs, br);
             if (d == 0) {
                 tb.add(t_adjc_e, jobs, be);
                 tb.add(t_adjc_f, jobs, bf);
@@ -617,7 +623,7 @@ void ccb_Batch::record_adjoint(int flags, hipStream_t st) {
     hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<int64_t>((nc + 255) / 256, 65536)), dim3(256), 0, st, adjCage, nc);
     for (int d = Dmax - 1; d >= 1; --d)
         timed_launch("adj_level", d, st,
-                     [&] { batch_adj_level(t_adjq[d], t_adj[d], arena, adj_arena, d, creased, params, st); });
+                     [&] { batch_adj_level(t_adjq[d], t_adj[d], t_adjr[d], arena, adj_arena, d, creased, params, st); });
     if (all_rings)
         timed_launch("adj_cage", 0, st, [&] {
             batch_adj_cage_gather(t_adj[0], t_adjc_e, t_adjc_f, t_adjc_v, arena, adj_arena, creased, st);

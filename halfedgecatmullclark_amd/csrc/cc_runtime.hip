@@ -498,6 +498,8 @@ bool ccd_Subd::adjoint(const void *gradFinest, void *gradCage, int32_t steps, hi
     pl.tab = tab;
     pl.lp = level_ptrs();
     pl.per_vertex_max = opt_adj_vertex;
+    pl.rare = rare;   // the forward's rare-vertex list, free between passes
+    pl.rare_cap = rare_cap;
     bool kept = erep != nullptr;
     for (int l = 1; l <= D - 2; ++l) kept = kept && erep_valid[l];
     pl.erep = kept ? erep : nullptr;
