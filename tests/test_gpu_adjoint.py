@@ -57,6 +57,28 @@ def test_dot_product_identity(capi, name, cage, depth):
     s.close()
 
 
+@pytest.mark.parametrize("depth", [1, 2, 3])
+@pytest.mark.parametrize("name,cage", [("irregular_l2", C.irregular(2, seed=3)), ("torus_r1.3",
+                                        C.torus(6, 5, seed=4, ring_sharpness=((0, 1.3), (2, 0.4))))],
+                         ids=["irregular_l2", "torus_r1.3"])
+def test_dot_product_identity_shallow(capi, name, cage, depth):
+    """Depths 1-3: the adjoint that copies the finest gradient (a cage
+    refined once: its cage step updates the gradient in place) and the ones
+    that read it where it lies, on a cage with a boundary and polygons other
+    than quads, and on creased rings."""
+    s = capi.HostSubd(cage, depth).run("ccs_RefineHalfedges", "ccs_RefineCreases")
+    rng = np.random.default_rng(depth)
+    v = rng.normal(size=(cage.V, 3))
+    Sv = forward64(cage, depth, v)
+    u = rng.normal(size=Sv.shape)
+    g64 = s.adjoint(u, np.float64)
+    lhs, rhs = float(np.sum(u * Sv)), float(np.sum(g64 * v))
+    assert abs(lhs - rhs) <= 1e-11 * (abs(lhs) + np.abs(u).sum()), (lhs, rhs)
+    g32 = s.adjoint(u.astype(np.float32), np.float32)
+    assert float(np.abs(g32 - g64).max()) <= 1e-5 * float(np.abs(g64).max())
+    s.close()
+
+
 def test_dot_product_identity_cage_fitting_workload(capi):
     """The same identity on the cage-fitting workload itself: torus 32x16
     (2,048 halfedges, crease rings of sharpness 1.5 and 4.0) at depth 5."""
@@ -115,16 +137,18 @@ def test_central_differences(capi):
                                                ("cube_d4", C.cube(1), 4)])
 @pytest.mark.parametrize("steps", [0, 32], ids=["creased", "nocreases"])
 def test_vertex_pass_orders_agree(capi, label, cage, depth, steps):
-    """The adjoint's vertex passes over the owners in halfedge order and one
-    thread per vertex visit the same rings in the same order, and every pass
-    adds in a fixed order (no atomics): the two give the same bits, and a
-    second run repeats them."""
+    """The adjoint's vertex passes over the owners in halfedge order (with
+    the rare-vertex kernel) and one thread per vertex visit the same rings
+    in the same order, and every pass adds in a fixed order (no atomics): the
+    two give the same bits, and a second run repeats them.  The device
+    gradient, read where it lies, is left unchanged."""
     import torch
     ds = capi.DeviceSubd(cage, depth)
     ds.refine(capi.ALL)
     n = ds.download()["counts"]
     gen = torch.Generator(device="cuda").manual_seed(5)
     g = torch.randn((n["V"][depth], 3), generator=gen, device="cuda", dtype=torch.float32)
+    g0 = g.clone()
     outs = []
     for per_vertex_max in (0, 1 << 30, 0):
         ds.set_option(capi.OPT_ADJ_VERTEX, per_vertex_max)
@@ -134,6 +158,7 @@ def test_vertex_pass_orders_agree(capi, label, cage, depth, steps):
         outs.append(out.cpu().numpy())
     assert outs[0].tobytes() == outs[1].tobytes(), label
     assert outs[0].tobytes() == outs[2].tobytes(), label
+    assert torch.equal(g, g0), label
     ds.close()
 
 
