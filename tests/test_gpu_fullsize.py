@@ -54,12 +54,23 @@ def capi():
 
 @pytest.mark.parametrize("label,make,depth,steps,flags", CASES, ids=[c[0] for c in CASES])
 def test_full_size_bitexact(capi, label, make, depth, steps, flags):
+    """The full refinement, then a points-only pass over the same topology
+    (the owner bits of the first pass kept, each halfedge's side of its edge
+    through LDS), both against the oracle."""
+    import torch
     cage = make()
     ds = capi.DeviceSubd(cage, depth)
     ds.refine(steps)
     got = ds.download()
+    out = torch.empty((ds.n["Vcum"][depth], 3), device="cuda", dtype=torch.float32)
+    ds.refine_points(None, out.data_ptr(), steps & (16 | 32))   # the gather and no-crease bits
+    ds.synchronize()
+    points_only = out.cpu().numpy()
+    del out
     ds.close()
     want = O.refine(cage, depth, flags, threads=THREADS)
+    assert_bits_equal(points_only, want["points"], f"{label} points-only pass")
+    del points_only
     for k in ("halfedges", "creases", "points"):
         assert_bits_equal(got[k], want[k], f"{label} {k}")
         got[k] = want[k] = None
