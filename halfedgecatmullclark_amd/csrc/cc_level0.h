@@ -101,6 +101,19 @@ __device__ inline void creases_at(const Crease *__restrict__ in, int32_t C, Crea
 }
 
 // ==== cage points (level 0 -> 1) ===========================================
+// Faces and vertex fans of at most kListFan halfedges take their halfedges
+// from the cage's ring lists and load everything at once; longer ones walk.
+constexpr int kListFan = 8;
+
+// rank of entry i among the first m entries of r (distinct halfedge IDs):
+// its position in ascending order
+__device__ __forceinline__ int list_rank(const int32_t (&r)[kListFan], int m, int i) {
+    int k = 0;
+#pragma unroll
+    for (int j = 0; j < kListFan; ++j) k += (j < m && r[j] < r[i]) ? 1 : 0;
+    return k;
+}
+
 // face points: gather CatmullClark.h:1408-1435; scatter :1437-1463 summed per
 // face in ascending halfedge order (the order of a sequential scatter)
 __device__ inline void cage_faces_at(DCage c, P3 *fp, int gather, int32_t f) {
@@ -115,6 +128,34 @@ __device__ inline void cage_faces_at(DCage c, P3 *fp, int gather, int32_t f) {
         }
         fp[f] = mul(acc, 1.0f / n);
         return;
+    }
+    if (c.fring) {
+        // faces of up to kListFan halfedges from the ring list: every point
+        // loaded at once, then added by the rank of its halfedge
+        const int32_t o = c.fring_off[f], m = c.fring_off[f + 1] - o;
+        if (m >= 1 && m <= kListFan) {
+            int32_t r[kListFan];
+            P3 pt[kListFan];
+#pragma unroll
+            for (int i = 0; i < kListFan; ++i) r[i] = c.fring[o + (i < m ? i : m - 1)];
+#pragma unroll
+            for (int i = 0; i < kListFan; ++i) pt[i] = c.pts[c.he[r[i]].vert];
+            const float n = (float)m;
+            P3 acc = zero3();
+#pragma unroll
+            for (int k = 0; k < kListFan; ++k) {
+                if (k >= m) break;
+                P3 v = pt[0];
+#pragma unroll
+                for (int i = 0; i < kListFan; ++i)
+                    if (i < m && list_rank(r, m, i) == k) v = pt[i];
+                acc.x += v.x / n;
+                acc.y += v.y / n;
+                acc.z += v.z / n;
+            }
+            fp[f] = acc;
+            return;
+        }
     }
     float n = 1.0f;
     int32_t lo = h0;
@@ -231,70 +272,156 @@ __device__ inline void cage_vertices_gather_at(DCage c, const P3 *fp, const P3 *
     else vp[v] = lerp(old, cr, sat(avgS * 0.5f));
 }
 
+// The scatter-form vertex rule of one cage halfedge from the results of its
+// ring walk: smooth (:1742-1748) from the valence n and s = 0 on a boundary;
+// creased (:1938-1987) from the walk's valence, crease count and summed
+// sharpness, open when the walk met a boundary.
+__device__ __forceinline__ P3 cage_smooth_share(P3 old, P3 e, P3 f, int32_t n, float s) {
+    const float w = 1.0f / (float)n;
+    P3 a;
+    a.x = w * (old.x + w * s * (4.0f * e.x - f.x - 3.0f * old.x));
+    a.y = w * (old.y + w * s * (4.0f * e.y - f.y - 3.0f * old.y));
+    a.z = w * (old.z + w * s * (4.0f * e.z - f.z - 3.0f * old.z));
+    return a;
+}
+
+__device__ __forceinline__ P3 cage_creased_share(P3 old, P3 E, P3 PE, P3 F, float cw, float pcw, float avgS, float cc,
+                                                 float n, bool open) {
+    const P3 corner = mul(old, 1.0f / n);
+    P3 cr = add(mul(old, 0.25f * cw), mul(E, 0.25f * cw));
+    P3 sm = add(mul(F, -1.0f), mul(E, 4.0f));
+    sm = add(sm, mul(old, n - 3.0f));
+    sm = mul(sm, 1.0f / (n * n));
+    if (open) {
+        cc = cc + cw;
+        n = n + 1.0f;
+        cr = add(cr, add(mul(old, 0.25f * pcw), mul(PE, 0.25f * pcw)));
+    }
+    if (cc <= 1.0f) return sm;
+    if (cc >= 3.0f || n == 2.0f) return corner;
+    return lerp(corner, cr, sat(avgS * 0.5f));
+}
+
 // scatter-form contribution of cage halfedge h to its vertex point (smooth
-// :1713-1756, creased :1872-1991)
+// :1713-1756, creased :1872-1991), walking h's ring as the reference does
 __device__ inline P3 cage_vertex_contrib(const DCage &c, const P3 *fp, const P3 *ep, int creased, int32_t h) {
     const CageHe x = c.he[h];
     const P3 old = c.pts[x.vert];
-    P3 a;
     if (!creased) {
         int32_t n = 1, fw, bw;
         for (fw = c.he[x.prev].twin; fw >= 0 && fw != h && n < kWalkCap; fw = c.he[c.he[fw].prev].twin) ++n;
         for (bw = x.twin >= 0 ? c.he[x.twin].next : -1; fw < 0 && bw >= 0 && bw != h && n < kWalkCap;
              bw = c.he[bw].twin >= 0 ? c.he[c.he[bw].twin].next : -1)
             ++n;
-        const float w = 1.0f / (float)n;
-        const float s = fw < 0 ? 0.0f : 1.0f;
-        const P3 f = fp[x.face], e = ep[x.edge];
-        a.x = w * (old.x + w * s * (4.0f * e.x - f.x - 3.0f * old.x));
-        a.y = w * (old.y + w * s * (4.0f * e.y - f.y - 3.0f * old.y));
-        a.z = w * (old.z + w * s * (4.0f * e.z - f.z - 3.0f * old.z));
-    } else {
-        const int32_t p = x.prev;
-        const float ts = c.cr[x.edge].sharp, ps = c.cr[c.he[p].edge].sharp;
-        const float cw = sgn(ts), pcw = sgn(ps);
-        const P3 PE = ep[c.he[p].edge], E = ep[x.edge], F = fp[x.face];
-        float avgS = ps, cc = pcw, n = 1.0f;
-        int32_t fw, bw;
-        for (fw = c.he[p].twin; fw >= 0 && fw != h && n < (float)kWalkCap; fw = c.he[fw].twin) {
-            const int32_t q = c.he[fw].prev;
-            const float qs = c.cr[c.he[q].edge].sharp;
-            n = n + 1.0f;
-            avgS = avgS + qs;
-            cc = cc + sgn(qs);
-            fw = q;
-        }
-        for (bw = x.twin; fw < 0 && bw >= 0 && bw != h && n < (float)kWalkCap; bw = c.he[bw].twin) {
-            const int32_t q = c.he[bw].next;
-            const float qs = c.cr[c.he[q].edge].sharp;
-            n = n + 1.0f;
-            avgS = avgS + qs;
-            cc = cc + sgn(qs);
-            bw = q;
-        }
-        const P3 corner = mul(old, 1.0f / n);
-        P3 cr = add(mul(old, 0.25f * cw), mul(E, 0.25f * cw));
-        P3 sm = add(mul(F, -1.0f), mul(E, 4.0f));
-        sm = add(sm, mul(old, n - 3.0f));
-        sm = mul(sm, 1.0f / (n * n));
-        if (fw < 0) {
-            cc = cc + cw;
-            n = n + 1.0f;
-            cr = add(cr, add(mul(old, 0.25f * pcw), mul(PE, 0.25f * pcw)));
-        }
-        if (cc <= 1.0f) a = sm;
-        else if (cc >= 3.0f || n == 2.0f) a = corner;
-        else a = lerp(corner, cr, sat(avgS * 0.5f));
+        return cage_smooth_share(old, ep[x.edge], fp[x.face], n, fw < 0 ? 0.0f : 1.0f);
     }
-    return a;
+    const int32_t p = x.prev;
+    const float ts = c.cr[x.edge].sharp, ps = c.cr[c.he[p].edge].sharp;
+    const P3 PE = ep[c.he[p].edge], E = ep[x.edge], F = fp[x.face];
+    float avgS = ps, cc = sgn(ps), n = 1.0f;
+    int32_t fw, bw;
+    for (fw = c.he[p].twin; fw >= 0 && fw != h && n < (float)kWalkCap; fw = c.he[fw].twin) {
+        const int32_t q = c.he[fw].prev;
+        const float qs = c.cr[c.he[q].edge].sharp;
+        n = n + 1.0f;
+        avgS = avgS + qs;
+        cc = cc + sgn(qs);
+        fw = q;
+    }
+    for (bw = x.twin; fw < 0 && bw >= 0 && bw != h && n < (float)kWalkCap; bw = c.he[bw].twin) {
+        const int32_t q = c.he[bw].next;
+        const float qs = c.cr[c.he[q].edge].sharp;
+        n = n + 1.0f;
+        avgS = avgS + qs;
+        cc = cc + sgn(qs);
+        bw = q;
+    }
+    return cage_creased_share(old, E, PE, F, sgn(ts), sgn(ps), avgS, cc, n, fw < 0);
+}
+
+// entry j (< kListFan) of v, by selects (no register indexed at run time)
+__device__ __forceinline__ float pick_list(const float (&v)[kListFan], int j) {
+    float x = v[0];
+#pragma unroll
+    for (int m = 1; m < kListFan; ++m) x = j == m ? v[m] : x;
+    return x;
+}
+
+// A closed fan of n <= kListFan halfedges from the cage's ring list R (in
+// walk order: R[j+1] = twin(prev(R[j])), indices mod n), when the ring holds
+// every halfedge leaving v.  The walk of R[k] visits R[k+1] .. R[k+n-1], and
+// the previous edge of R[j] is the edge of R[j+1]: its valence is n, its
+// crease count the sum over the ring's edges, and its summed sharpness
+// s[k+1] + s[k+2] + ... + s[k+n] in that order (s[j] the sharpness of R[j]'s
+// edge), so every contribution follows from the ring's records.
+__device__ inline bool cage_vertex_scatter_list(const DCage &c, const P3 *fp, const P3 *ep, P3 *vp, int creased,
+                                                int32_t v) {
+    const int32_t o = c.vring_off[v], n = c.vring_off[v + 1] - o;
+    if (c.vring_bnd[v] != 0 || n < 1 || n > kListFan || c.vout_off[v + 1] - c.vout_off[v] != n) return false;
+    int32_t r[kListFan], e[kListFan], f[kListFan];
+#pragma unroll
+    for (int i = 0; i < kListFan; ++i) r[i] = c.vring[o + (i < n ? i : n - 1)];
+#pragma unroll
+    for (int i = 0; i < kListFan; ++i) {
+        const CageHe x = c.he[r[i]];
+        e[i] = x.edge;
+        f[i] = x.face;
+    }
+    float s[kListFan];
+    float cc = 0.0f;
+#pragma unroll
+    for (int i = 0; i < kListFan; ++i) {
+        s[i] = creased ? c.cr[e[i]].sharp : 0.0f;
+        if (i < n) cc = cc + sgn(s[i]);
+    }
+    const P3 old = c.pts[v];
+    P3 con[kListFan];
+#pragma unroll
+    for (int k = 0; k < kListFan; ++k) {
+        const P3 E = ep[e[k]], F = fp[f[k]];
+        if (!creased) {
+            con[k] = cage_smooth_share(old, E, F, n, 1.0f);
+            continue;
+        }
+        // k < n, i <= n: k + i < 2n
+        const float ps = pick_list(s, k + 1 < n ? k + 1 : k + 1 - n);
+        float avgS = ps;
+#pragma unroll
+        for (int i = 2; i <= kListFan; ++i)
+            if (i <= n) avgS = avgS + pick_list(s, k + i < n ? k + i : k + i - n);
+        // the previous edge's point is read only on a boundary
+        con[k] = cage_creased_share(old, E, E, F, sgn(s[k]), sgn(ps), avgS, cc, (float)n, false);
+    }
+    P3 acc = zero3();
+#pragma unroll
+    for (int k = 0; k < kListFan; ++k) {
+        if (k >= n) break;
+        P3 ck = con[0];
+#pragma unroll
+        for (int i = 0; i < kListFan; ++i)
+            if (i < n && list_rank(r, n, i) == k) ck = con[i];
+        acc = add(acc, ck);
+    }
+    vp[v] = acc;
+    return true;
 }
 
 // cage vertex points, scatter: one thread per vertex adds the contributions
 // of the halfedges leaving it from zero in ascending halfedge order, the
 // order of the reference's sequential scatter (:1750, :1986-1987), so the
-// result does not depend on scheduling
+// result does not depend on scheduling.  With the ring lists: the list form
+// above, else every halfedge of the vertex's outgoing list; without them the
+// ring walked from v's halfedge.
 __device__ inline void cage_vertices_scatter_at(DCage c, const P3 *fp, const P3 *ep, P3 *vp, int creased, int32_t v) {
     if (v >= c.V) return;
+    if (c.vring && c.vout) {
+        if (cage_vertex_scatter_list(c, fp, ep, vp, creased, v)) return;
+        P3 acc = zero3();
+        for (int32_t i = c.vout_off[v]; i < c.vout_off[v + 1]; ++i)
+            acc = add(acc, cage_vertex_contrib(c, fp, ep, creased, c.vout[i]));
+        vp[v] = acc;
+        return;
+    }
     const int32_t h0 = c.v2h[v];
     P3 acc = zero3();
     int32_t last = -1;
