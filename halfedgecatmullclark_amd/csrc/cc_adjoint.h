@@ -46,7 +46,8 @@ struct AdjArgs {
     // when set: the vertex pass leaves the vertices whose ring does not
     // follow from the child numbering (cage vertices and faces, boundary
     // edge vertices) to the rare kernel through this list, rare[0] their
-    // count (cleared by the quad pass), rare[1 .. rare_cap] their IDs
+    // count (cleared by the quad pass), rare[1 .. rare_cap] their owner
+    // halfedges
     int32_t *rare = nullptr;
     int32_t rare_cap = 0;
 };

@@ -378,6 +378,40 @@ __device__ __forceinline__ void adj_quad_at(const AdjArgs<T> &a, int32_t h) {
 // (its own parameters, from the ring), the c0 and c1 shares of its incident
 // edges from the edge rule (edge gradients after pass 2) and a quarter of
 // each incident face (face gradients after pass 2)
+// the gather of vertex v whose interior ring of four, h[q] and their edges
+// e[q], follows from the child numbering (structural_ring)
+template <typename T, int CREASED>
+__device__ __forceinline__ void adj_gather_ring(const AdjArgs<T> &a, int32_t v, const int32_t (&h)[4],
+                                                const int32_t (&e)[4]) {
+    const V3<T> *gE = reinterpret_cast<const V3<T> *>(a.gEu);
+    const V3<T> *gF = reinterpret_cast<const V3<T> *>(a.gFu ? a.gFu : a.gF);
+    const V3<T> gv = reinterpret_cast<const V3<T> *>(a.gV)[v];
+    T alpha = 0;
+    V3<T> acc{0, 0, 0};
+    T s[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s[q] = CREASED ? sharp_at(a, e[q]) : (T)0;
+    V3<T> ge[4], gf[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        ge[q] = gE[e[q]];
+        gf[q] = gF[h[q] >> 2];
+    }
+    const VParam<T> vp = vparam_of<T>(ring_params4<T, CREASED>(e, s), CREASED);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int n1 = (i + 1) & 3;
+        alpha += vparam_weights<T>(vp, s[i], s[n1], e[i], CREASED).a;
+        T c0, c1, cf, d0, d1, df;
+        edge_weights<T, CREASED>(a, e[i], false, c0, c1, cf);
+        edge_weights<T, CREASED>(a, e[n1], false, d0, d1, df);
+        add_w(acc, c0, ge[i]);
+        add_w(acc, d1, ge[n1]);
+        add_w(acc, (T)0.25, gf[i]);
+    }
+    reinterpret_cast<V3<T> *>(a.gP)[v] = {alpha * gv.x + acc.x, alpha * gv.y + acc.y, alpha * gv.z + acc.z};
+}
+
 template <typename T, int CREASED>
 __device__ __forceinline__ void adj_gather_fan(const AdjArgs<T> &a, const DCage &c, const LevelTab &t, int d,
                                                const Fan &f, int32_t v) {
@@ -387,31 +421,12 @@ __device__ __forceinline__ void adj_gather_fan(const AdjArgs<T> &a, const DCage 
     const V3<T> gv = reinterpret_cast<const V3<T> *>(a.gV)[v];
     T alpha = 0;
     V3<T> acc{0, 0, 0};
-    int32_t h[4], e[4];
-    if (structural_ring(f, t, d, v, h, e)) {   // interior ring: no boundary edge
-        T s[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) s[q] = CREASED ? sharp_at(a, e[q]) : (T)0;
-        V3<T> ge[4], gf[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            ge[q] = gE[e[q]];
-            gf[q] = gF[h[q] >> 2];
+    {
+        int32_t h[4], e[4];
+        if (structural_ring(f, t, d, v, h, e)) {   // interior ring: no boundary edge
+            adj_gather_ring<T, CREASED>(a, v, h, e);
+            return;
         }
-        const VParam<T> vp = vparam_of<T>(ring_params4<T, CREASED>(e, s), CREASED);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int n1 = (i + 1) & 3;
-            alpha += vparam_weights<T>(vp, s[i], s[n1], e[i], CREASED).a;
-            T c0, c1, cf, d0, d1, df;
-            edge_weights<T, CREASED>(a, e[i], false, c0, c1, cf);
-            edge_weights<T, CREASED>(a, e[n1], false, d0, d1, df);
-            add_w(acc, c0, ge[i]);
-            add_w(acc, d1, ge[n1]);
-            add_w(acc, (T)0.25, gf[i]);
-        }
-        reinterpret_cast<V3<T> *>(a.gP)[v] = {alpha * gv.x + acc.x, alpha * gv.y + acc.y, alpha * gv.z + acc.z};
-        return;
     }
     const VParam<T> vp = vparam_of<T>(ring_params<T, CREASED>(a, c, f), CREASED);
     if (f.n <= 4) {
@@ -490,6 +505,7 @@ __device__ __forceinline__ void adj_vertex_pass(const AdjArgs<T> &a, const DCage
 
 template <typename T, int CREASED>
 struct RingParamsFn {
+    static constexpr bool kOwners = false;
     const AdjArgs<T> &a;
     const DCage &c;
     int d;
@@ -498,28 +514,35 @@ struct RingParamsFn {
     }
 };
 
-template <typename T, int CREASED>
+// Pass 3 for the owners of a block.  RING false: every owner's vertex, any
+// fan.  RING true: only vertices whose interior ring of four follows from the
+// child numbering (no general fan code in the kernel, so it keeps fewer
+// registers and runs more waves); the other owners go to the rare list
+// (a.rare, which must be set) for k_adj_vertices_rare.
+template <typename T, int CREASED, bool RING = false>
 struct VertexGatherFn {
+    static constexpr bool kOwners = RING;
     const AdjArgs<T> &a;
     const DCage &c;
     int d;
     __device__ void operator()(const LevelTab &t, const Fan &f, int32_t v) const {
-        if (a.rare) {
-            // a ring that does not follow from the child numbering walks
-            // records: one such lane would hold up its whole wave
-            int32_t h[4], e[4];
-            if (!structural_ring(f, t, d, v, h, e)) {
-                const int32_t slot = atomicAdd(a.rare, 1);
-                if (slot < a.rare_cap) a.rare[1 + slot] = v;
-                return;
-            }
-        }
         adj_gather_fan<T, CREASED>(a, c, t, d, f, v);
+    }
+    __device__ void owner(const LevelTab &t, const He *const *levels, int32_t h) const {
+        Fan f;
+        int32_t v, r[4], e[4];
+        if (!marked_ring_fan(h, d, c, t, levels, f, v) || !structural_ring(f, t, d, v, r, e)) {
+            const int32_t slot = atomicAdd(a.rare, 1);
+            if (slot < a.rare_cap) a.rare[1 + slot] = h;
+            return;
+        }
+        adj_gather_ring<T, CREASED>(a, v, r, e);
     }
 };
 
-// the vertices the vertex pass listed (a.rare), one thread each, with the
-// fan from the vertex ID; the caller passes this thread's list index
+// the vertices the vertex pass listed (a.rare: their owner halfedges), one
+// thread each, with the fan from the owner; the caller passes this thread's
+// list index
 template <typename T, int CREASED, class LP>
 __device__ __forceinline__ void adj_vertices_rare(const AdjArgs<T> &a, const DCage &c, const LevelTab &t, const LP &lp,
                                                   int d, int32_t i) {
@@ -530,9 +553,8 @@ __device__ __forceinline__ void adj_vertices_rare(const AdjArgs<T> &a, const DCa
     __syncthreads();
     const int32_t n = a.rare[0] < a.rare_cap ? a.rare[0] : a.rare_cap;
     if (i >= n) return;
-    const int32_t v = a.rare[1 + i];
     Fan f;
-    vertex_fan(v, d, c, st, slp.he, f);
+    const int32_t v = marked_fan(a.rare[1 + i], d, c, st, slp.he, f);
     adj_gather_fan<T, CREASED>(a, c, st, d, f, v);
 }
 
@@ -811,10 +833,10 @@ __global__ __launch_bounds__(256) void k_adj_quads(AdjArgs<T> a) {
     adj_quad_at<T, CREASED>(a, (int32_t)(xcd_block(blockIdx.x, gridDim.x, 1) * blockDim.x + threadIdx.x));
 }
 
-template <typename T, int CREASED>
+template <typename T, int CREASED, bool RING>
 __global__ __launch_bounds__(256) void k_adj_vertices(AdjArgs<T> a, DCage c, LevelTab t, LevelPtrs lp, int d) {
     owners_block<kAdjScan>(a.H, d, c, t, lp, xcd_block(blockIdx.x, gridDim.x, 1), a.erep,
-                           VertexGatherFn<T, CREASED>{a, c, d});
+                           VertexGatherFn<T, CREASED, RING>{a, c, d});
 }
 
 template <typename T, int CREASED>
@@ -966,7 +988,8 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
             }
             hipLaunchKernelGGL((k_adj_quads<T, 1>), g, b, 0, st, a);
             if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 1, 1>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
-            else hipLaunchKernelGGL((k_adj_vertices<T, 1>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            else if (a.rare) hipLaunchKernelGGL((k_adj_vertices<T, 1, true>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            else hipLaunchKernelGGL((k_adj_vertices<T, 1, false>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             if (a.rare) hipLaunchKernelGGL((k_adj_vertices_rare<T, 1>), gr, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
         } else {
             if (!params) {
@@ -977,7 +1000,8 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
             }
             hipLaunchKernelGGL((k_adj_quads<T, 0>), g, b, 0, st, a);
             if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 0, 1>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
-            else hipLaunchKernelGGL((k_adj_vertices<T, 0>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            else if (a.rare) hipLaunchKernelGGL((k_adj_vertices<T, 0, true>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            else hipLaunchKernelGGL((k_adj_vertices<T, 0, false>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             if (a.rare) hipLaunchKernelGGL((k_adj_vertices_rare<T, 0>), gr, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
         }
         // the next level reads P; the one after writes bufA (never the
@@ -1063,7 +1087,9 @@ struct OpAdjVertices {
         DCage c;
         unpack_adj(J, arena, adj, a, c);
         const JobLevelPtrs lp{J.lp, arena};
-        owners_block<kAdjScan>(a.H, d, c, J.t, lp, (uint32_t)(i >> 8), a.erep, VertexGatherFn<float, CREASED>{a, c, d});
+        // every job's level d >= 1 has a rare list
+        owners_block<kAdjScan>(a.H, d, c, J.t, lp, (uint32_t)(i >> 8), a.erep,
+                               VertexGatherFn<float, CREASED, true>{a, c, d});
     }
 };
 template <int CREASED>

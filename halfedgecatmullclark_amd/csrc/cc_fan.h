@@ -508,8 +508,10 @@ __device__ __forceinline__ int32_t marked_fan(int32_t h, int d, const DCage &c, 
 
 // The owners of level-d vertices among 256 * SCAN halfedges (structural
 // test, no scan of the records), collected in LDS so that every thread then
-// works on one, in halfedge order (neighbouring rings, shared cache lines).
-// The level tables go to LDS first; the first barrier orders the copy.
+// works on one, in halfedge order (neighbouring rings, shared cache lines):
+// fn(tables, fan, vertex), or with Fn::kOwners fn.owner(tables, level
+// records, owner halfedge), which finds the fan itself.  The level tables go
+// to LDS first; the first barrier orders the copy.
 template <int SCAN, class Fn, class LP>
 __device__ __forceinline__ void owners_block(int32_t H, int d, const DCage &c, const LevelTab &t, const LP &lp,
                                              uint32_t lb, const uint64_t *erep, const Fn &fn) {
@@ -530,9 +532,13 @@ __device__ __forceinline__ void owners_block(int32_t H, int d, const DCage &c, c
     __syncthreads();
     const int32_t count = nown;
     for (int32_t i = (int32_t)threadIdx.x; i < count; i += 256) {
-        Fan f;
-        const int32_t v = marked_fan(owners[i], d, c, st, slp.he, f);
-        fn(st, f, v);
+        if constexpr (Fn::kOwners) {
+            fn.owner(st, slp.he, owners[i]);
+        } else {
+            Fan f;
+            const int32_t v = marked_fan(owners[i], d, c, st, slp.he, f);
+            fn(st, f, v);
+        }
     }
 }
 
