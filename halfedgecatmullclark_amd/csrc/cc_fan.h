@@ -736,7 +736,7 @@ __device__ __forceinline__ P3 fan_vertex_point(const Fan &f, const Acc &acc, P3 
     // of the reference's sequential scatter (CatmullClark.h:2345-2347,
     // :2610-2611)
     P3 sum = zero3();
-    if (MAXN == 4) {
+    if constexpr (MAXN == 4) {
         // fans of at most four: every contribution in fan order, the rank of
         // each fan halfedge among the fan, then the sum by rank
         P3 con[4];
@@ -757,6 +757,36 @@ __device__ __forceinline__ P3 fan_vertex_point(const Fan &f, const Acc &acc, P3 
             if (k >= n) break;
             const P3 ck = rank[0] == k ? con[0] : (rank[1] == k ? con[1] : (rank[2] == k ? con[2] : con[3]));
             sum = add(sum, ck);
+        }
+        return sum;
+    }
+    if constexpr (MAXN == 8) {
+        // the same for fans of at most eight: the entries' points are dead
+        // once the contributions are formed
+        P3 con[8];
+        int32_t key[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const FanEntry e = acc.get(i);
+            con[i] = vertex_scatter_contrib(rp, old, e.E, e.PE, e.F, CREASED ? e.ts : 0.0f, CREASED ? e.ps : 0.0f,
+                                            e.re, CREASED);
+            key[i] = i < n ? e.r : INT32_MAX;
+        }
+        int rank[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            int r = 0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) r += key[j] < key[i] ? 1 : 0;
+            rank[i] = r;
+        }
+        // rank k belongs to exactly one entry: add it there (no select
+        // between entries, which would index con at run time)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                if (k < n && rank[i] == k) sum = add(sum, con[i]);
         }
         return sum;
     }
@@ -877,9 +907,41 @@ struct FanPreloaded8 {
     }
 };
 
+// The entries of a closed fan of n <= 8 halfedges r[q] in forward order
+// (r[q+1] = twin(prev(r[q])), indices mod n) and their edges re[q]: the
+// previous edge of r[q] is the edge of r[q+1], so one record per entry is
+// read and its edge point and sharpness serve two entries.  Entries q >= n
+// repeat entry 0 and are never summed.
+__device__ __forceinline__ FanPreloaded8 closed_fan8(const FusedArgs &a, const int32_t (&r)[8], const int32_t (&re)[8],
+                                                     int n) {
+    P3 E[8], F[8];
+    float s[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        E[q] = a.ep[re[q]];
+        F[q] = a.fp[r[q] >> 2];
+        s[q] = lsharp(a.cr, a.C, re[q]);
+    }
+    FanEntry e[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const bool wrap = q + 1 >= n;   // q + 1 < 8 whenever it is false
+        const int n1 = q + 1 < 8 ? q + 1 : 0;
+        e[q].r = r[q];
+        e[q].re = re[q];
+        e[q].pe = wrap ? re[0] : re[n1];
+        e[q].E = E[q];
+        e[q].PE = wrap ? E[0] : E[n1];
+        e[q].F = F[q];
+        e[q].ts = s[q];
+        e[q].ps = wrap ? s[0] : s[n1];
+    }
+    return FanPreloaded8{e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7]};
+}
+
 // vertex point of vertex v of level d with fan f: fans of at most four (and
-// with PRE8 of at most eight) load every entry up front, longer fans entry
-// by entry
+// with PRE8 closed fans of at most eight) load every entry up front, longer
+// fans entry by entry
 template <int GATHER, int CREASED, int PRE8 = 0>
 __device__ __forceinline__ void fan_vertex_store(const FusedArgs &a, const DCage &c, const LevelTab &t, int d,
                                                  const Fan &f, int32_t v) {
@@ -900,6 +962,13 @@ __device__ __forceinline__ void fan_vertex_store(const FusedArgs &a, const DCage
         for (int q = 0; q < 4; ++q) re[q] = he4[r[q]].y;
         const FanPreloaded pre = closed_fan(a, r, re);
         a.vp[v] = fan_vertex_point<GATHER, CREASED, 4>(f, pre, old);
+    } else if (PRE8 && !f.boundary && f.n <= 8) {
+        int32_t r[8], re[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) r[q] = fan_at(f, c, q < f.n ? q : 0);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) re[q] = he4[r[q]].y;
+        a.vp[v] = fan_vertex_point<GATHER, CREASED, 8>(f, closed_fan8(a, r, re, f.n), old);
     } else if (f.n <= 4) {
         int32_t r[4];
 #pragma unroll
@@ -916,26 +985,6 @@ __device__ __forceinline__ void fan_vertex_store(const FusedArgs &a, const DCage
         pre.e2 = fan_entry(a, r[2], rec[2], prv[2]);
         pre.e3 = fan_entry(a, r[3], rec[3], prv[3]);
         a.vp[v] = fan_vertex_point<GATHER, CREASED, 4>(f, pre, old);
-    } else if (PRE8 && f.n <= 8) {
-        int32_t r[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) r[q] = fan_at(f, c, q < f.n ? q : 0);
-        int4 rec[8], prv[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            rec[q] = he4[r[q]];
-            prv[q] = he4[qprev(r[q])];
-        }
-        FanPreloaded8 pre;
-        pre.e0 = fan_entry(a, r[0], rec[0], prv[0]);
-        pre.e1 = fan_entry(a, r[1], rec[1], prv[1]);
-        pre.e2 = fan_entry(a, r[2], rec[2], prv[2]);
-        pre.e3 = fan_entry(a, r[3], rec[3], prv[3]);
-        pre.e4 = fan_entry(a, r[4], rec[4], prv[4]);
-        pre.e5 = fan_entry(a, r[5], rec[5], prv[5]);
-        pre.e6 = fan_entry(a, r[6], rec[6], prv[6]);
-        pre.e7 = fan_entry(a, r[7], rec[7], prv[7]);
-        a.vp[v] = fan_vertex_point<GATHER, CREASED, 8>(f, pre, old);
     } else {
         FanOnDemand od{&a, &f, &c};
         a.vp[v] = fan_vertex_point<GATHER, CREASED, 0>(f, od, old);
