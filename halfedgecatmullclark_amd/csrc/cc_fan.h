@@ -808,9 +808,11 @@ __device__ __forceinline__ P3 fan_vertex_point(const Fan &f, const Acc &acc, P3 
     return sum;
 }
 
-// The four entries of a closed fan from its halfedges r[q] and their edges
-// re[q]: the previous edge of r[q] is the edge of r[q+1].
-__device__ __forceinline__ FanPreloaded closed_fan(const FusedArgs &a, const int32_t (&r)[4], const int32_t (&re)[4]) {
+// The entries of a closed fan of n <= 4 halfedges r[q] and their edges
+// re[q] (indices mod n): the previous edge of r[q] is the edge of r[q+1].
+// Entries q >= n repeat entry 0 and are never summed.
+__device__ __forceinline__ FanPreloaded closed_fan(const FusedArgs &a, const int32_t (&r)[4], const int32_t (&re)[4],
+                                                   int n = 4) {
     P3 E[4], F[4];
     float s[4];
 #pragma unroll
@@ -822,15 +824,16 @@ __device__ __forceinline__ FanPreloaded closed_fan(const FusedArgs &a, const int
     FanEntry e[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
+        const bool wrap = q + 1 >= n;   // q + 1 < 4 whenever it is false
         const int n1 = (q + 1) & 3;
         e[q].r = r[q];
         e[q].re = re[q];
-        e[q].pe = re[n1];
+        e[q].pe = wrap ? re[0] : re[n1];
         e[q].E = E[q];
-        e[q].PE = E[n1];
+        e[q].PE = wrap ? E[0] : E[n1];
         e[q].F = F[q];
         e[q].ts = s[q];
-        e[q].ps = s[n1];
+        e[q].ps = wrap ? s[0] : s[n1];
     }
     return FanPreloaded{e[0], e[1], e[2], e[3]};
 }
@@ -907,13 +910,15 @@ struct FanPreloaded8 {
     }
 };
 
-// The entries of a closed fan of n <= 8 halfedges r[q] in forward order
-// (r[q+1] = twin(prev(r[q])), indices mod n) and their edges re[q]: the
-// previous edge of r[q] is the edge of r[q+1], so one record per entry is
-// read and its edge point and sharpness serve two entries.  Entries q >= n
-// repeat entry 0 and are never summed.
-__device__ __forceinline__ FanPreloaded8 closed_fan8(const FusedArgs &a, const int32_t (&r)[8], const int32_t (&re)[8],
-                                                     int n) {
+// The entries of a fan of n <= 8 halfedges r[q] in forward order
+// (r[q+1] = twin(prev(r[q]))) and their edges re[q]: the previous edge of
+// r[q] is the edge of r[q+1], so one record per entry is read and its edge
+// point and sharpness serve two entries.  The last entry's previous edge is
+// the first entry's edge (closed fan, indices mod n) or, on a boundary,
+// plast, the edge of prev(r[n-1]).  Entries q >= n repeat entry 0 and are
+// never summed.
+__device__ __forceinline__ FanPreloaded8 fan8(const FusedArgs &a, const int32_t (&r)[8], const int32_t (&re)[8], int n,
+                                              bool boundary, int32_t plast) {
     P3 E[8], F[8];
     float s[8];
 #pragma unroll
@@ -922,26 +927,30 @@ __device__ __forceinline__ FanPreloaded8 closed_fan8(const FusedArgs &a, const i
         F[q] = a.fp[r[q] >> 2];
         s[q] = lsharp(a.cr, a.C, re[q]);
     }
+    const int32_t pl = boundary ? plast : re[0];
+    const P3 El = a.ep[pl];
+    const float sl = lsharp(a.cr, a.C, pl);
     FanEntry e[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        const bool wrap = q + 1 >= n;   // q + 1 < 8 whenever it is false
+        const bool last = q + 1 >= n;   // q + 1 < 8 whenever it is false
         const int n1 = q + 1 < 8 ? q + 1 : 0;
         e[q].r = r[q];
         e[q].re = re[q];
-        e[q].pe = wrap ? re[0] : re[n1];
+        e[q].pe = last ? pl : re[n1];
         e[q].E = E[q];
-        e[q].PE = wrap ? E[0] : E[n1];
+        e[q].PE = last ? El : E[n1];
         e[q].F = F[q];
         e[q].ts = s[q];
-        e[q].ps = wrap ? s[0] : s[n1];
+        e[q].ps = last ? sl : s[n1];
     }
     return FanPreloaded8{e[0], e[1], e[2], e[3], e[4], e[5], e[6], e[7]};
 }
 
 // vertex point of vertex v of level d with fan f: fans of at most four (and
-// with PRE8 closed fans of at most eight) load every entry up front, longer
-// fans entry by entry
+// with PRE8 of at most eight) load every entry up front, longer fans entry
+// by entry; with PRE8 closed fans and fans of five to eight read one record
+// per entry
 template <int GATHER, int CREASED, int PRE8 = 0>
 __device__ __forceinline__ void fan_vertex_store(const FusedArgs &a, const DCage &c, const LevelTab &t, int d,
                                                  const Fan &f, int32_t v) {
@@ -962,13 +971,21 @@ __device__ __forceinline__ void fan_vertex_store(const FusedArgs &a, const DCage
         for (int q = 0; q < 4; ++q) re[q] = he4[r[q]].y;
         const FanPreloaded pre = closed_fan(a, r, re);
         a.vp[v] = fan_vertex_point<GATHER, CREASED, 4>(f, pre, old);
-    } else if (PRE8 && !f.boundary && f.n <= 8) {
+    } else if (PRE8 && !f.boundary && f.n <= 4) {
+        int32_t r[4], re[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[q] = fan_at(f, c, q < f.n ? q : 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) re[q] = he4[r[q]].y;
+        a.vp[v] = fan_vertex_point<GATHER, CREASED, 4>(f, closed_fan(a, r, re, f.n), old);
+    } else if (PRE8 && f.n > 4 && f.n <= 8) {
         int32_t r[8], re[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) r[q] = fan_at(f, c, q < f.n ? q : 0);
 #pragma unroll
         for (int q = 0; q < 8; ++q) re[q] = he4[r[q]].y;
-        a.vp[v] = fan_vertex_point<GATHER, CREASED, 8>(f, closed_fan8(a, r, re, f.n), old);
+        const int32_t plast = f.boundary ? he4[qprev(r[f.n - 1])].y : 0;
+        a.vp[v] = fan_vertex_point<GATHER, CREASED, 8>(f, fan8(a, r, re, f.n, f.boundary, plast), old);
     } else if (f.n <= 4) {
         int32_t r[4];
 #pragma unroll
