@@ -428,7 +428,9 @@ __device__ __forceinline__ void adj_gather_fan(const AdjArgs<T> &a, const DCage 
             return;
         }
     }
-    const VParam<T> vp = vparam_of<T>(ring_params<T, CREASED>(a, c, f), CREASED);
+    // the parameters the P pass stored for v (every vertex of the level has
+    // them by now; the quad pass read them too)
+    const VParam<T> vp = a.VP[v];
     if (f.n <= 4) {
         // a fan of at most four: every entry's loads issued before any is
         // used, the same sums in the same order as below
