@@ -433,10 +433,22 @@ __device__ __forceinline__ bool marked_ring_fan(int32_t h, int d, const DCage &c
     f.ri = 0;
     f.n = 4;
     if (digit == 2) {
-        if (lvl < 2) return false;
+        if (lvl >= 2) {
+            f.kind = 1;
+            f.a0 = 4 * p + 2;
+            v = (int32_t)t.V[lvl - 1] + (p >> 2);
+            return true;
+        }
+        // the face vertex of cage face face(p): the same ring when the face
+        // is a quad whose halfedges are p .. p+3 in next order (p, its
+        // representative, first)
+        if (p + 3 >= c.H) return false;
+        const CageHe x = c.he[p];
+        const int32_t n1 = c.he[p + 1].next, n2 = c.he[p + 2].next, n3 = c.he[p + 3].next;
+        if (x.next != p + 1 || n1 != p + 2 || n2 != p + 3 || n3 != p) return false;
         f.kind = 1;
         f.a0 = 4 * p + 2;
-        v = (int32_t)t.V[lvl - 1] + (p >> 2);
+        v = c.V + x.face;
         return true;
     }
     if (digit != 1) return false;
@@ -845,7 +857,9 @@ __device__ __forceinline__ FanPreloaded closed_fan(const FusedArgs &a, const int
 __device__ __forceinline__ bool structural_ring(const Fan &f, const LevelTab &t, int d, int32_t v, int32_t (&h)[4],
                                                 int32_t (&e)[4]) {
     const int lvl = d - (f.shift >> 1);   // the level the vertex was born at
-    if (!((f.kind == 1 && lvl >= 2) || (f.kind == 3 && !f.boundary))) return false;
+    // (kind 1 at level 1: a cage quad with halfedges p .. p+3 in next order,
+    // as marked_ring_fan checked)
+    if (!(f.kind == 1 || (f.kind == 3 && !f.boundary))) return false;
     // The ring at level lvl are children of level-(lvl-1) halfedges (the
     // cage's for lvl = 1: the cage step numbers children the same way,
     // CatmullClark.h:2693-2740), and the ring at
