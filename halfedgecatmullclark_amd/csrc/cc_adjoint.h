@@ -44,8 +44,9 @@ struct AdjArgs {
     // edge-representative bits of levels 1..d-1 (FusedArgs::erep), or null
     const uint64_t *erep = nullptr;
     // when set: the vertex pass leaves the vertices whose ring does not
-    // follow from the child numbering (cage vertices and faces, boundary
-    // edge vertices) to the rare kernel through this list, rare[0] their
+    // follow from the child numbering (cage vertices, cage faces other than
+    // quads with consecutive halfedges, boundary edge vertices) to the rare
+    // kernel through this list, rare[0] their
     // count (cleared by the quad pass), rare[1 .. rare_cap] their owner
     // halfedges
     int32_t *rare = nullptr;

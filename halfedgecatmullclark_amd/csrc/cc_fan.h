@@ -418,9 +418,11 @@ __device__ __forceinline__ bool structural_owner(int32_t h, int d, const DCage &
 }
 
 // The fan and vertex ID of a marked owner h when its vertex is a face
-// vertex born at level >= 2 or an interior edge vertex (the fans
-// structural_ring accepts), reading at most the parent's record; false for
-// any other vertex (cage vertices and faces, boundary edges).
+// vertex born at level >= 2, the face vertex of a cage quad with halfedges
+// p .. p+3 in next order, or an interior edge vertex (the fans
+// structural_ring accepts), reading at most the parent's record (the cage
+// quad's four records); false for any other vertex (cage vertices, other
+// cage faces, boundary edges).
 __device__ __forceinline__ bool marked_ring_fan(int32_t h, int d, const DCage &c, const LevelTab &t,
                                                 const He *const *levels, Fan &f, int32_t &v) {
     const int k = zero_digits(h, d);
