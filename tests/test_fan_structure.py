@@ -146,12 +146,21 @@ def birth(h, d):
 def closed_ring(h, d, cage, counts, levels):
     """Mirror of marked_ring_fan + structural_ring in cc_fan.h: the ring
     (halfedges and their edges) of the vertex represented by level-d
-    halfedge h, for face vertices born at level >= 2 and interior edge
+    halfedge h, for face vertices born at level >= 2, face vertices of cage
+    quads whose halfedges are p .. p+3 in next order, and interior edge
     vertices, with the child-0 descent d - lvl levels down in closed form;
     None for any other vertex."""
     k, lvl, digit, p = birth(h, d)
     Ep2 = 2 * counts["E"][lvl - 1]
-    if digit == 2 and lvl >= 2:
+    if digit == 2 and lvl == 1:
+        he = cage.halfedges
+        if p + 3 >= len(he) or [int(he[p + q, 1]) for q in range(4)] != [p + 1, p + 2, p + 3, p]:
+            return None
+        v = counts["V"][0] + int(he[p, 3])
+        hs = [4 * (p + q) + 2 for q in range(4)]
+        es = [Ep2 + p + ((q + 3) & 3) for q in range(4)]
+        tws = [4 * (p + ((q + 3) & 3)) + 1 for q in range(4)]
+    elif digit == 2 and lvl >= 2:
         v = counts["V"][lvl - 1] + (p >> 2)
         hs = [4 * (p + q) + 2 for q in range(4)]
         es = [Ep2 + p + ((q + 3) & 3) for q in range(4)]
@@ -190,6 +199,7 @@ def test_closed_form_rings(name, cage, depth):
     out = O.refine(cage, depth, O.HALFEDGES)
     counts = out["counts"]
     levels = {d: O.level(out, "halfedges", d) for d in range(1, depth + 1)}
+    cage_quads = 0
     for d in range(1, depth + 1):
         lv = levels[d]
         seen = 0
@@ -208,7 +218,12 @@ def test_closed_form_rings(name, cage, depth):
                 assert int(lv["edgeID"][x]) == es[q], (d, h, q, int(lv["edgeID"][x]), es[q])
                 assert int(lv["edgeID"][qprev(x)]) == es[(q + 1) & 3], (d, h, q)
             seen += 1
-        assert seen > 0 or d == 1
+            cage_quads += 1 if counts["V"][0] <= v < counts["V"][0] + counts["F"][0] else 0
+        assert seen > 0
+    # every cage here but the irregular ones is built from quads with
+    # consecutive halfedges: their face vertices take the structural ring
+    if not name.startswith("irregular"):
+        assert cage_quads == depth * counts["F"][0], (cage_quads, counts["F"][0])
 
 
 def edge_rep_bits(cage, counts, levels, depth):
