@@ -253,7 +253,7 @@ __device__ __forceinline__ void adj_params_fan(const AdjArgs<T> &a, const DCage 
                                                const Fan &f, int32_t v) {
     int32_t h[4], e[4];
     RingT<T> rp;
-    if (structural_ring(f, t, d, v, h, e)) {
+    if (structural_ring(f, t, d, v, h, e, c)) {
         T s[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) s[q] = CREASED ? sharp_at(a, e[q]) : (T)0;
@@ -423,7 +423,7 @@ __device__ __forceinline__ void adj_gather_fan(const AdjArgs<T> &a, const DCage 
     V3<T> acc{0, 0, 0};
     {
         int32_t h[4], e[4];
-        if (structural_ring(f, t, d, v, h, e)) {   // interior ring: no boundary edge
+        if (structural_ring(f, t, d, v, h, e, c)) {   // interior ring: no boundary edge
             adj_gather_ring<T, CREASED>(a, v, h, e);
             return;
         }
@@ -533,7 +533,7 @@ struct VertexGatherFn {
     __device__ void owner(const LevelTab &t, const He *const *levels, int32_t h) const {
         Fan f;
         int32_t v, r[4], e[4];
-        if (!marked_ring_fan(h, d, c, t, levels, f, v) || !structural_ring(f, t, d, v, r, e)) {
+        if (!marked_ring_fan(h, d, c, t, levels, f, v) || !structural_ring(f, t, d, v, r, e, c)) {
             const int32_t slot = atomicAdd(a.rare, 1);
             if (slot < a.rare_cap) a.rare[1 + slot] = h;
             return;
@@ -992,7 +992,7 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
             if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 1, 1>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             else if (a.rare) hipLaunchKernelGGL((k_adj_vertices<T, 1, true>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             else hipLaunchKernelGGL((k_adj_vertices<T, 1, false>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
-            if (a.rare) hipLaunchKernelGGL((k_adj_vertices_rare<T, 1>), gr, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            if (a.rare && a.rare_cap > 0) hipLaunchKernelGGL((k_adj_vertices_rare<T, 1>), gr, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
         } else {
             if (!params) {
             } else if (pv) {
@@ -1004,7 +1004,7 @@ void adjoint_levels(const AdjPlan &pl, const T *gradFinest, T *gradCage, T *bufA
             if (pv) hipLaunchKernelGGL((k_adj_per_vertex<T, 0, 1>), gv, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             else if (a.rare) hipLaunchKernelGGL((k_adj_vertices<T, 0, true>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
             else hipLaunchKernelGGL((k_adj_vertices<T, 0, false>), gq, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
-            if (a.rare) hipLaunchKernelGGL((k_adj_vertices_rare<T, 0>), gr, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
+            if (a.rare && a.rare_cap > 0) hipLaunchKernelGGL((k_adj_vertices_rare<T, 0>), gr, b, 0, st, a, pl.cage, pl.tab, pl.lp, d);
         }
         // the next level reads P; the one after writes bufA (never the
         // caller's gradient), then the two alternate

@@ -419,10 +419,10 @@ __device__ __forceinline__ bool structural_owner(int32_t h, int d, const DCage &
 
 // The fan and vertex ID of a marked owner h when its vertex is a face
 // vertex born at level >= 2, the face vertex of a cage quad with halfedges
-// p .. p+3 in next order, or an interior edge vertex (the fans
-// structural_ring accepts), reading at most the parent's record (the cage
-// quad's four records); false for any other vertex (cage vertices, other
-// cage faces, boundary edges).
+// p .. p+3 in next order, a cage vertex with a closed ring of four, or an
+// interior edge vertex (the fans structural_ring accepts), reading at most
+// the parent's record (cage data for cage vertices and faces); false for any
+// other vertex (other cage vertices and faces, boundary edges).
 __device__ __forceinline__ bool marked_ring_fan(int32_t h, int d, const DCage &c, const LevelTab &t,
                                                 const He *const *levels, Fan &f, int32_t &v) {
     const int k = zero_digits(h, d);
@@ -451,6 +451,19 @@ __device__ __forceinline__ bool marked_ring_fan(int32_t h, int d, const DCage &c
         f.kind = 1;
         f.a0 = 4 * p + 2;
         v = c.V + x.face;
+        return true;
+    }
+    if (digit == 0) {
+        // a cage vertex (p, its representative, leaves it): the ring of four
+        // from the cage's ring list when that ring is closed
+        if (!c.vring) return false;
+        const int32_t vv = c.he[p].vert;
+        const int32_t o = c.vring_off[vv];
+        if (c.vring_off[vv + 1] - o != 4 || c.vring_bnd[vv] != 0) return false;
+        f.kind = 0;
+        f.a0 = p;
+        f.a1 = o;
+        v = vv;
         return true;
     }
     if (digit != 1) return false;
@@ -857,11 +870,12 @@ __device__ __forceinline__ FanPreloaded closed_fan(const FusedArgs &a, const int
 // halfedges h[q] in fan order and their edges e[q] (the previous edge of
 // h[q] is e[q+1]); false for any other fan.
 __device__ __forceinline__ bool structural_ring(const Fan &f, const LevelTab &t, int d, int32_t v, int32_t (&h)[4],
-                                                int32_t (&e)[4]) {
+                                                int32_t (&e)[4], const DCage &c) {
     const int lvl = d - (f.shift >> 1);   // the level the vertex was born at
     // (kind 1 at level 1: a cage quad with halfedges p .. p+3 in next order,
     // as marked_ring_fan checked)
-    if (!(f.kind == 1 || (f.kind == 3 && !f.boundary))) return false;
+    if (!(f.kind == 1 || (f.kind == 3 && !f.boundary) || (f.kind == 0 && f.n == 4 && !f.boundary && c.vring)))
+        return false;
     // The ring at level lvl are children of level-(lvl-1) halfedges (the
     // cage's for lvl = 1: the cage step numbers children the same way,
     // CatmullClark.h:2693-2740), and the ring at
@@ -877,7 +891,31 @@ __device__ __forceinline__ bool structural_ring(const Fan &f, const LevelTab &t,
     // next for lvl = 1).
     const int32_t Ep2 = 2 * (int32_t)t.E[lvl - 1];
     int32_t tw[4];
-    if (f.kind == 1) {
+    if (f.kind == 0) {
+        // A cage vertex with the closed ring R[q] (the cage's ring list):
+        // children 0 at level 1, 4 R[q], with edge 2 edge(R) + (R > twin(R)
+        // ? 0 : 1) and twin 4 next(twin(R)) + 3; the descent below needs
+        // that twin outside R's quad (next(twin(R)) != R).
+        int32_t R[4], w[4], en[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) R[q] = c.vring[f.a1 + q];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const CageHe x = c.he[R[q]];
+            w[q] = x.twin;
+            en[q] = x.edge;
+        }
+        bool ok = true;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int32_t nt = c.he[w[q] >= 0 ? w[q] : R[q]].next;
+            ok = ok && w[q] >= 0 && nt != R[q];
+            h[q] = 4 * R[q];
+            e[q] = 2 * en[q] + (R[q] > w[q] ? 0 : 1);
+            tw[q] = 4 * nt + 3;
+        }
+        if (!ok) return false;
+    } else if (f.kind == 1) {
         const int32_t p = (f.a0 - 2) >> 2;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -973,7 +1011,7 @@ __device__ __forceinline__ void fan_vertex_store(const FusedArgs &a, const DCage
     const HeView he4{a.he};
     const P3 old = a.old[v];
     int32_t h[4], e[4];
-    if (structural_ring(f, t, d, v, h, e)) {
+    if (structural_ring(f, t, d, v, h, e, c)) {
         const FanPreloaded pre = closed_fan(a, h, e);
         a.vp[v] = fan_vertex_point<GATHER, CREASED, 4>(f, pre, old);
     } else if (f.kind == 1 || (f.kind == 3 && !f.boundary)) {

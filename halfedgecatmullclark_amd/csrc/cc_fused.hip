@@ -330,16 +330,16 @@ void launch_level_vertices(const FusedArgs &a, const DCage &c, const LevelTab &t
         const dim3 g0((unsigned)((waves + 3) / 4)), gr((unsigned)((a.rare_cap + 255) / 256));
         if (gather && creased) {
             launch_ring<1, 1>(a, c, t, lp, d, waves, g0, st);
-            hipLaunchKernelGGL((k_level_vertices_rare<1, 1>), gr, block, 0, st, a, c, t, lp, d);
+            if (a.rare_cap > 0) hipLaunchKernelGGL((k_level_vertices_rare<1, 1>), gr, block, 0, st, a, c, t, lp, d);
         } else if (gather) {
             launch_ring<1, 0>(a, c, t, lp, d, waves, g0, st);
-            hipLaunchKernelGGL((k_level_vertices_rare<1, 0>), gr, block, 0, st, a, c, t, lp, d);
+            if (a.rare_cap > 0) hipLaunchKernelGGL((k_level_vertices_rare<1, 0>), gr, block, 0, st, a, c, t, lp, d);
         } else if (creased) {
             launch_ring<0, 1>(a, c, t, lp, d, waves, g0, st);
-            hipLaunchKernelGGL((k_level_vertices_rare<0, 1>), gr, block, 0, st, a, c, t, lp, d);
+            if (a.rare_cap > 0) hipLaunchKernelGGL((k_level_vertices_rare<0, 1>), gr, block, 0, st, a, c, t, lp, d);
         } else {
             launch_ring<0, 0>(a, c, t, lp, d, waves, g0, st);
-            hipLaunchKernelGGL((k_level_vertices_rare<0, 0>), gr, block, 0, st, a, c, t, lp, d);
+            if (a.rare_cap > 0) hipLaunchKernelGGL((k_level_vertices_rare<0, 0>), gr, block, 0, st, a, c, t, lp, d);
         }
         return;
     }

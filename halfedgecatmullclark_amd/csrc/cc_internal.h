@@ -92,6 +92,33 @@ inline CageRings build_cage_rings(const cc_Mesh *m) {
     return r;
 }
 
+// True when every vertex of every level of this cage takes a structural ring
+// in the vertex passes (marked_ring_fan + structural_ring), so nothing is
+// left for the rare-vertex kernels: no boundary, every cage vertex with a
+// closed ring of four whose halfedges R have a twin with next(twin(R)) != R,
+// and every cage face a quad whose halfedges are f2h .. f2h+3 in next order.
+inline bool rings_without_rare(const cc_Mesh *m, const CageRings &r) {
+    const int32_t V = m->vertexCount, F = m->faceCount, H = m->halfedgeCount;
+    const CageHe *he = reinterpret_cast<const CageHe *>(m->halfedges);
+    for (int32_t h = 0; h < H; ++h)
+        if (he[h].twin < 0 || he[h].twin >= H) return false;
+    for (int32_t v = 0; v < V; ++v) {
+        if (r.voff[v + 1] - r.voff[v] != 4 || r.vbnd[v] != 0) return false;
+        for (int32_t i = r.voff[v]; i < r.voff[v + 1]; ++i) {
+            const int32_t R = r.vring[i], w = he[R].twin;
+            const int32_t nt = he[w].next;
+            if (nt < 0 || nt >= H || nt == R) return false;
+        }
+    }
+    for (int32_t f = 0; f < F; ++f) {
+        const int32_t p = m->faceToHalfedgeIDs[f];
+        if (p < 0 || p + 3 >= H) return false;
+        if (he[p].next != p + 1 || he[p + 1].next != p + 2 || he[p + 2].next != p + 3 || he[p + 3].next != p)
+            return false;
+    }
+    return true;
+}
+
 // host-memory view of a cage with the device-view type (for the O(depth) maps)
 inline DCage host_cage_view(const cc_Mesh *m) {
     DCage c;

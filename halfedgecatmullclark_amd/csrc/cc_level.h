@@ -407,7 +407,7 @@ struct RingVertexFn {
     __device__ void operator()(int32_t h) const {
         Fan f;
         int32_t v, r[4], e[4];
-        if (!marked_ring_fan(h, d, c, t, lp.he, f, v) || !structural_ring(f, t, d, v, r, e)) {
+        if (!marked_ring_fan(h, d, c, t, lp.he, f, v) || !structural_ring(f, t, d, v, r, e, c)) {
             const int32_t slot = atomicAdd(a.rare, 1);
             if (slot < a.rare_cap) a.rare[1 + slot] = h;
             return;

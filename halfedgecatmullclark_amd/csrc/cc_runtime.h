@@ -38,6 +38,9 @@ struct ccd_Subd {
     int32_t *rare = nullptr;
     int32_t rare_cap = 0;
     bool rings_ok = false;
+    // no vertex of this cage can reach the rare lists (rings_without_rare):
+    // the vertex passes get a capacity of 0 and no rare kernel runs
+    bool rare_free = false;
     // levels 1..D, cumulative, laid out as cc_Subd's arrays
     cch::He *he = nullptr;
     cch::Crease *cr = nullptr;

@@ -156,6 +156,7 @@ bool ccd_Subd::upload_cage(const cc_Mesh *cage, hipStream_t st) {
     // vertex rings | face rings], each list padded by one entry
     const CageRings r = build_cage_rings(cage);
     rings_ok = r.ok;
+    rare_free = r.ok && rings_without_rare(cage, r);
     if (!r.ok) return true;
     std::vector<int32_t> rings_host;
     auto put = [&](const std::vector<int32_t> &v) {
@@ -338,7 +339,7 @@ void ccd_Subd::level_fused(int d, int32_t steps, int topo, int uvMode, P3 *point
         a.owner_mask = own_masks + own_off[d];
         a.owner_write = own_valid[d] && !topo ? 0 : 1;
         a.rare = rare;
-        a.rare_cap = rare_cap;
+        a.rare_cap = rare_free ? 0 : rare_cap;   // 0: the list stays empty, no rare kernel
     }
     // edge-representative bits: derive level d+1's while they are still
     // needed (levels up to D-2), read levels 1..d-1 in the owner test
@@ -499,7 +500,7 @@ bool ccd_Subd::adjoint(const void *gradFinest, void *gradCage, int32_t steps, hi
     pl.lp = level_ptrs();
     pl.per_vertex_max = opt_adj_vertex;
     pl.rare = rare;   // the forward's rare-vertex list, free between passes
-    pl.rare_cap = rare_cap;
+    pl.rare_cap = rare_free ? 0 : rare_cap;
     bool kept = erep != nullptr;
     for (int l = 1; l <= D - 2; ++l) kept = kept && erep_valid[l];
     pl.erep = kept ? erep : nullptr;

@@ -148,11 +148,29 @@ def closed_ring(h, d, cage, counts, levels):
     (halfedges and their edges) of the vertex represented by level-d
     halfedge h, for face vertices born at level >= 2, face vertices of cage
     quads whose halfedges are p .. p+3 in next order, and interior edge
-    vertices, with the child-0 descent d - lvl levels down in closed form;
-    None for any other vertex."""
+    vertices, and cage vertices with a closed ring of four, with the child-0
+    descent d - lvl levels down in closed form; None for any other vertex."""
     k, lvl, digit, p = birth(h, d)
     Ep2 = 2 * counts["E"][lvl - 1]
-    if digit == 2 and lvl == 1:
+    if digit == 0:
+        # a cage vertex: its closed ring of four (forward walk from p), each
+        # R with a twin whose next is not R itself
+        he = cage.halfedges
+        ring, g = [p], int(he[he[p, 2], 0])
+        while g >= 0 and g != p and len(ring) <= 4:
+            ring.append(g)
+            g = int(he[he[g, 2], 0])
+        if g != p or len(ring) != 4:
+            return None
+        tw0 = [int(he[R, 0]) for R in ring]
+        nts = [int(he[w, 1]) for w in tw0]
+        if any(nt == R for nt, R in zip(nts, ring)):
+            return None
+        v = int(he[p, 5])
+        hs = [4 * R for R in ring]
+        es = [2 * int(he[R, 4]) + (0 if R > w else 1) for R, w in zip(ring, tw0)]
+        tws = [4 * nt + 3 for nt in nts]
+    elif digit == 2 and lvl == 1:
         he = cage.halfedges
         if p + 3 >= len(he) or [int(he[p + q, 1]) for q in range(4)] != [p + 1, p + 2, p + 3, p]:
             return None
@@ -199,7 +217,7 @@ def test_closed_form_rings(name, cage, depth):
     out = O.refine(cage, depth, O.HALFEDGES)
     counts = out["counts"]
     levels = {d: O.level(out, "halfedges", d) for d in range(1, depth + 1)}
-    cage_quads = 0
+    cage_quads = cage_verts = 0
     for d in range(1, depth + 1):
         lv = levels[d]
         seen = 0
@@ -219,11 +237,15 @@ def test_closed_form_rings(name, cage, depth):
                 assert int(lv["edgeID"][qprev(x)]) == es[(q + 1) & 3], (d, h, q)
             seen += 1
             cage_quads += 1 if counts["V"][0] <= v < counts["V"][0] + counts["F"][0] else 0
+            cage_verts += 1 if v < counts["V"][0] else 0
         assert seen > 0
     # every cage here but the irregular ones is built from quads with
-    # consecutive halfedges: their face vertices take the structural ring
+    # consecutive halfedges: their face vertices take the structural ring;
+    # every torus vertex has a closed ring of four
     if not name.startswith("irregular"):
         assert cage_quads == depth * counts["F"][0], (cage_quads, counts["F"][0])
+    if name.startswith("torus"):
+        assert cage_verts == depth * counts["V"][0], (cage_verts, counts["V"][0])
 
 
 def edge_rep_bits(cage, counts, levels, depth):
